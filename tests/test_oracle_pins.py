@@ -1,0 +1,215 @@
+"""Pins of the oracle that the golden vectors leave open: known answers, closed forms, invariants.
+
+Each check is chosen so that a plausible slip (dropped term, wrong sign or index, transposed axis)
+fails it; none restates the formula under test.
+"""
+import numpy as np
+import pytest
+
+import golden as G
+import oracle as O
+
+
+# ---------------------------------------------------------------- FFT
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6, 8, 12, 16, 24, 30, 48, 64, 96, 7 * 11])
+def test_fft1_brute_force(n):
+    rng = np.random.default_rng(n)
+    x = rng.normal(size=n) + 1j * rng.normal(size=n)
+    m = np.arange(n)
+    dft = np.exp(-2j * np.pi * np.outer(m, m) / n) @ x
+    assert np.max(np.abs(O.fft1(x, -1) - dft)) < 1e-12 * max(1, np.max(np.abs(dft)))
+    idft = np.exp(2j * np.pi * np.outer(m, m) / n) @ x
+    assert np.max(np.abs(O.fft1(x, +1) - idft)) < 1e-12 * max(1, np.max(np.abs(idft)))
+
+
+@pytest.mark.parametrize("N", [4, 8, 12, 16, 24, 32])
+def test_rfft3_vs_numpy_layout(N):
+    # column-major (N,N,N) [ix,iy,iz] halved along z == numpy C-order [iz,iy,ix] rfftn over the
+    # first axis last: numpy.fft.rfftn(a.transpose(2,1,0)) is halved along the C-order last axis.
+    rng = np.random.default_rng(N)
+    f = rng.normal(size=(N, N, N))                  # [iz, iy, ix]
+    ours = O.rfft3(f)                                # [kz, ky, kx]
+    ref = np.fft.rfftn(f.transpose(2, 1, 0))         # [ix,iy,iz] -> [kx, ky, kz]
+    assert np.max(np.abs(ours - ref.transpose(2, 1, 0))) < 1e-12 * np.max(np.abs(ref))
+    back = O.brfft3(ours) / N**3
+    assert np.max(np.abs(back - f)) < 1e-13 * np.max(np.abs(f))
+
+
+def test_brfft3_takes_real_part_of_dc_and_nyquist_planes():
+    # c2r of a non-Hermitian half spectrum: numpy's irfftn (c2c on x, y, then c2r on z) is the
+    # semantics of the reference's FFTW/JAX path
+    N = 8
+    rng = np.random.default_rng(3)
+    s = rng.normal(size=(N // 2 + 1, N, N)) + 1j * rng.normal(size=(N // 2 + 1, N, N))
+    ref = np.fft.irfftn(s.transpose(2, 1, 0), s=(N, N, N), axes=(0, 1, 2)) * N**3
+    assert np.max(np.abs(O.brfft3(s) - ref.transpose(2, 1, 0))) < 1e-12 * np.max(np.abs(ref))
+
+
+def test_rfft3_float32_close_to_float64():
+    rng = np.random.default_rng(1)
+    f = rng.normal(size=(32, 32, 32))
+    a, b = O.rfft3(f), O.rfft3(f.astype(np.float32))
+    assert b.dtype == np.complex64
+    assert np.max(np.abs(a - b)) < 2e-5 * np.max(np.abs(a))
+
+
+# ---------------------------------------------------------------- ranlxd1 / N-GenIC
+def test_ranlxd1_known_answers():
+    # runtests.jl:409-414
+    k = G.KNOWN
+    u = O.ranlxd1_uniforms(42, 5)
+    assert np.max(np.abs(u - np.array(k["ranlxd1_seed42_first5"]))) <= k["ranlxd1_atol"]
+
+
+def test_ngenic_field_known_answer():
+    # runtests.jl:415-420
+    k = G.KNOWN
+    f = O.ngenic_field_gsl(42, 8)
+    assert f.shape[::-1] == tuple(k["ngenic_field_gsl_42_8_shape"])
+    assert abs(np.sqrt(np.sum(np.abs(f) ** 2)) - k["ngenic_field_gsl_42_8_norm"]) < k["ngenic_field_gsl_42_8_norm_atol"]
+    w = O.ngenic_wnoise_real(42, 8)
+    assert 0.5 < np.sqrt(np.mean(w**2)) < 1.5
+
+
+def test_ngenic_noise_unit_variance_and_hermitian():
+    w = O.ngenic_wnoise_real(7, 32)
+    assert abs(np.std(w) - 1) < 0.05 and abs(np.mean(w)) < 0.05
+    f = O.ngenic_field_gsl(7, 16)
+    # kz = 0 plane Hermitian: F(-kx, -ky) = conj F(kx, ky) away from the Nyquist lines
+    p = f[0]
+    for ky in range(1, 8):
+        for kx in range(1, 8):
+            assert p[ky, kx] == np.conj(p[(16 - ky) % 16, (16 - kx) % 16])
+
+
+# ---------------------------------------------------------------- transfer function and P(k)
+def test_eisenstein_hu_limits():
+    c = O.Cosmology(0.2621, 0.049, 0.6766, 0.8102, 0.9665)
+    T = O.eisenstein_hu(c, np.array([1e-6, 1e-5, 1e-4, 1.0, 10.0, 100.0]))
+    assert abs(T[0] - 1) < 1e-4 and abs(T[1] - 1) < 1e-3  # T -> 1 as k -> 0
+    assert np.all(np.diff(T) < 0) and T[-1] > 0            # monotone decreasing, positive
+
+
+def test_sigma8_normalisation_recomputed():
+    # runtests.jl:70-86: sigma8 recomputed from the normalised table by an independent integrator
+    c = O.Cosmology(0.2621, 0.049, 0.6766, 0.8102, 0.9665)
+    k, pk = O.linear_power_spectrum(c)
+    x = k * 8.0
+    W = 3 * (np.sin(x) - x * np.cos(x)) / x**3
+    lnk = np.log(k)
+    from scipy.integrate import simpson
+    s8 = np.sqrt(simpson(k**3 * pk * W**2, x=lnk) / (2 * np.pi**2))
+    assert s8 == pytest.approx(0.8102, rel=1e-2)
+
+
+def test_power_spectrum_single_mode():
+    # a single cosine mode delta = A cos(k1 x) has P = A^2 V / 4 in its bin and zero elsewhere
+    N, L, A = 32, 100.0, 0.3
+    x = np.arange(N) * L / N
+    field = np.broadcast_to(A * np.cos(2 * np.pi * 3 * x / L)[None, None, :], (N, N, N)).copy()
+    k, pk, nm = O.power_spectrum(field, L, bins=N, logarithmic=False)
+    kf = 2 * np.pi / L
+    edges = np.linspace(kf, np.pi * N / L * np.sqrt(3), N + 1)
+    b = np.searchsorted(edges, 3 * kf, side="right") - 1
+    # |F(+-k1)|^2 V / N^6 = A^2 V / 4 at the two kz = 0 entries kx = +-3 (weight 1/2 each); the bin
+    # average divides by the weight sum of all half-spectrum entries in the bin (enumerated here)
+    m = np.arange(N)
+    kx = np.where(m <= N // 2, m, m - N) * kf
+    kz = np.arange(N // 2 + 1) * kf
+    kk = np.sqrt(kz[:, None, None] ** 2 + kx[None, :, None] ** 2 + kx[None, None, :] ** 2)
+    w = np.ones_like(kk)
+    w[0] = w[-1] = 0.5
+    inbin = (kk >= edges[b]) & (kk < edges[b + 1])
+    assert pk[b] == pytest.approx(A**2 * L**3 / 4 * 1.0 / w[inbin].sum(), rel=1e-10)
+    assert nm[b] == inbin.sum()
+    assert np.sum(pk > 1e-20) == 1
+
+
+def test_power_spectrum_white_noise_amplitude():
+    # runtests.jl:174-186: unit white noise on N^3 cells of volume (L/N)^3 has P = (L/N)^3
+    N, L = 32, 200.0
+    w = np.random.default_rng(0).normal(size=(N, N, N))
+    k, pk, nm = O.power_spectrum_lin(w, L, W=0)
+    assert np.mean(pk) == pytest.approx((L / N) ** 3, rel=0.03)
+
+
+# ---------------------------------------------------------------- PM force invariants
+def test_lean_equals_solver():
+    # pm_lean_tests.jl:5-21 on other random inputs (the reference's MersenneTwister stream is Julia-only)
+    n, res, L = 24, 48, 120.0
+    psi = 3 * np.random.default_rng(5).normal(size=(3, n, n, n))
+    for W, g in ((2, 4), (3, 2), (4, 6)):
+        cfg = O.pm_config(res_pm=res, n_part=n, boxsize=L, worder=W, grad_order=g)
+        a = O.pm_acceleration(psi, cfg, "solver")
+        b = O.pm_acceleration(psi, cfg, "lean")
+        assert G.rel(b, a) < 1e-12
+        a32 = O.pm_acceleration(psi.astype(np.float32), cfg, "solver")
+        b32 = O.pm_acceleration(psi.astype(np.float32), cfg, "lean")
+        assert G.rel(b32, a32) < 2e-5
+    cfg = O.pm_config(res_pm=res, n_part=n, boxsize=L, worder=2, deconvolve=True)
+    assert G.rel(O.pm_acceleration(psi, cfg, "lean"), O.pm_acceleration(psi, cfg, "solver")) < 2e-5
+    with pytest.raises(ValueError):
+        O.pm_acceleration(psi, O.pm_config(res_pm=res, n_part=n, boxsize=L, antialias=1), "lean")
+
+
+def test_fd_stencil_equals_kspace_symbol():
+    # circulant identity (pm.jl:309-311): real-space FD gradient == k-space FD symbol, all orders
+    n, res, L = 16, 32, 100.0
+    psi = 2 * np.random.default_rng(9).normal(size=(3, n, n, n))
+    for g in (2, 4, 6):
+        cfg = O.pm_config(res_pm=res, n_part=n, boxsize=L, worder=3, grad_order=g, antialias=1)
+        assert G.rel(O.pm_acceleration(psi, cfg, "solver"), O.pm_acceleration(psi, cfg, "reference")) < 1e-12
+
+
+def test_cic_mass_conservation_and_gather_adjoint():
+    # runtests.jl:430; and <S x, m> = <x, G m>: scatter and gather are transposes
+    rng = np.random.default_rng(2)
+    X = rng.uniform(-50, 150, size=(3, 500))
+    for W in (2, 3, 4):
+        m = O.pm_scatter(X, 16, 100.0, W)
+        assert m.sum() == pytest.approx(500, rel=1e-12)
+        f = rng.normal(size=(16, 16, 16))
+        g = O.pm_gather(f, X, 100.0, W)
+        assert np.sum(m * f) == pytest.approx(np.sum(g), rel=1e-11)
+
+
+def test_fixed_point_deposit_matches_float_and_is_order_free():
+    rng = np.random.default_rng(4)
+    X = rng.uniform(0, 100, size=(3, 4000)).astype(np.float32)
+    a = O.pm_scatter_fixed(X, 16, np.float32(100.0), 3)
+    perm = rng.permutation(4000)
+    b = O.pm_scatter_fixed(np.ascontiguousarray(X[:, perm]), 16, np.float32(100.0), 3)
+    assert np.array_equal(a, b)
+    m = O.pm_scatter(X.astype(np.float64), 16, 100.0, 3)
+    assert np.max(np.abs(a / 2.0**32 - m)) < 1e-5
+
+
+def test_plane_wave_linear_growth():
+    # nbody_parity_tests.jl:68-90 for BullFrog with one and 16 steps and FastPM with 16 steps
+    c = O.Cosmology.named("Planck18EEBAOSN")
+    L, a0 = 100.0, 0.05
+    D0, D1 = c.Dplus(a0), c.Dplus(1.0)
+    err = {}
+    for n in (16, 32):
+        q = np.arange(n) * L / n
+        psi1 = np.zeros((3, n, n, n))
+        psi1[0] = 1e-3 * np.sin(2 * np.pi * q / L)[None, None, :]
+        for st, ns in (("bullfrog", 1), ("bullfrog", 16), ("fastpm", 16)):
+            cfg = O.pm_config(res_pm=2 * n, n_part=n, boxsize=L)
+            psi, _ = O.run_nbody(c, D0 * psi1, psi1.copy(), cfg, a0, 1.0, ns, stepper=st)
+            assert np.max(np.abs(psi[1:])) < 1e-12
+            err[(n, st, ns)] = abs(np.sum(psi[0] * psi1[0]) / np.sum(psi1[0] ** 2) / D1 - 1)
+    for st, ns in (("bullfrog", 1), ("bullfrog", 16), ("fastpm", 16)):
+        assert err[(32, st, ns)] < 1e-2
+        assert err[(16, st, ns)] / err[(32, st, ns)] > 3
+
+
+def test_callback_steps():
+    # pm_lean_tests.jl:30-33 shape: the callback sees k = 0..n_steps in order
+    c = O.Cosmology.named("Planck18EEBAOSN")
+    n = 8
+    psi = 0.05 * np.random.default_rng(2).normal(size=(3, n, n, n))
+    seen = []
+    O.run_nbody(c, psi, psi / 0.05, O.pm_config(res_pm=16, n_part=n, boxsize=100.0), 0.05, 0.5, 10,
+                lean=True, callback=lambda k, a: seen.append(k))
+    assert seen == list(range(11))
