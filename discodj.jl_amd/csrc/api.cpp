@@ -1,0 +1,706 @@
+// C ABI of the PM N-body step (include/discodj_hip.h): context set-up and orchestration.
+//
+// One force evaluation (pm_acceleration!, NR/src/nbody/pm.jl:466-521) per interlacing shift:
+//   zero mesh -> scatter -> XF -> YF -> ZM(K_phi) -> YI -> XI (potential) -> FD-fused gather
+// For grad_order 0 (ik gradient) the spectrum is kept after the forward z pass and three inverse
+// sequences produce -d_a phi, each gathered into one acceleration component (pm.jl:487-492).
+// In dj_run the gather of the last shift also applies the kick, the second drift and, unless a
+// callback needs the state in between, the next step's first drift.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/discodj_hip.h"
+#include "dj_internal.h"
+#include "fft_passes.h"
+
+using namespace dj;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+enum KernelId { K_ZERO = 0, K_SCATTER, K_XF, K_YF, K_ZM, K_YI, K_XI, K_GATHER, K_KICK, K_DRIFT, K_COUNT };
+const char *kernel_names[K_COUNT] = {"zero", "scatter", "fft_x_r2c", "fft_y_fwd", "fft_z_mult",
+                                     "fft_y_inv", "fft_x_c2r", "gather", "kick_drift", "drift"};
+
+struct ProfEvent {
+    int id;
+    hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct dj_ctx {
+    dj_pm_config cfg{};
+    int tsize = 4;  // bytes of the working precision
+    int dev = 0;
+    hipStream_t st = nullptr;
+    FftOps fft{};
+    PartOps part{};
+    int n = 0, res = 0;
+    int64_t np = 0, n3 = 0;
+    size_t M = 0, HC = 0;
+    void *psi = nullptr, *mom = nullptr, *acc = nullptr, *psi_in = nullptr, *q1 = nullptr;
+    void *mesh = nullptr, *meshfix = nullptr, *A = nullptr, *B = nullptr, *C = nullptr, *tw = nullptr;
+    void *tabs = nullptr;
+    alignas(16) unsigned char kmult[sizeof(KMult<double>)];
+    double shifts[16][3];
+    int nshifts = 1;
+    double fdc[3] = {0, 0, 0};
+    std::string err;
+    bool prof = false;
+    std::vector<ProfEvent> pending;
+    std::vector<hipEvent_t> pool;
+    int calls[K_COUNT] = {};
+    double ms[K_COUNT] = {};
+    double bytes[K_COUNT] = {};
+};
+
+namespace {
+
+dj_status fail(dj_ctx *c, dj_status code, const std::string &msg) {
+    if (c) c->err = msg;
+    else g_create_error = msg;
+    return code;
+}
+
+dj_status hip_fail(dj_ctx *c, hipError_t e, const char *where) {
+    return fail(c, e == hipErrorOutOfMemory ? DJ_ERR_OUT_OF_MEMORY : DJ_ERR_DEVICE,
+                std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCHK(ctx, expr)                                   \
+    do {                                                    \
+        hipError_t e_ = (expr);                             \
+        if (e_ != hipSuccess) return hip_fail(ctx, e_, #expr); \
+    } while (0)
+
+hipEvent_t take_event(dj_ctx *c) {
+    if (!c->pool.empty()) {
+        hipEvent_t e = c->pool.back();
+        c->pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+struct Prof {
+    dj_ctx *c;
+    int id;
+    hipEvent_t a = nullptr;
+    Prof(dj_ctx *c_, int id_) : c(c_), id(id_) {
+        if (c->prof) {
+            a = take_event(c);
+            if (a) hipEventRecord(a, c->st);
+        }
+    }
+    ~Prof() {
+        if (c->prof && a) {
+            hipEvent_t b = take_event(c);
+            if (b) {
+                hipEventRecord(b, c->st);
+                c->pending.push_back({id, a, b});
+            }
+        }
+    }
+};
+
+// collect finished event pairs (call after a stream synchronisation)
+void drain_prof(dj_ctx *c) {
+    for (auto &p : c->pending) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            c->ms[p.id] += ms;
+            c->calls[p.id] += 1;
+        }
+        c->pool.push_back(p.a);
+        c->pool.push_back(p.b);
+    }
+    c->pending.clear();
+}
+
+dj_status check_launch(dj_ctx *c, const char *what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(c, e, what);
+    return DJ_OK;
+}
+
+// ------------------------------------------------------------------------ host FP64 multiplier tables
+// Restates the 1-D k-space kernels of NR/src/nbody/pm.jl:119-183 in the working precision T.
+template <typename T>
+struct Tables {
+    std::vector<T> k2f, k2h, tf, th, mf, mh;
+    std::vector<cx<T>> gf, gh;
+};
+
+template <typename T>
+T maxabs(const std::vector<T> &k) {
+    T m = 0;
+    for (T v : k) m = std::fabs(v) > m ? std::fabs(v) : m;
+    return m;
+}
+
+template <typename T>
+void build_tables(const dj_pm_config &cfg, Tables<T> &tb) {
+    const int N = cfg.res_pm, H = N / 2 + 1;
+    const double L = cfg.boxsize;
+    std::vector<T> kf(N), kh(H);
+    for (int m = 0; m < N; m++) kf[m] = (T)((double)(m < N / 2 ? m : m - N) * 2 * M_PI / L);
+    for (int m = 0; m < H; m++) kh[m] = (T)((double)m * 2 * M_PI / L);
+    tb.k2f.resize(N);
+    tb.k2h.resize(H);
+    for (int m = 0; m < N; m++) tb.k2f[m] = kf[m] * kf[m];
+    for (int m = 0; m < H; m++) tb.k2h[m] = kh[m] * kh[m];
+    auto lap = [&](const std::vector<T> &k, std::vector<T> &t) {
+        t.resize(k.size());
+        T m = maxabs(k) / (T)M_PI;
+        for (size_t i = 0; i < k.size(); i++) {
+            T c1 = std::cos(1 * k[i] / m), c2 = std::cos(2 * k[i] / m), c3 = std::cos(3 * k[i] / m);
+            if (cfg.lap_order == 2) t[i] = -2 * (1 - c1) * (m * m);
+            else if (cfg.lap_order == 4) t[i] = -(c2 - 16 * c1 + 15) / 6 * (m * m);
+            else t[i] = -(-2 * c3 + 27 * c2 - 270 * c1 + 245) / 90 * (m * m);
+        }
+    };
+    if (cfg.lap_order != 0) {
+        lap(kf, tb.tf);
+        lap(kh, tb.th);
+    } else {
+        tb.tf.assign(N, 0);
+        tb.th.assign(H, 0);
+    }
+    auto mak = [&](const std::vector<T> &k, std::vector<T> &o) {
+        o.resize(k.size());
+        T m = maxabs(k) / (T)M_PI;
+        for (size_t i = 0; i < k.size(); i++) {
+            double x = (double)k[i] / (2 * M_PI) / (double)m;
+            T v = x == 0 ? (T)1 : (T)(std::sin(M_PI * x) / (M_PI * x));
+            o[i] = (T)std::pow((double)v, -2 * cfg.worder);
+        }
+    };
+    if (cfg.deconvolve) {
+        mak(kf, tb.mf);
+        mak(kh, tb.mh);
+    }
+    tb.gf.resize(N);
+    tb.gh.resize(H);
+    for (int m = 0; m < N; m++) tb.gf[m] = {(T)0, kf[m]};
+    for (int m = 0; m < H; m++) tb.gh[m] = {(T)0, kh[m]};
+    tb.gf[N / 2] = {0, 0};
+    tb.gh[H - 1] = {0, 0};
+}
+
+template <typename T>
+dj_status upload_tables(dj_ctx *c) {
+    Tables<T> tb;
+    build_tables<T>(c->cfg, tb);
+    const int N = c->res, H = N / 2 + 1;
+    size_t nreal = 2 * (size_t)N + 2 * H + 2 * (size_t)N + 2 * H + 2 * (size_t)N + 2 * H;
+    std::vector<T> buf(nreal, (T)0);
+    size_t o = 0;
+    auto put = [&](const std::vector<T> &v) {
+        size_t at = o;
+        for (size_t i = 0; i < v.size(); i++) buf[o + i] = v[i];
+        o += v.size();
+        return at;
+    };
+    size_t ok2f = put(tb.k2f), ok2h = put(tb.k2h), otf = put(tb.tf), oth = put(tb.th);
+    size_t omf = o, omh = o;
+    if (c->cfg.deconvolve) { omf = put(tb.mf); omh = put(tb.mh); }
+    o = (o + 1) & ~(size_t)1;
+    size_t ogf = o;
+    for (auto &g : tb.gf) { buf[o++] = g.x; buf[o++] = g.y; }
+    size_t ogh = o;
+    for (auto &g : tb.gh) { buf[o++] = g.x; buf[o++] = g.y; }
+    HIPCHK(c, hipMalloc(&c->tabs, sizeof(T) * o));
+    HIPCHK(c, hipMemcpy(c->tabs, buf.data(), sizeof(T) * o, hipMemcpyHostToDevice));
+    T *d = (T *)c->tabs;
+    KMult<T> K{};
+    K.k2f = d + ok2f; K.k2h = d + ok2h; K.tf = d + otf; K.th = d + oth;
+    K.mf = c->cfg.deconvolve ? d + omf : nullptr;
+    K.mh = c->cfg.deconvolve ? d + omh : nullptr;
+    K.gf = (const cx<T> *)(d + ogf);
+    K.gh = (const cx<T> *)(d + ogh);
+    double nn = (double)c->n;
+    K.rho = (T)(nn * nn * nn / ((double)N * N * N));
+    K.nsets = (T)(c->cfg.n_resample * c->cfg.n_resample * c->cfg.n_resample);
+    K.r3 = (T)N * (T)N * (T)N;
+    K.lap_order = c->cfg.lap_order;
+    std::memcpy(c->kmult, &K, sizeof(K));
+    // twiddles exp(-2 pi i k / N), k < N, in FP64 rounded to T
+    std::vector<cx<T>> tw(N);
+    for (int k = 0; k < N; k++) {
+        double a = -2.0 * M_PI * (double)k / (double)N;
+        tw[k] = {(T)std::cos(a), (T)std::sin(a)};
+    }
+    HIPCHK(c, hipMalloc(&c->tw, sizeof(cx<T>) * N));
+    HIPCHK(c, hipMemcpy(c->tw, tw.data(), sizeof(cx<T>) * N, hipMemcpyHostToDevice));
+    // Lagrangian lattice (pm.jl:186)
+    std::vector<T> q1(c->n);
+    for (int i = 0; i < c->n; i++) q1[i] = (T)((double)i * (c->cfg.boxsize / c->n));
+    HIPCHK(c, hipMalloc(&c->q1, sizeof(T) * c->n));
+    HIPCHK(c, hipMemcpy(c->q1, q1.data(), sizeof(T) * c->n, hipMemcpyHostToDevice));
+    return DJ_OK;
+}
+
+// pm.jl:237-246
+int aa_shifts(int aa, double dh, double (*s)[3]) {
+    if (aa == 0) { s[0][0] = s[0][1] = s[0][2] = 0; return 1; }
+    if (aa == -1) { s[0][0] = s[0][1] = s[0][2] = dh; return 1; }
+    if (aa == 1) { s[0][0] = s[0][1] = s[0][2] = 0; s[1][0] = s[1][1] = s[1][2] = dh; return 2; }
+    if (aa != 2 && aa != 3) return -1;
+    int n = 0;
+    double v[2] = {0.0, dh};
+    for (int a = 0; a < 2; a++) for (int b = 0; b < 2; b++) for (int c = 0; c < 2; c++) {
+        s[n][0] = v[a]; s[n][1] = v[b]; s[n][2] = v[c]; n++;
+    }
+    if (aa == 3) {
+        double q = dh / 2, u[2] = {-q, q};
+        for (int a = 0; a < 2; a++) for (int b = 0; b < 2; b++) for (int c = 0; c < 2; c++) {
+            s[n][0] = u[a]; s[n][1] = u[b]; s[n][2] = u[c]; n++;
+        }
+    }
+    return n;
+}
+
+void free_all(dj_ctx *c) {
+    void *bufs[] = {c->psi, c->mom, c->acc, c->psi_in, c->q1, c->mesh, c->meshfix, c->A, c->B, c->C, c->tw, c->tabs};
+    for (void *b : bufs)
+        if (b) hipFree(b);
+    for (auto &p : c->pending) { hipEventDestroy(p.a); hipEventDestroy(p.b); }
+    for (auto e : c->pool) hipEventDestroy(e);
+    if (c->st) hipStreamDestroy(c->st);
+}
+
+void set_bytes(dj_ctx *c) {
+    const double T = c->tsize, M = (double)c->M, HC = (double)c->HC, np = (double)c->np;
+    c->bytes[K_ZERO] = M * (c->cfg.deposit == DJ_DEPOSIT_FIXED64 ? 8 : T);
+    c->bytes[K_SCATTER] = 3 * T * np + M * (c->cfg.deposit == DJ_DEPOSIT_FIXED64 ? 8 : T);
+    c->bytes[K_XF] = M * (c->cfg.deposit == DJ_DEPOSIT_FIXED64 ? 8 : T) + 2 * T * HC;
+    c->bytes[K_YF] = 4 * T * HC;
+    c->bytes[K_ZM] = 4 * T * HC;
+    c->bytes[K_YI] = 4 * T * HC;
+    c->bytes[K_XI] = 2 * T * HC + T * M;
+    // FD-fused gather of the last pass with kick and drift: read psi, mom (and acc when averaging
+    // shifts), read phi once, write psi and mom
+    c->bytes[K_GATHER] = 3 * T * np * (c->nshifts > 1 ? 5 : 4) + T * M;
+    c->bytes[K_KICK] = 3 * T * np * 5;
+    c->bytes[K_DRIFT] = 3 * T * np * 3;
+}
+
+// ------------------------------------------------------------------------ one force evaluation
+enum { FORCE_ACC = 0, FORCE_KICK = 1 };
+
+dj_status force(dj_ctx *c, const void *psi, int mode, const KickArgs &kick) {
+    const dj_pm_config &cfg = c->cfg;
+    const bool fixed = cfg.deposit == DJ_DEPOSIT_FIXED64;
+    const int ns = c->nshifts;
+    const size_t mesh_bytes = c->M * (fixed ? 8 : c->tsize);
+    void *dens = fixed ? c->meshfix : c->mesh;
+    const double fscale = std::ldexp(1.0, -(cfg.fixed_frac_bits ? cfg.fixed_frac_bits : 32));
+    PartArgs pa{};
+    pa.psi = psi;
+    pa.mom = c->mom;
+    pa.acc = c->acc;
+    pa.q1 = c->q1;
+    pa.n = c->n;
+    pa.res = c->res;
+    pa.L = cfg.boxsize;
+    const bool ik = cfg.grad_order == 0;
+    // the acceleration buffer is needed when shifts are averaged, for the ik path, and as output
+    if (ik || mode == FORCE_ACC || ns > 1) {
+        if (!(mode == FORCE_ACC && ns == 1 && !ik)) {
+            Prof p(c, K_ZERO);
+            c->part.zero(c->acc, (size_t)c->n3 * c->tsize, c->st);
+        }
+    }
+    for (int s = 0; s < ns; s++) {
+        pa.shift[0] = c->shifts[s][0];
+        pa.shift[1] = c->shifts[s][1];
+        pa.shift[2] = c->shifts[s][2];
+        { Prof p(c, K_ZERO); c->part.zero(dens, mesh_bytes, c->st); }
+        { Prof p(c, K_SCATTER); c->part.scatter(pa, cfg.worder, fixed, cfg.fixed_frac_bits ? cfg.fixed_frac_bits : 32, dens, c->st); }
+        { Prof p(c, K_XF); c->fft.xf(dens, fixed, c->A, c->tw, fscale, c->st); }
+        { Prof p(c, K_YF); c->fft.yf(c->A, c->B, c->tw, c->st); }
+        if (!ik) {
+            { Prof p(c, K_ZM); c->fft.zm(ZM_POT, true, true, c->B, c->B, c->tw, c->kmult, c->st); }
+            { Prof p(c, K_YI); c->fft.yi(c->B, c->A, c->tw, c->st); }
+            { Prof p(c, K_XI); c->fft.xi(c->A, c->mesh, c->tw, c->st); }
+            const bool last = s == ns - 1;
+            int gmode = (mode == FORCE_KICK && last) ? GATHER_KICK : (ns == 1 && mode == FORCE_ACC ? GATHER_ASSIGN : GATHER_ACC);
+            Prof p(c, K_GATHER);
+            c->part.gather_fd(pa, cfg.worder, cfg.grad_order, c->mesh, c->fdc, gmode, kick, c->psi, c->st);
+        } else {
+            { Prof p(c, K_ZM); c->fft.zm(ZM_NONE, true, false, c->B, c->B, c->tw, c->kmult, c->st); }
+            for (int ax = 0; ax < 3; ax++) {
+                { Prof p(c, K_ZM); c->fft.zm(ZM_GRADX + ax, false, true, c->B, c->C, c->tw, c->kmult, c->st); }
+                { Prof p(c, K_YI); c->fft.yi(c->C, c->A, c->tw, c->st); }
+                { Prof p(c, K_XI); c->fft.xi(c->A, c->mesh, c->tw, c->st); }
+                Prof p(c, K_GATHER);
+                c->part.gather_comp(pa, cfg.worder, c->mesh, ax, c->st);
+            }
+        }
+        dj_status r = check_launch(c, "force pass");
+        if (r) return r;
+    }
+    if (ik && mode == FORCE_KICK) {
+        Prof p(c, K_KICK);
+        c->part.kick_drift(c->psi, c->mom, c->acc, c->n3, kick, c->st);
+    } else if (mode == FORCE_ACC && ns > 1) {
+        c->part.scale(c->acc, c->n3, 1.0 / ns, c->st);
+    }
+    return check_launch(c, "force");
+}
+
+template <typename T>
+dj_status setup_ops(dj_ctx *c) {
+    if (!fft_dispatch<T>(c->res, c->fft))
+        return fail(c, DJ_ERR_UNSUPPORTED, "res_pm " + std::to_string(c->res) +
+                                               " is not a built mesh size (16..1024, 2^a 3^b)");
+    part_ops<T>(c->part);
+    return upload_tables<T>(c);
+}
+
+dj_status validate(const dj_pm_config &cfg, std::string &msg) {
+    if (cfg.worder < 2 || cfg.worder > 4) { msg = "worder must be 2, 3 or 4"; return DJ_ERR_INVALID_CONFIG; }
+    if (cfg.grad_order != 0 && cfg.grad_order != 2 && cfg.grad_order != 4 && cfg.grad_order != 6) {
+        msg = "grad_order must be 0, 2, 4 or 6"; return DJ_ERR_INVALID_CONFIG;
+    }
+    if (cfg.lap_order != 0 && cfg.lap_order != 2 && cfg.lap_order != 4 && cfg.lap_order != 6) {
+        msg = "lap_order must be 0, 2, 4 or 6"; return DJ_ERR_INVALID_CONFIG;
+    }
+    if (cfg.antialias < -1 || cfg.antialias > 3) { msg = "antialias must be -1, 0, 1, 2 or 3"; return DJ_ERR_INVALID_CONFIG; }
+    if (cfg.n_part < 1 || cfg.res_pm < 2 || !(cfg.boxsize > 0)) { msg = "n_part, res_pm, boxsize must be positive"; return DJ_ERR_INVALID_CONFIG; }
+    if (cfg.resampling != 0 && cfg.resampling != 1) { msg = "resampling must be fourier (0) or linear (1)"; return DJ_ERR_INVALID_CONFIG; }
+    if (cfg.n_resample < 1) { msg = "n_resample must be >= 1"; return DJ_ERR_INVALID_CONFIG; }
+    if (cfg.lean && !(cfg.antialias == 0 && cfg.n_resample == 1 && cfg.grad_order != 0)) {
+        msg = "PMSolverLean supports antialias = 0, n_resample = 1 and finite-difference gradients (2/4/6)";
+        return DJ_ERR_INVALID_CONFIG;
+    }
+    if (cfg.precision != 32 && cfg.precision != 64) { msg = "precision must be 32 or 64"; return DJ_ERR_INVALID_CONFIG; }
+    if (cfg.deposit != DJ_DEPOSIT_FLOAT && cfg.deposit != DJ_DEPOSIT_FIXED64) { msg = "unknown deposit mode"; return DJ_ERR_INVALID_CONFIG; }
+    if (cfg.fixed_frac_bits < 0 || cfg.fixed_frac_bits > 40) { msg = "fixed_frac_bits must be in 0..40"; return DJ_ERR_INVALID_CONFIG; }
+    if (cfg.n_resample > 1) { msg = "sheet resampling (n_resample > 1) is not built into this library"; return DJ_ERR_UNSUPPORTED; }
+    if (cfg.n_part > 1024) { msg = "n_part above 1024 is not supported"; return DJ_ERR_UNSUPPORTED; }
+    return DJ_OK;
+}
+
+dj_status upload_particles(dj_ctx *c, void *dst, const void *src, hipMemcpyKind kind) {
+    size_t bytes = (size_t)c->n3 * c->tsize;
+    if (src) HIPCHK(c, hipMemcpyAsync(dst, src, bytes, kind, c->st));
+    else c->part.zero(dst, bytes, c->st);
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return DJ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *dj_version(void) { return "discodj_hip 0.1 (gfx950)"; }
+
+dj_status dj_create(const dj_pm_config *cfg, dj_ctx **out) {
+    if (!cfg || !out) return fail(nullptr, DJ_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    std::string msg;
+    dj_status v = validate(*cfg, msg);
+    if (v) return fail(nullptr, v, msg);
+    dj_ctx *c = new (std::nothrow) dj_ctx();
+    if (!c) return fail(nullptr, DJ_ERR_OUT_OF_MEMORY, "host allocation failed");
+    c->cfg = *cfg;
+    c->tsize = cfg->precision == 64 ? 8 : 4;
+    c->dev = cfg->device;
+    c->n = cfg->n_part;
+    c->res = cfg->res_pm;
+    c->np = (int64_t)c->n * c->n * c->n;
+    c->n3 = 3 * c->np;
+    c->M = (size_t)c->res * c->res * c->res;
+    c->HC = (size_t)c->res * c->res * (c->res / 2 + 1);
+    c->nshifts = aa_shifts(cfg->antialias, 0.5 * cfg->boxsize / cfg->res_pm, c->shifts);
+    double h = cfg->boxsize / cfg->res_pm;
+    if (cfg->grad_order == 2) { c->fdc[0] = 1 / (2 * h); }
+    else if (cfg->grad_order == 4) { c->fdc[0] = 8 / (12 * h); c->fdc[1] = -1 / (12 * h); }
+    else if (cfg->grad_order == 6) { c->fdc[0] = 45 / (60 * h); c->fdc[1] = -9 / (60 * h); c->fdc[2] = 1 / (60 * h); }
+    dj_status r = DJ_OK;
+    auto bail = [&](dj_status code) {
+        g_create_error = c->err;
+        free_all(c);
+        delete c;
+        return code;
+    };
+    hipError_t e = hipSetDevice(c->dev);
+    if (e != hipSuccess) { hip_fail(c, e, "hipSetDevice"); return bail(DJ_ERR_DEVICE); }
+    e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
+    if (e != hipSuccess) { hip_fail(c, e, "hipStreamCreate"); return bail(DJ_ERR_DEVICE); }
+    r = c->tsize == 8 ? setup_ops<double>(c) : setup_ops<float>(c);
+    if (r) return bail(r);
+    size_t pbytes = (size_t)c->n3 * c->tsize, cbytes = c->HC * 2 * c->tsize;
+    void **bufs[] = {&c->psi, &c->mom, &c->acc};
+    for (void **b : bufs) {
+        e = hipMalloc(b, pbytes);
+        if (e != hipSuccess) { hip_fail(c, e, "hipMalloc(particles)"); return bail(DJ_ERR_OUT_OF_MEMORY); }
+    }
+    e = hipMalloc(&c->mesh, c->M * c->tsize);
+    if (e == hipSuccess && cfg->deposit == DJ_DEPOSIT_FIXED64) e = hipMalloc(&c->meshfix, c->M * 8);
+    if (e == hipSuccess) e = hipMalloc(&c->A, cbytes);
+    if (e == hipSuccess) e = hipMalloc(&c->B, cbytes);
+    if (e == hipSuccess && cfg->grad_order == 0) e = hipMalloc(&c->C, cbytes);
+    if (e != hipSuccess) { hip_fail(c, e, "hipMalloc(mesh)"); return bail(DJ_ERR_OUT_OF_MEMORY); }
+    c->part.zero(c->psi, pbytes, c->st);
+    c->part.zero(c->mom, pbytes, c->st);
+    e = hipStreamSynchronize(c->st);
+    if (e != hipSuccess) { hip_fail(c, e, "initialise"); return bail(DJ_ERR_DEVICE); }
+    set_bytes(c);
+    *out = c;
+    return DJ_OK;
+}
+
+dj_status dj_destroy(dj_ctx *c) {
+    if (!c) return DJ_OK;
+    hipSetDevice(c->dev);
+    if (c->st) hipStreamSynchronize(c->st);
+    free_all(c);
+    delete c;
+    return DJ_OK;
+}
+
+const char *dj_last_error(const dj_ctx *c) { return c ? c->err.c_str() : g_create_error.c_str(); }
+
+dj_status dj_set_state(dj_ctx *c, const void *psi, const void *pi) {
+    if (!c || !psi) return fail(c, DJ_ERR_INVALID_ARGUMENT, "null argument");
+    HIPCHK(c, hipSetDevice(c->dev));
+    dj_status r = upload_particles(c, c->psi, psi, hipMemcpyHostToDevice);
+    return r ? r : upload_particles(c, c->mom, pi, hipMemcpyHostToDevice);
+}
+
+dj_status dj_set_state_device(dj_ctx *c, const void *psi, const void *pi) {
+    if (!c || !psi) return fail(c, DJ_ERR_INVALID_ARGUMENT, "null argument");
+    HIPCHK(c, hipSetDevice(c->dev));
+    dj_status r = upload_particles(c, c->psi, psi, hipMemcpyDeviceToDevice);
+    return r ? r : upload_particles(c, c->mom, pi, hipMemcpyDeviceToDevice);
+}
+
+static dj_status get_state(dj_ctx *c, void *psi, void *pi, hipMemcpyKind kind) {
+    HIPCHK(c, hipSetDevice(c->dev));
+    size_t bytes = (size_t)c->n3 * c->tsize;
+    if (psi) HIPCHK(c, hipMemcpyAsync(psi, c->psi, bytes, kind, c->st));
+    if (pi) HIPCHK(c, hipMemcpyAsync(pi, c->mom, bytes, kind, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    drain_prof(c);
+    return DJ_OK;
+}
+
+dj_status dj_get_state(dj_ctx *c, void *psi, void *pi) {
+    if (!c) return fail(c, DJ_ERR_INVALID_ARGUMENT, "null context");
+    return get_state(c, psi, pi, hipMemcpyDeviceToHost);
+}
+
+dj_status dj_get_state_device(dj_ctx *c, void *psi, void *pi) {
+    if (!c) return fail(c, DJ_ERR_INVALID_ARGUMENT, "null context");
+    return get_state(c, psi, pi, hipMemcpyDeviceToDevice);
+}
+
+dj_status dj_acceleration(dj_ctx *c, const void *psi, void *acc_out) {
+    if (!c || !acc_out) return fail(c, DJ_ERR_INVALID_ARGUMENT, "null argument");
+    HIPCHK(c, hipSetDevice(c->dev));
+    const void *src = c->psi;
+    if (psi) {
+        if (!c->psi_in) HIPCHK(c, hipMalloc(&c->psi_in, (size_t)c->n3 * c->tsize));
+        HIPCHK(c, hipMemcpyAsync(c->psi_in, psi, (size_t)c->n3 * c->tsize, hipMemcpyHostToDevice, c->st));
+        src = c->psi_in;
+    }
+    KickArgs k{};
+    k.nshifts = c->nshifts;
+    dj_status r = force(c, src, FORCE_ACC, k);
+    if (r) return r;
+    HIPCHK(c, hipMemcpyAsync(acc_out, c->acc, (size_t)c->n3 * c->tsize, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    drain_prof(c);
+    return DJ_OK;
+}
+
+dj_status dj_step(dj_ctx *c, double dd1, double alpha, double beta, double dd2) {
+    if (!c) return fail(c, DJ_ERR_INVALID_ARGUMENT, "null context");
+    HIPCHK(c, hipSetDevice(c->dev));
+    { Prof p(c, K_DRIFT); c->part.drift(c->psi, c->mom, c->n3, dd1, c->st); }
+    KickArgs k{alpha, beta, dd2, 0.0, 0, c->nshifts};
+    dj_status r = force(c, c->psi, FORCE_KICK, k);
+    if (r) return r;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    drain_prof(c);
+    return DJ_OK;
+}
+
+dj_status dj_run(dj_ctx *c, int32_t n_steps, const double *dd1, const double *alpha, const double *beta,
+                 const double *dd2, const uint8_t *want, dj_step_cb cb, void *user) {
+    if (!c || n_steps < 0 || (n_steps > 0 && (!dd1 || !alpha || !beta || !dd2)))
+        return fail(c, DJ_ERR_INVALID_ARGUMENT, "invalid run arguments");
+    HIPCHK(c, hipSetDevice(c->dev));
+    auto wants = [&](int k) { return cb && (!want || want[k]); };
+    if (wants(0)) {
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        cb(0, user);
+    }
+    if (n_steps == 0) return DJ_OK;
+    { Prof p(c, K_DRIFT); c->part.drift(c->psi, c->mom, c->n3, dd1[0], c->st); }
+    for (int s = 0; s < n_steps; s++) {
+        const bool stop = wants(s + 1);
+        const bool has_next = s + 1 < n_steps && !stop;
+        KickArgs k{alpha[s], beta[s], dd2[s], has_next ? dd1[s + 1] : 0.0, has_next ? 1 : 0, c->nshifts};
+        dj_status r = force(c, c->psi, FORCE_KICK, k);
+        if (r) return r;
+        if (stop) {
+            HIPCHK(c, hipStreamSynchronize(c->st));
+            cb(s + 1, user);
+            if (s + 1 < n_steps) {
+                Prof p(c, K_DRIFT);
+                c->part.drift(c->psi, c->mom, c->n3, dd1[s + 1], c->st);
+            }
+        }
+    }
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    drain_prof(c);
+    return check_launch(c, "run");
+}
+
+dj_status dj_deposit(dj_ctx *c, const void *psi, const double *shift, void *mesh_out) {
+    if (!c || !mesh_out) return fail(c, DJ_ERR_INVALID_ARGUMENT, "null argument");
+    HIPCHK(c, hipSetDevice(c->dev));
+    const bool fixed = c->cfg.deposit == DJ_DEPOSIT_FIXED64;
+    const void *src = c->psi;
+    if (psi) {
+        if (!c->psi_in) HIPCHK(c, hipMalloc(&c->psi_in, (size_t)c->n3 * c->tsize));
+        HIPCHK(c, hipMemcpyAsync(c->psi_in, psi, (size_t)c->n3 * c->tsize, hipMemcpyHostToDevice, c->st));
+        src = c->psi_in;
+    }
+    PartArgs pa{};
+    pa.psi = src;
+    pa.q1 = c->q1;
+    pa.n = c->n;
+    pa.res = c->res;
+    pa.L = c->cfg.boxsize;
+    for (int a = 0; a < 3; a++) pa.shift[a] = shift ? shift[a] : 0.0;
+    void *dens = fixed ? c->meshfix : c->mesh;
+    size_t bytes = c->M * (fixed ? 8 : c->tsize);
+    { Prof p(c, K_ZERO); c->part.zero(dens, bytes, c->st); }
+    { Prof p(c, K_SCATTER); c->part.scatter(pa, c->cfg.worder, fixed, c->cfg.fixed_frac_bits ? c->cfg.fixed_frac_bits : 32, dens, c->st); }
+    dj_status r = check_launch(c, "deposit");
+    if (r) return r;
+    HIPCHK(c, hipMemcpyAsync(mesh_out, dens, bytes, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    drain_prof(c);
+    return DJ_OK;
+}
+
+dj_status dj_check_finite(dj_ctx *c) {
+    if (!c) return fail(c, DJ_ERR_INVALID_ARGUMENT, "null context");
+    size_t bytes = (size_t)c->n3 * c->tsize;
+    std::vector<unsigned char> h(bytes);
+    void *bufs[2] = {c->psi, c->mom};
+    for (void *b : bufs) {
+        HIPCHK(c, hipMemcpyAsync(h.data(), b, bytes, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        for (int64_t i = 0; i < c->n3; i++) {
+            double v = c->tsize == 8 ? ((double *)h.data())[i] : (double)((float *)h.data())[i];
+            if (!std::isfinite(v)) return fail(c, DJ_ERR_NONFINITE, "non-finite particle state");
+        }
+    }
+    return DJ_OK;
+}
+
+dj_status dj_synchronize(dj_ctx *c) {
+    if (!c) return fail(c, DJ_ERR_INVALID_ARGUMENT, "null context");
+    HIPCHK(c, hipSetDevice(c->dev));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    drain_prof(c);
+    return DJ_OK;
+}
+
+dj_status dj_set_profiling(dj_ctx *c, int32_t on) {
+    if (!c) return fail(c, DJ_ERR_INVALID_ARGUMENT, "null context");
+    c->prof = on != 0;
+    return DJ_OK;
+}
+
+dj_status dj_reset_stats(dj_ctx *c) {
+    if (!c) return fail(c, DJ_ERR_INVALID_ARGUMENT, "null context");
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    drain_prof(c);
+    for (int i = 0; i < K_COUNT; i++) { c->calls[i] = 0; c->ms[i] = 0; }
+    return DJ_OK;
+}
+
+dj_status dj_kernel_stats(dj_ctx *c, dj_kernel_stat *out, int32_t cap, int32_t *n_out) {
+    if (!c || !n_out) return fail(c, DJ_ERR_INVALID_ARGUMENT, "null argument");
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    drain_prof(c);
+    int n = 0;
+    for (int i = 0; i < K_COUNT && n < cap; i++) {
+        if (!out) break;
+        std::snprintf(out[n].name, sizeof(out[n].name), "%s", kernel_names[i]);
+        out[n].calls = c->calls[i];
+        out[n].ms_total = c->ms[i];
+        out[n].bytes_per_call = c->bytes[i];
+        n++;
+    }
+    *n_out = out ? n : K_COUNT;
+    return DJ_OK;
+}
+
+dj_status dj_fft3d(int32_t N, int32_t precision, int32_t direction, const void *in, void *out, int32_t device) {
+    if (!in || !out || (direction != -1 && direction != 1) || (precision != 32 && precision != 64))
+        return fail(nullptr, DJ_ERR_INVALID_ARGUMENT, "invalid fft arguments");
+    dj_pm_config cfg{};
+    cfg.res_pm = N; cfg.n_part = 1; cfg.boxsize = 1.0; cfg.worder = 2; cfg.grad_order = 2;
+    cfg.n_resample = 1; cfg.precision = precision; cfg.device = device;
+    dj_ctx *c = nullptr;
+    dj_status r = dj_create(&cfg, &c);
+    if (r) return r;
+    const int T = c->tsize, H = N / 2 + 1;
+    const size_t M = c->M, HC = c->HC;
+    std::vector<unsigned char> hb(HC * 2 * T);
+    auto fin = [&](dj_status code) {
+        if (code) g_create_error = c->err;
+        dj_destroy(c);
+        return code;
+    };
+    if (direction < 0) {
+        HIPCHK(c, hipMemcpy(c->mesh, in, M * T, hipMemcpyHostToDevice));
+        c->fft.xf(c->mesh, false, c->A, c->tw, 1.0, c->st);
+        c->fft.yf(c->A, c->B, c->tw, c->st);
+        c->fft.zm(ZM_NONE, true, false, c->B, c->B, c->tw, c->kmult, c->st);
+        if ((r = check_launch(c, "fft3d"))) return fin(r);
+        HIPCHK(c, hipMemcpyAsync(hb.data(), c->B, HC * 2 * T, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        // B[kz + N (kx + H ky)] -> out[kx + H (ky + N kz)]
+        for (int ky = 0; ky < N; ky++)
+            for (int kx = 0; kx < H; kx++)
+                for (int kz = 0; kz < N; kz++)
+                    std::memcpy((unsigned char *)out + 2 * T * (kx + (size_t)H * (ky + (size_t)N * kz)),
+                                hb.data() + 2 * T * (kz + (size_t)N * (kx + (size_t)H * ky)), 2 * T);
+    } else {
+        for (int ky = 0; ky < N; ky++)
+            for (int kx = 0; kx < H; kx++)
+                for (int kz = 0; kz < N; kz++)
+                    std::memcpy(hb.data() + 2 * T * (kz + (size_t)N * (kx + (size_t)H * ky)),
+                                (const unsigned char *)in + 2 * T * (kx + (size_t)H * (ky + (size_t)N * kz)), 2 * T);
+        HIPCHK(c, hipMemcpy(c->B, hb.data(), HC * 2 * T, hipMemcpyHostToDevice));
+        c->fft.zm(ZM_NONE, false, true, c->B, c->B, c->tw, c->kmult, c->st);
+        c->fft.yi(c->B, c->A, c->tw, c->st);
+        c->fft.xi(c->A, c->mesh, c->tw, c->st);
+        if ((r = check_launch(c, "fft3d"))) return fin(r);
+        HIPCHK(c, hipMemcpyAsync(out, c->mesh, M * T, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(c, hipStreamSynchronize(c->st));
+    }
+    return fin(DJ_OK);
+}
+
+}  // extern "C"

@@ -1,0 +1,62 @@
+// Internal interfaces between the C-ABI layer (api.cpp) and the kernel translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace dj {
+
+// FFT passes of one mesh size and precision (fft.hip)
+struct FftOps {
+    void (*xf)(const void *mesh, bool fixed, void *A, const void *tw, double fscale, hipStream_t st);
+    void (*xi)(const void *A, void *mesh, const void *tw, hipStream_t st);
+    void (*yf)(const void *A, void *B, const void *tw, hipStream_t st);
+    void (*yi)(const void *B, void *A, const void *tw, hipStream_t st);
+    void (*zm)(int mode, bool fwd, bool inv, const void *in, void *out, const void *tw, const void *K,
+               hipStream_t st);
+    int lines_per_block;
+};
+template <typename T> bool fft_dispatch(int N, FftOps &ops);
+
+// particle-side kernels (particles.hip)
+struct PartArgs {
+    const void *psi;      // (3, Np) displacement, component-major
+    void *mom;            // (3, Np) momentum
+    void *acc;            // (3, Np) acceleration accumulator
+    const void *q1;       // (n) Lagrangian lattice coordinates (i * L/n in FP64, rounded to T)
+    int n;                // particles per side
+    int res;              // mesh cells per side
+    double L;             // box size
+    double shift[3];      // interlacing shift of this pass
+};
+
+enum { GATHER_ACC = 0, GATHER_ASSIGN = 1, GATHER_KICK = 2 };
+
+struct KickArgs {
+    double alpha, beta, dd2, dd1_next;  // FP64 step coefficients, rounded to T at use
+    int has_next;                       // apply the next step's first drift in the same pass
+    int nshifts;                        // interlacing passes averaged
+};
+
+struct PartOps {
+    // unit-mass deposit of X = q + psi + shift (float atomics, or fixed-point int64 when fixed)
+    void (*scatter)(const PartArgs &a, int W, bool fixed, int frac_bits, void *mesh, hipStream_t st);
+    // gather -D phi (FD order 2/4/6) at X; mode GATHER_ACC (acc += s), GATHER_ASSIGN (acc = s, lean
+    // per-axis semantics are equal), GATHER_KICK (final pass fused with kick and drift)
+    void (*gather_fd)(const PartArgs &a, int W, int order, const void *phi, const double *c, int mode,
+                      const KickArgs &k, void *psi_out, hipStream_t st);
+    // gather one component of a mesh field at X into acc[comp] (+=)
+    void (*gather_comp)(const PartArgs &a, int W, const void *field, int comp, hipStream_t st);
+    // psi += dd * mom
+    void (*drift)(void *psi, const void *mom, int64_t n3, double dd, hipStream_t st);
+    // mom = alpha mom + beta acc/ns ; psi += dd2 mom [; psi += dd1_next mom]
+    void (*kick_drift)(void *psi, void *mom, const void *acc, int64_t n3, const KickArgs &k, hipStream_t st);
+    // x *= s
+    void (*scale)(void *x, int64_t n, double s, hipStream_t st);
+    // x = 0 (a fill kernel, graph-capture safe)
+    void (*zero)(void *x, size_t bytes, hipStream_t st);
+};
+template <typename T> void part_ops(PartOps &ops);
+
+}  // namespace dj

@@ -1,0 +1,154 @@
+// Device entry of the FFT passes (fft_passes.h) and their launchers for every supported mesh size.
+#include <hip/hip_runtime.h>
+
+#include "dj_internal.h"
+#include "fft_passes.h"
+
+namespace dj {
+
+template <typename T, int N> struct FftShape {
+    // lines per workgroup: about 64 KiB of LDS for the y/z tiles, at most 16, dividing N
+    static constexpr int target = 65536 / (N * (int)sizeof(cx<T>));
+    static constexpr int B = (N % 16 == 0 && target >= 16) ? 16
+                           : (N % 8 == 0 && target >= 8) ? 8
+                           : (N % 4 == 0 && target >= 4) ? 4
+                           : (N % 2 == 0 && target >= 2) ? 2 : 1;
+    static constexpr int NT = 256;
+    using P = Passes<T, N, B, NT>;
+};
+
+template <typename T, int N, typename IN>
+__global__ __launch_bounds__(256) void k_xf(const IN *__restrict__ mesh, cx<T> *__restrict__ A,
+                                            const cx<T> *__restrict__ tw, double fixed_scale) {
+    using S = FftShape<T, N>;
+    using P = typename S::P;
+    __shared__ cx<T> lds[P::EX::LDS_ELEMS];
+    const int yb = blockIdx.x % (N / S::B), z = blockIdx.x / (N / S::B);
+    P::xf_load(mesh, lds, yb, z, threadIdx.x, fixed_scale);
+    __syncthreads();
+    P::EX::template run<-1, 2>(lds, tw, threadIdx.x);
+    P::xf_store(lds, tw, A, yb, z, threadIdx.x);
+}
+
+template <typename T, int N>
+__global__ __launch_bounds__(256) void k_xi(const cx<T> *__restrict__ A, T *__restrict__ mesh,
+                                            const cx<T> *__restrict__ tw) {
+    using S = FftShape<T, N>;
+    using P = typename S::P;
+    __shared__ cx<T> lds[P::EX::LDS_ELEMS];
+    const int yb = blockIdx.x % (N / S::B), z = blockIdx.x / (N / S::B);
+    P::xi_load(A, lds, yb, z, threadIdx.x);
+    __syncthreads();
+    P::xi_pre(lds, tw, threadIdx.x);
+    __syncthreads();
+    P::EX::template run<+1, 2>(lds, tw, threadIdx.x);
+    P::xi_store(lds, mesh, yb, z, threadIdx.x);
+}
+
+template <typename T, int N>
+__global__ __launch_bounds__(256) void k_yf(const cx<T> *__restrict__ A, cx<T> *__restrict__ Bm,
+                                            const cx<T> *__restrict__ tw) {
+    using S = FftShape<T, N>;
+    using P = typename S::P;
+    __shared__ cx<T> lds[P::EF::LDS_ELEMS];
+    const int zb = blockIdx.x % (N / S::B), kx = blockIdx.x / (N / S::B);
+    P::y_load_a(A, lds, zb, kx, threadIdx.x);
+    __syncthreads();
+    P::EF::template run<-1, 1>(lds, tw, threadIdx.x);
+    P::y_store_b(lds, Bm, zb, kx, threadIdx.x);
+}
+
+template <typename T, int N>
+__global__ __launch_bounds__(256) void k_yi(const cx<T> *__restrict__ Bm, cx<T> *__restrict__ A,
+                                            const cx<T> *__restrict__ tw) {
+    using S = FftShape<T, N>;
+    using P = typename S::P;
+    __shared__ cx<T> lds[P::EF::LDS_ELEMS];
+    const int zb = blockIdx.x % (N / S::B), kx = blockIdx.x / (N / S::B);
+    P::y_load_b(Bm, lds, zb, kx, threadIdx.x);
+    __syncthreads();
+    P::EF::template run<+1, 1>(lds, tw, threadIdx.x);
+    P::y_store_a(lds, A, zb, kx, threadIdx.x);
+}
+
+// z lines of B: optional forward transform, multiplier, optional inverse; in may equal out
+template <typename T, int N, int MODE, bool FWD, bool INV>
+__global__ __launch_bounds__(256) void k_zm(const cx<T> *in, cx<T> *out, const cx<T> *__restrict__ tw,
+                                            KMult<T> K) {
+    using S = FftShape<T, N>;
+    using P = typename S::P;
+    __shared__ cx<T> lds[P::EF::LDS_ELEMS];
+    const int b = blockIdx.x;
+    P::z_load(in, lds, b, threadIdx.x);
+    __syncthreads();
+    if constexpr (FWD) P::EF::template run<-1, 1>(lds, tw, threadIdx.x);
+    if constexpr (MODE != ZM_NONE) {
+        P::template z_mult<MODE>(lds, K, b, threadIdx.x);
+        __syncthreads();
+    }
+    if constexpr (INV) P::EF::template run<+1, 1>(lds, tw, threadIdx.x);
+    P::z_store(lds, out, b, threadIdx.x);
+}
+
+template <typename T, int N>
+struct FftN {
+    using S = FftShape<T, N>;
+    using P = typename S::P;
+    static void xf(const void *mesh, bool fixed, void *A, const void *tw, double fscale, hipStream_t st) {
+        if (fixed)
+            k_xf<T, N, long long><<<P::XBLOCKS(), 256, 0, st>>>((const long long *)mesh, (cx<T> *)A,
+                                                               (const cx<T> *)tw, fscale);
+        else
+            k_xf<T, N, T><<<P::XBLOCKS(), 256, 0, st>>>((const T *)mesh, (cx<T> *)A, (const cx<T> *)tw, fscale);
+    }
+    static void xi(const void *A, void *mesh, const void *tw, hipStream_t st) {
+        k_xi<T, N><<<P::XBLOCKS(), 256, 0, st>>>((const cx<T> *)A, (T *)mesh, (const cx<T> *)tw);
+    }
+    static void yf(const void *A, void *Bm, const void *tw, hipStream_t st) {
+        k_yf<T, N><<<P::YBLOCKS(), 256, 0, st>>>((const cx<T> *)A, (cx<T> *)Bm, (const cx<T> *)tw);
+    }
+    static void yi(const void *Bm, void *A, const void *tw, hipStream_t st) {
+        k_yi<T, N><<<P::YBLOCKS(), 256, 0, st>>>((const cx<T> *)Bm, (cx<T> *)A, (const cx<T> *)tw);
+    }
+    static void zm(int mode, bool fwd, bool inv, const void *in, void *out, const void *tw, const void *Kp,
+                   hipStream_t st) {
+        const KMult<T> &K = *(const KMult<T> *)Kp;
+        const cx<T> *i = (const cx<T> *)in;
+        cx<T> *o = (cx<T> *)out;
+        const cx<T> *t = (const cx<T> *)tw;
+        dim3 g(P::ZBLOCKS()), b(256);
+#define ZM(M, F, I) k_zm<T, N, M, F, I><<<g, b, 0, st>>>(i, o, t, K)
+        if (mode == ZM_POT && fwd && inv) ZM(ZM_POT, true, true);
+        else if (mode == ZM_NONE && fwd && !inv) ZM(ZM_NONE, true, false);
+        else if (mode == ZM_NONE && !fwd && inv) ZM(ZM_NONE, false, true);
+        else if (mode == ZM_NONE && fwd && inv) ZM(ZM_NONE, true, true);
+        else if (mode == ZM_GRADX && !fwd && inv) ZM(ZM_GRADX, false, true);
+        else if (mode == ZM_GRADY && !fwd && inv) ZM(ZM_GRADY, false, true);
+        else if (mode == ZM_GRADZ && !fwd && inv) ZM(ZM_GRADZ, false, true);
+#undef ZM
+    }
+};
+
+template <typename T>
+bool fft_dispatch(int N, FftOps &ops) {
+#define CASE(n)                         \
+    case n:                             \
+        ops.xf = FftN<T, n>::xf;        \
+        ops.xi = FftN<T, n>::xi;        \
+        ops.yf = FftN<T, n>::yf;        \
+        ops.yi = FftN<T, n>::yi;        \
+        ops.zm = FftN<T, n>::zm;        \
+        ops.lines_per_block = FftN<T, n>::S::B; \
+        return true;
+    switch (N) {
+        CASE(16) CASE(24) CASE(32) CASE(48) CASE(64) CASE(96) CASE(128) CASE(192) CASE(256) CASE(384)
+        CASE(512) CASE(768) CASE(1024)
+    }
+#undef CASE
+    return false;
+}
+
+template bool fft_dispatch<float>(int, FftOps &);
+template bool fft_dispatch<double>(int, FftOps &);
+
+}  // namespace dj

@@ -1,0 +1,199 @@
+// The five passes of the PM Poisson solve on an N^3 mesh, as barrier-separated phases.
+//
+// HBM layouts (complex half spectrum, x halved: H = N/2 + 1):
+//   mesh  real   R[x + N (y + N z)]                the density / potential (column-major N^3)
+//   A     cx     A[y + N (kx + H z)]               after the x pass: y lines contiguous
+//   B     cx     B[z + N (kx + H ky)]              after the y pass: z lines contiguous
+// Pass sequence of one Poisson solve (each pass reads and writes HBM once):
+//   XF  r2c along x      mesh -> A   (lines of B consecutive y; stores transposed, B-wide chunks)
+//   YF  c2c along y      A -> B      (lines of B consecutive z; stores transposed)
+//   ZM  c2c along z, multiplier, inverse along z, in place on B (no transpose)
+//   YI  inverse along y  B -> A
+//   XI  c2r along x      A -> mesh   (Re of the kx = 0 and kx = N/2 planes, as c2r does)
+// The x axis is halved (the reference halves its slowest axis): every multiplier of the PM force
+// is even in k, so the choice of halved axis only changes round-off.
+//
+// Phases are __host__ __device__ so that tests can run a whole workgroup on the host; the
+// __global__ wrappers in fft.hip call them with __syncthreads() between phases.
+#pragma once
+
+#include <type_traits>
+
+#include "fft_engine.h"
+
+namespace dj {
+
+// k-space multiplier of one Poisson solve, from 1-D tables built on the host in FP64 and rounded
+// to T (the reference builds them in T as well): K_phi = invlap * mak / rho / nsets / N^3, DC = 0
+template <typename T>
+struct KMult {
+    const T *k2f, *k2h;        // (k^2) per full / half axis index      (lap_order == 0)
+    const T *tf, *th;          // FD Laplace symbol per full / half index (lap_order != 0)
+    const T *mf, *mh;          // deconvolution window^-2W, or null
+    const cx<T> *gf, *gh;      // gradient kernels (ik path) full / half
+    T rho, nsets, r3;          // n^3/N^3, n_resample^3, N^3 (all in T)
+    int lap_order;
+
+    DJ_HD T kphi(int kx, int ky, int kz) const {
+        if (kx == 0 && ky == 0 && kz == 0) return (T)0;
+        T il;
+        if (lap_order == 0) {
+            T k2 = k2h[kx] + k2f[ky] + k2f[kz];
+            il = (T)-1 / k2;
+        } else {
+            T inv = th[kx] + tf[ky] + tf[kz];
+            il = (T)1 / inv;
+        }
+        T v = il;
+        if (mf) v = il * (mh[kx] * mf[ky] * mf[kz]);
+        return v / rho / nsets / r3;
+    }
+};
+
+enum { ZM_POT = 0, ZM_GRADX = 1, ZM_GRADY = 2, ZM_GRADZ = 3, ZM_NONE = 4 };
+
+template <typename T, int N, int B, int NT>
+struct Passes {
+    static constexpr int NC = N / 2;      // complex length of the packed real x lines
+    static constexpr int H = N / 2 + 1;
+    using EX = LineFFT<T, NC, B, NT>;     // x pass engine (packed real lines)
+    using EF = LineFFT<T, N, B, NT>;      // y and z pass engine
+    static_assert(N % B == 0, "B must divide N");
+
+    // ------------------------------------------------------------------ XF: mesh -> A
+    // block (yb, z): lines y = yb*B + l.  IN is T (float mesh) or int64 (fixed-point mesh).
+    template <typename IN>
+    DJ_HD static void xf_load(const IN *mesh, cx<T> *lds, int yb, int z, int tid, double fixed_scale) {
+        size_t base = (size_t)N * N * z + (size_t)N * yb * B;
+        for (int i = tid; i < B * NC; i += NT) {
+            int l = i / NC, m = i - l * NC;
+            size_t g = base + (size_t)N * l + 2 * m;
+            cx<T> v;
+            if constexpr (sizeof(IN) == 8 && !std::is_floating_point<IN>::value) {
+                v = {(T)((double)mesh[g] * fixed_scale), (T)((double)mesh[g + 1] * fixed_scale)};
+            } else {
+                v = {(T)mesh[g], (T)mesh[g + 1]};
+            }
+            lds[EX::idx(l, m)] = v;
+        }
+    }
+    // X[k] = E[k] + W_N^k O[k], E = (Z[k] + conj Z[NC-k]) / 2, O = (Z[k] - conj Z[NC-k]) / 2i
+    DJ_HD static void xf_store(const cx<T> *lds, const cx<T> *twN, cx<T> *A, int yb, int z, int tid) {
+        for (int i = tid; i < B * H; i += NT) {
+            int kx = i / B, l = i - kx * B;
+            cx<T> z1 = lds[EX::idx(l, kx == NC ? 0 : kx)];
+            cx<T> z2 = conj(lds[EX::idx(l, kx == 0 ? 0 : NC - kx)]);
+            cx<T> e = scale(z1 + z2, (T)0.5);
+            cx<T> d = z1 - z2;
+            cx<T> o = {d.y * (T)0.5, -d.x * (T)0.5};
+            cx<T> x = e + cmul(o, twN[kx]);
+            A[(size_t)(yb * B + l) + (size_t)N * (kx + (size_t)H * z)] = x;
+        }
+    }
+
+    // ------------------------------------------------------------------ XI: A -> mesh
+    DJ_HD static void xi_load(const cx<T> *A, cx<T> *lds, int yb, int z, int tid) {
+        for (int i = tid; i < B * H; i += NT) {
+            int kx = i / B, l = i - kx * B;
+            lds[EX::idx(l, kx)] = A[(size_t)(yb * B + l) + (size_t)N * (kx + (size_t)H * z)];
+        }
+    }
+    // Z[k] = (X[k] + conj X[NC-k]) + i (X[k] - conj X[NC-k]) W_N^-k, for k and NC-k together;
+    // the imaginary parts of X[0] and X[NC] are dropped (c2r of a Hermitian line)
+    DJ_HD static void xi_pre(cx<T> *lds, const cx<T> *twN, int tid) {
+        constexpr int HP = NC / 2 + 1;
+        for (int i = tid; i < B * HP; i += NT) {
+            int l = i / HP, k = i - l * HP;
+            cx<T> a = lds[EX::idx(l, k)], b = lds[EX::idx(l, NC - k)];
+            if (k == 0) { a.y = 0; b.y = 0; }
+            // Z[k]
+            cx<T> e = a + conj(b);
+            cx<T> o = cmulc(a - conj(b), twN[k]);
+            cx<T> zk = {e.x - o.y, e.y + o.x};
+            if (k != 0 && 2 * k != NC) {
+                cx<T> e2 = b + conj(a);
+                cx<T> o2 = cmulc(b - conj(a), twN[NC - k]);
+                lds[EX::idx(l, NC - k)] = {e2.x - o2.y, e2.y + o2.x};
+            }
+            lds[EX::idx(l, k)] = zk;
+        }
+    }
+    DJ_HD static void xi_store(const cx<T> *lds, T *mesh, int yb, int z, int tid) {
+        size_t base = (size_t)N * N * z + (size_t)N * yb * B;
+        for (int i = tid; i < B * NC; i += NT) {
+            int l = i / NC, m = i - l * NC;
+            cx<T> v = lds[EX::idx(l, m)];
+            size_t g = base + (size_t)N * l + 2 * m;
+            mesh[g] = v.x;
+            mesh[g + 1] = v.y;
+        }
+    }
+
+    // ------------------------------------------------------------------ YF: A -> B, YI: B -> A
+    // block (zb, kx): lines z = zb*B + l
+    DJ_HD static void y_load_a(const cx<T> *A, cx<T> *lds, int zb, int kx, int tid) {
+        for (int i = tid; i < B * N; i += NT) {
+            int l = i / N, y = i - l * N;
+            lds[EF::idx(l, y)] = A[(size_t)y + (size_t)N * (kx + (size_t)H * (zb * B + l))];
+        }
+    }
+    DJ_HD static void y_store_b(const cx<T> *lds, cx<T> *Bm, int zb, int kx, int tid) {
+        for (int i = tid; i < B * N; i += NT) {
+            int ky = i / B, l = i - ky * B;
+            Bm[(size_t)(zb * B + l) + (size_t)N * (kx + (size_t)H * ky)] = lds[EF::idx(l, ky)];
+        }
+    }
+    DJ_HD static void y_load_b(const cx<T> *Bm, cx<T> *lds, int zb, int kx, int tid) {
+        for (int i = tid; i < B * N; i += NT) {
+            int ky = i / B, l = i - ky * B;
+            lds[EF::idx(l, ky)] = Bm[(size_t)(zb * B + l) + (size_t)N * (kx + (size_t)H * ky)];
+        }
+    }
+    DJ_HD static void y_store_a(const cx<T> *lds, cx<T> *A, int zb, int kx, int tid) {
+        for (int i = tid; i < B * N; i += NT) {
+            int l = i / N, y = i - l * N;
+            A[(size_t)y + (size_t)N * (kx + (size_t)H * (zb * B + l))] = lds[EF::idx(l, y)];
+        }
+    }
+
+    // ------------------------------------------------------------------ ZM: on B lines
+    // block b: line ids L = b*B + l, kx = L % H, ky = L / H
+    DJ_HD static void z_load(const cx<T> *Bm, cx<T> *lds, int b, int tid) {
+        size_t base = (size_t)N * B * b;
+        for (int i = tid; i < B * N; i += NT) {
+            int l = i / N, zz = i - l * N;
+            lds[EF::idx(l, zz)] = Bm[base + (size_t)N * l + zz];
+        }
+    }
+    template <int MODE>
+    DJ_HD static void z_mult(cx<T> *lds, const KMult<T> &K, int b, int tid) {
+        for (int i = tid; i < B * N; i += NT) {
+            int l = i / N, kz = i - l * N;
+            int L = b * B + l, ky = L / H, kx = L - ky * H;
+            cx<T> v = lds[EF::idx(l, kz)];
+            T kp = K.kphi(kx, ky, kz);
+            if constexpr (MODE == ZM_POT) {
+                v = scale(v, kp);
+            } else if constexpr (MODE == ZM_GRADX || MODE == ZM_GRADY || MODE == ZM_GRADZ) {
+                cx<T> g = MODE == ZM_GRADX ? K.gh[kx] : MODE == ZM_GRADY ? K.gf[ky] : K.gf[kz];
+                cx<T> gk = scale(g, kp);
+                cx<T> r = {gk.x * v.x - gk.y * v.y, gk.x * v.y + gk.y * v.x};
+                v = {-r.x, -r.y};
+            }
+            lds[EF::idx(l, kz)] = v;
+        }
+    }
+    DJ_HD static void z_store(const cx<T> *lds, cx<T> *Bm, int b, int tid) {
+        size_t base = (size_t)N * B * b;
+        for (int i = tid; i < B * N; i += NT) {
+            int l = i / N, zz = i - l * N;
+            Bm[base + (size_t)N * l + zz] = lds[EF::idx(l, zz)];
+        }
+    }
+
+    static constexpr int XBLOCKS() { return (N / B) * N; }
+    static constexpr int YBLOCKS() { return (N / B) * H; }
+    static constexpr int ZBLOCKS() { return (N * H) / B; }
+};
+
+}  // namespace dj

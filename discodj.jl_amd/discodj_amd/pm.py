@@ -1,0 +1,175 @@
+"""PM force on the MI355X library, mirroring NR/src/nbody/pm.jl's interface.
+
+    PMConfig(res_pm=, n_part=, boxsize=, worder=2, deconvolve=False, antialias=0, grad_order=4,
+             lap_order=0, n_resample=1, resampling="fourier")               (pm.jl:35-46)
+    S = PMSolverHIP(cfg, precision=32)                                        (PMSolver, pm.jl:385)
+    acc = S.pm_acceleration(psi)                                              (pm_acceleration!, pm.jl:507)
+    acc = pm_acceleration(psi, cfg)                                           (pm.jl:528)
+
+Particle arrays are numpy (3, n, n, n) in C order, i.e. the reference's column-major (n,n,n,3)
+[i,j,k,c] with c slowest; they are handed to the library without reordering.  Invalid options
+raise DJError (the reference raises ErrorException from the same checks, pm.jl:136,146,570).
+"""
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import DJConfig, DJError, KernelStat, check
+
+
+@dataclass
+class PMConfig:
+    res_pm: int
+    n_part: int
+    boxsize: float
+    worder: int = 2
+    deconvolve: bool = False
+    antialias: int = 0
+    grad_order: int = 4
+    lap_order: int = 0
+    n_resample: int = 1
+    resampling: str = "fourier"
+
+
+def _c_config(cfg, precision, deposit, lean, device, fixed_frac_bits):
+    if cfg.resampling not in ("fourier", "linear"):
+        raise DJError(-1, "resampling must be :fourier or :linear")
+    return DJConfig(cfg.res_pm, cfg.n_part, float(cfg.boxsize), cfg.worder, int(bool(cfg.deconvolve)),
+                    cfg.antialias, cfg.grad_order, cfg.lap_order, cfg.n_resample,
+                    0 if cfg.resampling == "fourier" else 1, precision, _lib.DEPOSIT[deposit],
+                    fixed_frac_bits, int(bool(lean)), device)
+
+
+class PMSolverHIP:
+    """Device-resident PM solver (a library context): owns the particle state (Psi, Pi), the mesh and
+    spectrum buffers and the k-space multipliers.  precision 32 keeps mesh, FFT, forces and the
+    particle state in FP32 (FP64 step coefficients); precision 64 runs the same kernels in FP64."""
+
+    def __init__(self, cfg, precision=32, deposit="float", lean=False, device=0, fixed_frac_bits=0):
+        self.cfg = cfg
+        self.precision = precision
+        self.dtype = np.float32 if precision == 32 else np.float64
+        self.n = cfg.n_part
+        self.shape = (3, self.n, self.n, self.n)
+        self.deposit_mode = deposit
+        L = _lib.lib()
+        ptr = C.c_void_p()
+        c = _c_config(cfg, precision, deposit, lean, device, fixed_frac_bits)
+        check(L.dj_create(C.byref(c), C.byref(ptr)))
+        self._ctx = ptr
+        self._L = L
+
+    def close(self):
+        if getattr(self, "_ctx", None):
+            self._L.dj_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _arr(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        if a.shape != self.shape:
+            a = a.reshape(self.shape)
+        return a
+
+    def _chk(self, rc):
+        return check(rc, self._ctx)
+
+    # state ------------------------------------------------------------------------------------
+    def set_state(self, psi, pi=None):
+        psi = self._arr(psi)
+        pi = None if pi is None else self._arr(pi)
+        self._chk(self._L.dj_set_state(self._ctx, psi.ctypes.data, None if pi is None else pi.ctypes.data))
+
+    def get_state(self):
+        psi = np.empty(self.shape, self.dtype)
+        pi = np.empty(self.shape, self.dtype)
+        self._chk(self._L.dj_get_state(self._ctx, psi.ctypes.data, pi.ctypes.data))
+        return psi, pi
+
+    # force ------------------------------------------------------------------------------------
+    def pm_acceleration(self, psi=None):
+        """pm_acceleration!(S, psi) (pm.jl:507-521): -grad phi at q + psi, nabla^2 phi = delta."""
+        acc = np.empty(self.shape, self.dtype)
+        src = None if psi is None else self._arr(psi)
+        self._chk(self._L.dj_acceleration(self._ctx, None if src is None else src.ctypes.data, acc.ctypes.data))
+        return acc
+
+    def deposit(self, psi=None, shift=None):
+        """pm_scatter! of q + psi (+ shift): unnormalised unit-mass density (pm.jl:97-106); int64
+        fixed-point values in the fixed64 deposit mode."""
+        N = self.cfg.res_pm
+        out = np.empty((N, N, N), np.int64 if self.deposit_mode == "fixed64" else self.dtype)
+        src = None if psi is None else self._arr(psi)
+        sh = None if shift is None else np.ascontiguousarray(shift, np.float64)
+        self._chk(self._L.dj_deposit(self._ctx, None if src is None else src.ctypes.data,
+                                     None if sh is None else sh.ctypes.data, out.ctypes.data))
+        return out
+
+    # stepping ---------------------------------------------------------------------------------
+    def step(self, dd1, alpha, beta, dd2):
+        self._chk(self._L.dj_step(self._ctx, float(dd1), float(alpha), float(beta), float(dd2)))
+
+    def run(self, dd1, alpha, beta, dd2, want=None, callback=None):
+        """n_steps DKD steps with FP64 coefficient arrays; callback(k) after the steps want[k] marks."""
+        arrs = [np.ascontiguousarray(a, np.float64) for a in (dd1, alpha, beta, dd2)]
+        n = len(arrs[0])
+        w = None if want is None else np.ascontiguousarray(want, np.uint8)
+        cb = _lib.STEP_CB(lambda k, u: callback(int(k))) if callback else _lib.STEP_CB()
+        self._chk(self._L.dj_run(self._ctx, n, *(a.ctypes.data for a in arrs),
+                                 None if w is None else w.ctypes.data, cb, None))
+
+    def check_finite(self):
+        self._chk(self._L.dj_check_finite(self._ctx))
+
+    # measurement ------------------------------------------------------------------------------
+    def synchronize(self):
+        self._chk(self._L.dj_synchronize(self._ctx))
+
+    def set_profiling(self, on=True):
+        self._chk(self._L.dj_set_profiling(self._ctx, int(on)))
+
+    def reset_stats(self):
+        self._chk(self._L.dj_reset_stats(self._ctx))
+
+    def kernel_stats(self):
+        arr = (KernelStat * 32)()
+        n = C.c_int32()
+        self._chk(self._L.dj_kernel_stats(self._ctx, arr, 32, C.byref(n)))
+        return {arr[i].name.decode(): {"calls": arr[i].calls, "ms_total": arr[i].ms_total,
+                                       "bytes_per_call": arr[i].bytes_per_call} for i in range(n.value)}
+
+
+def pm_acceleration(psi, cfg, precision=None, **kw):
+    """pm_acceleration(psi, cfg) (pm.jl:528-529) on the library; precision follows psi's dtype."""
+    psi = np.asarray(psi)
+    if precision is None:
+        precision = 64 if psi.dtype == np.float64 else 32
+    with PMSolverHIP(cfg, precision=precision, **kw) as S:
+        return S.pm_acceleration(psi)
+
+
+def fft3d(x, direction=-1, device=0):
+    """The library's 3-D transform (tests): real (N,N,N) [iz,iy,ix] -> half spectrum (N,N,N//2+1)
+    [kz,ky,kx] (x halved), or the unnormalised inverse."""
+    L = _lib.lib()
+    x = np.ascontiguousarray(x)
+    if direction < 0:
+        prec = 64 if x.dtype == np.float64 else 32
+        N = x.shape[0]
+        out = np.empty((N, N, N // 2 + 1), np.complex128 if prec == 64 else np.complex64)
+    else:
+        prec = 64 if x.dtype == np.complex128 else 32
+        N = x.shape[0]
+        out = np.empty((N, N, N), np.float64 if prec == 64 else np.float32)
+    check(L.dj_fft3d(N, prec, direction, x.ctypes.data, out.ctypes.data, device))
+    return out

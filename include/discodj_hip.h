@@ -1,0 +1,132 @@
+/*
+ * discodj_hip — C ABI of the MI355X (gfx950) particle-mesh N-body step.
+ *
+ * Drop-in boundary for DiscoDJNative's PM path (reference: native/DiscoDJNative/src/nbody/).
+ * The reference selects a solver object and calls it through one generic function
+ * (run_nbody.jl:140-141):
+ *       solver = lean ? PMSolverLean(cfg, psi_ini) : PMSolver(cfg, psi_ini)
+ *       acc(x) = pm_acceleration!(solver, x)
+ * and steps  Psi += dd1 Mom;  Mom = alpha Mom + beta acc(Psi);  Psi += dd2 Mom  (run_nbody.jl:155-158)
+ * with FP64 coefficients from stepper_coefficients (run_nbody.jl:48-101) kept on the host.
+ * A third solver type on the Julia side (PMSolverHIP, see the integration notes) ccalls these entry
+ * points; the particle state stays resident on the device between steps.
+ *
+ * Conventions
+ *   - Every function returns dj_status: 0 on success, a negative DJ_ERR_* code otherwise; no C++
+ *     exception crosses the ABI.  dj_last_error() describes the last failure of a context.
+ *   - Particle arrays are the reference's column-major (n,n,n,3) arrays: element (i,j,k,c) at
+ *     i + n*j + n^2*k + n^3*c, in the context's precision (float for precision 32, double for 64).
+ *   - Meshes are column-major (N,N,N): cell (ix,iy,iz) at ix + N*iy + N^2*iz.
+ *   - Host pointers are borrowed for the duration of the call; the call copies and blocks.
+ *   - The library allocates and owns all device memory.  A context owns one HIP stream of default
+ *     priority and is single-threaded; separate contexts may be used from separate threads.
+ */
+#ifndef DISCODJ_HIP_H
+#define DISCODJ_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t dj_status;
+
+#define DJ_OK 0
+#define DJ_ERR_INVALID_CONFIG (-1)   /* option outside what the reference accepts (pm.jl error paths) */
+#define DJ_ERR_INVALID_ARGUMENT (-2) /* null pointer, bad size */
+#define DJ_ERR_UNSUPPORTED (-3)      /* valid reference option not built into this library */
+#define DJ_ERR_OUT_OF_MEMORY (-4)    /* device allocation failed */
+#define DJ_ERR_DEVICE (-5)           /* HIP runtime error */
+#define DJ_ERR_NONFINITE (-6)        /* NaN/Inf found in the particle state (dj_check_finite) */
+
+#define DJ_DEPOSIT_FLOAT 0   /* atomic adds in the working precision (order-dependent round-off) */
+#define DJ_DEPOSIT_FIXED64 1 /* int64 fixed-point weights: bit-reproducible density */
+
+/* PMConfig (pm.jl:35-46) plus the device-side choices.  Field meanings and accepted values are the
+ * reference's: worder 2 (CIC) / 3 (TSC) / 4 (PCS); antialias -1, 0, 1, 2, 3; grad_order 0 (ik) or
+ * 2/4/6 (finite differences); lap_order 0 (-1/k^2) or 2/4/6; resampling 0 = fourier, 1 = linear. */
+typedef struct dj_pm_config {
+    int32_t res_pm;
+    int32_t n_part;
+    double boxsize;
+    int32_t worder;
+    int32_t deconvolve;
+    int32_t antialias;
+    int32_t grad_order;
+    int32_t lap_order;
+    int32_t n_resample;
+    int32_t resampling;
+    int32_t precision;       /* 32 or 64 */
+    int32_t deposit;         /* DJ_DEPOSIT_FLOAT or DJ_DEPOSIT_FIXED64 */
+    int32_t fixed_frac_bits; /* fractional bits of the fixed-point deposit (0 -> 32) */
+    int32_t lean;            /* 1: PMSolverLean semantics and option checks (pm.jl:569-571) */
+    int32_t device;          /* HIP device ordinal */
+} dj_pm_config;
+
+typedef struct dj_ctx dj_ctx;
+
+/* Called by dj_run after step k (k = 0 before the first step) when want[k] is set; the callback may
+ * call dj_get_state.  a_k is not known to the library: the host keeps all_a. */
+typedef void (*dj_step_cb)(int32_t k, void *user);
+
+/* Per-kernel timing of a context (dj_set_profiling): HIP events on the context's stream. */
+typedef struct dj_kernel_stat {
+    char name[32];
+    int32_t calls;
+    double ms_total;        /* summed event time */
+    double bytes_per_call;  /* algorithmic HBM bytes of one launch */
+} dj_kernel_stat;
+
+/* PMSolver(cfg, like) / PMSolverLean(cfg, like) (pm.jl:385-414, 569-585): validates the options,
+ * builds the k-space multipliers in FP64 on the host, allocates the device buffers. */
+dj_status dj_create(const dj_pm_config *cfg, dj_ctx **out);
+dj_status dj_destroy(dj_ctx *ctx);
+/* message of the last failing call on ctx (ctx NULL: of the last failing dj_create) */
+const char *dj_last_error(const dj_ctx *ctx);
+
+/* particle state (Psi, Pi): host arrays of 3 n^3 elements; pi NULL sets zero momentum */
+dj_status dj_set_state(dj_ctx *ctx, const void *psi, const void *pi);
+dj_status dj_get_state(dj_ctx *ctx, void *psi, void *pi);
+/* the same from / to device memory (same layout, same precision) */
+dj_status dj_set_state_device(dj_ctx *ctx, const void *d_psi, const void *d_pi);
+dj_status dj_get_state_device(dj_ctx *ctx, void *d_psi, void *d_pi);
+
+/* pm_acceleration!(S, psi) (pm.jl:507-521, 623-641): -grad phi at X = q + psi, nabla^2 phi = delta,
+ * averaged over the interlacing shifts.  psi NULL uses the context's Psi; acc_out is a host array. */
+dj_status dj_acceleration(dj_ctx *ctx, const void *psi, void *acc_out);
+
+/* one DKD step of run_nbody (run_nbody.jl:156-158) on the resident state, FP64 coefficients */
+dj_status dj_step(dj_ctx *ctx, double dd1, double alpha, double beta, double dd2);
+
+/* the step loop of run_nbody (run_nbody.jl:151-163): n_steps DKD steps with per-step coefficients
+ * (arrays of n_steps).  want (n_steps + 1 entries, or NULL) selects the steps after which cb runs
+ * (callback_when); between such steps the next drift is fused into the kick. */
+dj_status dj_run(dj_ctx *ctx, int32_t n_steps, const double *dd1, const double *alpha, const double *beta,
+                 const double *dd2, const uint8_t *want, dj_step_cb cb, void *user);
+
+/* pm_scatter! (pm.jl:97-106) of X = q + psi + shift (shift NULL = 0): the unit-mass density
+ * mesh, unnormalised.  mesh_out is a host array of N^3 elements of the context's precision, or of
+ * int64 fixed-point values (fixed_frac_bits fractional bits) in DJ_DEPOSIT_FIXED64 mode. */
+dj_status dj_deposit(dj_ctx *ctx, const void *psi, const double *shift, void *mesh_out);
+
+/* NaN/Inf scan of the particle state (failure detection); DJ_ERR_NONFINITE if any */
+dj_status dj_check_finite(dj_ctx *ctx);
+
+dj_status dj_synchronize(dj_ctx *ctx);
+dj_status dj_set_profiling(dj_ctx *ctx, int32_t on);
+dj_status dj_reset_stats(dj_ctx *ctx);
+dj_status dj_kernel_stats(dj_ctx *ctx, dj_kernel_stat *out, int32_t cap, int32_t *n_out);
+
+/* Stand-alone 3-D transform with the library's kernels (tests and tools): rfft (direction -1)
+ * of a real (N,N,N) host array into the x-halved spectrum out[kx + (N/2+1)*(ky + N*kz)], or the
+ * unnormalised inverse (direction +1) back to real.  precision 32 or 64. */
+dj_status dj_fft3d(int32_t N, int32_t precision, int32_t direction, const void *in, void *out, int32_t device);
+
+/* library version string */
+const char *dj_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
