@@ -1,0 +1,156 @@
+#!/usr/bin/env python3
+"""Throughput of the PM N-body step on one MI355X: particle-updates/s at 512^3 particles on a 1024^3
+mesh (box 1000 Mpc/h, TSC with interlacing, 6th-order gradient and Laplace kernels, FastPM 20 steps
+from z = 19 to z = 0), FP32 mesh / FFT / forces with FP64 step coefficients.
+
+A step is one FastPM DKD step over all particles (two interlaced force passes).  The initial
+conditions are synthetic: the N-GenIC white noise of seed 42, the Eisenstein-Hu P(k) of the
+workload's cosmology, 2LPT at z = 19, generated on the device (outside the timed region).  The
+particle state is resident in HBM when the timed region starts.
+
+Reported beside the throughput:
+  accuracy       max |P32(k)/P64(k) - 1| for kf <= k < k_Nyq/2 and rms(Psi32 - Psi64)/(L/n) of the
+                 FP32 run against the same run in FP64 (same kernels, same ICs)
+  roofline       the dominant kernel's algorithmic bytes per launch / its mean launch time (HIP
+                 events on the library's stream over the timed region) against 8 TB/s
+  cpu_baseline   the CPU oracle (the C restatement of the reference path, FP32, OpenMP) on the host
+                 cores: one interlacing pass of the same force, the step counted as two passes
+
+usage: python bench.py [--gpus 1] [--steps 19] [--warmup 1] [--no-accuracy] [--no-cpu] [--small]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "discodj.jl_amd"))
+
+HBM_PEAK_GBS = 8000.0
+COSMO = dict(Omega_c=0.2621, Omega_b=0.049, h=0.6766, sigma8=0.8102, n_s=0.9665)
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=19)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--small", action="store_true", help="128^3 particles on a 256^3 mesh (quick check)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: every box of this build has one MI355X; --gpus {args.gpus} "
+              f"is not run (the multi-GPU slab/pencil decomposition is a design, not a build)", file=sys.stderr)
+        return 2
+
+    import numpy as np
+
+    import discodj_amd as D
+
+    metric = json.load(open(os.path.join(ROOT, "BASELINE.json")))["metric"]
+    n, N, L = (128, 256, 250.0) if args.small else (512, 1024, 1000.0)
+    a_ini, a_end, seed = 0.05, 1.0, 42
+    W, K = args.warmup, args.steps
+    total = max(20, W + K)
+    cfg = D.PMConfig(res_pm=N, n_part=n, boxsize=L, worder=3, antialias=1, grad_order=6, lap_order=6)
+    c = D.compute_timetables(D.Cosmology(**COSMO))
+    k_tab, pk_tab = D.linear_power_spectrum(c)
+    co = D.stepper_coefficients(c, "fastpm", a_ini, a_end, total, "D")
+    Dg = c.Dplus(a_ini)
+    coef = [co["ddrift1"], co["alpha"], co["beta"], co["ddrift2"]]
+
+    log(f"workload {n}^3 particles, {N}^3 mesh, L={L}, FastPM {total} steps; ICs on the device")
+    S = D.PMSolverHIP(cfg, precision=32)
+    t = time.perf_counter()
+    S.ics_lpt(k_tab, pk_tab, Dg, order=2, seed=seed)
+    log(f"2LPT ICs in {time.perf_counter() - t:.1f}s")
+    ic_psi, ic_pi = S.get_state() if not args.no_cpu else (None, None)
+
+    if W:
+        S.run(*(a[:W] for a in coef))
+    S.synchronize()
+    S.set_profiling(True)
+    S.reset_stats()
+    log(f"timing {K} steps after {W} warm-up steps")
+    t0 = time.perf_counter()
+    S.run(*(a[W:W + K] for a in coef))
+    S.synchronize()
+    dt = time.perf_counter() - t0
+    stats = S.kernel_stats()
+    S.set_profiling(False)
+    value = n**3 * K / dt
+    log(f"{dt / K * 1e3:.2f} ms/step, {value:.3e} particle-updates/s")
+    if W + K < total:
+        S.run(*(a[W + K:] for a in coef))
+
+    kernels = {}
+    for name, s in stats.items():
+        if s["calls"]:
+            ms = s["ms_total"] / s["calls"]
+            gbs = s["bytes_per_call"] / (ms * 1e-3) / 1e9
+            kernels[name] = {"calls": s["calls"], "ms_per_call": round(ms, 4), "ms_total": round(s["ms_total"], 3),
+                             "algorithmic_GB": round(s["bytes_per_call"] / 1e9, 4), "achieved_GBs": round(gbs, 1),
+                             "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    dom = max(kernels, key=lambda k: kernels[k]["ms_total"])
+    roof = {"kernel": dom, "bound": "hbm", "achieved": kernels[dom]["achieved_GBs"], "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": kernels[dom]["frac"], "traffic": None,
+            "algorithmic_bytes_per_launch": int(kernels[dom]["algorithmic_GB"] * 1e9)}
+
+    accuracy = None
+    if not args.no_accuracy:
+        psi32, _ = S.get_state()
+        S.close()
+        log("FP64 run of the same workload for the accuracy reference")
+        S64 = D.PMSolverHIP(cfg, precision=64)
+        S64.ics_lpt(k_tab, pk_tab, Dg, order=2, seed=seed)
+        S64.run(*coef)
+        psi64, _ = S64.get_state()
+        k, p64, nm = S64.power_spectrum(W=2)
+        _, p32, _ = S64.power_spectrum(psi=psi32.astype(np.float64), W=2)
+        S64.close()
+        ok = nm > 0
+        rel = np.abs(p32[ok] / p64[ok] - 1)
+        disp = float(np.sqrt(np.mean((psi32.astype(np.float64) - psi64) ** 2)) / (L / n))
+        accuracy = {"pk_max_rel_err": float(rel.max()), "pk_k_range": [float(k[ok][0]), float(k[ok][-1])],
+                    "pk_bins": int(ok.sum()), "disp_rms_over_spacing": disp, "reference": "same run in FP64"}
+        log(f"P(k) max rel err {rel.max():.2e}, displacement rms / spacing {disp:.2e}")
+        del psi32, psi64
+    else:
+        S.close()
+
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        ocfg = O.pm_config(res_pm=N, n_part=n, boxsize=L, worder=3, antialias=0, grad_order=6, lap_order=6)
+        log(f"CPU oracle: one force pass on {threads} threads")
+        t = time.perf_counter()
+        O.pm_acceleration(ic_psi, ocfg, "solver")
+        t_pass = time.perf_counter() - t
+        cpu = {"value": n**3 / (2 * t_pass), "unit": "particle-updates/s", "cores": threads, "kind": "port",
+               "sample": f"one interlacing pass (deposit, FFT Poisson solve, FD gradient, gather) of the "
+                         f"{n}^3 / {N}^3 FP32 force at the z=19 ICs, {t_pass:.1f} s; a step is two passes"}
+        log(f"CPU pass {t_pass:.1f}s -> {cpu['value']:.3e} particle-updates/s")
+
+    out = {"metric": metric, "value": value, "unit": "particle-updates/s", "n_gpus": 1, "steps": K, "warmup": W,
+           "ms_per_step": dt / K * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"PM N-body, {n}^3 particles on a {N}^3 mesh, L={L} Mpc/h, TSC, interlacing "
+                                  f"(2 passes), grad/Laplace order 6, FastPM {total} steps z=19->0, 2LPT ICs "
+                                  f"(N-GenIC seed {seed}), FP32 state and mesh, FP64 coefficients",
+                      "particles": n**3, "mesh": N**3, "boxsize": L, "stepper": "fastpm", "n_steps": total},
+           "roofline": roof, "kernels": kernels, "accuracy": accuracy, "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -600,6 +600,86 @@ dj_status dj_deposit(dj_ctx *c, const void *psi, const double *shift, void *mesh
     return DJ_OK;
 }
 
+dj_status dj_ics_lpt(dj_ctx *c, uint32_t seed, const double *white, const double *k_tab, const double *pk_tab,
+                     int32_t ntab, double D, int32_t order) {
+    if (!c || !k_tab || !pk_tab || ntab < 2 || (order != 1 && order != 2))
+        return fail(c, DJ_ERR_INVALID_ARGUMENT, "invalid IC arguments");
+    const int n = c->n;
+    FftOps probe{};
+    if (n % 2 || !fft_dispatch<double>(n, probe) || (order == 2 && !fft_dispatch<double>(3 * n / 2, probe)))
+        return fail(c, DJ_ERR_UNSUPPORTED, "n_part " + std::to_string(n) + " has no built IC transform size");
+    HIPCHK(c, hipSetDevice(c->dev));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    std::vector<double> field;
+    if (!white) {
+        field.resize(2 * (size_t)n * n * (n / 2 + 1));
+        ngenic_field(seed, n, field.data());
+    }
+    int e = ics_lpt_device(n, c->cfg.boxsize, order, white, white ? nullptr : field.data(), k_tab, pk_tab, ntab, D,
+                           c->cfg.precision, c->psi, c->mom, c->st);
+    if (e) return hip_fail(c, (hipError_t)e, "dj_ics_lpt");
+    return DJ_OK;
+}
+
+dj_status dj_power_spectrum(dj_ctx *c, const void *psi, int32_t W, int32_t nbins, double *k_out, double *pk_out,
+                            int64_t *nmodes_out) {
+    if (!c || !k_out || !pk_out || W < 0 || W > 4 || nbins < 1 || nbins > 2048)
+        return fail(c, DJ_ERR_INVALID_ARGUMENT, "invalid power spectrum arguments");
+    HIPCHK(c, hipSetDevice(c->dev));
+    const int dw = W == 0 ? 2 : W;
+    const void *src = c->psi;
+    if (psi) {
+        if (!c->psi_in) HIPCHK(c, hipMalloc(&c->psi_in, (size_t)c->n3 * c->tsize));
+        HIPCHK(c, hipMemcpyAsync(c->psi_in, psi, (size_t)c->n3 * c->tsize, hipMemcpyHostToDevice, c->st));
+        src = c->psi_in;
+    }
+    PartArgs pa{};
+    pa.psi = src;
+    pa.q1 = c->q1;
+    pa.n = c->n;
+    pa.res = c->res;
+    pa.L = c->cfg.boxsize;
+    c->part.zero(c->mesh, c->M * c->tsize, c->st);
+    c->part.scatter(pa, dw, false, 32, c->mesh, c->st);
+    c->fft.xf(c->mesh, false, c->A, c->tw, 1.0, c->st);
+    c->fft.yf(c->A, c->B, c->tw, c->st);
+    c->fft.zm(ZM_NONE, true, false, c->B, c->B, c->tw, c->kmult, c->st);
+    void *sums = nullptr;
+    HIPCHK(c, hipMalloc(&sums, (size_t)nbins * 24));
+    c->part.zero(sums, (size_t)nbins * 24, c->st);
+    double *sp = (double *)sums, *sw = sp + nbins;
+    unsigned long long *cnt = (unsigned long long *)(sw + nbins);
+    const double np6 = (double)c->np * (double)c->np, V = c->cfg.boxsize * c->cfg.boxsize * c->cfg.boxsize;
+    if (c->tsize == 8) pk_bin<double>(c->B, c->res, c->cfg.boxsize, W, nbins, V / np6, sp, sw, cnt, c->st);
+    else pk_bin<float>(c->B, c->res, c->cfg.boxsize, W, nbins, V / np6, sp, sw, cnt, c->st);
+    std::vector<double> h(3 * (size_t)nbins);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(h.data(), sums, (size_t)nbins * 24, hipMemcpyDeviceToHost, c->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    (void)hipFree(sums);
+    if (e != hipSuccess) return hip_fail(c, e, "dj_power_spectrum");
+    const double kf = 2 * M_PI / c->cfg.boxsize;
+    const unsigned long long *hc = (const unsigned long long *)(h.data() + 2 * nbins);
+    for (int b = 0; b < nbins; b++) {
+        k_out[b] = kf * (b + 1);
+        pk_out[b] = h[nbins + b] > 0 ? h[b] / h[nbins + b] : 0.0;
+        if (nmodes_out) nmodes_out[b] = (int64_t)hc[b];
+    }
+    return DJ_OK;
+}
+
+dj_status dj_ngenic_field(uint32_t seed, int32_t res, double *out) {
+    if (!out || res < 2 || res % 2) return fail(nullptr, DJ_ERR_INVALID_ARGUMENT, "invalid N-GenIC arguments");
+    ngenic_field(seed, res, out);
+    return DJ_OK;
+}
+
+dj_status dj_ranlxd1_uniforms(uint32_t seed, int32_t n, double *out) {
+    if (!out || n < 0) return fail(nullptr, DJ_ERR_INVALID_ARGUMENT, "invalid arguments");
+    ranlxd1_uniforms(seed, n, out);
+    return DJ_OK;
+}
+
 dj_status dj_check_finite(dj_ctx *c) {
     if (!c) return fail(c, DJ_ERR_INVALID_ARGUMENT, "null context");
     size_t bytes = (size_t)c->n3 * c->tsize;

@@ -59,4 +59,16 @@ struct PartOps {
 };
 template <typename T> void part_ops(PartOps &ops);
 
+// power spectrum binning of an x-halved B-layout spectrum (analysis.hip)
+template <typename T>
+void pk_bin(const void *B, int N, double L, int W, int nbins, double scale, double *sum_p, double *sum_w,
+            unsigned long long *cnt, hipStream_t st);
+
+// initial conditions (ics.hip, ngenic.cpp)
+void ranlxd1_uniforms(uint32_t seed, int n, double *out);
+void ngenic_field(uint32_t seed, int res, double *out);
+int ics_lpt_device(int n, double L, int order, const double *white_host, const double *field_host,
+                   const double *kt, const double *pt, int ntab, double D, int prec, void *psi_out,
+                   void *pi_out, hipStream_t st);
+
 }  // namespace dj

@@ -1,0 +1,281 @@
+// Initial conditions of the PM path on the device, in FP64 (the IC pipeline of the reference):
+//   white noise w ............ N-GenIC field (host, ngenic.cpp) -> real space by the library's c2r
+//                              (ngenic_wnoise_real, NR/src/ics/ngenic_gsl.jl:174-177)
+//   phi(k) ................... rfft(w) * (-sqrt(P(k) (n/L)^3) / k^2), 0 at k = 0 (grf.jl:118-144)
+//   psi_1 .................... irfft(-(i k_c) phi)  (nlpt_core.jl:285-297)
+//   psi_2 (n_order = 2) ...... de-aliased on ext = 3n/2: A_ik = irfft_ext(i k_k pad(psi1_i)), the
+//                              mu2 source A11 A22 + A11 A33 + A22 A33 - A12 A21 - A13 A31 - A23 A32
+//                              times -3/7, crop(rfft_ext(.)), psi_2 = irfft(-1/k^2 (i k_c) .)
+//                              (nlpt_fast.jl:27-64, 208-249; nlpt_core.jl:126-177)
+//   Psi = D psi1 + D^2 psi2, Pi = psi1 + 2 D psi2 (run_nbody.jl:200-206)
+// Spectra are in the library's x-halved B layout  B[kz + N (kx + H ky)].
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <vector>
+
+#include "dj_internal.h"
+#include "fft_passes.h"
+
+namespace dj {
+
+struct BIdx {
+    int kx, ky, kz;
+};
+__device__ __forceinline__ BIdx bidx(size_t e, int N) {
+    const int H = N / 2 + 1;
+    BIdx r;
+    r.kz = (int)(e % N);
+    size_t t = e / N;
+    r.kx = (int)(t % H);
+    r.ky = (int)(t / H);
+    return r;
+}
+__device__ __forceinline__ int sgn_full(int m, int N) { return m < N / 2 ? m : m - N; }
+
+// phi(k) = F(w)(k) * scale(k), k grid of grf.jl:190-216, P interpolated as np.interp (clamped)
+__global__ void k_ic_scale(cx<double> *B, int N, double L, const double *kt, const double *pt, int ntab) {
+    const int H = N / 2 + 1;
+    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x, tot = (size_t)N * H * N;
+    if (e >= tot) return;
+    BIdx b = bidx(e, N);
+    double dk = 2 * M_PI / L;
+    // grf.jl indexes the full axes as i <= N/2 ? i : i - N (only k^2 enters)
+    double kx = b.kx * dk, ky = (b.ky <= N / 2 ? b.ky : b.ky - N) * dk, kz = (b.kz <= N / 2 ? b.kz : b.kz - N) * dk;
+    double k2 = kx * kx + ky * ky + kz * kz;
+    double s = 0;
+    if (k2 != 0) {
+        double k = sqrt(k2), P;
+        if (k <= kt[0]) P = pt[0];
+        else if (k >= kt[ntab - 1]) P = pt[ntab - 1];
+        else {
+            int lo = 0, hi = ntab - 1;  // kt[lo] < k <= kt[hi]
+            while (hi - lo > 1) {
+                int mid = (lo + hi) / 2;
+                if (kt[mid] < k) lo = mid; else hi = mid;
+            }
+            double t = (k - kt[hi - 1]) / (kt[hi] - kt[hi - 1]);
+            P = pt[hi - 1] * (1 - t) + pt[hi] * t;
+        }
+        double rb = (double)N / L;
+        s = -sqrt(P * (rb * rb * rb)) / k2;
+    }
+    B[e] = scale(B[e], s);
+}
+
+// i k_c along a physical axis with the Nyquist entry zeroed (nlpt_core.jl:49-67)
+__device__ __forceinline__ double kdir(const BIdx &b, int c, int N, double dk) {
+    if (c == 0) return b.kx == N / 2 ? 0.0 : b.kx * dk;
+    int m = c == 1 ? b.ky : b.kz;
+    return m == N / 2 ? 0.0 : sgn_full(m, N) * dk;
+}
+
+enum { IC_PSI1 = 0, IC_DERIV = 1, IC_PSI2 = 2 };
+
+// out = mult(k) in:  PSI1: -(i k_c) ;  DERIV: i k_c ;  PSI2: (-1/k^2) (i k_c)
+template <int MODE>
+__global__ void k_ic_mult(const cx<double> *in, cx<double> *out, int N, double L, int c) {
+    const int H = N / 2 + 1;
+    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x, tot = (size_t)N * H * N;
+    if (e >= tot) return;
+    BIdx b = bidx(e, N);
+    double dk = 2 * M_PI / L;
+    double g = kdir(b, c, N, dk);
+    cx<double> v = in[e];
+    cx<double> r = {-g * v.y, g * v.x};  // (i g) v
+    if constexpr (MODE == IC_PSI1) r = {-r.x, -r.y};
+    if constexpr (MODE == IC_PSI2) {
+        double kx = b.kx * dk, ky = sgn_full(b.ky, N) * dk, kz = sgn_full(b.kz, N) * dk;
+        double k2 = (kx * kx + ky * ky) + kz * kz;
+        double il = -1.0 / (e == 0 ? 1.0 : k2);
+        r = scale(r, il);
+    }
+    out[e] = r;
+}
+
+// zero-pad (n -> e) or crop (e -> n) between B layouts, Nyquist planes dropped, amplitude rescaled
+__global__ void k_ic_pad(const cx<double> *in, cx<double> *out, int n, int ext, double s) {
+    const int He = ext / 2 + 1, Hn = n / 2 + 1, h = n / 2;
+    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x, tot = (size_t)ext * He * ext;
+    if (e >= tot) return;
+    BIdx b = bidx(e, ext);
+    auto src = [&](int m) { return m < h ? m : (m > ext - h ? m - ext + n : -1); };
+    int sx = b.kx < h ? b.kx : -1, sy = src(b.ky), sz = src(b.kz);
+    cx<double> v = {0, 0};
+    if (sx >= 0 && sy >= 0 && sz >= 0) v = scale(in[(size_t)sz + (size_t)n * (sx + (size_t)Hn * sy)], s);
+    out[e] = v;
+}
+__global__ void k_ic_crop(const cx<double> *in, cx<double> *out, int n, int ext, double s) {
+    const int He = ext / 2 + 1, Hn = n / 2 + 1, h = n / 2;
+    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x, tot = (size_t)n * Hn * n;
+    if (e >= tot) return;
+    BIdx b = bidx(e, n);
+    auto src = [&](int m) { return m < h ? m : (m > h ? m - n + ext : -1); };
+    int sx = b.kx < h ? b.kx : -1, sy = src(b.ky), sz = src(b.kz);
+    cx<double> v = {0, 0};
+    if (sx >= 0 && sy >= 0 && sz >= 0) v = scale(in[(size_t)sz + (size_t)ext * (sx + (size_t)He * sy)], s);
+    out[e] = v;
+}
+
+// mu op= a * b  (op 0 assign, 1 add, 2 subtract); the products and sums in the reference's order
+__global__ void k_ic_mu(double *mu, const double *a, const double *b, size_t n, int op) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double p = a[i] * b[i];
+    mu[i] = op == 0 ? p : op == 1 ? mu[i] + p : mu[i] - p;
+}
+
+__global__ void k_ic_scale_real(double *x, size_t n, double s) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] *= s;
+}
+
+template <typename T>
+__global__ void k_ic_combine(const double *p1, const double *p2, T *psi, T *pi, size_t n3, double D, int order) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n3) return;
+    double a = 0.0 + D * p1[i], b = 0.0 + 1.0 * p1[i];
+    if (order >= 2) {
+        a = a + (D * D) * p2[i];
+        b = b + (2 * D) * p2[i];
+    }
+    psi[i] = (T)a;
+    pi[i] = (T)b;
+}
+
+static unsigned g1(size_t n) { return (unsigned)((n + 255) / 256); }
+
+// host orchestration; returns a HIP error code (0 = success)
+int ics_lpt_device(int n, double L, int order, const double *white_host, const double *field_host,
+                   const double *kt_h, const double *pt_h, int ntab, double D, int prec, void *psi_out,
+                   void *pi_out, hipStream_t st) {
+    FftOps fn{}, fe{};
+    const int ext = 3 * n / 2;
+    if (!fft_dispatch<double>(n, fn)) return (int)hipErrorInvalidValue;
+    if (order >= 2 && !fft_dispatch<double>(ext, fe)) return (int)hipErrorInvalidValue;
+    const int Hn = n / 2 + 1, He = ext / 2 + 1;
+    const size_t M = (size_t)n * n * n, HC = (size_t)n * Hn * n, ME = (size_t)ext * ext * ext,
+                 HE = (size_t)ext * He * ext;
+    std::vector<void *> allocs;
+    hipError_t err = hipSuccess;
+    auto alloc = [&](size_t bytes) -> void * {
+        void *p = nullptr;
+        if (err == hipSuccess) err = hipMalloc(&p, bytes);
+        if (p) allocs.push_back(p);
+        return p;
+    };
+    auto twiddles = [&](int N) -> void * {
+        std::vector<cx<double>> tw(N);
+        for (int k = 0; k < N; k++) {
+            double a = -2.0 * M_PI * (double)k / (double)N;
+            tw[k] = {std::cos(a), std::sin(a)};
+        }
+        void *d = alloc(sizeof(cx<double>) * N);
+        if (d) err = hipMemcpy(d, tw.data(), sizeof(cx<double>) * N, hipMemcpyHostToDevice);
+        return d;
+    };
+    void *twn = twiddles(n);
+    cx<double> *A = (cx<double> *)alloc(HC * 16), *Bm = (cx<double> *)alloc(HC * 16), *F = (cx<double> *)alloc(HC * 16);
+    double *w = (double *)alloc(M * 8), *p1 = (double *)alloc(3 * M * 8), *p2 = (double *)alloc(3 * M * 8);
+    double *kt = (double *)alloc(ntab * 8), *pt = (double *)alloc(ntab * 8);
+    if (err == hipSuccess) err = hipMemcpy(kt, kt_h, ntab * 8, hipMemcpyHostToDevice);
+    if (err == hipSuccess) err = hipMemcpy(pt, pt_h, ntab * 8, hipMemcpyHostToDevice);
+    const KMult<double> none{};  // the plain transforms apply no multiplier
+    auto fwd = [&](FftOps &f, const double *real, cx<double> *a, cx<double> *b, void *tw) {
+        f.xf(real, false, a, tw, 1.0, st);
+        f.yf(a, b, tw, st);
+        f.zm(ZM_NONE, true, false, b, b, tw, &none, st);
+    };
+    auto inv = [&](FftOps &f, cx<double> *b, cx<double> *a, double *real, void *tw) {  // destroys b
+        f.zm(ZM_NONE, false, true, b, b, tw, &none, st);
+        f.yi(b, a, tw, st);
+        f.xi(a, real, tw, st);
+    };
+    if (err != hipSuccess) goto done;
+    // white noise
+    if (white_host) {
+        err = hipMemcpy(w, white_host, M * 8, hipMemcpyHostToDevice);
+    } else {
+        // N-GenIC field (z halved) -> B layout (x halved) by Hermitian symmetry, times n^(3/2)
+        std::vector<cx<double>> hb(HC);
+        const double sc = std::pow((double)n, 1.5);
+        const cx<double> *f = (const cx<double> *)field_host;
+#pragma omp parallel for collapse(2)
+        for (int ky = 0; ky < n; ky++)
+            for (int kx = 0; kx < Hn; kx++)
+                for (int kz = 0; kz < n; kz++) {
+                    cx<double> v;
+                    if (kz <= n / 2) {
+                        v = f[(size_t)kx + n * ((size_t)ky + (size_t)n * kz)];
+                    } else {
+                        int mx = (n - kx) % n, my = (n - ky) % n, mz = n - kz;
+                        v = conj(f[(size_t)mx + n * ((size_t)my + (size_t)n * mz)]);
+                    }
+                    hb[(size_t)kz + (size_t)n * (kx + (size_t)Hn * ky)] = scale(v, sc);
+                }
+        err = hipMemcpy(Bm, hb.data(), HC * 16, hipMemcpyHostToDevice);
+        if (err == hipSuccess) {
+            inv(fn, Bm, A, w, twn);
+            k_ic_scale_real<<<g1(M), 256, 0, st>>>(w, M, 1.0 / (double)M);
+        }
+    }
+    if (err != hipSuccess) goto done;
+    // phi(k)
+    fwd(fn, w, A, F, twn);
+    k_ic_scale<<<g1(HC), 256, 0, st>>>(F, n, L, kt, pt, ntab);
+    for (int c = 0; c < 3; c++) {
+        k_ic_mult<IC_PSI1><<<g1(HC), 256, 0, st>>>(F, Bm, n, L, c);
+        inv(fn, Bm, A, p1 + c * M, twn);
+        k_ic_scale_real<<<g1(M), 256, 0, st>>>(p1 + c * M, M, 1.0 / (double)M);
+    }
+    if (order >= 2) {
+        void *twe = twiddles(ext);
+        cx<double> *Ae = (cx<double> *)alloc(HE * 16), *Be = (cx<double> *)alloc(HE * 16), *Pe = (cx<double> *)alloc(HE * 16);
+        double *mu = (double *)alloc(ME * 8);
+        double *fld[3] = {(double *)alloc(ME * 8), (double *)alloc(ME * 8), (double *)alloc(ME * 8)};
+        if (err != hipSuccess) goto done;
+        const double pad_s = std::pow((double)ext / n, 3), crop_s = std::pow((double)n / ext, 3);
+        // A_ik = irfft_ext(i k_k pad(psi1_i)); psi1_i spectrum = -(i k_i) phi
+        auto deriv = [&](int i, int k, double *out) {
+            k_ic_mult<IC_PSI1><<<g1(HC), 256, 0, st>>>(F, Bm, n, L, i);
+            k_ic_pad<<<g1(HE), 256, 0, st>>>(Bm, Pe, n, ext, pad_s);
+            k_ic_mult<IC_DERIV><<<g1(HE), 256, 0, st>>>(Pe, Be, ext, L, k);
+            inv(fe, Be, Ae, out, twe);
+            k_ic_scale_real<<<g1(ME), 256, 0, st>>>(out, ME, 1.0 / (double)ME);
+        };
+        // mu = A11 A22 + A11 A33 + A22 A33 - A12 A21 - A13 A31 - A23 A32 (evaluated left to right)
+        deriv(0, 0, fld[0]);
+        deriv(1, 1, fld[1]);
+        deriv(2, 2, fld[2]);
+        k_ic_mu<<<g1(ME), 256, 0, st>>>(mu, fld[0], fld[1], ME, 0);
+        k_ic_mu<<<g1(ME), 256, 0, st>>>(mu, fld[0], fld[2], ME, 1);
+        k_ic_mu<<<g1(ME), 256, 0, st>>>(mu, fld[1], fld[2], ME, 1);
+        const int pairs[3][2][2] = {{{0, 1}, {1, 0}}, {{0, 2}, {2, 0}}, {{1, 2}, {2, 1}}};
+        for (auto &pr : pairs) {
+            deriv(pr[0][0], pr[0][1], fld[0]);
+            deriv(pr[1][0], pr[1][1], fld[1]);
+            k_ic_mu<<<g1(ME), 256, 0, st>>>(mu, fld[0], fld[1], ME, 2);
+        }
+        // the order-2 factor ((3-2)/2 - 2) / ((2 + 3/2)(2 - 1)) = -3/7
+        k_ic_scale_real<<<g1(ME), 256, 0, st>>>(mu, ME, ((3.0 - 2) / 2 - 2.0) / ((2 + 1.5) * 1.0));
+        fwd(fe, mu, Ae, Be, twe);
+        k_ic_crop<<<g1(HC), 256, 0, st>>>(Be, A, n, ext, crop_s);  // fL in A
+        for (int c = 0; c < 3; c++) {
+            k_ic_mult<IC_PSI2><<<g1(HC), 256, 0, st>>>(A, Bm, n, L, c);
+            inv(fn, Bm, F, p2 + c * M, twn);  // F is free now
+            k_ic_scale_real<<<g1(M), 256, 0, st>>>(p2 + c * M, M, 1.0 / (double)M);
+        }
+    }
+    if (prec == 64)
+        k_ic_combine<double><<<g1(3 * M), 256, 0, st>>>(p1, p2, (double *)psi_out, (double *)pi_out, 3 * M, D, order);
+    else
+        k_ic_combine<float><<<g1(3 * M), 256, 0, st>>>(p1, p2, (float *)psi_out, (float *)pi_out, 3 * M, D, order);
+    err = hipGetLastError();
+    if (err == hipSuccess) err = hipStreamSynchronize(st);
+done:
+    hipStreamSynchronize(st);
+    for (void *p : allocs) (void)hipFree(p);
+    return (int)err;
+}
+
+}  // namespace dj

@@ -6,4 +6,4 @@ run_nbody, stepper_coefficients) on top of the C ABI of libdiscodj_hip.so (inclu
 from ._lib import DJError, LIB_PATH, header_exports, lib  # noqa: F401
 from .cosmology import Cosmology, compute_timetables, linear_power_spectrum, stepper_coefficients  # noqa: F401
 from .nbody import add_q, nbody_ics_lpt, run_nbody  # noqa: F401
-from .pm import PMConfig, PMSolverHIP, fft3d, pm_acceleration  # noqa: F401
+from .pm import PMConfig, PMSolverHIP, fft3d, ngenic_field, pm_acceleration  # noqa: F401

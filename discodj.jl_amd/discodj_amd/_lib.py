@@ -59,6 +59,12 @@ _SIGS = {
     "dj_kernel_stats": (C.c_int32, [C.c_void_p, C.POINTER(KernelStat), C.c_int32, C.POINTER(C.c_int32)]),
     "dj_fft3d": (C.c_int32, [C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32]),
     "dj_version": (C.c_char_p, []),
+    "dj_ics_lpt": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32,
+                               C.c_double, C.c_int32]),
+    "dj_power_spectrum": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
+                                      C.c_void_p]),
+    "dj_ngenic_field": (C.c_int32, [C.c_uint32, C.c_int32, C.c_void_p]),
+    "dj_ranlxd1_uniforms": (C.c_int32, [C.c_uint32, C.c_int32, C.c_void_p]),
 }
 
 _lib = None

@@ -128,6 +128,25 @@ class PMSolverHIP:
         self._chk(self._L.dj_run(self._ctx, n, *(a.ctypes.data for a in arrs),
                                  None if w is None else w.ctypes.data, cb, None))
 
+    def ics_lpt(self, k_tab, pk_tab, D, order=2, seed=42, white=None):
+        """Initial conditions on the device (N-GenIC noise of `seed`, or the given real white noise
+        (n,n,n) [k,j,i]), 1LPT/2LPT at growth factor D, into the solver's state (dj_ics_lpt)."""
+        kt = np.ascontiguousarray(k_tab, np.float64)
+        pt = np.ascontiguousarray(pk_tab, np.float64)
+        w = None if white is None else np.ascontiguousarray(white, np.float64)
+        self._chk(self._L.dj_ics_lpt(self._ctx, seed, None if w is None else w.ctypes.data, kt.ctypes.data,
+                                     pt.ctypes.data, len(kt), float(D), order))
+
+    def power_spectrum(self, psi=None, W=2, nbins=None):
+        """(k, P, nmodes) of the CIC density of q + psi on the solver's mesh, window-deconvolved, linear
+        bins of width kf up to k_Nyq/2 by default (dj_power_spectrum)."""
+        nbins = nbins or self.cfg.res_pm // 4 - 1
+        k, pk, nm = np.empty(nbins), np.empty(nbins), np.empty(nbins, np.int64)
+        src = None if psi is None else self._arr(psi)
+        self._chk(self._L.dj_power_spectrum(self._ctx, None if src is None else src.ctypes.data, W, nbins,
+                                            k.ctypes.data, pk.ctypes.data, nm.ctypes.data))
+        return k, pk, nm
+
     def check_finite(self):
         self._chk(self._L.dj_check_finite(self._ctx))
 
@@ -172,4 +191,11 @@ def fft3d(x, direction=-1, device=0):
         N = x.shape[0]
         out = np.empty((N, N, N), np.float64 if prec == 64 else np.float32)
     check(L.dj_fft3d(N, prec, direction, x.ctypes.data, out.ctypes.data, device))
+    return out
+
+
+def ngenic_field(seed, res):
+    """N-GenIC Fourier white noise (host): (res//2+1, res, res) complex [kz, j, i] (halved along z)."""
+    out = np.empty((res // 2 + 1, res, res), np.complex128)
+    check(_lib.lib().dj_ngenic_field(seed, res, out.ctypes.data))
     return out

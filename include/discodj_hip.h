@@ -110,6 +110,31 @@ dj_status dj_run(dj_ctx *ctx, int32_t n_steps, const double *dd1, const double *
  * int64 fixed-point values (fixed_frac_bits fractional bits) in DJ_DEPOSIT_FIXED64 mode. */
 dj_status dj_deposit(dj_ctx *ctx, const void *psi, const double *shift, void *mesh_out);
 
+/* Initial conditions into the context's state, computed on the device in FP64 and rounded to the
+ * context's precision: the IC pipeline of the reference (ngenic_ic, ngenic_gsl.jl:187-191; ic_operator /
+ * white_noise_to_fphi, grf.jl:118-144; compute_core, nlpt_core.jl:311-362 and nlpt_fast.jl:208-249;
+ * nbody_ics_lpt, run_nbody.jl:180-208).  White noise: `white` (host, n^3 real values, column-major) or,
+ * when NULL, the N-GenIC field of `seed` (GSL ranlxd1).  phi(k) = rfft(w) sqrt(P(k) (n/L)^3) (-1/k^2)
+ * with P interpolated linearly in the table (k_tab, pk_tab; ntab entries, k ascending).  order 1 or 2
+ * (de-aliased on the 3n/2 grid).  Psi = sum_m D^m psi_m, Pi = sum_m m D^(m-1) psi_m (EdS growth). */
+dj_status dj_ics_lpt(dj_ctx *ctx, uint32_t seed, const double *white, const double *k_tab, const double *pk_tab,
+                     int32_t ntab, double D, int32_t order);
+
+/* Power spectrum of the CIC (W = 2; TSC 3, PCS 4) density of X = q + psi on the context's mesh
+ * (psi NULL: the context's Psi): delta = mesh N^3/n^3 - 1, P(k) = V <|delta_k|^2 / |W(k)|^2> / N^6
+ * averaged with the half-spectrum mode weights over linear bins [kf (b + 1/2), kf (b + 3/2)),
+ * b < nbins, kf = 2 pi / L (evaluate_power_spectrum, power_spectrum.jl:25-79, with the window
+ * deconvolution of the study estimator).  W = 0 deposits CIC without deconvolution.  Outputs k_out
+ * (bin centres kf (b + 1)), pk_out and the mode counts (nmodes_out may be NULL); nbins <= 2048. */
+dj_status dj_power_spectrum(dj_ctx *ctx, const void *psi, int32_t W, int32_t nbins, double *k_out, double *pk_out,
+                            int64_t *nmodes_out);
+
+/* Host helpers of the IC pipeline: the N-GenIC Fourier white noise of `seed` (ngenic_field_gsl,
+ * ngenic_gsl.jl:126-165), (res, res, res/2+1) interleaved complex, halved along z, element (i,j,k)
+ * at i + res*(j + res*k); and the GSL ranlxd1 stream (ngenic_gsl.jl:34-98). */
+dj_status dj_ngenic_field(uint32_t seed, int32_t res, double *out);
+dj_status dj_ranlxd1_uniforms(uint32_t seed, int32_t n, double *out);
+
 /* NaN/Inf scan of the particle state (failure detection); DJ_ERR_NONFINITE if any */
 dj_status dj_check_finite(dj_ctx *ctx);
 
