@@ -8,8 +8,10 @@
 // callback needs the state in between, the next step's first drift.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <complex>
+#include <initializer_list>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -26,9 +28,9 @@ namespace {
 
 thread_local std::string g_create_error;
 
-enum KernelId { K_ZERO = 0, K_SCATTER, K_XF, K_YF, K_ZM, K_YI, K_XI, K_GATHER, K_KICK, K_DRIFT, K_COUNT };
+enum KernelId { K_ZERO = 0, K_SCATTER, K_XF, K_YF, K_ZM, K_YI, K_XI, K_GATHER, K_KICK, K_DRIFT, K_SORT, K_COUNT };
 const char *kernel_names[K_COUNT] = {"zero", "scatter", "fft_x_r2c", "fft_y_fwd", "fft_z_mult",
-                                     "fft_y_inv", "fft_x_c2r", "gather", "kick_drift", "drift"};
+                                     "fft_y_inv", "fft_x_c2r", "gather", "kick_drift", "drift", "bin"};
 
 struct ProfEvent {
     int id;
@@ -54,6 +56,12 @@ struct dj_ctx {
     double shifts[16][3];
     int nshifts = 1;
     double fdc[3] = {0, 0, 0};
+    TileOps tiles{};
+    TileGeom tg{};
+    unsigned *tcnt = nullptr, *tstart = nullptr, *tcur = nullptr;
+    void *tent = nullptr;
+    size_t tent_cap = 0;   // entries
+    double *d_shifts = nullptr;  // (16 + 1, 3): the interlacing shifts, then one free slot
     std::string err;
     bool prof = false;
     std::vector<ProfEvent> pending;
@@ -272,7 +280,8 @@ int aa_shifts(int aa, double dh, double (*s)[3]) {
 }
 
 void free_all(dj_ctx *c) {
-    void *bufs[] = {c->psi, c->mom, c->acc, c->psi_in, c->q1, c->mesh, c->meshfix, c->A, c->B, c->C, c->tw, c->tabs};
+    void *bufs[] = {c->psi, c->mom, c->acc, c->psi_in, c->q1, c->mesh, c->meshfix, c->A, c->B, c->C, c->tw, c->tabs,
+                    c->tcnt, c->tstart, c->tcur, c->tent, c->d_shifts};
     for (void *b : bufs)
         if (b) hipFree(b);
     for (auto &p : c->pending) { hipEventDestroy(p.a); hipEventDestroy(p.b); }
@@ -294,6 +303,50 @@ void set_bytes(dj_ctx *c) {
     c->bytes[K_GATHER] = 3 * T * np * (c->nshifts > 1 ? 5 : 4) + T * M;
     c->bytes[K_KICK] = 3 * T * np * 5;
     c->bytes[K_DRIFT] = 3 * T * np * 3;
+    // binning: read psi twice (count, fill), write one entry per (particle, tile)
+    c->bytes[K_SORT] = 2 * 3 * T * np + (double)c->tiles.entry_bytes * np;
+}
+
+double fscale_dep(const dj_pm_config &cfg) { return std::ldexp(1.0, cfg.fixed_frac_bits ? cfg.fixed_frac_bits : 32); }
+
+// ------------------------------------------------------------------------ tile binning
+TileArgs tile_args(dj_ctx *c, const void *psi, const double *d_shifts, int ns) {
+    TileArgs a{};
+    a.psi = psi;
+    a.mom = c->mom;
+    a.acc = c->acc;
+    a.q1 = c->q1;
+    a.n = c->n;
+    a.L = c->cfg.boxsize;
+    a.shifts = d_shifts;
+    a.ns = ns;
+    a.g = c->tg;
+    a.cnt = c->tcnt;
+    a.start = c->tstart;
+    a.cur = c->tcur;
+    a.ent = c->tent;
+    return a;
+}
+
+// counting sort of the particles by mesh tile for the given shifts (device array of ns x 3)
+dj_status tile_sort(dj_ctx *c, const void *psi, const double *d_shifts, int ns, int W) {
+    Prof p(c, K_SORT);
+    TileArgs a = tile_args(c, psi, d_shifts, ns);
+    c->part.zero(c->tcnt, sizeof(unsigned) * 2 * c->tg.nt, c->st);
+    c->tiles.count(a, W, c->st);
+    unsigned total = 0;
+    HIPCHK(c, hipMemcpyAsync(&total, c->tstart + c->tg.nt, sizeof(unsigned), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    if (total > c->tent_cap) {
+        if (c->tent) HIPCHK(c, hipFree(c->tent));
+        c->tent = nullptr;
+        size_t cap = (size_t)total + total / 4 + 1024;
+        HIPCHK(c, hipMalloc(&c->tent, cap * c->tiles.entry_bytes));
+        c->tent_cap = cap;
+        a.ent = c->tent;
+    }
+    c->tiles.fill(a, W, c->st);
+    return check_launch(c, "tile binning");
 }
 
 // ------------------------------------------------------------------------ one force evaluation
@@ -322,12 +375,15 @@ dj_status force(dj_ctx *c, const void *psi, int mode, const KickArgs &kick) {
             c->part.zero(c->acc, (size_t)c->n3 * c->tsize, c->st);
         }
     }
+    dj_status rs = tile_sort(c, psi, c->d_shifts, ns, cfg.worder);
+    if (rs) return rs;
+    (void)mesh_bytes;
     for (int s = 0; s < ns; s++) {
         pa.shift[0] = c->shifts[s][0];
         pa.shift[1] = c->shifts[s][1];
         pa.shift[2] = c->shifts[s][2];
-        { Prof p(c, K_ZERO); c->part.zero(dens, mesh_bytes, c->st); }
-        { Prof p(c, K_SCATTER); c->part.scatter(pa, cfg.worder, fixed, cfg.fixed_frac_bits ? cfg.fixed_frac_bits : 32, dens, c->st); }
+        TileArgs ta = tile_args(c, psi, c->d_shifts, ns);
+        { Prof p(c, K_SCATTER); c->tiles.deposit(ta, cfg.worder, fixed, c->shifts[s], dens, fscale_dep(cfg), c->st); }
         { Prof p(c, K_XF); c->fft.xf(dens, fixed, c->A, c->tw, fscale, c->st); }
         { Prof p(c, K_YF); c->fft.yf(c->A, c->B, c->tw, c->st); }
         if (!ik) {
@@ -337,7 +393,7 @@ dj_status force(dj_ctx *c, const void *psi, int mode, const KickArgs &kick) {
             const bool last = s == ns - 1;
             int gmode = (mode == FORCE_KICK && last) ? GATHER_KICK : (ns == 1 && mode == FORCE_ACC ? GATHER_ASSIGN : GATHER_ACC);
             Prof p(c, K_GATHER);
-            c->part.gather_fd(pa, cfg.worder, cfg.grad_order, c->mesh, c->fdc, gmode, kick, c->psi, c->st);
+            c->tiles.gather(ta, cfg.worder, cfg.grad_order, c->shifts[s], c->mesh, c->fdc, gmode, kick, c->psi, c->st);
         } else {
             { Prof p(c, K_ZM); c->fft.zm(ZM_NONE, true, false, c->B, c->B, c->tw, c->kmult, c->st); }
             for (int ax = 0; ax < 3; ax++) {
@@ -366,7 +422,43 @@ dj_status setup_ops(dj_ctx *c) {
         return fail(c, DJ_ERR_UNSUPPORTED, "res_pm " + std::to_string(c->res) +
                                                " is not a built mesh size (16..1024, 2^a 3^b)");
     part_ops<T>(c->part);
+    tile_ops<T>(c->tiles);
     return upload_tables<T>(c);
+}
+
+// tile shape (FP32: 32 x 16 x 16 cells, FP64: 16^3) and the stencil reach over the shifts
+dj_status setup_tiles(dj_ctx *c) {
+    const int N = c->res;
+    auto pick = [N](std::initializer_list<int> cand) {
+        for (int t : cand)
+            if (t <= N && N % t == 0) return t;
+        return N;
+    };
+    TileGeom &g = c->tg;
+    g.N = N;
+    if (c->tsize == 4) { g.TX = pick({32, 16, 8}); g.TY = pick({16, 8}); g.TZ = pick({16, 8}); }
+    else { g.TX = pick({16, 8}); g.TY = pick({16, 8}); g.TZ = pick({16, 8}); }
+    g.ntx = N / g.TX; g.nty = N / g.TY; g.ntz = N / g.TZ;
+    g.nt = g.ntx * g.nty * g.ntz;
+    // cells of the stencil relative to floor(g): [floor(s_min) + lo, floor(s_max) + (W odd ? 2 : 1) + hi]
+    const int W = c->cfg.worder, lo = W == 2 ? 0 : -1, hi = lo + W - 1;
+    double smin = 0, smax = 0;
+    for (int s = 0; s < c->nshifts; s++)
+        for (int a = 0; a < 3; a++) {
+            double sig = c->shifts[s][a] * N / c->cfg.boxsize;
+            smin = std::min(smin, sig);
+            smax = std::max(smax, sig);
+        }
+    g.RL = -((int)std::floor(smin) + lo);
+    g.RR = (int)std::floor(smax) + (W % 2 ? 2 : 1) + hi;
+    HIPCHK(c, hipMalloc(&c->tcnt, sizeof(unsigned) * 2 * g.nt));
+    HIPCHK(c, hipMalloc(&c->tcur, sizeof(unsigned) * 2 * g.nt));
+    HIPCHK(c, hipMalloc(&c->tstart, sizeof(unsigned) * (g.nt + 1)));
+    HIPCHK(c, hipMalloc(&c->d_shifts, sizeof(double) * 17 * 3));
+    HIPCHK(c, hipMemcpy(c->d_shifts, c->shifts, sizeof(double) * 16 * 3, hipMemcpyHostToDevice));
+    c->tent_cap = (size_t)(2.0 * (double)c->np) + 1024;
+    HIPCHK(c, hipMalloc(&c->tent, c->tent_cap * c->tiles.entry_bytes));
+    return DJ_OK;
 }
 
 dj_status validate(const dj_pm_config &cfg, std::string &msg) {
@@ -441,6 +533,8 @@ dj_status dj_create(const dj_pm_config *cfg, dj_ctx **out) {
     e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
     if (e != hipSuccess) { hip_fail(c, e, "hipStreamCreate"); return bail(DJ_ERR_DEVICE); }
     r = c->tsize == 8 ? setup_ops<double>(c) : setup_ops<float>(c);
+    if (r) return bail(r);
+    r = setup_tiles(c);
     if (r) return bail(r);
     size_t pbytes = (size_t)c->n3 * c->tsize, cbytes = c->HC * 2 * c->tsize;
     void **bufs[] = {&c->psi, &c->mom, &c->acc};
@@ -590,9 +684,12 @@ dj_status dj_deposit(dj_ctx *c, const void *psi, const double *shift, void *mesh
     for (int a = 0; a < 3; a++) pa.shift[a] = shift ? shift[a] : 0.0;
     void *dens = fixed ? c->meshfix : c->mesh;
     size_t bytes = c->M * (fixed ? 8 : c->tsize);
-    { Prof p(c, K_ZERO); c->part.zero(dens, bytes, c->st); }
-    { Prof p(c, K_SCATTER); c->part.scatter(pa, c->cfg.worder, fixed, c->cfg.fixed_frac_bits ? c->cfg.fixed_frac_bits : 32, dens, c->st); }
-    dj_status r = check_launch(c, "deposit");
+    HIPCHK(c, hipMemcpyAsync(c->d_shifts + 48, pa.shift, sizeof(double) * 3, hipMemcpyHostToDevice, c->st));
+    dj_status r = tile_sort(c, src, c->d_shifts + 48, 1, c->cfg.worder);
+    if (r) return r;
+    TileArgs ta = tile_args(c, src, c->d_shifts + 48, 1);
+    { Prof p(c, K_SCATTER); c->tiles.deposit(ta, c->cfg.worder, fixed, pa.shift, dens, fscale_dep(c->cfg), c->st); }
+    r = check_launch(c, "deposit");
     if (r) return r;
     HIPCHK(c, hipMemcpyAsync(mesh_out, dens, bytes, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
@@ -639,8 +736,12 @@ dj_status dj_power_spectrum(dj_ctx *c, const void *psi, int32_t W, int32_t nbins
     pa.n = c->n;
     pa.res = c->res;
     pa.L = c->cfg.boxsize;
-    c->part.zero(c->mesh, c->M * c->tsize, c->st);
-    c->part.scatter(pa, dw, false, 32, c->mesh, c->st);
+    const double zero3[3] = {0, 0, 0};
+    HIPCHK(c, hipMemcpyAsync(c->d_shifts + 48, zero3, sizeof(double) * 3, hipMemcpyHostToDevice, c->st));
+    dj_status rs = tile_sort(c, src, c->d_shifts + 48, 1, dw);
+    if (rs) return rs;
+    TileArgs ta = tile_args(c, src, c->d_shifts + 48, 1);
+    c->tiles.deposit(ta, dw, false, zero3, c->mesh, 1.0, c->st);
     c->fft.xf(c->mesh, false, c->A, c->tw, 1.0, c->st);
     c->fft.yf(c->A, c->B, c->tw, c->st);
     c->fft.zm(ZM_NONE, true, false, c->B, c->B, c->tw, c->kmult, c->st);

@@ -59,6 +59,36 @@ struct PartOps {
 };
 template <typename T> void part_ops(PartOps &ops);
 
+// tile binning (tiles.hip): mesh tiles of TX x TY x TZ cells; RL / RR: stencil reach in cells left
+// and right of a particle's cell over all interlacing shifts (sizes the staged potential of the gather)
+struct TileGeom {
+    int N, TX, TY, TZ, ntx, nty, ntz, nt, RL, RR;
+};
+struct TileArgs {
+    const void *psi;       // (3, Np) displacement
+    void *mom, *acc;       // (3, Np)
+    const void *q1;        // lattice coordinates
+    int n;
+    double L;
+    const double *shifts;  // device (ns, 3) interlacing shifts used by the binning
+    int ns;
+    TileGeom g;
+    unsigned *cnt;         // (2 nt) own then ghost counts
+    unsigned *start;       // (nt + 1) first entry of each tile
+    unsigned *cur;         // (2 nt) fill cursors
+    void *ent;             // entries (X, particle index), own then ghosts per tile
+};
+struct TileOps {
+    void (*count)(const TileArgs &a, int W, hipStream_t st);  // counts + scan
+    void (*fill)(const TileArgs &a, int W, hipStream_t st);
+    void (*deposit)(const TileArgs &a, int W, bool fixed, const double *shift, void *mesh, double fscale,
+                    hipStream_t st);
+    void (*gather)(const TileArgs &a, int W, int order, const double *shift, const void *phi, const double *c,
+                   int mode, const KickArgs &k, void *psi_out, hipStream_t st);
+    size_t entry_bytes;
+};
+template <typename T> void tile_ops(TileOps &ops);
+
 // power spectrum binning of an x-halved B-layout spectrum (analysis.hip)
 template <typename T>
 void pk_bin(const void *B, int N, double L, int W, int nbins, double scale, double *sum_p, double *sum_w,

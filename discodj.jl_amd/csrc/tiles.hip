@@ -1,0 +1,475 @@
+// Tile-binned mass assignment and force gather.
+//
+// The mesh is cut into tiles of TX x TY x TZ cells.  Once per force evaluation the particles are
+// binned by tile (a counting sort): each particle is listed in the tile of its cell c0 = floor(g)
+// ("own") and, as a ghost, in every other tile that a cell of its mass-assignment stencil falls in
+// for any of the interlacing shifts.  The stencil cells are computed with exactly the arithmetic
+// of the deposit (same X, same shift, same base rule), so a tile's list holds every particle that
+// deposits into it.  Per-workgroup LDS hash tables aggregate the counters, so the sort issues one
+// global atomic per (workgroup, tile) instead of one per particle.
+//
+// Deposit: one workgroup per tile accumulates its list in LDS (float / double / int64 atomics) and
+// writes the tile with plain coalesced stores: no global atomics, no zero fill of the mesh.
+// Gather: one workgroup per tile stages the potential of the tile plus the stencil reach and the
+// finite-difference halo in LDS, evaluates -D phi at the 27 (64) stencil cells from LDS for its own
+// particles, and applies the kick and drifts in place.  A particle whose stencil leaves the staged
+// region (not expected; kept for safety) reads the potential from global memory instead.
+#include <hip/hip_runtime.h>
+
+#include "dj_internal.h"
+
+namespace dj {
+
+template <typename T> struct Entry;
+template <> struct alignas(16) Entry<float> { float x, y, z; int idx; };
+template <> struct alignas(16) Entry<double> { double x, y, z; int idx; int pad; };
+
+template <typename T> __device__ __forceinline__ T tjmod(T x, T L) {
+    T r = fmod(x, L);
+    if (r == (T)0) return copysign(r, L);
+    if ((r > (T)0) != (L > (T)0)) return r + L;
+    return r;
+}
+template <int W, typename T> __device__ __forceinline__ int twbase(T g) {
+    return (W % 2 == 0) ? (int)floor(g) : (int)rint(g);
+}
+template <int W> __device__ __forceinline__ constexpr int twlo() { return W == 2 ? 0 : -1; }
+__device__ __forceinline__ int timod(int a, int n) {
+    int r = a % n;
+    return r < 0 ? r + n : r;
+}
+template <int W, typename T> __device__ __forceinline__ T twk(T d) {
+    if constexpr (W == 2) {
+        return d > (T)1 ? (T)0 : (T)1 - d;
+    } else if constexpr (W == 3) {
+        if (d > (T)1.5) return (T)0;
+        if (d > (T)0.5) { T t = (T)1.5 - d; return (T)0.5 * (t * t); }
+        return (T)0.75 - d * d;
+    } else {
+        if (d > (T)2) return (T)0;
+        if (d > (T)1) { T t = (T)2 - d; return t * t * t / (T)6; }
+        return ((T)4 - (T)6 * (d * d) + (T)3 * (d * d * d)) / (T)6;
+    }
+}
+
+// ------------------------------------------------------------------------------------ binning
+constexpr int HASH = 512;          // LDS hash slots per workgroup (keys: tile * 2 + ghost)
+constexpr unsigned EMPTY = 0xffffffffu;
+
+__device__ __forceinline__ int hash_slot(unsigned *keys, unsigned key) {
+    unsigned h = (key * 2654435761u) & (HASH - 1);
+    for (int probe = 0; probe < HASH; probe++) {
+        unsigned s = (h + probe) & (HASH - 1);
+        unsigned k = keys[s];
+        if (k == key) return (int)s;
+        if (k == EMPTY) {
+            unsigned old = atomicCAS(&keys[s], EMPTY, key);
+            if (old == EMPTY || old == key) return (int)s;
+        }
+    }
+    return -1;  // table full: the caller falls back to a global atomic
+}
+
+// tiles of one particle: own first, then distinct ghosts; returns the count (<= 9)
+template <typename T, int W>
+__device__ __forceinline__ int particle_tiles(const T X[3], const TileGeom &g, const double *shifts, int ns, T L,
+                                              unsigned *tiles) {
+    int tl[3], th[3], to[3];
+    const int Td[3] = {g.TX, g.TY, g.TZ}, nt[3] = {g.ntx, g.nty, g.ntz};
+    for (int a = 0; a < 3; a++) {
+        T g0 = tjmod(X[a], L) / L * (T)g.N;
+        int c0 = timod((int)floor(g0), g.N);
+        int rlo = 1 << 20, rhi = -(1 << 20);
+        for (int s = 0; s < ns; s++) {
+            T xs = X[a] + (T)shifts[3 * s + a];
+            T gs = tjmod(xs, L) / L * (T)g.N;
+            int b = twbase<W>(gs);
+            int d = timod(b - c0 + g.N / 2, g.N) - g.N / 2;
+            rlo = min(rlo, d + twlo<W>());
+            rhi = max(rhi, d + twlo<W>() + W - 1);
+        }
+        to[a] = c0 / Td[a];
+        tl[a] = timod(c0 + rlo, g.N) / Td[a];
+        th[a] = timod(c0 + rhi, g.N) / Td[a];
+        (void)nt;
+    }
+    int n = 0;
+    tiles[n++] = (unsigned)(to[0] + g.ntx * (to[1] + g.nty * to[2]));
+    for (int cz = 0; cz < 2; cz++)
+        for (int cy = 0; cy < 2; cy++)
+            for (int cx = 0; cx < 2; cx++) {
+                int tx = cx ? th[0] : tl[0], ty = cy ? th[1] : tl[1], tz = cz ? th[2] : tl[2];
+                unsigned t = (unsigned)(tx + g.ntx * (ty + g.nty * tz));
+                bool dup = false;
+                for (int m = 0; m < n; m++) dup |= tiles[m] == t;
+                if (!dup) tiles[n++] = t;
+            }
+    return n;
+}
+
+template <typename T, int W, bool FILL>
+__global__ __launch_bounds__(256) void k_tile_bin(const T *__restrict__ psi, const T *__restrict__ q1, int n,
+                                                  const double *__restrict__ shifts, int ns, T L, TileGeom g,
+                                                  unsigned *cnt, const unsigned *__restrict__ start, unsigned *cur,
+                                                  Entry<T> *__restrict__ ent) {
+    __shared__ unsigned keys[HASH], vals[HASH], base[HASH];
+    for (int s = threadIdx.x; s < HASH; s += 256) { keys[s] = EMPTY; vals[s] = 0; }
+    __syncthreads();
+    // 16 x 4 x 4 Lagrangian block of particles per workgroup
+    const int i = blockIdx.x * 16 + (threadIdx.x & 15), j = blockIdx.y * 4 + ((threadIdx.x >> 4) & 3),
+              k = blockIdx.z * 4 + (threadIdx.x >> 6);
+    const bool ok = i < n && j < n && k < n;
+    unsigned tiles[9];
+    int slot[9], rank[9], nt = 0;
+    T X[3];
+    size_t p = 0;
+    if (ok) {
+        const size_t np = (size_t)n * n * n;
+        p = (size_t)i + (size_t)n * ((size_t)j + (size_t)n * k);
+        X[0] = psi[p] + q1[i];
+        X[1] = psi[p + np] + q1[j];
+        X[2] = psi[p + 2 * np] + q1[k];
+        nt = particle_tiles<T, W>(X, g, shifts, ns, L, tiles);
+        for (int m = 0; m < nt; m++) {
+            unsigned key = tiles[m] * 2u + (m > 0 ? 1u : 0u);
+            int s = hash_slot(keys, key);
+            slot[m] = s;
+            if (s >= 0) rank[m] = (int)atomicAdd(&vals[s], 1u);
+            else rank[m] = -1;
+        }
+    }
+    __syncthreads();
+    // one global atomic per distinct (tile, own/ghost) of the workgroup
+    for (int s = threadIdx.x; s < HASH; s += 256) {
+        unsigned key = keys[s];
+        if (key == EMPTY) continue;
+        unsigned t = key >> 1, gh = key & 1u;
+        if (!FILL) atomicAdd(&cnt[gh * g.nt + t], vals[s]);
+        else base[s] = atomicAdd(&cur[gh * g.nt + t], vals[s]);
+    }
+    if (!FILL) {
+        // overflow pairs (table full) count directly
+        if (ok)
+            for (int m = 0; m < nt; m++)
+                if (slot[m] < 0) atomicAdd(&cnt[(m > 0 ? g.nt : 0) + tiles[m]], 1u);
+        return;
+    }
+    __syncthreads();
+    if (!ok) return;
+    for (int m = 0; m < nt; m++) {
+        unsigned t = tiles[m];
+        unsigned off = m > 0 ? cnt[t] : 0u;  // ghosts follow the tile's own particles
+        unsigned r = slot[m] >= 0 ? base[slot[m]] + (unsigned)rank[m] : atomicAdd(&cur[(m > 0 ? g.nt : 0) + t], 1u);
+        Entry<T> e;
+        e.x = X[0];
+        e.y = X[1];
+        e.z = X[2];
+        e.idx = (int)p;
+        ent[start[t] + off + r] = e;
+    }
+}
+
+// start[t] = sum_{u<t} (own[u] + ghost[u]); start[nt] = total.  One workgroup of 1024 threads.
+__global__ __launch_bounds__(1024) void k_tile_scan(const unsigned *__restrict__ cnt, int nt, unsigned *__restrict__ start,
+                                                    unsigned *__restrict__ cur) {
+    __shared__ unsigned long long part[1024];
+    const int tid = threadIdx.x, chunk = (nt + 1023) / 1024;
+    const int a = min(nt, tid * chunk), b = min(nt, a + chunk);
+    unsigned long long s = 0;
+    for (int t = a; t < b; t++) s += (unsigned long long)cnt[t] + cnt[nt + t];
+    part[tid] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        unsigned long long v = tid >= off ? part[tid - off] : 0ull;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    unsigned long long run = part[tid] - s;
+    for (int t = a; t < b; t++) {
+        start[t] = (unsigned)run;
+        run += (unsigned long long)cnt[t] + cnt[nt + t];
+        cur[t] = 0;
+        cur[nt + t] = 0;
+    }
+    if (tid == 1023) start[nt] = (unsigned)part[1023];
+}
+
+// ------------------------------------------------------------------------------------ deposit
+template <typename T, int W, bool FIXED>
+__global__ __launch_bounds__(256) void k_tile_deposit(const Entry<T> *__restrict__ ent, const unsigned *__restrict__ start,
+                                                      TileGeom g, T sx, T sy, T sz, T L, void *mesh, double fscale) {
+    using Acc = typename std::conditional<FIXED, unsigned long long, T>::type;
+    extern __shared__ unsigned char smem[];
+    Acc *acc = (Acc *)smem;
+    const int t = blockIdx.x, TV = g.TX * g.TY * g.TZ;
+    for (int c = threadIdx.x; c < TV; c += blockDim.x) acc[c] = 0;
+    const int tx = t % g.ntx, ty = (t / g.ntx) % g.nty, tz = t / (g.ntx * g.nty);
+    const int ox = tx * g.TX, oy = ty * g.TY, oz = tz * g.TZ;
+    __syncthreads();
+    const unsigned e0 = start[t], e1 = start[t + 1];
+    for (unsigned e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
+        const Entry<T> en = ent[e];
+        const T xs[3] = {en.x + sx, en.y + sy, en.z + sz};
+        T w[3][W];
+        int loc[3][W];
+        const int o3[3] = {ox, oy, oz}, T3[3] = {g.TX, g.TY, g.TZ};
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            T gg = tjmod(xs[a], L) / L * (T)g.N;
+            int b = twbase<W>(gg);
+#pragma unroll
+            for (int o = 0; o < W; o++) {
+                int bo = b + twlo<W>() + o;
+                w[a][o] = twk<W>(fabs(gg - (T)bo));
+                int l = timod(bo, g.N) - o3[a];
+                loc[a][o] = (l >= 0 && l < T3[a]) ? l : -1;
+            }
+        }
+#pragma unroll
+        for (int iz = 0; iz < W; iz++) {
+            if (loc[2][iz] < 0) continue;
+#pragma unroll
+            for (int iy = 0; iy < W; iy++) {
+                if (loc[1][iy] < 0) continue;
+                const int row = g.TX * (loc[1][iy] + g.TY * loc[2][iz]);
+#pragma unroll
+                for (int ix = 0; ix < W; ix++) {
+                    if (loc[0][ix] < 0) continue;
+                    T wv = w[0][ix] * w[1][iy] * w[2][iz];
+                    if constexpr (FIXED) {
+                        atomicAdd(&acc[row + loc[0][ix]], (unsigned long long)__double2ll_rn((double)wv * fscale));
+                    } else {
+                        atomicAdd(&acc[row + loc[0][ix]], wv);
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < TV; c += blockDim.x) {
+        const int lx = c % g.TX, ly = (c / g.TX) % g.TY, lz = c / (g.TX * g.TY);
+        const size_t gi = (size_t)(ox + lx) + (size_t)g.N * ((size_t)(oy + ly) + (size_t)g.N * (oz + lz));
+        ((Acc *)mesh)[gi] = acc[c];
+    }
+}
+
+// ------------------------------------------------------------------------------------ gather
+// -(c1 d1 + c2 d2 + c3 d3) of a line of potential values v[0 .. W+2H) at stencil point o (index o + H)
+template <typename T, int ORDER>
+__device__ __forceinline__ T fd_line(const T *v, int i, T c1, T c2, T c3) {
+    const T d1 = v[i + 1] - v[i - 1];
+    if constexpr (ORDER == 2) return -(c1 * d1);
+    const T d2 = v[i + 2] - v[i - 2];
+    if constexpr (ORDER == 4) return -(c1 * d1 + c2 * d2);
+    const T d3 = v[i + 3] - v[i - 3];
+    return -(c1 * d1 + c2 * d2 + c3 * d3);
+}
+
+// Sums over the W^3 stencil cells of (-D_a phi) w for the three components.  ld(x, y, z) returns phi at
+// stencil-relative cell (x, y, z), the stencil's first cell at H.  Each component is summed along its
+// own lines (x rows for F_x, y columns for F_y, z columns for F_z), so one line of W + 2H values is
+// live at a time; the cells' values and weights are the reference's, only the order of the 27 (64)
+// additions differs for F_y and F_z.
+template <typename T, int W, int ORDER, typename LD>
+__device__ __forceinline__ void gather_sums(const LD &ld, const T (&w)[3][W], T c1, T c2, T c3, T &s1, T &s2, T &s3) {
+    constexpr int H = ORDER / 2, Q = W + 2 * H;
+#pragma unroll
+    for (int oz = 0; oz < W; oz++)
+#pragma unroll
+        for (int oy = 0; oy < W; oy++) {
+            T v[Q];
+#pragma unroll
+            for (int q = 0; q < Q; q++) v[q] = ld(q, oy + H, oz + H);
+#pragma unroll
+            for (int ox = 0; ox < W; ox++) s1 += fd_line<T, ORDER>(v, ox + H, c1, c2, c3) * (w[0][ox] * w[1][oy] * w[2][oz]);
+        }
+#pragma unroll
+    for (int oz = 0; oz < W; oz++)
+#pragma unroll
+        for (int ox = 0; ox < W; ox++) {
+            T v[Q];
+#pragma unroll
+            for (int q = 0; q < Q; q++) v[q] = ld(ox + H, q, oz + H);
+#pragma unroll
+            for (int oy = 0; oy < W; oy++) s2 += fd_line<T, ORDER>(v, oy + H, c1, c2, c3) * (w[0][ox] * w[1][oy] * w[2][oz]);
+        }
+#pragma unroll
+    for (int oy = 0; oy < W; oy++)
+#pragma unroll
+        for (int ox = 0; ox < W; ox++) {
+            T v[Q];
+#pragma unroll
+            for (int q = 0; q < Q; q++) v[q] = ld(ox + H, oy + H, q);
+#pragma unroll
+            for (int oz = 0; oz < W; oz++) s3 += fd_line<T, ORDER>(v, oz + H, c1, c2, c3) * (w[0][ox] * w[1][oy] * w[2][oz]);
+        }
+}
+
+// 1024 threads when the unrolled stencil fits 128 VGPRs (FP32 CIC/TSC), else 512 (256 VGPRs)
+template <typename T, int W> constexpr int gather_threads() { return (sizeof(T) == 4 && W <= 3) ? 1024 : 512; }
+
+template <typename T, int W, int ORDER, int MODE>
+__global__ __launch_bounds__((gather_threads<T, W>())) void k_tile_gather(const Entry<T> *__restrict__ ent, const unsigned *__restrict__ start,
+                                                      const unsigned *__restrict__ own, TileGeom g, T sx, T sy, T sz,
+                                                      T L, const T *__restrict__ phi, T c1, T c2, T c3, const T *psi_in,
+                                                      T *psi_out, T *__restrict__ mom, T *__restrict__ acc, size_t np,
+                                                      T alpha, T beta, T dd2, T dd1n, int has_next, int ns) {
+    constexpr int H = ORDER / 2, Q = W + 2 * H;
+    extern __shared__ unsigned char smem[];
+    T *ph = (T *)smem;
+    const int t = blockIdx.x;
+    const int tx = t % g.ntx, ty = (t / g.ntx) % g.nty, tz = t / (g.ntx * g.nty);
+    // staged region: tile + stencil reach (RL left, RR right) + FD halo H
+    const int r0[3] = {tx * g.TX - g.RL - H, ty * g.TY - g.RL - H, tz * g.TZ - g.RL - H};
+    const int S[3] = {g.TX + g.RL + g.RR + 2 * H, g.TY + g.RL + g.RR + 2 * H, g.TZ + g.RL + g.RR + 2 * H};
+    const int SV = S[0] * S[1] * S[2], N = g.N, SX = S[0], SXY = S[0] * S[1];
+    const bool region_ok = S[0] < N && S[1] < N && S[2] < N;  // no periodic self-overlap of the region
+    if (region_ok) {
+        for (int c = threadIdx.x; c < SV; c += blockDim.x) {
+            const int lx = c % S[0], ly = (c / S[0]) % S[1], lz = c / SXY;
+            const int gx = timod(r0[0] + lx, N), gy = timod(r0[1] + ly, N), gz = timod(r0[2] + lz, N);
+            ph[c] = phi[(size_t)gx + (size_t)N * ((size_t)gy + (size_t)N * gz)];
+        }
+    }
+    __syncthreads();
+    const unsigned e0 = start[t], e1 = e0 + own[t];
+    const T xsh[3] = {sx, sy, sz};
+    for (unsigned e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
+        const Entry<T> en = ent[e];
+        const T xe[3] = {en.x, en.y, en.z};
+        T w[3][W];
+        int c0[3], lb[3];
+        bool staged = region_ok;
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            const T gg = tjmod(xe[a] + xsh[a], L) / L * (T)N;
+            const int b = twbase<W>(gg);
+#pragma unroll
+            for (int o = 0; o < W; o++) w[a][o] = twk<W>(fabs(gg - (T)(b + twlo<W>() + o)));
+            c0[a] = b + twlo<W>() - H;  // first cell of the stencil's FD neighbourhood (unwrapped)
+            lb[a] = timod(c0[a] - r0[a], N);
+            staged &= lb[a] + Q <= S[a];
+        }
+        T s1 = 0, s2 = 0, s3 = 0;
+        if (staged) {
+            const int base = lb[0] + SX * lb[1] + SXY * lb[2];
+            auto ld = [&](int x, int y, int z) { return ph[base + x + SX * y + SXY * z]; };
+            gather_sums<T, W, ORDER>(ld, w, c1, c2, c3, s1, s2, s3);
+        } else {
+            auto ld = [&](int x, int y, int z) {
+                return phi[(size_t)timod(c0[0] + x, N) +
+                           (size_t)N * ((size_t)timod(c0[1] + y, N) + (size_t)N * timod(c0[2] + z, N))];
+            };
+            gather_sums<T, W, ORDER>(ld, w, c1, c2, c3, s1, s2, s3);
+        }
+        const size_t p = (size_t)en.idx;
+        if constexpr (MODE == GATHER_ACC) {
+            acc[p] += s1;
+            acc[p + np] += s2;
+            acc[p + 2 * np] += s3;
+        } else if constexpr (MODE == GATHER_ASSIGN) {
+            acc[p] = s1;
+            acc[p + np] = s2;
+            acc[p + 2 * np] = s3;
+        } else {
+            const T a3[3] = {s1, s2, s3};
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const size_t q = p + c * np;
+                T ac = a3[c];
+                if (ns > 1) ac = (acc[q] + ac) / (T)ns;
+                T m = alpha * mom[q] + beta * ac;
+                T s = psi_in[q] + dd2 * m;
+                if (has_next) s = s + dd1n * m;
+                mom[q] = m;
+                psi_out[q] = s;
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------ launchers
+template <typename T>
+struct TileLaunch {
+    template <int W>
+    static void bin_w(const TileArgs &a, hipStream_t st) {
+        dim3 grid((a.n + 15) / 16, (a.n + 3) / 4, (a.n + 3) / 4);
+        const unsigned *cnt = a.cnt;
+        k_tile_bin<T, W, false><<<grid, 256, 0, st>>>((const T *)a.psi, (const T *)a.q1, a.n, a.shifts, a.ns, (T)a.L,
+                                                      a.g, a.cnt, a.start, a.cur, (Entry<T> *)a.ent);
+        k_tile_scan<<<1, 1024, 0, st>>>(cnt, a.g.nt, a.start, a.cur);
+    }
+    template <int W>
+    static void fill_w(const TileArgs &a, hipStream_t st) {
+        dim3 grid((a.n + 15) / 16, (a.n + 3) / 4, (a.n + 3) / 4);
+        k_tile_bin<T, W, true><<<grid, 256, 0, st>>>((const T *)a.psi, (const T *)a.q1, a.n, a.shifts, a.ns, (T)a.L,
+                                                     a.g, a.cnt, a.start, a.cur, (Entry<T> *)a.ent);
+    }
+    static void count(const TileArgs &a, int W, hipStream_t st) {
+        if (W == 2) bin_w<2>(a, st); else if (W == 3) bin_w<3>(a, st); else bin_w<4>(a, st);
+    }
+    static void fill(const TileArgs &a, int W, hipStream_t st) {
+        if (W == 2) fill_w<2>(a, st); else if (W == 3) fill_w<3>(a, st); else fill_w<4>(a, st);
+    }
+    template <int W, bool FIXED>
+    static void dep_w(const TileArgs &a, const double *sh, void *mesh, double fscale, hipStream_t st) {
+        size_t shm = (size_t)a.g.TX * a.g.TY * a.g.TZ * (FIXED ? 8 : sizeof(T));
+        k_tile_deposit<T, W, FIXED><<<a.g.nt, 256, shm, st>>>((const Entry<T> *)a.ent, a.start, a.g, (T)sh[0],
+                                                              (T)sh[1], (T)sh[2], (T)a.L, mesh, fscale);
+    }
+    static void deposit(const TileArgs &a, int W, bool fixed, const double *sh, void *mesh, double fscale,
+                        hipStream_t st) {
+#define D(w) (fixed ? dep_w<w, true>(a, sh, mesh, fscale, st) : dep_w<w, false>(a, sh, mesh, fscale, st))
+        if (W == 2) D(2); else if (W == 3) D(3); else D(4);
+#undef D
+    }
+    template <int W, int O, int M>
+    static void gat_wom(const TileArgs &a, const double *sh, const void *phi, const double *c, const KickArgs &k,
+                        void *psi_out, hipStream_t st) {
+        constexpr int H = O / 2;
+        const int SX = a.g.TX + a.g.RL + a.g.RR + 2 * H, SY = a.g.TY + a.g.RL + a.g.RR + 2 * H,
+                  SZ = a.g.TZ + a.g.RL + a.g.RR + 2 * H;
+        // a region as large as the mesh is not staged (the kernel reads the potential directly)
+        size_t shm = (SX < a.g.N && SY < a.g.N && SZ < a.g.N) ? sizeof(T) * (size_t)SX * SY * SZ : sizeof(T);
+        auto kern = k_tile_gather<T, W, O, M>;
+        (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+        kern<<<a.g.nt, gather_threads<T, W>(), shm, st>>>((const Entry<T> *)a.ent, a.start, a.cnt, a.g, (T)sh[0], (T)sh[1], (T)sh[2],
+                                        (T)a.L, (const T *)phi, (T)c[0], (T)c[1], (T)c[2], (const T *)a.psi,
+                                        (T *)psi_out, (T *)a.mom, (T *)a.acc, (size_t)a.n * a.n * a.n, (T)k.alpha,
+                                        (T)k.beta, (T)k.dd2, (T)k.dd1_next, k.has_next, k.nshifts);
+    }
+    template <int W, int O>
+    static void gat_wo(const TileArgs &a, const double *sh, const void *phi, const double *c, int mode,
+                       const KickArgs &k, void *psi_out, hipStream_t st) {
+        if (mode == GATHER_ACC) gat_wom<W, O, GATHER_ACC>(a, sh, phi, c, k, psi_out, st);
+        else if (mode == GATHER_ASSIGN) gat_wom<W, O, GATHER_ASSIGN>(a, sh, phi, c, k, psi_out, st);
+        else gat_wom<W, O, GATHER_KICK>(a, sh, phi, c, k, psi_out, st);
+    }
+    template <int W>
+    static void gat_w(const TileArgs &a, int order, const double *sh, const void *phi, const double *c, int mode,
+                      const KickArgs &k, void *psi_out, hipStream_t st) {
+        if (order == 2) gat_wo<W, 2>(a, sh, phi, c, mode, k, psi_out, st);
+        else if (order == 4) gat_wo<W, 4>(a, sh, phi, c, mode, k, psi_out, st);
+        else gat_wo<W, 6>(a, sh, phi, c, mode, k, psi_out, st);
+    }
+    static void gather(const TileArgs &a, int W, int order, const double *sh, const void *phi, const double *c,
+                       int mode, const KickArgs &k, void *psi_out, hipStream_t st) {
+        if (W == 2) gat_w<2>(a, order, sh, phi, c, mode, k, psi_out, st);
+        else if (W == 3) gat_w<3>(a, order, sh, phi, c, mode, k, psi_out, st);
+        else gat_w<4>(a, order, sh, phi, c, mode, k, psi_out, st);
+    }
+};
+
+template <typename T> void tile_ops(TileOps &ops) {
+    ops.count = TileLaunch<T>::count;
+    ops.fill = TileLaunch<T>::fill;
+    ops.deposit = TileLaunch<T>::deposit;
+    ops.gather = TileLaunch<T>::gather;
+    ops.entry_bytes = sizeof(Entry<T>);
+}
+
+template void tile_ops<float>(TileOps &);
+template void tile_ops<double>(TileOps &);
+
+}  // namespace dj
