@@ -237,10 +237,9 @@ dj_status upload_tables(dj_ctx *c) {
     K.mh = c->cfg.deconvolve ? d + omh : nullptr;
     K.gf = (const cx<T> *)(d + ogf);
     K.gh = (const cx<T> *)(d + ogh);
-    double nn = (double)c->n;
-    K.rho = (T)(nn * nn * nn / ((double)N * N * N));
-    K.nsets = (T)(c->cfg.n_resample * c->cfg.n_resample * c->cfg.n_resample);
-    K.r3 = (T)N * (T)N * (T)N;
+    double nn = (double)c->n, nr = c->cfg.n_resample;
+    // 1 / (rho nsets N^3) = 1 / (n^3 nsets)
+    K.scale = (T)(1.0 / (nn * nn * nn * (nr * nr * nr)));
     K.lap_order = c->cfg.lap_order;
     std::memcpy(c->kmult, &K, sizeof(K));
     // twiddles exp(-2 pi i k / N), k < N, in FP64 rounded to T
@@ -438,7 +437,11 @@ dj_status setup_tiles(dj_ctx *c) {
     g.N = N;
     if (c->tsize == 4) { g.TX = pick({32, 16, 8}); g.TY = pick({16, 8}); g.TZ = pick({16, 8}); }
     else { g.TX = pick({16, 8}); g.TY = pick({16, 8}); g.TZ = pick({16, 8}); }
+    if ((g.TX & (g.TX - 1)) || (g.TY & (g.TY - 1)) || (g.TZ & (g.TZ - 1)))
+        return fail(c, DJ_ERR_UNSUPPORTED, "mesh size needs power-of-two tiles (multiple of 8)");
     g.ntx = N / g.TX; g.nty = N / g.TY; g.ntz = N / g.TZ;
+    auto lg = [](int v) { int l = 0; while ((1 << l) < v) l++; return l; };
+    g.lT[0] = lg(g.TX); g.lT[1] = lg(g.TY); g.lT[2] = lg(g.TZ);
     g.nt = g.ntx * g.nty * g.ntz;
     // cells of the stencil relative to floor(g): [floor(s_min) + lo, floor(s_max) + (W odd ? 2 : 1) + hi]
     const int W = c->cfg.worder, lo = W == 2 ? 0 : -1, hi = lo + W - 1;

@@ -63,6 +63,7 @@ template <typename T> void part_ops(PartOps &ops);
 // and right of a particle's cell over all interlacing shifts (sizes the staged potential of the gather)
 struct TileGeom {
     int N, TX, TY, TZ, ntx, nty, ntz, nt, RL, RR;
+    int lT[3];  // log2 of the (power-of-two) tile dimensions
 };
 struct TileArgs {
     const void *psi;       // (3, Np) displacement

@@ -119,7 +119,7 @@ static int run(int op, const void *in, void *out, const void *ktab, int lap_orde
     const T *kt = (const T *)ktab;
     KMult<T> K{};
     K.k2f = kt; K.k2h = kt + N; K.tf = kt + N + H; K.th = kt + 2 * N + H;
-    K.rho = (T)rho; K.nsets = (T)nsets; K.r3 = (T)N * (T)N * (T)N; K.lap_order = lap_order;
+    K.scale = (T)(1.0 / (rho * nsets * (double)N * N * N)); K.lap_order = lap_order;
     E::forward((const T *)in, A.data(), Bm.data(), tw.data(), false);
     E::template zm<ZM_POT>(Bm.data(), tw.data(), K, 1, 1);
     E::inverse_yx(Bm.data(), A.data(), (T *)out, tw.data());

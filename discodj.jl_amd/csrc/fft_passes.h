@@ -24,14 +24,14 @@
 namespace dj {
 
 // k-space multiplier of one Poisson solve, from 1-D tables built on the host in FP64 and rounded
-// to T (the reference builds them in T as well): K_phi = invlap * mak / rho / nsets / N^3, DC = 0
+// to T (the reference builds them in T as well): K_phi = invlap * mak / (rho nsets N^3), DC = 0
 template <typename T>
 struct KMult {
     const T *k2f, *k2h;        // (k^2) per full / half axis index      (lap_order == 0)
     const T *tf, *th;          // FD Laplace symbol per full / half index (lap_order != 0)
     const T *mf, *mh;          // deconvolution window^-2W, or null
     const cx<T> *gf, *gh;      // gradient kernels (ik path) full / half
-    T rho, nsets, r3;          // n^3/N^3, n_resample^3, N^3 (all in T)
+    T scale;                   // 1 / (rho nsets N^3), rho = n^3/N^3, built in FP64 and rounded once
     int lap_order;
 
     DJ_HD T kphi(int kx, int ky, int kz) const {
@@ -46,7 +46,7 @@ struct KMult {
         }
         T v = il;
         if (mf) v = il * (mh[kx] * mf[ky] * mf[kz]);
-        return v / rho / nsets / r3;
+        return v * scale;
     }
 };
 

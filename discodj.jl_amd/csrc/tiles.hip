@@ -24,12 +24,20 @@ template <typename T> struct Entry;
 template <> struct alignas(16) Entry<float> { float x, y, z; int idx; };
 template <> struct alignas(16) Entry<double> { double x, y, z; int idx; int pad; };
 
+// Julia's float mod(x, L) for L > 0.  On [-L, 2L) it is computed without fmod: there fmod returns x,
+// x - L (exact, Sterbenz) or x (for -L < x < 0, then shifted by L), and every branch below gives
+// the same bits as the general rule (rem, then + L for a negative remainder, +0 for a zero one).
 template <typename T> __device__ __forceinline__ T tjmod(T x, T L) {
+    if (x >= (T)0 && x < L) return x + (T)0;  // + 0 turns -0 into +0, as copysign(0, L) does
+    if (x >= L && x < (T)2 * L) return x - L;
+    if (x < (T)0 && x >= -L) return x + L;
     T r = fmod(x, L);
     if (r == (T)0) return copysign(r, L);
     if ((r > (T)0) != (L > (T)0)) return r + L;
     return r;
 }
+// a mod n for a in [-n, 2n)
+__device__ __forceinline__ int wrapn(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
 template <int W, typename T> __device__ __forceinline__ int twbase(T g) {
     return (W % 2 == 0) ? (int)floor(g) : (int)rint(g);
 }
@@ -78,19 +86,20 @@ __device__ __forceinline__ int particle_tiles(const T X[3], const TileGeom &g, c
     const int Td[3] = {g.TX, g.TY, g.TZ}, nt[3] = {g.ntx, g.nty, g.ntz};
     for (int a = 0; a < 3; a++) {
         T g0 = tjmod(X[a], L) / L * (T)g.N;
-        int c0 = timod((int)floor(g0), g.N);
+        int c0 = wrapn((int)floor(g0), g.N);
         int rlo = 1 << 20, rhi = -(1 << 20);
         for (int s = 0; s < ns; s++) {
             T xs = X[a] + (T)shifts[3 * s + a];
             T gs = tjmod(xs, L) / L * (T)g.N;
             int b = twbase<W>(gs);
-            int d = timod(b - c0 + g.N / 2, g.N) - g.N / 2;
+            int d = wrapn(b - c0 + g.N / 2, g.N) - g.N / 2;  // b in [0, N], c0 in [0, N)
             rlo = min(rlo, d + twlo<W>());
             rhi = max(rhi, d + twlo<W>() + W - 1);
         }
-        to[a] = c0 / Td[a];
-        tl[a] = timod(c0 + rlo, g.N) / Td[a];
-        th[a] = timod(c0 + rhi, g.N) / Td[a];
+        to[a] = c0 >> g.lT[a];
+        tl[a] = wrapn(c0 + rlo, g.N) >> g.lT[a];
+        th[a] = wrapn(c0 + rhi, g.N) >> g.lT[a];
+        (void)Td;
         (void)nt;
     }
     int n = 0;
@@ -205,7 +214,7 @@ __global__ __launch_bounds__(256) void k_tile_deposit(const Entry<T> *__restrict
     const int t = blockIdx.x, TV = g.TX * g.TY * g.TZ;
     for (int c = threadIdx.x; c < TV; c += blockDim.x) acc[c] = 0;
     const int tx = t % g.ntx, ty = (t / g.ntx) % g.nty, tz = t / (g.ntx * g.nty);
-    const int ox = tx * g.TX, oy = ty * g.TY, oz = tz * g.TZ;
+    const int ox = tx << g.lT[0], oy = ty << g.lT[1], oz = tz << g.lT[2];
     __syncthreads();
     const unsigned e0 = start[t], e1 = start[t + 1];
     for (unsigned e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
@@ -222,7 +231,7 @@ __global__ __launch_bounds__(256) void k_tile_deposit(const Entry<T> *__restrict
             for (int o = 0; o < W; o++) {
                 int bo = b + twlo<W>() + o;
                 w[a][o] = twk<W>(fabs(gg - (T)bo));
-                int l = timod(bo, g.N) - o3[a];
+                int l = wrapn(bo, g.N) - o3[a];
                 loc[a][o] = (l >= 0 && l < T3[a]) ? l : -1;
             }
         }
@@ -248,7 +257,7 @@ __global__ __launch_bounds__(256) void k_tile_deposit(const Entry<T> *__restrict
     }
     __syncthreads();
     for (int c = threadIdx.x; c < TV; c += blockDim.x) {
-        const int lx = c % g.TX, ly = (c / g.TX) % g.TY, lz = c / (g.TX * g.TY);
+        const int lx = c & (g.TX - 1), ly = (c >> g.lT[0]) & (g.TY - 1), lz = c >> (g.lT[0] + g.lT[1]);
         const size_t gi = (size_t)(ox + lx) + (size_t)g.N * ((size_t)(oy + ly) + (size_t)g.N * (oz + lz));
         ((Acc *)mesh)[gi] = acc[c];
     }
@@ -321,15 +330,24 @@ __global__ __launch_bounds__((gather_threads<T, W>())) void k_tile_gather(const 
     const int t = blockIdx.x;
     const int tx = t % g.ntx, ty = (t / g.ntx) % g.nty, tz = t / (g.ntx * g.nty);
     // staged region: tile + stencil reach (RL left, RR right) + FD halo H
-    const int r0[3] = {tx * g.TX - g.RL - H, ty * g.TY - g.RL - H, tz * g.TZ - g.RL - H};
+    const int r0[3] = {(tx << g.lT[0]) - g.RL - H, (ty << g.lT[1]) - g.RL - H, (tz << g.lT[2]) - g.RL - H};
     const int S[3] = {g.TX + g.RL + g.RR + 2 * H, g.TY + g.RL + g.RR + 2 * H, g.TZ + g.RL + g.RR + 2 * H};
-    const int SV = S[0] * S[1] * S[2], N = g.N, SX = S[0], SXY = S[0] * S[1];
-    const bool region_ok = S[0] < N && S[1] < N && S[2] < N;  // no periodic self-overlap of the region
+    const int N = g.N, SX = S[0], SXY = S[0] * S[1];
+    // stage only regions smaller than the mesh (no periodic self-overlap) with x rows of <= 64 cells
+    const bool region_ok = S[0] <= 64 && S[0] < N && S[1] < N && S[2] < N;
     if (region_ok) {
-        for (int c = threadIdx.x; c < SV; c += blockDim.x) {
-            const int lx = c % S[0], ly = (c / S[0]) % S[1], lz = c / SXY;
-            const int gx = timod(r0[0] + lx, N), gy = timod(r0[1] + ly, N), gz = timod(r0[2] + lz, N);
-            ph[c] = phi[(size_t)gx + (size_t)N * ((size_t)gy + (size_t)N * gz)];
+        // rows along x: 64 lanes per row (S[0] <= 64), NT / 64 rows at a time
+        constexpr int NT = gather_threads<T, W>(), RPI = NT / 64;
+        const int lx = threadIdx.x & 63, rstart = threadIdx.x >> 6, nrows = S[1] * S[2];
+        int ly = rstart % S[1], lz = rstart / S[1];
+        const int gx = wrapn(r0[0] + lx, N);
+        for (int row = rstart; row < nrows; row += RPI) {
+            if (lx < S[0]) {
+                const int gy = wrapn(r0[1] + ly, N), gz = wrapn(r0[2] + lz, N);
+                ph[lx + SX * ly + SXY * lz] = phi[(size_t)gx + (size_t)N * ((size_t)gy + (size_t)N * gz)];
+            }
+            ly += RPI;
+            while (ly >= S[1]) { ly -= S[1]; lz++; }
         }
     }
     __syncthreads();
@@ -348,7 +366,7 @@ __global__ __launch_bounds__((gather_threads<T, W>())) void k_tile_gather(const 
 #pragma unroll
             for (int o = 0; o < W; o++) w[a][o] = twk<W>(fabs(gg - (T)(b + twlo<W>() + o)));
             c0[a] = b + twlo<W>() - H;  // first cell of the stencil's FD neighbourhood (unwrapped)
-            lb[a] = timod(c0[a] - r0[a], N);
+            lb[a] = wrapn(c0[a] - r0[a], N);
             staged &= lb[a] + Q <= S[a];
         }
         T s1 = 0, s2 = 0, s3 = 0;
@@ -358,8 +376,8 @@ __global__ __launch_bounds__((gather_threads<T, W>())) void k_tile_gather(const 
             gather_sums<T, W, ORDER>(ld, w, c1, c2, c3, s1, s2, s3);
         } else {
             auto ld = [&](int x, int y, int z) {
-                return phi[(size_t)timod(c0[0] + x, N) +
-                           (size_t)N * ((size_t)timod(c0[1] + y, N) + (size_t)N * timod(c0[2] + z, N))];
+                return phi[(size_t)wrapn(c0[0] + x, N) +
+                           (size_t)N * ((size_t)wrapn(c0[1] + y, N) + (size_t)N * wrapn(c0[2] + z, N))];
             };
             gather_sums<T, W, ORDER>(ld, w, c1, c2, c3, s1, s2, s3);
         }
@@ -431,7 +449,7 @@ struct TileLaunch {
         const int SX = a.g.TX + a.g.RL + a.g.RR + 2 * H, SY = a.g.TY + a.g.RL + a.g.RR + 2 * H,
                   SZ = a.g.TZ + a.g.RL + a.g.RR + 2 * H;
         // a region as large as the mesh is not staged (the kernel reads the potential directly)
-        size_t shm = (SX < a.g.N && SY < a.g.N && SZ < a.g.N) ? sizeof(T) * (size_t)SX * SY * SZ : sizeof(T);
+        size_t shm = (SX <= 64 && SX < a.g.N && SY < a.g.N && SZ < a.g.N) ? sizeof(T) * (size_t)SX * SY * SZ : sizeof(T);
         auto kern = k_tile_gather<T, W, O, M>;
         (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
         kern<<<a.g.nt, gather_threads<T, W>(), shm, st>>>((const Entry<T> *)a.ent, a.start, a.cnt, a.g, (T)sh[0], (T)sh[1], (T)sh[2],
