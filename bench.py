@@ -43,7 +43,20 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--small", action="store_true", help="128^3 particles on a 256^3 mesh (quick check)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device copy kernel three times (the reference point that scales "
+                         "the FETCH_SIZE / WRITE_SIZE counters of a profiling run to bytes)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+
+        a = torch.ones(2**28, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            torch.neg(a, out=b)  # one elementwise kernel: reads 1 GiB, writes 1 GiB
+        torch.cuda.synchronize()
+        log("copy calibration: 3 launches of a 1 GiB read + 1 GiB write elementwise kernel")
+        return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every box of this build has one MI355X; --gpus {args.gpus} "
               f"is not run (the multi-GPU slab/pencil decomposition is a design, not a build)", file=sys.stderr)

@@ -151,31 +151,29 @@ struct Tables {
 };
 
 template <typename T>
-T maxabs(const std::vector<T> &k) {
-    T m = 0;
-    for (T v : k) m = std::fabs(v) > m ? std::fabs(v) : m;
-    return m;
-}
-
-template <typename T>
 void build_tables(const dj_pm_config &cfg, Tables<T> &tb) {
+    // Every table is evaluated in FP64 and rounded once to T.  pm.jl:140-146 evaluates the
+    // finite-difference Laplacian in T; in FP32 its cosine sums cancel to a relative error of about
+    // 3 eps / (k dx)^2 at small k (5e-3 at the fundamental mode of a 1024^3 mesh), which shows up as
+    // a percent-level error of the large-scale power after a run.  For T = double nothing changes.
     const int N = cfg.res_pm, H = N / 2 + 1;
     const double L = cfg.boxsize;
-    std::vector<T> kf(N), kh(H);
-    for (int m = 0; m < N; m++) kf[m] = (T)((double)(m < N / 2 ? m : m - N) * 2 * M_PI / L);
-    for (int m = 0; m < H; m++) kh[m] = (T)((double)m * 2 * M_PI / L);
+    std::vector<double> kf(N), kh(H);
+    for (int m = 0; m < N; m++) kf[m] = (double)(m < N / 2 ? m : m - N) * 2 * M_PI / L;
+    for (int m = 0; m < H; m++) kh[m] = (double)m * 2 * M_PI / L;
     tb.k2f.resize(N);
     tb.k2h.resize(H);
-    for (int m = 0; m < N; m++) tb.k2f[m] = kf[m] * kf[m];
-    for (int m = 0; m < H; m++) tb.k2h[m] = kh[m] * kh[m];
-    auto lap = [&](const std::vector<T> &k, std::vector<T> &t) {
+    for (int m = 0; m < N; m++) tb.k2f[m] = (T)(kf[m] * kf[m]);
+    for (int m = 0; m < H; m++) tb.k2h[m] = (T)(kh[m] * kh[m]);
+    const double mres = (double)N / L;  // max |k| / pi over the full axis (pm.jl:131,141)
+    auto lap = [&](const std::vector<double> &k, std::vector<T> &t) {
         t.resize(k.size());
-        T m = maxabs(k) / (T)M_PI;
         for (size_t i = 0; i < k.size(); i++) {
-            T c1 = std::cos(1 * k[i] / m), c2 = std::cos(2 * k[i] / m), c3 = std::cos(3 * k[i] / m);
-            if (cfg.lap_order == 2) t[i] = -2 * (1 - c1) * (m * m);
-            else if (cfg.lap_order == 4) t[i] = -(c2 - 16 * c1 + 15) / 6 * (m * m);
-            else t[i] = -(-2 * c3 + 27 * c2 - 270 * c1 + 245) / 90 * (m * m);
+            double c1 = std::cos(1 * k[i] / mres), c2 = std::cos(2 * k[i] / mres), c3 = std::cos(3 * k[i] / mres), v;
+            if (cfg.lap_order == 2) v = -2 * (1 - c1) * (mres * mres);
+            else if (cfg.lap_order == 4) v = -(c2 - 16 * c1 + 15) / 6 * (mres * mres);
+            else v = -(-2 * c3 + 27 * c2 - 270 * c1 + 245) / 90 * (mres * mres);
+            t[i] = (T)v;
         }
     };
     if (cfg.lap_order != 0) {
@@ -185,13 +183,12 @@ void build_tables(const dj_pm_config &cfg, Tables<T> &tb) {
         tb.tf.assign(N, 0);
         tb.th.assign(H, 0);
     }
-    auto mak = [&](const std::vector<T> &k, std::vector<T> &o) {
+    auto mak = [&](const std::vector<double> &k, std::vector<T> &o) {
         o.resize(k.size());
-        T m = maxabs(k) / (T)M_PI;
         for (size_t i = 0; i < k.size(); i++) {
-            double x = (double)k[i] / (2 * M_PI) / (double)m;
-            T v = x == 0 ? (T)1 : (T)(std::sin(M_PI * x) / (M_PI * x));
-            o[i] = (T)std::pow((double)v, -2 * cfg.worder);
+            double x = k[i] / (2 * M_PI) / mres;
+            double v = x == 0 ? 1.0 : std::sin(M_PI * x) / (M_PI * x);
+            o[i] = (T)std::pow(v, -2 * cfg.worder);
         }
     };
     if (cfg.deconvolve) {
@@ -200,8 +197,8 @@ void build_tables(const dj_pm_config &cfg, Tables<T> &tb) {
     }
     tb.gf.resize(N);
     tb.gh.resize(H);
-    for (int m = 0; m < N; m++) tb.gf[m] = {(T)0, kf[m]};
-    for (int m = 0; m < H; m++) tb.gh[m] = {(T)0, kh[m]};
+    for (int m = 0; m < N; m++) tb.gf[m] = {(T)0, (T)kf[m]};
+    for (int m = 0; m < H; m++) tb.gh[m] = {(T)0, (T)kh[m]};
     tb.gf[N / 2] = {0, 0};
     tb.gh[H - 1] = {0, 0};
 }
