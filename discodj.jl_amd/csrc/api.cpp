@@ -1,7 +1,8 @@
 // C ABI of the PM N-body step (include/discodj_hip.h): context set-up and orchestration.
 //
-// One force evaluation (pm_acceleration!, NR/src/nbody/pm.jl:466-521) per interlacing shift:
-//   zero mesh -> scatter -> XF -> YF -> ZM(K_phi) -> YI -> XI (potential) -> FD-fused gather
+// One force evaluation (pm_acceleration!, NR/src/nbody/pm.jl:466-521): the particles are binned by
+// mesh tile once, then per interlacing shift
+//   tile deposit -> XF -> YF -> ZM(K_phi) -> YI -> XI (potential) -> FD-fused tile gather
 // For grad_order 0 (ik gradient) the spectrum is kept after the forward z pass and three inverse
 // sequences produce -d_a phi, each gathered into one acceleration component (pm.jl:487-492).
 // In dj_run the gather of the last shift also applies the kick, the second drift and, unless a
@@ -28,9 +29,9 @@ namespace {
 
 thread_local std::string g_create_error;
 
-enum KernelId { K_ZERO = 0, K_SCATTER, K_XF, K_YF, K_ZM, K_YI, K_XI, K_GATHER, K_KICK, K_DRIFT, K_SORT, K_COUNT };
-const char *kernel_names[K_COUNT] = {"zero", "scatter", "fft_x_r2c", "fft_y_fwd", "fft_z_mult",
-                                     "fft_y_inv", "fft_x_c2r", "gather", "kick_drift", "drift", "bin"};
+enum KernelId { K_ZERO = 0, K_SCATTER, K_XF, K_YF, K_ZM, K_YI, K_XI, K_GATHER, K_GATHER_KICK, K_KICK, K_DRIFT, K_SORT, K_COUNT };
+const char *kernel_names[K_COUNT] = {"zero", "scatter", "fft_x_r2c", "fft_y_fwd", "fft_z_mult", "fft_y_inv",
+                                     "fft_x_c2r", "gather", "gather_kick", "kick_drift", "drift", "bin"};
 
 struct ProfEvent {
     int id;
@@ -143,7 +144,8 @@ dj_status check_launch(dj_ctx *c, const char *what) {
 }
 
 // ------------------------------------------------------------------------ host FP64 multiplier tables
-// Restates the 1-D k-space kernels of NR/src/nbody/pm.jl:119-183 in the working precision T.
+// Restates the 1-D k-space kernels of NR/src/nbody/pm.jl:119-183, evaluated in FP64 and rounded to
+// the working precision T.
 template <typename T>
 struct Tables {
     std::vector<T> k2f, k2h, tf, th, mf, mh;
@@ -287,16 +289,20 @@ void free_all(dj_ctx *c) {
 
 void set_bytes(dj_ctx *c) {
     const double T = c->tsize, M = (double)c->M, HC = (double)c->HC, np = (double)c->np;
-    c->bytes[K_ZERO] = M * (c->cfg.deposit == DJ_DEPOSIT_FIXED64 ? 8 : T);
+    // algorithmic bytes per launch: what the operation must read and write at least, with the
+    // particle positions counted as 3 values per particle (the tile lists are this library's choice)
+    c->bytes[K_ZERO] = 3 * T * np;
     c->bytes[K_SCATTER] = 3 * T * np + M * (c->cfg.deposit == DJ_DEPOSIT_FIXED64 ? 8 : T);
     c->bytes[K_XF] = M * (c->cfg.deposit == DJ_DEPOSIT_FIXED64 ? 8 : T) + 2 * T * HC;
     c->bytes[K_YF] = 4 * T * HC;
     c->bytes[K_ZM] = 4 * T * HC;
     c->bytes[K_YI] = 4 * T * HC;
     c->bytes[K_XI] = 2 * T * HC + T * M;
-    // FD-fused gather of the last pass with kick and drift: read psi, mom (and acc when averaging
+    // FD gather of a pass that stores the acceleration: read the positions and phi, write acc
+    c->bytes[K_GATHER] = 3 * T * np * 2 + T * M;
+    // FD gather of the last pass fused with kick and drift: read psi, mom (and acc when averaging
     // shifts), read phi once, write psi and mom
-    c->bytes[K_GATHER] = 3 * T * np * (c->nshifts > 1 ? 5 : 4) + T * M;
+    c->bytes[K_GATHER_KICK] = 3 * T * np * (c->nshifts > 1 ? 5 : 4) + T * M;
     c->bytes[K_KICK] = 3 * T * np * 5;
     c->bytes[K_DRIFT] = 3 * T * np * 3;
     // binning: read psi twice (count, fill), write one entry per (particle, tile)
@@ -364,12 +370,10 @@ dj_status force(dj_ctx *c, const void *psi, int mode, const KickArgs &kick) {
     pa.res = c->res;
     pa.L = cfg.boxsize;
     const bool ik = cfg.grad_order == 0;
-    // the acceleration buffer is needed when shifts are averaged, for the ik path, and as output
-    if (ik || mode == FORCE_ACC || ns > 1) {
-        if (!(mode == FORCE_ACC && ns == 1 && !ik)) {
-            Prof p(c, K_ZERO);
-            c->part.zero(c->acc, (size_t)c->n3 * c->tsize, c->st);
-        }
+    // the ik path accumulates the components into acc; the FD path assigns it in its first pass
+    if (ik) {
+        Prof p(c, K_ZERO);
+        c->part.zero(c->acc, (size_t)c->n3 * c->tsize, c->st);
     }
     dj_status rs = tile_sort(c, psi, c->d_shifts, ns, cfg.worder);
     if (rs) return rs;
@@ -387,8 +391,9 @@ dj_status force(dj_ctx *c, const void *psi, int mode, const KickArgs &kick) {
             { Prof p(c, K_YI); c->fft.yi(c->B, c->A, c->tw, c->st); }
             { Prof p(c, K_XI); c->fft.xi(c->A, c->mesh, c->tw, c->st); }
             const bool last = s == ns - 1;
-            int gmode = (mode == FORCE_KICK && last) ? GATHER_KICK : (ns == 1 && mode == FORCE_ACC ? GATHER_ASSIGN : GATHER_ACC);
-            Prof p(c, K_GATHER);
+            // first pass stores acc, later passes add to it; the last one of a step kicks and drifts
+            const int gmode = (mode == FORCE_KICK && last) ? GATHER_KICK : (s == 0 ? GATHER_ASSIGN : GATHER_ACC);
+            Prof p(c, gmode == GATHER_KICK ? K_GATHER_KICK : K_GATHER);
             c->tiles.gather(ta, cfg.worder, cfg.grad_order, c->shifts[s], c->mesh, c->fdc, gmode, kick, c->psi, c->st);
         } else {
             { Prof p(c, K_ZM); c->fft.zm(ZM_NONE, true, false, c->B, c->B, c->tw, c->kmult, c->st); }

@@ -205,10 +205,20 @@ __global__ __launch_bounds__(1024) void k_tile_scan(const unsigned *__restrict__
 }
 
 // ------------------------------------------------------------------------------------ deposit
-template <typename T, int W, bool FIXED>
+// Accumulators of a tile in LDS:
+//   ACC_T      T atomics (FP64 meshes)
+//   ACC_Q31    FP32 meshes: each weight w <= 1 enters as the integer round(w 2^31) (exact in 32 bits),
+//              summed in 64-bit integers and rounded once to T at the tile store.  Integer LDS atomics
+//              run at about five times the rate of float ones on gfx950, the sum is independent of
+//              the order of the additions (the deposit is reproducible bit for bit), and its error
+//              (2^-32 per term, then one rounding) is below that of an FP32 running sum.
+//   ACC_FIXED  the fixed-point mode: llrint(w 2^frac) in 64-bit integers, stored as integers
+enum { ACC_T = 0, ACC_Q31 = 1, ACC_FIXED = 2 };
+
+template <typename T, int W, int ACC>
 __global__ __launch_bounds__(256) void k_tile_deposit(const Entry<T> *__restrict__ ent, const unsigned *__restrict__ start,
                                                       TileGeom g, T sx, T sy, T sz, T L, void *mesh, double fscale) {
-    using Acc = typename std::conditional<FIXED, unsigned long long, T>::type;
+    using Acc = typename std::conditional<ACC == ACC_T, T, unsigned long long>::type;
     extern __shared__ unsigned char smem[];
     Acc *acc = (Acc *)smem;
     const int t = blockIdx.x, TV = g.TX * g.TY * g.TZ;
@@ -246,8 +256,10 @@ __global__ __launch_bounds__(256) void k_tile_deposit(const Entry<T> *__restrict
                 for (int ix = 0; ix < W; ix++) {
                     if (loc[0][ix] < 0) continue;
                     T wv = w[0][ix] * w[1][iy] * w[2][iz];
-                    if constexpr (FIXED) {
+                    if constexpr (ACC == ACC_FIXED) {
                         atomicAdd(&acc[row + loc[0][ix]], (unsigned long long)__double2ll_rn((double)wv * fscale));
+                    } else if constexpr (ACC == ACC_Q31) {
+                        atomicAdd(&acc[row + loc[0][ix]], (unsigned long long)(unsigned)rintf(wv * 2147483648.0f));
                     } else {
                         atomicAdd(&acc[row + loc[0][ix]], wv);
                     }
@@ -259,12 +271,14 @@ __global__ __launch_bounds__(256) void k_tile_deposit(const Entry<T> *__restrict
     for (int c = threadIdx.x; c < TV; c += blockDim.x) {
         const int lx = c & (g.TX - 1), ly = (c >> g.lT[0]) & (g.TY - 1), lz = c >> (g.lT[0] + g.lT[1]);
         const size_t gi = (size_t)(ox + lx) + (size_t)g.N * ((size_t)(oy + ly) + (size_t)g.N * (oz + lz));
-        ((Acc *)mesh)[gi] = acc[c];
+        if constexpr (ACC == ACC_Q31) ((T *)mesh)[gi] = (T)((double)acc[c] * 0x1p-31);
+        else ((Acc *)mesh)[gi] = acc[c];
     }
 }
 
 // ------------------------------------------------------------------------------------ gather
-// -(c1 d1 + c2 d2 + c3 d3) of a line of potential values v[0 .. W+2H) at stencil point o (index o + H)
+// -(c1 d1 + c2 d2 + c3 d3) of a line of potential values v[0 .. W+2H) at stencil point o (index o + H):
+// the reference's finite-difference gradient at one cell (global-memory path)
 template <typename T, int ORDER>
 __device__ __forceinline__ T fd_line(const T *v, int i, T c1, T c2, T c3) {
     const T d1 = v[i + 1] - v[i - 1];
@@ -275,11 +289,9 @@ __device__ __forceinline__ T fd_line(const T *v, int i, T c1, T c2, T c3) {
     return -(c1 * d1 + c2 * d2 + c3 * d3);
 }
 
-// Sums over the W^3 stencil cells of (-D_a phi) w for the three components.  ld(x, y, z) returns phi at
-// stencil-relative cell (x, y, z), the stencil's first cell at H.  Each component is summed along its
-// own lines (x rows for F_x, y columns for F_y, z columns for F_z), so one line of W + 2H values is
-// live at a time; the cells' values and weights are the reference's, only the order of the 27 (64)
-// additions differs for F_y and F_z.
+// Sums over the W^3 stencil cells of (-D_a phi) w for the three components, one line of W + 2H values
+// live at a time (x rows for F_x, y columns for F_y, z columns for F_z); ld(x, y, z) returns phi at
+// stencil-relative cell (x, y, z), the stencil's first cell at H.
 template <typename T, int W, int ORDER, typename LD>
 __device__ __forceinline__ void gather_sums(const LD &ld, const T (&w)[3][W], T c1, T c2, T c3, T &s1, T &s2, T &s3) {
     constexpr int H = ORDER / 2, Q = W + 2 * H;
@@ -315,8 +327,103 @@ __device__ __forceinline__ void gather_sums(const LD &ld, const T (&w)[3][W], T 
         }
 }
 
-// 1024 threads when the unrolled stencil fits 128 VGPRs (FP32 CIC/TSC), else 512 (256 VGPRs)
-template <typename T, int W> constexpr int gather_threads() { return (sizeof(T) == 4 && W <= 3) ? 1024 : 512; }
+// The same sums with the finite difference and the interpolation along the line folded into one
+// filter: sum_o w[o] fd(v, o + H) = sum_q u[q] v[q] with u[o+H+t] -= w[o] c_t, u[o+H-t] += w[o] c_t.
+// A line then costs Q fused multiply-adds instead of W finite differences; the value equals the
+// per-cell form up to rounding (the LDS path).
+template <typename T, int W, int ORDER>
+__device__ __forceinline__ void fd_filter(const T (&w)[W], T c1, T c2, T c3, T (&u)[W + ORDER]) {
+    constexpr int H = ORDER / 2, Q = W + 2 * H;
+    const T ct[3] = {c1, c2, c3};
+#pragma unroll
+    for (int q = 0; q < Q; q++) u[q] = 0;
+#pragma unroll
+    for (int o = 0; o < W; o++)
+#pragma unroll
+        for (int t = 1; t <= H; t++) {
+            u[o + H + t] = fma(-w[o], ct[t - 1], u[o + H + t]);
+            u[o + H - t] = fma(w[o], ct[t - 1], u[o + H - t]);
+        }
+}
+
+template <typename T, int W, int ORDER, typename LD>
+__device__ __forceinline__ void gather_sums_filtered(const LD &ld, const T (&w)[3][W], T c1, T c2, T c3, T &s1, T &s2,
+                                                     T &s3) {
+    constexpr int H = ORDER / 2, Q = W + 2 * H;
+    T u[Q];
+    fd_filter<T, W, ORDER>(w[0], c1, c2, c3, u);
+#pragma unroll
+    for (int oz = 0; oz < W; oz++)
+#pragma unroll
+        for (int oy = 0; oy < W; oy++) {
+            T l = 0;
+#pragma unroll
+            for (int q = 0; q < Q; q++) l = fma(u[q], ld(q, oy + H, oz + H), l);
+            s1 = fma(w[1][oy] * w[2][oz], l, s1);
+        }
+    fd_filter<T, W, ORDER>(w[1], c1, c2, c3, u);
+#pragma unroll
+    for (int oz = 0; oz < W; oz++)
+#pragma unroll
+        for (int ox = 0; ox < W; ox++) {
+            T l = 0;
+#pragma unroll
+            for (int q = 0; q < Q; q++) l = fma(u[q], ld(ox + H, q, oz + H), l);
+            s2 = fma(w[0][ox] * w[2][oz], l, s2);
+        }
+    fd_filter<T, W, ORDER>(w[2], c1, c2, c3, u);
+#pragma unroll
+    for (int oy = 0; oy < W; oy++)
+#pragma unroll
+        for (int ox = 0; ox < W; ox++) {
+            T l = 0;
+#pragma unroll
+            for (int q = 0; q < Q; q++) l = fma(u[q], ld(ox + H, oy + H, q), l);
+            s3 = fma(w[0][ox] * w[1][oy], l, s3);
+        }
+}
+
+// workgroup size by register need: 1024 threads leave 128 VGPRs (FP32 CIC/TSC), 512 leave 256 (FP32
+// PCS, FP64 CIC/TSC), 256 leave 512 (FP64 PCS)
+template <typename T, int W> constexpr int gather_threads() {
+    return sizeof(T) == 4 ? (W <= 3 ? 1024 : 512) : (W <= 3 ? 512 : 256);
+}
+
+// a particle's tile entry and the state its gather mode reads
+template <typename T, int MODE>
+__device__ __forceinline__ void gather_fetch(const Entry<T> *__restrict__ ent, unsigned e, const T *psi_in,
+                                             const T *__restrict__ mom, const T *__restrict__ acc, size_t np, int ns,
+                                             Entry<T> &en, T (&m3)[3], T (&p3)[3], T (&a3)[3]) {
+    en = ent[e];
+    const size_t p = (size_t)en.idx;
+    if constexpr (MODE == GATHER_KICK) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            m3[c] = mom[p + c * np];
+            p3[c] = psi_in[p + c * np];
+            if (ns > 1) a3[c] = acc[p + c * np];
+        }
+    } else if constexpr (MODE == GATHER_ACC) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) a3[c] = acc[p + c * np];
+    }
+}
+
+// The hardware hands workgroup b to XCD b % 8.  Tile (b / 8) + (b % 8) (nt / 8) keeps runs of
+// consecutive tiles (x neighbours, then the next row of tiles) on one XCD, so the halo that
+// neighbouring tiles share is fetched into that XCD's L2 once.
+__device__ __forceinline__ int xcd_tile(int b, int nt) { return (nt & 7) ? b : (b >> 3) + (b & 7) * (nt >> 3); }
+
+// staged-region geometry shared by the kernel and its launcher: x rows of S0 cells, S0 * sizeof(T)
+// / 4 <= 64 dwords so that one wave loads a row, and no periodic self-overlap
+template <typename T>
+__host__ __device__ __forceinline__ bool gather_staged(const TileGeom &g, int H) {
+    const int S0 = g.TX + g.RL + g.RR + 2 * H, S1 = g.TY + g.RL + g.RR + 2 * H, S2 = g.TZ + g.RL + g.RR + 2 * H;
+    return S0 * (int)(sizeof(T) / 4) <= 64 && S0 < g.N && S1 < g.N && S2 < g.N;
+}
+
+typedef __attribute__((address_space(3))) void lds_void_t;
+typedef __attribute__((address_space(1))) void glb_void_t;
 
 template <typename T, int W, int ORDER, int MODE>
 __global__ __launch_bounds__((gather_threads<T, W>())) void k_tile_gather(const Entry<T> *__restrict__ ent, const unsigned *__restrict__ start,
@@ -324,37 +431,44 @@ __global__ __launch_bounds__((gather_threads<T, W>())) void k_tile_gather(const 
                                                       T L, const T *__restrict__ phi, T c1, T c2, T c3, const T *psi_in,
                                                       T *psi_out, T *__restrict__ mom, T *__restrict__ acc, size_t np,
                                                       T alpha, T beta, T dd2, T dd1n, int has_next, int ns) {
-    constexpr int H = ORDER / 2, Q = W + 2 * H;
+    constexpr int H = ORDER / 2, Q = W + 2 * H, NT = gather_threads<T, W>(), DW = sizeof(T) / 4;
     extern __shared__ unsigned char smem[];
     T *ph = (T *)smem;
-    const int t = blockIdx.x;
+    const int t = xcd_tile(blockIdx.x, g.nt);
     const int tx = t % g.ntx, ty = (t / g.ntx) % g.nty, tz = t / (g.ntx * g.nty);
     // staged region: tile + stencil reach (RL left, RR right) + FD halo H
     const int r0[3] = {(tx << g.lT[0]) - g.RL - H, (ty << g.lT[1]) - g.RL - H, (tz << g.lT[2]) - g.RL - H};
     const int S[3] = {g.TX + g.RL + g.RR + 2 * H, g.TY + g.RL + g.RR + 2 * H, g.TZ + g.RL + g.RR + 2 * H};
     const int N = g.N, SX = S[0], SXY = S[0] * S[1];
-    // stage only regions smaller than the mesh (no periodic self-overlap) with x rows of <= 64 cells
-    const bool region_ok = S[0] <= 64 && S[0] < N && S[1] < N && S[2] < N;
+    const bool region_ok = gather_staged<T>(g, H);
+
+    // this thread's first particle and its state, loaded before the staging so both are in flight together
+    const unsigned e0 = start[t], e1 = e0 + own[t];
+    unsigned e = e0 + threadIdx.x;
+    Entry<T> en{};
+    T m3[3] = {0, 0, 0}, p3[3] = {0, 0, 0}, a3[3] = {0, 0, 0};
+    if (e < e1) gather_fetch<T, MODE>(ent, e, psi_in, mom, acc, np, ns, en, m3, p3, a3);
+
     if (region_ok) {
-        // rows along x: 64 lanes per row (S[0] <= 64), NT / 64 rows at a time
-        constexpr int NT = gather_threads<T, W>(), RPI = NT / 64;
-        const int lx = threadIdx.x & 63, rstart = threadIdx.x >> 6, nrows = S[1] * S[2];
-        int ly = rstart % S[1], lz = rstart / S[1];
-        const int gx = wrapn(r0[0] + lx, N);
-        for (int row = rstart; row < nrows; row += RPI) {
-            if (lx < S[0]) {
-                const int gy = wrapn(r0[1] + ly, N), gz = wrapn(r0[2] + lz, N);
-                ph[lx + SX * ly + SXY * lz] = phi[(size_t)gx + (size_t)N * ((size_t)gy + (size_t)N * gz)];
-            }
-            ly += RPI;
-            while (ly >= S[1]) { ly -= S[1]; lz++; }
+        // one wave per x row: lane l loads dword l of the row straight into LDS (lanes past the row idle)
+        const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+        const int nrows = S[1] * S[2];
+        const bool active = lane < S[0] * DW;
+        const int gx = wrapn(r0[0] + lane / DW, N);
+        const unsigned *src0 = (const unsigned *)phi + (size_t)gx * DW + (lane % DW);
+        for (int row = wave; row < nrows; row += NT / 64) {
+            const int ly = row % S[1], lz = row / S[1];
+            const int gy = wrapn(r0[1] + ly, N), gz = wrapn(r0[2] + lz, N);
+            const size_t roff = ((size_t)gy + (size_t)N * gz) * (size_t)N * DW;
+            if (active)
+                __builtin_amdgcn_global_load_lds((glb_void_t *)(src0 + roff), (lds_void_t *)(ph + (size_t)row * SX), 4, 0, 0);
         }
+        __builtin_amdgcn_s_waitcnt(0);
     }
     __syncthreads();
-    const unsigned e0 = start[t], e1 = e0 + own[t];
+
     const T xsh[3] = {sx, sy, sz};
-    for (unsigned e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
-        const Entry<T> en = ent[e];
+    while (e < e1) {
         const T xe[3] = {en.x, en.y, en.z};
         T w[3][W];
         int c0[3], lb[3];
@@ -373,7 +487,7 @@ __global__ __launch_bounds__((gather_threads<T, W>())) void k_tile_gather(const 
         if (staged) {
             const int base = lb[0] + SX * lb[1] + SXY * lb[2];
             auto ld = [&](int x, int y, int z) { return ph[base + x + SX * y + SXY * z]; };
-            gather_sums<T, W, ORDER>(ld, w, c1, c2, c3, s1, s2, s3);
+            gather_sums_filtered<T, W, ORDER>(ld, w, c1, c2, c3, s1, s2, s3);
         } else {
             auto ld = [&](int x, int y, int z) {
                 return phi[(size_t)wrapn(c0[0] + x, N) +
@@ -382,28 +496,27 @@ __global__ __launch_bounds__((gather_threads<T, W>())) void k_tile_gather(const 
             gather_sums<T, W, ORDER>(ld, w, c1, c2, c3, s1, s2, s3);
         }
         const size_t p = (size_t)en.idx;
+        const T f3[3] = {s1, s2, s3};
         if constexpr (MODE == GATHER_ACC) {
-            acc[p] += s1;
-            acc[p + np] += s2;
-            acc[p + 2 * np] += s3;
+#pragma unroll
+            for (int c = 0; c < 3; c++) acc[p + c * np] = a3[c] + f3[c];
         } else if constexpr (MODE == GATHER_ASSIGN) {
-            acc[p] = s1;
-            acc[p + np] = s2;
-            acc[p + 2 * np] = s3;
+#pragma unroll
+            for (int c = 0; c < 3; c++) acc[p + c * np] = f3[c];
         } else {
-            const T a3[3] = {s1, s2, s3};
 #pragma unroll
             for (int c = 0; c < 3; c++) {
-                const size_t q = p + c * np;
-                T ac = a3[c];
-                if (ns > 1) ac = (acc[q] + ac) / (T)ns;
-                T m = alpha * mom[q] + beta * ac;
-                T s = psi_in[q] + dd2 * m;
+                T ac = f3[c];
+                if (ns > 1) ac = (a3[c] + ac) / (T)ns;
+                T m = alpha * m3[c] + beta * ac;
+                T s = p3[c] + dd2 * m;
                 if (has_next) s = s + dd1n * m;
-                mom[q] = m;
-                psi_out[q] = s;
+                mom[p + c * np] = m;
+                psi_out[p + c * np] = s;
             }
         }
+        e += NT;
+        if (e < e1) gather_fetch<T, MODE>(ent, e, psi_in, mom, acc, np, ns, en, m3, p3, a3);
     }
 }
 
@@ -430,17 +543,25 @@ struct TileLaunch {
     static void fill(const TileArgs &a, int W, hipStream_t st) {
         if (W == 2) fill_w<2>(a, st); else if (W == 3) fill_w<3>(a, st); else fill_w<4>(a, st);
     }
-    template <int W, bool FIXED>
+    template <int W, int ACC>
     static void dep_w(const TileArgs &a, const double *sh, void *mesh, double fscale, hipStream_t st) {
-        size_t shm = (size_t)a.g.TX * a.g.TY * a.g.TZ * (FIXED ? 8 : sizeof(T));
-        k_tile_deposit<T, W, FIXED><<<a.g.nt, 256, shm, st>>>((const Entry<T> *)a.ent, a.start, a.g, (T)sh[0],
-                                                              (T)sh[1], (T)sh[2], (T)a.L, mesh, fscale);
+        size_t shm = (size_t)a.g.TX * a.g.TY * a.g.TZ * (ACC == ACC_T ? sizeof(T) : 8);
+        auto kern = k_tile_deposit<T, W, ACC>;
+        (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+        kern<<<a.g.nt, 256, shm, st>>>((const Entry<T> *)a.ent, a.start, a.g, (T)sh[0], (T)sh[1], (T)sh[2], (T)a.L,
+                                       mesh, fscale);
+    }
+    template <int W>
+    static void dep_sel(const TileArgs &a, bool fixed, const double *sh, void *mesh, double fscale, hipStream_t st) {
+        constexpr int FLOAT_ACC = sizeof(T) == 4 ? ACC_Q31 : ACC_T;
+        if (fixed) dep_w<W, ACC_FIXED>(a, sh, mesh, fscale, st);
+        else dep_w<W, FLOAT_ACC>(a, sh, mesh, fscale, st);
     }
     static void deposit(const TileArgs &a, int W, bool fixed, const double *sh, void *mesh, double fscale,
                         hipStream_t st) {
-#define D(w) (fixed ? dep_w<w, true>(a, sh, mesh, fscale, st) : dep_w<w, false>(a, sh, mesh, fscale, st))
-        if (W == 2) D(2); else if (W == 3) D(3); else D(4);
-#undef D
+        if (W == 2) dep_sel<2>(a, fixed, sh, mesh, fscale, st);
+        else if (W == 3) dep_sel<3>(a, fixed, sh, mesh, fscale, st);
+        else dep_sel<4>(a, fixed, sh, mesh, fscale, st);
     }
     template <int W, int O, int M>
     static void gat_wom(const TileArgs &a, const double *sh, const void *phi, const double *c, const KickArgs &k,
@@ -448,8 +569,8 @@ struct TileLaunch {
         constexpr int H = O / 2;
         const int SX = a.g.TX + a.g.RL + a.g.RR + 2 * H, SY = a.g.TY + a.g.RL + a.g.RR + 2 * H,
                   SZ = a.g.TZ + a.g.RL + a.g.RR + 2 * H;
-        // a region as large as the mesh is not staged (the kernel reads the potential directly)
-        size_t shm = (SX <= 64 && SX < a.g.N && SY < a.g.N && SZ < a.g.N) ? sizeof(T) * (size_t)SX * SY * SZ : sizeof(T);
+        // a region that is not staged (as large as the mesh) is read from global memory by the kernel
+        size_t shm = gather_staged<T>(a.g, H) ? sizeof(T) * (size_t)SX * SY * SZ : sizeof(T);
         auto kern = k_tile_gather<T, W, O, M>;
         (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
         kern<<<a.g.nt, gather_threads<T, W>(), shm, st>>>((const Entry<T> *)a.ent, a.start, a.cnt, a.g, (T)sh[0], (T)sh[1], (T)sh[2],
