@@ -40,7 +40,9 @@ typedef int32_t dj_status;
 #define DJ_ERR_DEVICE (-5)           /* HIP runtime error */
 #define DJ_ERR_NONFINITE (-6)        /* NaN/Inf found in the particle state (dj_check_finite) */
 
-#define DJ_DEPOSIT_FLOAT 0   /* atomic adds in the working precision (order-dependent round-off) */
+#define DJ_DEPOSIT_FLOAT 0   /* density mesh in the working precision.  FP32: each tile sums the
+                              * weights as integers round(w 2^31) in 64 bits and rounds once to
+                              * float (independent of the order of the adds); FP64: double atomics */
 #define DJ_DEPOSIT_FIXED64 1 /* int64 fixed-point weights: bit-reproducible density */
 
 /* PMConfig (pm.jl:35-46) plus the device-side choices.  Field meanings and accepted values are the

@@ -133,6 +133,43 @@ def test_power_spectrum_white_noise_amplitude():
     assert np.mean(pk) == pytest.approx((L / N) ** 3, rel=0.03)
 
 
+def test_ic_operator_amplitude_and_sign():
+    # grf.jl:118-144 maps white noise omega to the potential phi of a density with spectrum P(k).
+    # Pinned without restating the map: (1) delta = -k^2 phi has the input P(k) on average over the
+    # modes of each bin (amplitude: the square root and the (N/L)^3 factor); (2) delta_k / omega_k is
+    # real and positive (the density follows the noise; the Poisson sign); (3) the first-order
+    # displacement of the golden-pinned compute_core converges on those overdensities.
+    n, L = 64, 400.0
+    c = O.Cosmology(0.2621, 0.049, 0.6766, 0.8102, 0.9665)
+    k, pk = O.linear_power_spectrum(c)
+    w = O.ngenic_wnoise_real(42, n)
+    fphi = O.white_noise_to_fphi(O.ic_operator(n, L, k, pk), w)
+    kf = 2 * np.pi / L
+    kfull = np.fft.fftfreq(n, 1.0 / n) * kf
+    kz = np.arange(n // 2 + 1) * kf
+    k2 = kz[:, None, None] ** 2 + kfull[None, :, None] ** 2 + kfull[None, None, :] ** 2
+    delta_k = -k2 * fphi
+    Wk = O.rfft3(w)
+    live = (k2 > 0) & (np.abs(Wk) > 1e-9 * np.max(np.abs(Wk)))  # N-GenIC leaves some modes empty
+    assert live.mean() > 0.9
+    ratio = delta_k[live] / Wk[live]
+    assert np.max(np.abs(ratio.imag)) <= 1e-10 * np.max(np.abs(ratio.real))
+    assert np.all(ratio.real > 0)
+    delta = O.brfft3(delta_k) / n**3
+    kb, pb, nm = O.power_spectrum_lin(delta, L, W=0)
+    ok = nm > 400
+    pin = np.exp(np.interp(np.log(kb[ok]), np.log(k), np.log(pk)))
+    r = pb[ok] / pin
+    assert ok.sum() >= 8
+    assert abs(np.mean(r) - 1) < 0.03
+    assert np.all(np.abs(r - 1) < 6 * np.sqrt(2.0 / nm[ok]))
+    psi1, _ = O.compute_core2(fphi, L)
+    div = sum(1j * kk * O.rfft3(psi1[a]) for a, kk in ((0, kfull[None, None, :]), (1, kfull[None, :, None]),
+                                                       (2, kz[:, None, None])))
+    conv = -O.brfft3(div) / n**3
+    assert np.corrcoef(conv.ravel(), delta.ravel())[0, 1] > 0.99
+
+
 # ---------------------------------------------------------------- PM force invariants
 def test_lean_equals_solver():
     # pm_lean_tests.jl:5-21 on other random inputs (the reference's MersenneTwister stream is Julia-only)
