@@ -109,14 +109,14 @@ struct Prof {
     Prof(dj_ctx *c_, int id_) : c(c_), id(id_) {
         if (c->prof) {
             a = take_event(c);
-            if (a) hipEventRecord(a, c->st);
+            if (a) (void)hipEventRecord(a, c->st);
         }
     }
     ~Prof() {
         if (c->prof && a) {
             hipEvent_t b = take_event(c);
             if (b) {
-                hipEventRecord(b, c->st);
+                (void)hipEventRecord(b, c->st);
                 c->pending.push_back({id, a, b});
             }
         }
@@ -280,11 +280,12 @@ int aa_shifts(int aa, double dh, double (*s)[3]) {
 void free_all(dj_ctx *c) {
     void *bufs[] = {c->psi, c->mom, c->acc, c->psi_in, c->q1, c->mesh, c->meshfix, c->A, c->B, c->C, c->tw, c->tabs,
                     c->tcnt, c->tstart, c->tcur, c->tent, c->d_shifts};
+    // teardown: failures here have no caller to report to
     for (void *b : bufs)
-        if (b) hipFree(b);
-    for (auto &p : c->pending) { hipEventDestroy(p.a); hipEventDestroy(p.b); }
-    for (auto e : c->pool) hipEventDestroy(e);
-    if (c->st) hipStreamDestroy(c->st);
+        if (b) (void)hipFree(b);
+    for (auto &p : c->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
+    for (auto e : c->pool) (void)hipEventDestroy(e);
+    if (c->st) (void)hipStreamDestroy(c->st);
 }
 
 void set_bytes(dj_ctx *c) {
@@ -327,6 +328,7 @@ TileArgs tile_args(dj_ctx *c, const void *psi, const double *d_shifts, int ns) {
     a.start = c->tstart;
     a.cur = c->tcur;
     a.ent = c->tent;
+    a.ent_cap = c->tent_cap;
     return a;
 }
 
@@ -346,6 +348,7 @@ dj_status tile_sort(dj_ctx *c, const void *psi, const double *d_shifts, int ns, 
         HIPCHK(c, hipMalloc(&c->tent, cap * c->tiles.entry_bytes));
         c->tent_cap = cap;
         a.ent = c->tent;
+        a.ent_cap = c->tent_cap;
     }
     c->tiles.fill(a, W, c->st);
     return check_launch(c, "tile binning");
@@ -564,8 +567,8 @@ dj_status dj_create(const dj_pm_config *cfg, dj_ctx **out) {
 
 dj_status dj_destroy(dj_ctx *c) {
     if (!c) return DJ_OK;
-    hipSetDevice(c->dev);
-    if (c->st) hipStreamSynchronize(c->st);
+    (void)hipSetDevice(c->dev);
+    if (c->st) (void)hipStreamSynchronize(c->st);
     free_all(c);
     delete c;
     return DJ_OK;

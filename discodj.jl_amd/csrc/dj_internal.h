@@ -78,6 +78,7 @@ struct TileArgs {
     unsigned *start;       // (nt + 1) first entry of each tile
     unsigned *cur;         // (2 nt) fill cursors
     void *ent;             // entries (X, particle index), own then ghosts per tile
+    size_t ent_cap;        // entries the buffer holds
 };
 struct TileOps {
     void (*count)(const TileArgs &a, int W, hipStream_t st);  // counts + scan

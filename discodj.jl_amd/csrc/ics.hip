@@ -273,7 +273,7 @@ int ics_lpt_device(int n, double L, int order, const double *white_host, const d
     err = hipGetLastError();
     if (err == hipSuccess) err = hipStreamSynchronize(st);
 done:
-    hipStreamSynchronize(st);
+    (void)hipStreamSynchronize(st);
     for (void *p : allocs) (void)hipFree(p);
     return (int)err;
 }

@@ -120,7 +120,7 @@ template <typename T, int W, bool FILL>
 __global__ __launch_bounds__(256) void k_tile_bin(const T *__restrict__ psi, const T *__restrict__ q1, int n,
                                                   const double *__restrict__ shifts, int ns, T L, TileGeom g,
                                                   unsigned *cnt, const unsigned *__restrict__ start, unsigned *cur,
-                                                  Entry<T> *__restrict__ ent) {
+                                                  Entry<T> *__restrict__ ent, size_t ent_cap) {
     __shared__ unsigned keys[HASH], vals[HASH], base[HASH];
     for (int s = threadIdx.x; s < HASH; s += 256) { keys[s] = EMPTY; vals[s] = 0; }
     __syncthreads();
@@ -139,12 +139,36 @@ __global__ __launch_bounds__(256) void k_tile_bin(const T *__restrict__ psi, con
         X[1] = psi[p + np] + q1[j];
         X[2] = psi[p + 2 * np] + q1[k];
         nt = particle_tiles<T, W>(X, g, shifts, ns, L, tiles);
-        for (int m = 0; m < nt; m++) {
-            unsigned key = tiles[m] * 2u + (m > 0 ? 1u : 0u);
-            int s = hash_slot(keys, key);
-            slot[m] = s;
-            if (s >= 0) rank[m] = (int)atomicAdd(&vals[s], 1u);
-            else rank[m] = -1;
+    }
+    // Wave-aggregated ranks: per list position m, the lanes that share a key form one group; its
+    // first lane makes the hash lookup and one LDS atomic for the whole group, and each lane's rank
+    // is the group base plus its position in the group.  A wave's particles thus stay contiguous
+    // (in Lagrangian order) within their tile's list.
+    const int lane = __lane_id();
+    const unsigned long long below = (1ull << lane) - 1ull;
+#pragma unroll
+    for (int m = 0; m < 9; m++) {
+        const bool act = m < nt;
+        const unsigned key = act ? tiles[m] * 2u + (m > 0 ? 1u : 0u) : EMPTY;
+        unsigned long long pend = __ballot(act);
+        while (pend) {
+            const int leader = __ffsll((long long)pend) - 1;
+            const unsigned lk = (unsigned)__shfl((int)key, leader);
+            const bool mine = act && key == lk;
+            const unsigned long long grp = __ballot(mine);
+            int s = -1;
+            unsigned b = 0;
+            if (lane == leader) {
+                s = hash_slot(keys, lk);
+                if (s >= 0) b = atomicAdd(&vals[s], (unsigned)__popcll(grp));
+            }
+            s = __shfl(s, leader);
+            b = (unsigned)__shfl((int)b, leader);
+            if (mine) {
+                slot[m] = s;
+                rank[m] = s >= 0 ? (int)(b + (unsigned)__popcll(grp & below)) : -1;
+            }
+            pend &= ~grp;
         }
     }
     __syncthreads();
@@ -174,7 +198,8 @@ __global__ __launch_bounds__(256) void k_tile_bin(const T *__restrict__ psi, con
         e.y = X[1];
         e.z = X[2];
         e.idx = (int)p;
-        ent[start[t] + off + r] = e;
+        const size_t at = (size_t)start[t] + off + r;
+        if (at < ent_cap) ent[at] = e;  // always true when the counts and the fill agree
     }
 }
 
@@ -528,14 +553,14 @@ struct TileLaunch {
         dim3 grid((a.n + 15) / 16, (a.n + 3) / 4, (a.n + 3) / 4);
         const unsigned *cnt = a.cnt;
         k_tile_bin<T, W, false><<<grid, 256, 0, st>>>((const T *)a.psi, (const T *)a.q1, a.n, a.shifts, a.ns, (T)a.L,
-                                                      a.g, a.cnt, a.start, a.cur, (Entry<T> *)a.ent);
+                                                      a.g, a.cnt, a.start, a.cur, (Entry<T> *)a.ent, a.ent_cap);
         k_tile_scan<<<1, 1024, 0, st>>>(cnt, a.g.nt, a.start, a.cur);
     }
     template <int W>
     static void fill_w(const TileArgs &a, hipStream_t st) {
         dim3 grid((a.n + 15) / 16, (a.n + 3) / 4, (a.n + 3) / 4);
         k_tile_bin<T, W, true><<<grid, 256, 0, st>>>((const T *)a.psi, (const T *)a.q1, a.n, a.shifts, a.ns, (T)a.L,
-                                                     a.g, a.cnt, a.start, a.cur, (Entry<T> *)a.ent);
+                                                     a.g, a.cnt, a.start, a.cur, (Entry<T> *)a.ent, a.ent_cap);
     }
     static void count(const TileArgs &a, int W, hipStream_t st) {
         if (W == 2) bin_w<2>(a, st); else if (W == 3) bin_w<3>(a, st); else bin_w<4>(a, st);
