@@ -13,9 +13,28 @@ template <typename T, int N> struct FftShape {
                            : (N % 8 == 0 && target >= 8) ? 8
                            : (N % 4 == 0 && target >= 4) ? 4
                            : (N % 2 == 0 && target >= 2) ? 2 : 1;
+    // x passes (half-length packed lines, about 32 KiB of LDS) run 256 threads; the y/z passes run
+    // 512 once a radix-16 pass has at least that many butterflies, so that the two workgroups their
+    // LDS tile allows per CU still hold 16 waves
     static constexpr int NT = 256;
+    static constexpr int NTY = (B * N / 16 >= 512) ? 512 : 256;
     using P = Passes<T, N, B, NT>;
+    using PY = Passes<T, N, B, NTY>;
+    // the N-entry twiddle table moves into LDS when two workgroups of the y/z tile plus table
+    // still fit a CU (160 KiB); otherwise the passes read it through the caches
+    static constexpr bool TWL = (PY::EF::LDS_ELEMS + N) * sizeof(cx<T>) <= 80 * 1024;
 };
+
+// the twiddle table for a workgroup: a copy in LDS, or the global table
+template <typename T, int N, bool TWL>
+__device__ __forceinline__ const cx<T> *stage_twiddles(const cx<T> *__restrict__ tw, cx<T> *twl, int tid, int nt) {
+    if constexpr (TWL) {
+        for (int i = tid; i < N; i += nt) twl[i] = tw[i];
+        return twl;  // the caller's barrier after the tile load covers this copy
+    } else {
+        return tw;
+    }
+}
 
 template <typename T, int N, typename IN>
 __global__ __launch_bounds__(256) void k_xf(const IN *__restrict__ mesh, cx<T> *__restrict__ A,
@@ -46,47 +65,53 @@ __global__ __launch_bounds__(256) void k_xi(const cx<T> *__restrict__ A, T *__re
 }
 
 template <typename T, int N>
-__global__ __launch_bounds__(256) void k_yf(const cx<T> *__restrict__ A, cx<T> *__restrict__ Bm,
-                                            const cx<T> *__restrict__ tw) {
+__global__ __launch_bounds__((FftShape<T, N>::NTY)) void k_yf(const cx<T> *__restrict__ A, cx<T> *__restrict__ Bm,
+                                                             const cx<T> *__restrict__ tw) {
     using S = FftShape<T, N>;
-    using P = typename S::P;
+    using P = typename S::PY;
     __shared__ cx<T> lds[P::EF::LDS_ELEMS];
+    __shared__ cx<T> twl[S::TWL ? N : 1];
     const int zb = blockIdx.x % (N / S::B), kx = blockIdx.x / (N / S::B);
+    const cx<T> *t = stage_twiddles<T, N, S::TWL>(tw, twl, threadIdx.x, S::NTY);
     P::y_load_a(A, lds, zb, kx, threadIdx.x);
     __syncthreads();
-    P::EF::template run<-1, 1>(lds, tw, threadIdx.x);
+    P::EF::template run<-1, 1>(lds, t, threadIdx.x);
     P::y_store_b(lds, Bm, zb, kx, threadIdx.x);
 }
 
 template <typename T, int N>
-__global__ __launch_bounds__(256) void k_yi(const cx<T> *__restrict__ Bm, cx<T> *__restrict__ A,
-                                            const cx<T> *__restrict__ tw) {
+__global__ __launch_bounds__((FftShape<T, N>::NTY)) void k_yi(const cx<T> *__restrict__ Bm, cx<T> *__restrict__ A,
+                                                             const cx<T> *__restrict__ tw) {
     using S = FftShape<T, N>;
-    using P = typename S::P;
+    using P = typename S::PY;
     __shared__ cx<T> lds[P::EF::LDS_ELEMS];
+    __shared__ cx<T> twl[S::TWL ? N : 1];
     const int zb = blockIdx.x % (N / S::B), kx = blockIdx.x / (N / S::B);
+    const cx<T> *t = stage_twiddles<T, N, S::TWL>(tw, twl, threadIdx.x, S::NTY);
     P::y_load_b(Bm, lds, zb, kx, threadIdx.x);
     __syncthreads();
-    P::EF::template run<+1, 1>(lds, tw, threadIdx.x);
+    P::EF::template run<+1, 1>(lds, t, threadIdx.x);
     P::y_store_a(lds, A, zb, kx, threadIdx.x);
 }
 
 // z lines of B: optional forward transform, multiplier, optional inverse; in may equal out
 template <typename T, int N, int MODE, bool FWD, bool INV>
-__global__ __launch_bounds__(256) void k_zm(const cx<T> *in, cx<T> *out, const cx<T> *__restrict__ tw,
-                                            KMult<T> K) {
+__global__ __launch_bounds__((FftShape<T, N>::NTY)) void k_zm(const cx<T> *in, cx<T> *out, const cx<T> *__restrict__ tw,
+                                                             KMult<T> K) {
     using S = FftShape<T, N>;
-    using P = typename S::P;
+    using P = typename S::PY;
     __shared__ cx<T> lds[P::EF::LDS_ELEMS];
+    __shared__ cx<T> twl[S::TWL ? N : 1];
     const int b = blockIdx.x;
+    const cx<T> *t = stage_twiddles<T, N, S::TWL>(tw, twl, threadIdx.x, S::NTY);
     P::z_load(in, lds, b, threadIdx.x);
     __syncthreads();
-    if constexpr (FWD) P::EF::template run<-1, 1>(lds, tw, threadIdx.x);
+    if constexpr (FWD) P::EF::template run<-1, 1>(lds, t, threadIdx.x);
     if constexpr (MODE != ZM_NONE) {
         P::template z_mult<MODE>(lds, K, b, threadIdx.x);
         __syncthreads();
     }
-    if constexpr (INV) P::EF::template run<+1, 1>(lds, tw, threadIdx.x);
+    if constexpr (INV) P::EF::template run<+1, 1>(lds, t, threadIdx.x);
     P::z_store(lds, out, b, threadIdx.x);
 }
 
@@ -105,10 +130,10 @@ struct FftN {
         k_xi<T, N><<<P::XBLOCKS(), 256, 0, st>>>((const cx<T> *)A, (T *)mesh, (const cx<T> *)tw);
     }
     static void yf(const void *A, void *Bm, const void *tw, hipStream_t st) {
-        k_yf<T, N><<<P::YBLOCKS(), 256, 0, st>>>((const cx<T> *)A, (cx<T> *)Bm, (const cx<T> *)tw);
+        k_yf<T, N><<<P::YBLOCKS(), S::NTY, 0, st>>>((const cx<T> *)A, (cx<T> *)Bm, (const cx<T> *)tw);
     }
     static void yi(const void *Bm, void *A, const void *tw, hipStream_t st) {
-        k_yi<T, N><<<P::YBLOCKS(), 256, 0, st>>>((const cx<T> *)Bm, (cx<T> *)A, (const cx<T> *)tw);
+        k_yi<T, N><<<P::YBLOCKS(), S::NTY, 0, st>>>((const cx<T> *)Bm, (cx<T> *)A, (const cx<T> *)tw);
     }
     static void zm(int mode, bool fwd, bool inv, const void *in, void *out, const void *tw, const void *Kp,
                    hipStream_t st) {
@@ -116,7 +141,7 @@ struct FftN {
         const cx<T> *i = (const cx<T> *)in;
         cx<T> *o = (cx<T> *)out;
         const cx<T> *t = (const cx<T> *)tw;
-        dim3 g(P::ZBLOCKS()), b(256);
+        dim3 g(P::ZBLOCKS()), b(S::NTY);
 #define ZM(M, F, I) k_zm<T, N, M, F, I><<<g, b, 0, st>>>(i, o, t, K)
         if (mode == ZM_POT && fwd && inv) ZM(ZM_POT, true, true);
         else if (mode == ZM_NONE && fwd && !inv) ZM(ZM_NONE, true, false);

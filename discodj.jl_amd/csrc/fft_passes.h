@@ -52,6 +52,24 @@ struct KMult {
 
 enum { ZM_POT = 0, ZM_GRADX = 1, ZM_GRADY = 2, ZM_GRADZ = 3, ZM_NONE = 4 };
 
+// Tile copy in two phases: a thread issues all of its TOTAL / NT reads (src(i)) before its first
+// write (dst(i, v)), so the memory latencies overlap instead of adding up per element.
+template <typename V, int TOTAL, int NT, typename SRC, typename DST>
+DJ_HD void copy_batched(int tid, const SRC &src, const DST &dst) {
+    constexpr int CNT = (TOTAL + NT - 1) / NT;
+    V v[CNT];
+#pragma unroll
+    for (int c = 0; c < CNT; c++) {
+        const int i = tid + c * NT;
+        if (TOTAL % NT == 0 || i < TOTAL) v[c] = src(i);
+    }
+#pragma unroll
+    for (int c = 0; c < CNT; c++) {
+        const int i = tid + c * NT;
+        if (TOTAL % NT == 0 || i < TOTAL) dst(i, v[c]);
+    }
+}
+
 template <typename T, int N, int B, int NT>
 struct Passes {
     static constexpr int NC = N / 2;      // complex length of the packed real x lines
@@ -64,39 +82,53 @@ struct Passes {
     // block (yb, z): lines y = yb*B + l.  IN is T (float mesh) or int64 (fixed-point mesh).
     template <typename IN>
     DJ_HD static void xf_load(const IN *mesh, cx<T> *lds, int yb, int z, int tid, double fixed_scale) {
-        size_t base = (size_t)N * N * z + (size_t)N * yb * B;
-        for (int i = tid; i < B * NC; i += NT) {
-            int l = i / NC, m = i - l * NC;
-            size_t g = base + (size_t)N * l + 2 * m;
-            cx<T> v;
-            if constexpr (sizeof(IN) == 8 && !std::is_floating_point<IN>::value) {
-                v = {(T)((double)mesh[g] * fixed_scale), (T)((double)mesh[g + 1] * fixed_scale)};
-            } else {
-                v = {(T)mesh[g], (T)mesh[g + 1]};
-            }
-            lds[EX::idx(l, m)] = v;
-        }
+        const size_t base = (size_t)N * N * z + (size_t)N * yb * B;
+        copy_batched<cx<T>, B * NC, NT>(
+            tid,
+            [&](int i) {
+                const int l = i / NC, m = i - l * NC;
+                const size_t g = base + (size_t)N * l + 2 * m;
+                if constexpr (sizeof(IN) == 8 && !std::is_floating_point<IN>::value)
+                    return cx<T>{(T)((double)mesh[g] * fixed_scale), (T)((double)mesh[g + 1] * fixed_scale)};
+                else
+                    return cx<T>{(T)mesh[g], (T)mesh[g + 1]};
+            },
+            [&](int i, cx<T> v) {
+                const int l = i / NC, m = i - l * NC;
+                lds[EX::idx(l, m)] = v;
+            });
     }
     // X[k] = E[k] + W_N^k O[k], E = (Z[k] + conj Z[NC-k]) / 2, O = (Z[k] - conj Z[NC-k]) / 2i
     DJ_HD static void xf_store(const cx<T> *lds, const cx<T> *twN, cx<T> *A, int yb, int z, int tid) {
-        for (int i = tid; i < B * H; i += NT) {
-            int kx = i / B, l = i - kx * B;
-            cx<T> z1 = lds[EX::idx(l, kx == NC ? 0 : kx)];
-            cx<T> z2 = conj(lds[EX::idx(l, kx == 0 ? 0 : NC - kx)]);
-            cx<T> e = scale(z1 + z2, (T)0.5);
-            cx<T> d = z1 - z2;
-            cx<T> o = {d.y * (T)0.5, -d.x * (T)0.5};
-            cx<T> x = e + cmul(o, twN[kx]);
-            A[(size_t)(yb * B + l) + (size_t)N * (kx + (size_t)H * z)] = x;
-        }
+        copy_batched<cx<T>, B * H, NT>(
+            tid,
+            [&](int i) {
+                const int kx = i / B, l = i - kx * B;
+                const cx<T> z1 = lds[EX::idx(l, kx == NC ? 0 : kx)];
+                const cx<T> z2 = conj(lds[EX::idx(l, kx == 0 ? 0 : NC - kx)]);
+                const cx<T> e = scale(z1 + z2, (T)0.5);
+                const cx<T> d = z1 - z2;
+                const cx<T> o = {d.y * (T)0.5, -d.x * (T)0.5};
+                return e + cmul(o, twN[kx]);
+            },
+            [&](int i, cx<T> x) {
+                const int kx = i / B, l = i - kx * B;
+                A[(size_t)(yb * B + l) + (size_t)N * (kx + (size_t)H * z)] = x;
+            });
     }
 
     // ------------------------------------------------------------------ XI: A -> mesh
     DJ_HD static void xi_load(const cx<T> *A, cx<T> *lds, int yb, int z, int tid) {
-        for (int i = tid; i < B * H; i += NT) {
-            int kx = i / B, l = i - kx * B;
-            lds[EX::idx(l, kx)] = A[(size_t)(yb * B + l) + (size_t)N * (kx + (size_t)H * z)];
-        }
+        copy_batched<cx<T>, B * H, NT>(
+            tid,
+            [&](int i) {
+                const int kx = i / B, l = i - kx * B;
+                return A[(size_t)(yb * B + l) + (size_t)N * (kx + (size_t)H * z)];
+            },
+            [&](int i, cx<T> v) {
+                const int kx = i / B, l = i - kx * B;
+                lds[EX::idx(l, kx)] = v;
+            });
     }
     // Z[k] = (X[k] + conj X[NC-k]) + i (X[k] - conj X[NC-k]) W_N^-k, for k and NC-k together;
     // the imaginary parts of X[0] and X[NC] are dropped (c2r of a Hermitian line)
@@ -119,55 +151,62 @@ struct Passes {
         }
     }
     DJ_HD static void xi_store(const cx<T> *lds, T *mesh, int yb, int z, int tid) {
-        size_t base = (size_t)N * N * z + (size_t)N * yb * B;
-        for (int i = tid; i < B * NC; i += NT) {
-            int l = i / NC, m = i - l * NC;
-            cx<T> v = lds[EX::idx(l, m)];
-            size_t g = base + (size_t)N * l + 2 * m;
-            mesh[g] = v.x;
-            mesh[g + 1] = v.y;
-        }
+        const size_t base = (size_t)N * N * z + (size_t)N * yb * B;
+        copy_batched<cx<T>, B * NC, NT>(
+            tid,
+            [&](int i) {
+                const int l = i / NC, m = i - l * NC;
+                return lds[EX::idx(l, m)];
+            },
+            [&](int i, cx<T> v) {
+                const int l = i / NC, m = i - l * NC;
+                const size_t g = base + (size_t)N * l + 2 * m;
+                mesh[g] = v.x;
+                mesh[g + 1] = v.y;
+            });
     }
 
     // ------------------------------------------------------------------ YF: A -> B, YI: B -> A
     // block (zb, kx): lines z = zb*B + l
+    DJ_HD static size_t a_at(int y, int kx, int z) { return (size_t)y + (size_t)N * (kx + (size_t)H * z); }
+    DJ_HD static size_t b_at(int z, int kx, int ky) { return (size_t)z + (size_t)N * (kx + (size_t)H * ky); }
     DJ_HD static void y_load_a(const cx<T> *A, cx<T> *lds, int zb, int kx, int tid) {
-        for (int i = tid; i < B * N; i += NT) {
-            int l = i / N, y = i - l * N;
-            lds[EF::idx(l, y)] = A[(size_t)y + (size_t)N * (kx + (size_t)H * (zb * B + l))];
-        }
+        copy_batched<cx<T>, B * N, NT>(
+            tid, [&](int i) { const int l = i / N, y = i - l * N; return A[a_at(y, kx, zb * B + l)]; },
+            [&](int i, cx<T> v) { const int l = i / N, y = i - l * N; lds[EF::idx(l, y)] = v; });
     }
     DJ_HD static void y_store_b(const cx<T> *lds, cx<T> *Bm, int zb, int kx, int tid) {
-        for (int i = tid; i < B * N; i += NT) {
-            int ky = i / B, l = i - ky * B;
-            Bm[(size_t)(zb * B + l) + (size_t)N * (kx + (size_t)H * ky)] = lds[EF::idx(l, ky)];
-        }
+        copy_batched<cx<T>, B * N, NT>(
+            tid, [&](int i) { const int ky = i / B, l = i - ky * B; return lds[EF::idx(l, ky)]; },
+            [&](int i, cx<T> v) { const int ky = i / B, l = i - ky * B; Bm[b_at(zb * B + l, kx, ky)] = v; });
     }
     DJ_HD static void y_load_b(const cx<T> *Bm, cx<T> *lds, int zb, int kx, int tid) {
-        for (int i = tid; i < B * N; i += NT) {
-            int ky = i / B, l = i - ky * B;
-            lds[EF::idx(l, ky)] = Bm[(size_t)(zb * B + l) + (size_t)N * (kx + (size_t)H * ky)];
-        }
+        copy_batched<cx<T>, B * N, NT>(
+            tid, [&](int i) { const int ky = i / B, l = i - ky * B; return Bm[b_at(zb * B + l, kx, ky)]; },
+            [&](int i, cx<T> v) { const int ky = i / B, l = i - ky * B; lds[EF::idx(l, ky)] = v; });
     }
     DJ_HD static void y_store_a(const cx<T> *lds, cx<T> *A, int zb, int kx, int tid) {
-        for (int i = tid; i < B * N; i += NT) {
-            int l = i / N, y = i - l * N;
-            A[(size_t)y + (size_t)N * (kx + (size_t)H * (zb * B + l))] = lds[EF::idx(l, y)];
-        }
+        copy_batched<cx<T>, B * N, NT>(
+            tid, [&](int i) { const int l = i / N, y = i - l * N; return lds[EF::idx(l, y)]; },
+            [&](int i, cx<T> v) { const int l = i / N, y = i - l * N; A[a_at(y, kx, zb * B + l)] = v; });
     }
 
     // ------------------------------------------------------------------ ZM: on B lines
     // block b: line ids L = b*B + l, kx = L % H, ky = L / H
     DJ_HD static void z_load(const cx<T> *Bm, cx<T> *lds, int b, int tid) {
-        size_t base = (size_t)N * B * b;
-        for (int i = tid; i < B * N; i += NT) {
-            int l = i / N, zz = i - l * N;
-            lds[EF::idx(l, zz)] = Bm[base + (size_t)N * l + zz];
-        }
+        const cx<T> *src = Bm + (size_t)N * B * b;
+        copy_batched<cx<T>, B * N, NT>(
+            tid, [&](int i) { return src[i]; },
+            [&](int i, cx<T> v) { const int l = i / N, zz = i - l * N; lds[EF::idx(l, zz)] = v; });
     }
     template <int MODE>
     DJ_HD static void z_mult(cx<T> *lds, const KMult<T> &K, int b, int tid) {
-        for (int i = tid; i < B * N; i += NT) {
+        // fixed trip count, unrolled: the multiplier-table reads of all iterations are in flight together
+        constexpr int CNT = (B * N + NT - 1) / NT;
+#pragma unroll
+        for (int c = 0; c < CNT; c++) {
+            const int i = tid + c * NT;
+            if (!((B * N) % NT == 0 || i < B * N)) continue;
             int l = i / N, kz = i - l * N;
             int L = b * B + l, ky = L / H, kx = L - ky * H;
             cx<T> v = lds[EF::idx(l, kz)];
@@ -184,11 +223,10 @@ struct Passes {
         }
     }
     DJ_HD static void z_store(const cx<T> *lds, cx<T> *Bm, int b, int tid) {
-        size_t base = (size_t)N * B * b;
-        for (int i = tid; i < B * N; i += NT) {
-            int l = i / N, zz = i - l * N;
-            Bm[base + (size_t)N * l + zz] = lds[EF::idx(l, zz)];
-        }
+        cx<T> *dst = Bm + (size_t)N * B * b;
+        copy_batched<cx<T>, B * N, NT>(
+            tid, [&](int i) { const int l = i / N, zz = i - l * N; return lds[EF::idx(l, zz)]; },
+            [&](int i, cx<T> v) { dst[i] = v; });
     }
 
     static constexpr int XBLOCKS() { return (N / B) * N; }
