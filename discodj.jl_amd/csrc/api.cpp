@@ -440,7 +440,9 @@ dj_status setup_tiles(dj_ctx *c) {
     };
     TileGeom &g = c->tg;
     g.N = N;
-    if (c->tsize == 4) { g.TX = pick({32, 16, 8}); g.TY = pick({16, 8}); g.TZ = pick({16, 8}); }
+    // FP32: 32 x 16 x 8 cells, so that the gather's staged region (tile + reach + FD halo, 79 KB for
+    // TSC at order 6) lets two workgroups share a CU: one stages while the other computes
+    if (c->tsize == 4) { g.TX = pick({32, 16, 8}); g.TY = pick({16, 8}); g.TZ = pick({8}); }
     else { g.TX = pick({16, 8}); g.TY = pick({16, 8}); g.TZ = pick({16, 8}); }
     if ((g.TX & (g.TX - 1)) || (g.TY & (g.TY - 1)) || (g.TZ & (g.TZ - 1)))
         return fail(c, DJ_ERR_UNSUPPORTED, "mesh size needs power-of-two tiles (multiple of 8)");

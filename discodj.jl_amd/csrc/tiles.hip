@@ -203,13 +203,16 @@ __global__ __launch_bounds__(256) void k_tile_bin(const T *__restrict__ psi, con
     }
 }
 
-// start[t] = sum_{u<t} (own[u] + ghost[u]); start[nt] = total.  One workgroup of 1024 threads.
+// start[t] = sum_{u<t} (own[u] + ghost[u]); start[nt] = total.  One workgroup of 1024 threads; each
+// thread owns a contiguous chunk of tiles whose counters it reads with independent (unrolled)
+// loads, then a block scan of the chunk sums.
 __global__ __launch_bounds__(1024) void k_tile_scan(const unsigned *__restrict__ cnt, int nt, unsigned *__restrict__ start,
                                                     unsigned *__restrict__ cur) {
     __shared__ unsigned long long part[1024];
     const int tid = threadIdx.x, chunk = (nt + 1023) / 1024;
     const int a = min(nt, tid * chunk), b = min(nt, a + chunk);
     unsigned long long s = 0;
+#pragma unroll 16
     for (int t = a; t < b; t++) s += (unsigned long long)cnt[t] + cnt[nt + t];
     part[tid] = s;
     __syncthreads();
@@ -220,6 +223,7 @@ __global__ __launch_bounds__(1024) void k_tile_scan(const unsigned *__restrict__
         __syncthreads();
     }
     unsigned long long run = part[tid] - s;
+#pragma unroll 16
     for (int t = a; t < b; t++) {
         start[t] = (unsigned)run;
         run += (unsigned long long)cnt[t] + cnt[nt + t];
@@ -408,11 +412,11 @@ __device__ __forceinline__ void gather_sums_filtered(const LD &ld, const T (&w)[
         }
 }
 
-// workgroup size by register need: 1024 threads leave 128 VGPRs (FP32 CIC/TSC), 512 leave 256 (FP32
-// PCS, FP64 CIC/TSC), 256 leave 512 (FP64 PCS)
-template <typename T, int W> constexpr int gather_threads() {
-    return sizeof(T) == 4 ? (W <= 3 ? 1024 : 512) : (W <= 3 ? 512 : 256);
-}
+// workgroup size: FP32 tiles hold ~512 particles, one per thread; FP64 PCS needs 256 threads to keep
+// its stencil in registers
+template <typename T, int W> constexpr int gather_threads() { return sizeof(T) == 4 ? 512 : (W <= 3 ? 512 : 256); }
+// waves per SIMD the register allocation must allow: two FP32 workgroups per CU (128 VGPRs)
+template <typename T, int W> constexpr int gather_min_waves() { return sizeof(T) == 4 && W <= 3 ? 4 : 1; }
 
 // a particle's tile entry and the state its gather mode reads
 template <typename T, int MODE>
@@ -451,7 +455,7 @@ typedef __attribute__((address_space(3))) void lds_void_t;
 typedef __attribute__((address_space(1))) void glb_void_t;
 
 template <typename T, int W, int ORDER, int MODE>
-__global__ __launch_bounds__((gather_threads<T, W>())) void k_tile_gather(const Entry<T> *__restrict__ ent, const unsigned *__restrict__ start,
+__global__ __launch_bounds__((gather_threads<T, W>()), (gather_min_waves<T, W>())) void k_tile_gather(const Entry<T> *__restrict__ ent, const unsigned *__restrict__ start,
                                                       const unsigned *__restrict__ own, TileGeom g, T sx, T sy, T sz,
                                                       T L, const T *__restrict__ phi, T c1, T c2, T c3, const T *psi_in,
                                                       T *psi_out, T *__restrict__ mom, T *__restrict__ acc, size_t np,
@@ -477,16 +481,20 @@ __global__ __launch_bounds__((gather_threads<T, W>())) void k_tile_gather(const 
     if (region_ok) {
         // one wave per x row: lane l loads dword l of the row straight into LDS (lanes past the row idle)
         const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+        constexpr int NW = NT / 64;
         const int nrows = S[1] * S[2];
         const bool active = lane < S[0] * DW;
         const int gx = wrapn(r0[0] + lane / DW, N);
         const unsigned *src0 = (const unsigned *)phi + (size_t)gx * DW + (lane % DW);
-        for (int row = wave; row < nrows; row += NT / 64) {
-            const int ly = row % S[1], lz = row / S[1];
+        // row = ly + S1 lz, advanced by NW rows per step without a division (scalar registers)
+        int ly = wave % S[1], lz = wave / S[1];
+        for (int row = wave; row < nrows; row += NW) {
             const int gy = wrapn(r0[1] + ly, N), gz = wrapn(r0[2] + lz, N);
             const size_t roff = ((size_t)gy + (size_t)N * gz) * (size_t)N * DW;
             if (active)
                 __builtin_amdgcn_global_load_lds((glb_void_t *)(src0 + roff), (lds_void_t *)(ph + (size_t)row * SX), 4, 0, 0);
+            ly += NW;
+            while (ly >= S[1]) { ly -= S[1]; lz++; }
         }
         __builtin_amdgcn_s_waitcnt(0);
     }
