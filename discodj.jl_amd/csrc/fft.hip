@@ -36,6 +36,9 @@ __device__ __forceinline__ const cx<T> *stage_twiddles(const cx<T> *__restrict__
     }
 }
 
+// Each kernel runs its line FFTs wave by wave (run_wave) when the engine's pass shapes allow it,
+// with workgroup barriers only around the twiddle table and the transposed (strided) side of a pass;
+// otherwise every FFT pass ends with a workgroup barrier (run).
 template <typename T, int N, typename IN>
 __global__ __launch_bounds__(256) void k_xf(const IN *__restrict__ mesh, cx<T> *__restrict__ A,
                                             const cx<T> *__restrict__ tw, double fixed_scale) {
@@ -43,9 +46,16 @@ __global__ __launch_bounds__(256) void k_xf(const IN *__restrict__ mesh, cx<T> *
     using P = typename S::P;
     __shared__ cx<T> lds[P::EX::LDS_ELEMS];
     const int yb = blockIdx.x % (N / S::B), z = blockIdx.x / (N / S::B);
-    P::xf_load(mesh, lds, yb, z, threadIdx.x, fixed_scale);
-    __syncthreads();
-    P::EX::template run<-1, 2>(lds, tw, threadIdx.x);
+    if constexpr (P::EX::wave_ok()) {
+        P::xf_load_w(mesh, lds, yb, z, threadIdx.x, fixed_scale);
+        P::EX::wave_sync();
+        P::EX::template run_wave<-1, 2>(lds, tw, threadIdx.x);
+        __syncthreads();
+    } else {
+        P::xf_load(mesh, lds, yb, z, threadIdx.x, fixed_scale);
+        __syncthreads();
+        P::EX::template run<-1, 2>(lds, tw, threadIdx.x);
+    }
     P::xf_store(lds, tw, A, yb, z, threadIdx.x);
 }
 
@@ -58,10 +68,17 @@ __global__ __launch_bounds__(256) void k_xi(const cx<T> *__restrict__ A, T *__re
     const int yb = blockIdx.x % (N / S::B), z = blockIdx.x / (N / S::B);
     P::xi_load(A, lds, yb, z, threadIdx.x);
     __syncthreads();
-    P::xi_pre(lds, tw, threadIdx.x);
-    __syncthreads();
-    P::EX::template run<+1, 2>(lds, tw, threadIdx.x);
-    P::xi_store(lds, mesh, yb, z, threadIdx.x);
+    if constexpr (P::EX::wave_ok()) {
+        P::xi_pre_w(lds, tw, threadIdx.x);
+        P::EX::wave_sync();
+        P::EX::template run_wave<+1, 2>(lds, tw, threadIdx.x);
+        P::xi_store_w(lds, mesh, yb, z, threadIdx.x);
+    } else {
+        P::xi_pre(lds, tw, threadIdx.x);
+        __syncthreads();
+        P::EX::template run<+1, 2>(lds, tw, threadIdx.x);
+        P::xi_store(lds, mesh, yb, z, threadIdx.x);
+    }
 }
 
 template <typename T, int N>
@@ -73,9 +90,16 @@ __global__ __launch_bounds__((FftShape<T, N>::NTY)) void k_yf(const cx<T> *__res
     __shared__ cx<T> twl[S::TWL ? N : 1];
     const int zb = blockIdx.x % (N / S::B), kx = blockIdx.x / (N / S::B);
     const cx<T> *t = stage_twiddles<T, N, S::TWL>(tw, twl, threadIdx.x, S::NTY);
-    P::y_load_a(A, lds, zb, kx, threadIdx.x);
-    __syncthreads();
-    P::EF::template run<-1, 1>(lds, t, threadIdx.x);
+    if constexpr (P::EF::wave_ok()) {
+        P::y_load_a_w(A, lds, zb, kx, threadIdx.x);
+        __syncthreads();
+        P::EF::template run_wave<-1, 1>(lds, t, threadIdx.x);
+        __syncthreads();
+    } else {
+        P::y_load_a(A, lds, zb, kx, threadIdx.x);
+        __syncthreads();
+        P::EF::template run<-1, 1>(lds, t, threadIdx.x);
+    }
     P::y_store_b(lds, Bm, zb, kx, threadIdx.x);
 }
 
@@ -90,8 +114,13 @@ __global__ __launch_bounds__((FftShape<T, N>::NTY)) void k_yi(const cx<T> *__res
     const cx<T> *t = stage_twiddles<T, N, S::TWL>(tw, twl, threadIdx.x, S::NTY);
     P::y_load_b(Bm, lds, zb, kx, threadIdx.x);
     __syncthreads();
-    P::EF::template run<+1, 1>(lds, t, threadIdx.x);
-    P::y_store_a(lds, A, zb, kx, threadIdx.x);
+    if constexpr (P::EF::wave_ok()) {
+        P::EF::template run_wave<+1, 1>(lds, t, threadIdx.x);
+        P::y_store_a_w(lds, A, zb, kx, threadIdx.x);
+    } else {
+        P::EF::template run<+1, 1>(lds, t, threadIdx.x);
+        P::y_store_a(lds, A, zb, kx, threadIdx.x);
+    }
 }
 
 // z lines of B: optional forward transform, multiplier, optional inverse; in may equal out
@@ -104,15 +133,27 @@ __global__ __launch_bounds__((FftShape<T, N>::NTY)) void k_zm(const cx<T> *in, c
     __shared__ cx<T> twl[S::TWL ? N : 1];
     const int b = blockIdx.x;
     const cx<T> *t = stage_twiddles<T, N, S::TWL>(tw, twl, threadIdx.x, S::NTY);
-    P::z_load(in, lds, b, threadIdx.x);
-    __syncthreads();
-    if constexpr (FWD) P::EF::template run<-1, 1>(lds, t, threadIdx.x);
-    if constexpr (MODE != ZM_NONE) {
-        P::template z_mult<MODE>(lds, K, b, threadIdx.x);
+    if constexpr (P::EF::wave_ok()) {
+        P::z_load_w(in, lds, b, threadIdx.x);
+        __syncthreads();  // the twiddle table; from here on every wave works on its own lines
+        if constexpr (FWD) P::EF::template run_wave<-1, 1>(lds, t, threadIdx.x);
+        if constexpr (MODE != ZM_NONE) {
+            P::template z_mult_w<MODE>(lds, K, b, threadIdx.x);
+            P::EF::wave_sync();
+        }
+        if constexpr (INV) P::EF::template run_wave<+1, 1>(lds, t, threadIdx.x);
+        P::z_store_w(lds, out, b, threadIdx.x);
+    } else {
+        P::z_load(in, lds, b, threadIdx.x);
         __syncthreads();
+        if constexpr (FWD) P::EF::template run<-1, 1>(lds, t, threadIdx.x);
+        if constexpr (MODE != ZM_NONE) {
+            P::template z_mult<MODE>(lds, K, b, threadIdx.x);
+            __syncthreads();
+        }
+        if constexpr (INV) P::EF::template run<+1, 1>(lds, t, threadIdx.x);
+        P::z_store(lds, out, b, threadIdx.x);
     }
-    if constexpr (INV) P::EF::template run<+1, 1>(lds, t, threadIdx.x);
-    P::z_store(lds, out, b, threadIdx.x);
 }
 
 template <typename T, int N>

@@ -52,6 +52,24 @@ struct KMult {
 
 enum { ZM_POT = 0, ZM_GRADX = 1, ZM_GRADY = 2, ZM_GRADZ = 3, ZM_NONE = 4 };
 
+// Wave-local tile copy: wave w moves elements [w E, (w + 1) E) of a workgroup's tile (E = the wave's
+// lines times the line length), 64 consecutive ones per step, all reads before the first write.
+template <typename V, int E, typename SRC, typename DST>
+DJ_HD void copy_wave(int wave, int lane, const SRC &src, const DST &dst) {
+    constexpr int CNT = (E + 63) / 64;
+    V v[CNT];
+#pragma unroll
+    for (int c = 0; c < CNT; c++) {
+        const int i = lane + 64 * c;
+        if (E % 64 == 0 || i < E) v[c] = src(wave * E + i);
+    }
+#pragma unroll
+    for (int c = 0; c < CNT; c++) {
+        const int i = lane + 64 * c;
+        if (E % 64 == 0 || i < E) dst(wave * E + i, v[c]);
+    }
+}
+
 // Tile copy in two phases: a thread issues all of its TOTAL / NT reads (src(i)) before its first
 // write (dst(i, v)), so the memory latencies overlap instead of adding up per element.
 template <typename V, int TOTAL, int NT, typename SRC, typename DST>
@@ -199,6 +217,23 @@ struct Passes {
             tid, [&](int i) { return src[i]; },
             [&](int i, cx<T> v) { const int l = i / N, zz = i - l * N; lds[EF::idx(l, zz)] = v; });
     }
+    // multiplier of element i (line l = i / N, kz = i % N) of block b's tile
+    template <int MODE>
+    DJ_HD static void z_mult_one(cx<T> *lds, const KMult<T> &K, int b, int i) {
+        const int l = i / N, kz = i - l * N;
+        const int L = b * B + l, ky = L / H, kx = L - ky * H;
+        cx<T> v = lds[EF::idx(l, kz)];
+        const T kp = K.kphi(kx, ky, kz);
+        if constexpr (MODE == ZM_POT) {
+            v = scale(v, kp);
+        } else if constexpr (MODE == ZM_GRADX || MODE == ZM_GRADY || MODE == ZM_GRADZ) {
+            const cx<T> g = MODE == ZM_GRADX ? K.gh[kx] : MODE == ZM_GRADY ? K.gf[ky] : K.gf[kz];
+            const cx<T> gk = scale(g, kp);
+            const cx<T> r = {gk.x * v.x - gk.y * v.y, gk.x * v.y + gk.y * v.x};
+            v = {-r.x, -r.y};
+        }
+        lds[EF::idx(l, kz)] = v;
+    }
     template <int MODE>
     DJ_HD static void z_mult(cx<T> *lds, const KMult<T> &K, int b, int tid) {
         // fixed trip count, unrolled: the multiplier-table reads of all iterations are in flight together
@@ -206,20 +241,7 @@ struct Passes {
 #pragma unroll
         for (int c = 0; c < CNT; c++) {
             const int i = tid + c * NT;
-            if (!((B * N) % NT == 0 || i < B * N)) continue;
-            int l = i / N, kz = i - l * N;
-            int L = b * B + l, ky = L / H, kx = L - ky * H;
-            cx<T> v = lds[EF::idx(l, kz)];
-            T kp = K.kphi(kx, ky, kz);
-            if constexpr (MODE == ZM_POT) {
-                v = scale(v, kp);
-            } else if constexpr (MODE == ZM_GRADX || MODE == ZM_GRADY || MODE == ZM_GRADZ) {
-                cx<T> g = MODE == ZM_GRADX ? K.gh[kx] : MODE == ZM_GRADY ? K.gf[ky] : K.gf[kz];
-                cx<T> gk = scale(g, kp);
-                cx<T> r = {gk.x * v.x - gk.y * v.y, gk.x * v.y + gk.y * v.x};
-                v = {-r.x, -r.y};
-            }
-            lds[EF::idx(l, kz)] = v;
+            if ((B * N) % NT == 0 || i < B * N) z_mult_one<MODE>(lds, K, b, i);
         }
     }
     DJ_HD static void z_store(const cx<T> *lds, cx<T> *Bm, int b, int tid) {
@@ -227,6 +249,98 @@ struct Passes {
         copy_batched<cx<T>, B * N, NT>(
             tid, [&](int i) { const int l = i / N, zz = i - l * N; return lds[EF::idx(l, zz)]; },
             [&](int i, cx<T> v) { dst[i] = v; });
+    }
+
+    // ------------------------------------------------------------------ wave-local variants
+    // The same element maps as above for the line-contiguous sides of each pass, split so that
+    // every wave touches only the lines it transforms (run_wave); tid = 64 wave + lane.
+    template <typename IN>
+    DJ_HD static void xf_load_w(const IN *mesh, cx<T> *lds, int yb, int z, int tid, double fixed_scale) {
+        const size_t base = (size_t)N * N * z + (size_t)N * yb * B;
+        copy_wave<cx<T>, EX::LW * NC>(
+            tid >> 6, tid & 63,
+            [&](int i) {
+                const int l = i / NC, m = i - l * NC;
+                const size_t g = base + (size_t)N * l + 2 * m;
+                if constexpr (sizeof(IN) == 8 && !std::is_floating_point<IN>::value)
+                    return cx<T>{(T)((double)mesh[g] * fixed_scale), (T)((double)mesh[g + 1] * fixed_scale)};
+                else
+                    return cx<T>{(T)mesh[g], (T)mesh[g + 1]};
+            },
+            [&](int i, cx<T> v) {
+                const int l = i / NC, m = i - l * NC;
+                lds[EX::idx(l, m)] = v;
+            });
+    }
+    DJ_HD static void xi_pre_w(cx<T> *lds, const cx<T> *twN, int tid) {
+        constexpr int HP = NC / 2 + 1, E = EX::LW * HP, CNT = (E + 63) / 64;
+        const int wave = tid >> 6, lane = tid & 63;
+#pragma unroll
+        for (int c = 0; c < CNT; c++) {
+            const int il = lane + 64 * c;
+            if (!(E % 64 == 0 || il < E)) continue;
+            const int i = wave * E + il;
+            const int l = i / HP, k = i - l * HP;
+            cx<T> a = lds[EX::idx(l, k)], b = lds[EX::idx(l, NC - k)];
+            if (k == 0) { a.y = 0; b.y = 0; }
+            cx<T> e = a + conj(b);
+            cx<T> o = cmulc(a - conj(b), twN[k]);
+            cx<T> zk = {e.x - o.y, e.y + o.x};
+            if (k != 0 && 2 * k != NC) {
+                cx<T> e2 = b + conj(a);
+                cx<T> o2 = cmulc(b - conj(a), twN[NC - k]);
+                lds[EX::idx(l, NC - k)] = {e2.x - o2.y, e2.y + o2.x};
+            }
+            lds[EX::idx(l, k)] = zk;
+        }
+    }
+    DJ_HD static void xi_store_w(const cx<T> *lds, T *mesh, int yb, int z, int tid) {
+        const size_t base = (size_t)N * N * z + (size_t)N * yb * B;
+        copy_wave<cx<T>, EX::LW * NC>(
+            tid >> 6, tid & 63,
+            [&](int i) {
+                const int l = i / NC, m = i - l * NC;
+                return lds[EX::idx(l, m)];
+            },
+            [&](int i, cx<T> v) {
+                const int l = i / NC, m = i - l * NC;
+                const size_t g = base + (size_t)N * l + 2 * m;
+                mesh[g] = v.x;
+                mesh[g + 1] = v.y;
+            });
+    }
+    DJ_HD static void y_load_a_w(const cx<T> *A, cx<T> *lds, int zb, int kx, int tid) {
+        copy_wave<cx<T>, EF::LW * N>(
+            tid >> 6, tid & 63, [&](int i) { const int l = i / N, y = i - l * N; return A[a_at(y, kx, zb * B + l)]; },
+            [&](int i, cx<T> v) { const int l = i / N, y = i - l * N; lds[EF::idx(l, y)] = v; });
+    }
+    DJ_HD static void y_store_a_w(const cx<T> *lds, cx<T> *A, int zb, int kx, int tid) {
+        copy_wave<cx<T>, EF::LW * N>(
+            tid >> 6, tid & 63, [&](int i) { const int l = i / N, y = i - l * N; return lds[EF::idx(l, y)]; },
+            [&](int i, cx<T> v) { const int l = i / N, y = i - l * N; A[a_at(y, kx, zb * B + l)] = v; });
+    }
+    DJ_HD static void z_load_w(const cx<T> *Bm, cx<T> *lds, int b, int tid) {
+        const cx<T> *src = Bm + (size_t)N * B * b;
+        copy_wave<cx<T>, EF::LW * N>(
+            tid >> 6, tid & 63, [&](int i) { return src[i]; },
+            [&](int i, cx<T> v) { const int l = i / N, zz = i - l * N; lds[EF::idx(l, zz)] = v; });
+    }
+    DJ_HD static void z_store_w(const cx<T> *lds, cx<T> *Bm, int b, int tid) {
+        cx<T> *dst = Bm + (size_t)N * B * b;
+        copy_wave<cx<T>, EF::LW * N>(
+            tid >> 6, tid & 63, [&](int i) { const int l = i / N, zz = i - l * N; return lds[EF::idx(l, zz)]; },
+            [&](int i, cx<T> v) { dst[i] = v; });
+    }
+    template <int MODE>
+    DJ_HD static void z_mult_w(cx<T> *lds, const KMult<T> &K, int b, int tid) {
+        constexpr int E = EF::LW * N, CNT = (E + 63) / 64;
+        const int wave = tid >> 6, lane = tid & 63;
+#pragma unroll
+        for (int c = 0; c < CNT; c++) {
+            const int il = lane + 64 * c;
+            if (!(E % 64 == 0 || il < E)) continue;
+            z_mult_one<MODE>(lds, K, b, wave * E + il);
+        }
     }
 
     static constexpr int XBLOCKS() { return (N / B) * N; }

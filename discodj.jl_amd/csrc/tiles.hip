@@ -46,17 +46,21 @@ __device__ __forceinline__ int timod(int a, int n) {
     int r = a % n;
     return r < 0 ? r + n : r;
 }
+// 1-D assignment weights (pm.jl:49-51).  Every piece is evaluated with the reference's operations
+// and the applicable one selected, so the code has no branches and the values are the same bits.
 template <int W, typename T> __device__ __forceinline__ T twk(T d) {
     if constexpr (W == 2) {
-        return d > (T)1 ? (T)0 : (T)1 - d;
+        const T a = (T)1 - d;
+        return d > (T)1 ? (T)0 : a;
     } else if constexpr (W == 3) {
-        if (d > (T)1.5) return (T)0;
-        if (d > (T)0.5) { T t = (T)1.5 - d; return (T)0.5 * (t * t); }
-        return (T)0.75 - d * d;
+        const T t = (T)1.5 - d;
+        const T outer = (T)0.5 * (t * t), inner = (T)0.75 - d * d;
+        return d > (T)1.5 ? (T)0 : (d > (T)0.5 ? outer : inner);
     } else {
-        if (d > (T)2) return (T)0;
-        if (d > (T)1) { T t = (T)2 - d; return t * t * t / (T)6; }
-        return ((T)4 - (T)6 * (d * d) + (T)3 * (d * d * d)) / (T)6;
+        const T t = (T)2 - d;
+        const T outer = t * t * t / (T)6;
+        const T inner = ((T)4 - (T)6 * (d * d) + (T)3 * (d * d * d)) / (T)6;
+        return d > (T)2 ? (T)0 : (d > (T)1 ? outer : inner);
     }
 }
 
@@ -78,40 +82,41 @@ __device__ __forceinline__ int hash_slot(unsigned *keys, unsigned key) {
     return -1;  // table full: the caller falls back to a global atomic
 }
 
-// tiles of one particle: own first, then distinct ghosts; returns the count (<= 9)
+// tiles of one particle: own first, then the distinct ghosts; returns the count (<= 8).  The stencil
+// cells of every shift along an axis lie within c0 + [rlo, rhi], a span shorter than the tile, so the
+// axis touches at most the two tiles {tile(c0 + rlo), tile(c0 + rhi)}, one of which is the own tile;
+// the ghosts are the other combinations of those per-axis tiles.  The cells are computed with the
+// arithmetic of the deposit (same X, same shift, same base rule).
 template <typename T, int W>
 __device__ __forceinline__ int particle_tiles(const T X[3], const TileGeom &g, const double *shifts, int ns, T L,
                                               unsigned *tiles) {
     int tl[3], th[3], to[3];
-    const int Td[3] = {g.TX, g.TY, g.TZ}, nt[3] = {g.ntx, g.nty, g.ntz};
     for (int a = 0; a < 3; a++) {
-        T g0 = tjmod(X[a], L) / L * (T)g.N;
-        int c0 = wrapn((int)floor(g0), g.N);
+        const T g0 = tjmod(X[a], L) / L * (T)g.N;
+        const int c0 = wrapn((int)floor(g0), g.N);
         int rlo = 1 << 20, rhi = -(1 << 20);
         for (int s = 0; s < ns; s++) {
-            T xs = X[a] + (T)shifts[3 * s + a];
-            T gs = tjmod(xs, L) / L * (T)g.N;
-            int b = twbase<W>(gs);
-            int d = wrapn(b - c0 + g.N / 2, g.N) - g.N / 2;  // b in [0, N], c0 in [0, N)
+            const T sh = (T)shifts[3 * s + a];
+            // X + 0 == X: the unshifted pass reuses g0
+            const T gs = sh == (T)0 ? g0 : tjmod(X[a] + sh, L) / L * (T)g.N;
+            const int b = twbase<W>(gs);
+            const int d = wrapn(b - c0 + g.N / 2, g.N) - g.N / 2;  // b in [0, N], c0 in [0, N)
             rlo = min(rlo, d + twlo<W>());
             rhi = max(rhi, d + twlo<W>() + W - 1);
         }
         to[a] = c0 >> g.lT[a];
         tl[a] = wrapn(c0 + rlo, g.N) >> g.lT[a];
         th[a] = wrapn(c0 + rhi, g.N) >> g.lT[a];
-        (void)Td;
-        (void)nt;
     }
     int n = 0;
     tiles[n++] = (unsigned)(to[0] + g.ntx * (to[1] + g.nty * to[2]));
-    for (int cz = 0; cz < 2; cz++)
-        for (int cy = 0; cy < 2; cy++)
-            for (int cx = 0; cx < 2; cx++) {
-                int tx = cx ? th[0] : tl[0], ty = cy ? th[1] : tl[1], tz = cz ? th[2] : tl[2];
-                unsigned t = (unsigned)(tx + g.ntx * (ty + g.nty * tz));
-                bool dup = false;
-                for (int m = 0; m < n; m++) dup |= tiles[m] == t;
-                if (!dup) tiles[n++] = t;
+    const int nz = th[2] != tl[2] ? 2 : 1, ny = th[1] != tl[1] ? 2 : 1, nx = th[0] != tl[0] ? 2 : 1;
+    for (int cz = 0; cz < nz; cz++)
+        for (int cy = 0; cy < ny; cy++)
+            for (int cx = 0; cx < nx; cx++) {
+                const int tx = cx ? th[0] : tl[0], ty = cy ? th[1] : tl[1], tz = cz ? th[2] : tl[2];
+                if (tx == to[0] && ty == to[1] && tz == to[2]) continue;
+                tiles[n++] = (unsigned)(tx + g.ntx * (ty + g.nty * tz));
             }
     return n;
 }
@@ -128,8 +133,8 @@ __global__ __launch_bounds__(256) void k_tile_bin(const T *__restrict__ psi, con
     const int i = blockIdx.x * 16 + (threadIdx.x & 15), j = blockIdx.y * 4 + ((threadIdx.x >> 4) & 3),
               k = blockIdx.z * 4 + (threadIdx.x >> 6);
     const bool ok = i < n && j < n && k < n;
-    unsigned tiles[9];
-    int slot[9], rank[9], nt = 0;
+    unsigned tiles[8];
+    int slot[8], rank[8], nt = 0;
     T X[3];
     size_t p = 0;
     if (ok) {
@@ -147,7 +152,7 @@ __global__ __launch_bounds__(256) void k_tile_bin(const T *__restrict__ psi, con
     const int lane = __lane_id();
     const unsigned long long below = (1ull << lane) - 1ull;
 #pragma unroll
-    for (int m = 0; m < 9; m++) {
+    for (int m = 0; m < 8; m++) {
         const bool act = m < nt;
         const unsigned key = act ? tiles[m] * 2u + (m > 0 ? 1u : 0u) : EMPTY;
         unsigned long long pend = __ballot(act);
@@ -274,6 +279,11 @@ __global__ __launch_bounds__(256) void k_tile_deposit(const Entry<T> *__restrict
                 loc[a][o] = (l >= 0 && l < T3[a]) ? l : -1;
             }
         }
+        if constexpr (ACC == ACC_Q31) {
+            // x weights scaled by 2^31 (exact); one w_y w_z product per row
+#pragma unroll
+            for (int ix = 0; ix < W; ix++) w[0][ix] *= (T)2147483648.0f;
+        }
 #pragma unroll
         for (int iz = 0; iz < W; iz++) {
             if (loc[2][iz] < 0) continue;
@@ -281,15 +291,18 @@ __global__ __launch_bounds__(256) void k_tile_deposit(const Entry<T> *__restrict
             for (int iy = 0; iy < W; iy++) {
                 if (loc[1][iy] < 0) continue;
                 const int row = g.TX * (loc[1][iy] + g.TY * loc[2][iz]);
+                const T wyz = w[1][iy] * w[2][iz];
 #pragma unroll
                 for (int ix = 0; ix < W; ix++) {
                     if (loc[0][ix] < 0) continue;
-                    T wv = w[0][ix] * w[1][iy] * w[2][iz];
                     if constexpr (ACC == ACC_FIXED) {
+                        // the oracle's product order, (wx wy) wz, then the fixed-point scale in FP64
+                        const T wv = w[0][ix] * w[1][iy] * w[2][iz];
                         atomicAdd(&acc[row + loc[0][ix]], (unsigned long long)__double2ll_rn((double)wv * fscale));
                     } else if constexpr (ACC == ACC_Q31) {
-                        atomicAdd(&acc[row + loc[0][ix]], (unsigned long long)(unsigned)rintf(wv * 2147483648.0f));
+                        atomicAdd(&acc[row + loc[0][ix]], (unsigned long long)(unsigned)rintf(w[0][ix] * wyz));
                     } else {
+                        const T wv = w[0][ix] * w[1][iy] * w[2][iz];
                         atomicAdd(&acc[row + loc[0][ix]], wv);
                     }
                 }
@@ -426,11 +439,11 @@ __device__ __forceinline__ void gather_fetch(const Entry<T> *__restrict__ ent, u
     en = ent[e];
     const size_t p = (size_t)en.idx;
     if constexpr (MODE == GATHER_KICK) {
+        // acc (the earlier passes' sum) is read after the stencil, to keep registers for it
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             m3[c] = mom[p + c * np];
             p3[c] = psi_in[p + c * np];
-            if (ns > 1) a3[c] = acc[p + c * np];
         }
     } else if constexpr (MODE == GATHER_ACC) {
 #pragma unroll
@@ -454,13 +467,46 @@ __host__ __device__ __forceinline__ bool gather_staged(const TileGeom &g, int H)
 typedef __attribute__((address_space(3))) void lds_void_t;
 typedef __attribute__((address_space(1))) void glb_void_t;
 
+// Staging of the region (S0 x S1 x S2 cells, x rows contiguous in LDS at stride S0) straight from
+// global memory into LDS.  global_load_lds writes each active lane's dword at the wave's LDS base
+// + 4 lane, so the region is read as a flat array of dwords, 64 consecutive ones per wave
+// instruction (rows may change within an instruction); each lane tracks its (column, row, plane)
+// by increments and wraps its own source cell periodically.  The address arithmetic is per lane
+// (vector ALU), which keeps the CU's single scalar unit free.
+template <typename T, int NT>
+__device__ __forceinline__ void stage_flat(const T *phi, T *ph, const int (&r0)[3], const int (&S)[3], int N, int lane,
+                                           int wave) {
+    constexpr int NW = NT / 64, DW = sizeof(T) / 4;
+    const int RD = S[0] * DW;                  // dwords per row
+    const int total = RD * S[1] * S[2];
+    int f = wave * 64 + lane;                  // this lane's dword of the current instruction
+    int row = f / RD, cd = f - row * RD;       // dword column within the row
+    int ly = row % S[1], lz = row / S[1];
+    const int step = NW * 64, dcd = step % RD, drow = step / RD;
+    const unsigned *src0 = (const unsigned *)phi;
+    unsigned *dst0 = (unsigned *)ph;
+    for (int c = wave; c * 64 < total; c += NW) {
+        if (f < total) {
+            const int gx = wrapn(r0[0] + cd / DW, N), gy = wrapn(r0[1] + ly, N), gz = wrapn(r0[2] + lz, N);
+            const unsigned *src = src0 + ((size_t)gx + (size_t)N * ((size_t)gy + (size_t)N * gz)) * DW + (cd % DW);
+            __builtin_amdgcn_global_load_lds((glb_void_t *)src, (lds_void_t *)(dst0 + (size_t)c * 64), 4, 0, 0);
+        }
+        f += step;
+        cd += dcd;
+        int dr = drow;
+        if (cd >= RD) { cd -= RD; dr++; }
+        ly += dr;
+        while (ly >= S[1]) { ly -= S[1]; lz++; }
+    }
+}
+
 template <typename T, int W, int ORDER, int MODE>
 __global__ __launch_bounds__((gather_threads<T, W>()), (gather_min_waves<T, W>())) void k_tile_gather(const Entry<T> *__restrict__ ent, const unsigned *__restrict__ start,
                                                       const unsigned *__restrict__ own, TileGeom g, T sx, T sy, T sz,
                                                       T L, const T *__restrict__ phi, T c1, T c2, T c3, const T *psi_in,
                                                       T *psi_out, T *__restrict__ mom, T *__restrict__ acc, size_t np,
                                                       T alpha, T beta, T dd2, T dd1n, int has_next, int ns) {
-    constexpr int H = ORDER / 2, Q = W + 2 * H, NT = gather_threads<T, W>(), DW = sizeof(T) / 4;
+    constexpr int H = ORDER / 2, Q = W + 2 * H, NT = gather_threads<T, W>();
     extern __shared__ unsigned char smem[];
     T *ph = (T *)smem;
     const int t = xcd_tile(blockIdx.x, g.nt);
@@ -479,23 +525,8 @@ __global__ __launch_bounds__((gather_threads<T, W>()), (gather_min_waves<T, W>()
     if (e < e1) gather_fetch<T, MODE>(ent, e, psi_in, mom, acc, np, ns, en, m3, p3, a3);
 
     if (region_ok) {
-        // one wave per x row: lane l loads dword l of the row straight into LDS (lanes past the row idle)
         const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-        constexpr int NW = NT / 64;
-        const int nrows = S[1] * S[2];
-        const bool active = lane < S[0] * DW;
-        const int gx = wrapn(r0[0] + lane / DW, N);
-        const unsigned *src0 = (const unsigned *)phi + (size_t)gx * DW + (lane % DW);
-        // row = ly + S1 lz, advanced by NW rows per step without a division (scalar registers)
-        int ly = wave % S[1], lz = wave / S[1];
-        for (int row = wave; row < nrows; row += NW) {
-            const int gy = wrapn(r0[1] + ly, N), gz = wrapn(r0[2] + lz, N);
-            const size_t roff = ((size_t)gy + (size_t)N * gz) * (size_t)N * DW;
-            if (active)
-                __builtin_amdgcn_global_load_lds((glb_void_t *)(src0 + roff), (lds_void_t *)(ph + (size_t)row * SX), 4, 0, 0);
-            ly += NW;
-            while (ly >= S[1]) { ly -= S[1]; lz++; }
-        }
+        stage_flat<T, NT>(phi, ph, r0, S, N, lane, wave);
         __builtin_amdgcn_s_waitcnt(0);
     }
     __syncthreads();
@@ -540,7 +571,7 @@ __global__ __launch_bounds__((gather_threads<T, W>()), (gather_min_waves<T, W>()
 #pragma unroll
             for (int c = 0; c < 3; c++) {
                 T ac = f3[c];
-                if (ns > 1) ac = (a3[c] + ac) / (T)ns;
+                if (ns > 1) ac = (acc[p + c * np] + ac) / (T)ns;
                 T m = alpha * m3[c] + beta * ac;
                 T s = p3[c] + dd2 * m;
                 if (has_next) s = s + dd1n * m;

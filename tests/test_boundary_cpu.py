@@ -106,29 +106,34 @@ def _emu():
     return _EMU
 
 
-@pytest.mark.parametrize("N", [8, 12, 16, 24, 32, 48, 64, 96])
+@pytest.mark.parametrize("N", [8, 12, 16, 24, 32, 48, 64, 96, -64])
 @pytest.mark.parametrize("prec", [32, 64])
 def test_fft_pass_kernels_emulated(N, prec):
     """The device kernels' phase code run workgroup by workgroup on the host: the x-halved B-layout
-    spectrum equals numpy's rfftn, and the inverse passes return the field."""
+    spectrum equals numpy's rfftn, and the inverse passes return the field.  N = -64 runs the
+    wave-local schedule (each wave transforms only its own lines) on a 64^3 mesh."""
+    code = N
+    N = abs(N)
     dt, ct = (np.float64, np.complex128) if prec == 64 else (np.float32, np.complex64)
     H = N // 2 + 1
     x = np.random.default_rng(N).normal(size=(N, N, N)).astype(dt)
     B = np.zeros(N * H * N, ct)
-    assert _emu().emu_fft(N, prec, 0, x.ctypes.data, B.ctypes.data, None, 0, 1.0, 1.0) == 0
+    assert _emu().emu_fft(code, prec, 0, x.ctypes.data, B.ctypes.data, None, 0, 1.0, 1.0) == 0
     ref = np.fft.rfftn(x.astype(np.float64), axes=(0, 1, 2))
     got = B.reshape(N, H, N).transpose(2, 0, 1)
     tol = 1e-14 if prec == 64 else 1e-6
     assert np.max(np.abs(got - ref)) < tol * np.max(np.abs(ref))
     Bc = np.ascontiguousarray(ref.transpose(1, 2, 0)).astype(ct)
     out = np.zeros((N, N, N), dt)
-    _emu().emu_fft(N, prec, 1, Bc.ctypes.data, out.ctypes.data, None, 0, 1.0, 1.0)
+    assert _emu().emu_fft(code, prec, 1, Bc.ctypes.data, out.ctypes.data, None, 0, 1.0, 1.0) == 0
     assert np.max(np.abs(out / N**3 - x)) < tol * np.max(np.abs(x))
 
 
-def test_fft_poisson_emulated_vs_oracle():
-    """Poisson solve (XF, YF, ZM with K_phi, YI, XI) against the oracle's FP64 FFT and multiplier."""
-    N, L, n = 32, 100.0, 16
+@pytest.mark.parametrize("code", [32, -64])
+def test_fft_poisson_emulated_vs_oracle(code):
+    """Poisson solve (XF, YF, ZM with K_phi, YI, XI) against the oracle's FP64 FFT and multiplier, with
+    the barrier schedule (32^3) and the wave-local schedule (64^3)."""
+    N, L, n = abs(code), 100.0, 16
     H = N // 2 + 1
     x = np.random.default_rng(1).normal(size=(N, N, N))
     dk = 2 * np.pi / L
@@ -137,7 +142,7 @@ def test_fft_poisson_emulated_vs_oracle():
     ktab = np.concatenate([kf**2, kh**2, np.zeros(N), np.zeros(H)])
     out = np.zeros((N, N, N))
     rho = n**3 / N**3
-    _emu().emu_fft(N, 64, 2, x.ctypes.data, out.ctypes.data, ktab.ctypes.data, 0, rho, 1.0)
+    assert _emu().emu_fft(code, 64, 2, x.ctypes.data, out.ctypes.data, ktab.ctypes.data, 0, rho, 1.0) == 0
     # reference: brfft(rfft(x) * -1/k^2 / rho / N^3)
     F = O.rfft3(x)
     kz = kh[:, None, None]
