@@ -461,6 +461,16 @@ dj_status setup_tiles(dj_ctx *c) {
         }
     g.RL = -((int)std::floor(smin) + lo);
     g.RR = (int)std::floor(smax) + (W % 2 ? 2 : 1) + hi;
+    // The gather stages the reach that exact arithmetic allows: base cells of g in [c0 + s, c0 + 1 + s)
+    // are at most c0 + ceil(1.5 + s) - 1 (odd W, round) or c0 + ceil(1 + s) - 1 (even W, floor).
+    // A particle that rounding pushes one cell further is gathered from global memory.
+    int rs = 0;
+    for (int s = 0; s < c->nshifts; s++)
+        for (int a = 0; a < 3; a++) {
+            const double sig = c->shifts[s][a] * N / c->cfg.boxsize;
+            rs = std::max(rs, (int)std::ceil((W % 2 ? 1.5 : 1.0) + sig) - 1);
+        }
+    g.RS = std::min(g.RR, rs + hi);
     HIPCHK(c, hipMalloc(&c->tcnt, sizeof(unsigned) * 2 * g.nt));
     HIPCHK(c, hipMalloc(&c->tcur, sizeof(unsigned) * 2 * g.nt));
     HIPCHK(c, hipMalloc(&c->tstart, sizeof(unsigned) * (g.nt + 1)));

@@ -64,6 +64,7 @@ template <typename T> void part_ops(PartOps &ops);
 struct TileGeom {
     int N, TX, TY, TZ, ntx, nty, ntz, nt, RL, RR;
     int lT[3];  // log2 of the (power-of-two) tile dimensions
+    int RS;     // right reach of the gather's staged region (RR without the rounding-tie margin)
 };
 struct TileArgs {
     const void *psi;       // (3, Np) displacement

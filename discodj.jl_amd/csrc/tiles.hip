@@ -456,45 +456,62 @@ __device__ __forceinline__ void gather_fetch(const Entry<T> *__restrict__ ent, u
 // neighbouring tiles share is fetched into that XCD's L2 once.
 __device__ __forceinline__ int xcd_tile(int b, int nt) { return (nt & 7) ? b : (b >> 3) + (b & 7) * (nt >> 3); }
 
-// staged-region geometry shared by the kernel and its launcher: x rows of S0 cells, S0 * sizeof(T)
-// / 4 <= 64 dwords so that one wave loads a row, and no periodic self-overlap
+// Staged region of a tile's gather, shared by the kernel and its launcher: cells
+// [tile + RL + H left, tile + RS + H right] along each axis, the x extent widened to whole 16-byte
+// groups (4 floats, 2 doubles) that start at a multiple of the group width, so that the region is
+// staged with 16-byte LDS DMA and no group straddles the periodic x edge.  A region is staged only
+// when it is smaller than the mesh (no periodic self-overlap).
+struct StageGeom {
+    int xl;        // cells left of the tile's first x cell
+    int s[3];      // extents (x padded to whole groups)
+    int lo[3];     // cells left of the tile's first cell per axis (xl for x)
+    bool ok;
+};
 template <typename T>
-__host__ __device__ __forceinline__ bool gather_staged(const TileGeom &g, int H) {
-    const int S0 = g.TX + g.RL + g.RR + 2 * H, S1 = g.TY + g.RL + g.RR + 2 * H, S2 = g.TZ + g.RL + g.RR + 2 * H;
-    return S0 * (int)(sizeof(T) / 4) <= 64 && S0 < g.N && S1 < g.N && S2 < g.N;
+__host__ __device__ __forceinline__ StageGeom stage_geom(const TileGeom &g, int H) {
+    constexpr int CPG = 16 / (int)sizeof(T);
+    StageGeom q;
+    q.xl = ((g.RL + H + CPG - 1) / CPG) * CPG;
+    q.s[0] = ((q.xl + g.TX + g.RS + H + CPG - 1) / CPG) * CPG;
+    q.s[1] = g.TY + g.RL + g.RS + 2 * H;
+    q.s[2] = g.TZ + g.RL + g.RS + 2 * H;
+    q.lo[0] = q.xl;
+    q.lo[1] = q.lo[2] = g.RL + H;
+    q.ok = q.s[0] < g.N && q.s[1] < g.N && q.s[2] < g.N && g.N % CPG == 0;
+    return q;
 }
 
 typedef __attribute__((address_space(3))) void lds_void_t;
 typedef __attribute__((address_space(1))) void glb_void_t;
 
 // Staging of the region (S0 x S1 x S2 cells, x rows contiguous in LDS at stride S0) straight from
-// global memory into LDS.  global_load_lds writes each active lane's dword at the wave's LDS base
-// + 4 lane, so the region is read as a flat array of dwords, 64 consecutive ones per wave
-// instruction (rows may change within an instruction); each lane tracks its (column, row, plane)
-// by increments and wraps its own source cell periodically.  The address arithmetic is per lane
-// (vector ALU), which keeps the CU's single scalar unit free.
+// global memory into LDS: global_load_lds of 16 bytes writes each active lane's group at the wave's
+// LDS base + 16 lane, so the region is read as a flat array of 16-byte groups, 64 consecutive ones
+// per wave instruction (rows may change within an instruction).  Each lane tracks its (group, row,
+// plane) by increments and wraps its own source cell periodically; the arithmetic is per lane (vector
+// ALU), which keeps the CU's single scalar unit free.
 template <typename T, int NT>
-__device__ __forceinline__ void stage_flat(const T *phi, T *ph, const int (&r0)[3], const int (&S)[3], int N, int lane,
-                                           int wave) {
-    constexpr int NW = NT / 64, DW = sizeof(T) / 4;
-    const int RD = S[0] * DW;                  // dwords per row
-    const int total = RD * S[1] * S[2];
-    int f = wave * 64 + lane;                  // this lane's dword of the current instruction
-    int row = f / RD, cd = f - row * RD;       // dword column within the row
+__device__ __forceinline__ void stage_groups(const T *phi, T *ph, const int (&r0)[3], const int (&S)[3], int N, int lane,
+                                             int wave) {
+    constexpr int NW = NT / 64, CPG = 16 / (int)sizeof(T);
+    const int GR = S[0] / CPG;                 // groups per row
+    const int total = GR * S[1] * S[2];
+    int f = wave * 64 + lane;                  // this lane's group of the current instruction
+    int row = f / GR, cg = f - row * GR;
     int ly = row % S[1], lz = row / S[1];
-    const int step = NW * 64, dcd = step % RD, drow = step / RD;
-    const unsigned *src0 = (const unsigned *)phi;
+    const int step = NW * 64, dcg = step % GR, drow = step / GR;
     unsigned *dst0 = (unsigned *)ph;
     for (int c = wave; c * 64 < total; c += NW) {
         if (f < total) {
-            const int gx = wrapn(r0[0] + cd / DW, N), gy = wrapn(r0[1] + ly, N), gz = wrapn(r0[2] + lz, N);
-            const unsigned *src = src0 + ((size_t)gx + (size_t)N * ((size_t)gy + (size_t)N * gz)) * DW + (cd % DW);
-            __builtin_amdgcn_global_load_lds((glb_void_t *)src, (lds_void_t *)(dst0 + (size_t)c * 64), 4, 0, 0);
+            const unsigned gx = (unsigned)wrapn(r0[0] + cg * CPG, N), gy = (unsigned)wrapn(r0[1] + ly, N),
+                           gz = (unsigned)wrapn(r0[2] + lz, N);
+            const T *src = phi + (gx + (unsigned)N * (gy + (unsigned)N * gz));  // < N^3 <= 2^30 cells
+            __builtin_amdgcn_global_load_lds((glb_void_t *)src, (lds_void_t *)(dst0 + (size_t)c * 256), 16, 0, 0);
         }
         f += step;
-        cd += dcd;
+        cg += dcg;
         int dr = drow;
-        if (cd >= RD) { cd -= RD; dr++; }
+        if (cg >= GR) { cg -= GR; dr++; }
         ly += dr;
         while (ly >= S[1]) { ly -= S[1]; lz++; }
     }
@@ -511,11 +528,12 @@ __global__ __launch_bounds__((gather_threads<T, W>()), (gather_min_waves<T, W>()
     T *ph = (T *)smem;
     const int t = xcd_tile(blockIdx.x, g.nt);
     const int tx = t % g.ntx, ty = (t / g.ntx) % g.nty, tz = t / (g.ntx * g.nty);
-    // staged region: tile + stencil reach (RL left, RR right) + FD halo H
-    const int r0[3] = {(tx << g.lT[0]) - g.RL - H, (ty << g.lT[1]) - g.RL - H, (tz << g.lT[2]) - g.RL - H};
-    const int S[3] = {g.TX + g.RL + g.RR + 2 * H, g.TY + g.RL + g.RR + 2 * H, g.TZ + g.RL + g.RR + 2 * H};
+    // staged region: tile + stencil reach + FD halo H (stage_geom)
+    const StageGeom q = stage_geom<T>(g, H);
+    const int r0[3] = {(tx << g.lT[0]) - q.lo[0], (ty << g.lT[1]) - q.lo[1], (tz << g.lT[2]) - q.lo[2]};
+    const int S[3] = {q.s[0], q.s[1], q.s[2]};
     const int N = g.N, SX = S[0], SXY = S[0] * S[1];
-    const bool region_ok = gather_staged<T>(g, H);
+    const bool region_ok = q.ok;
 
     // this thread's first particle and its state, loaded before the staging so both are in flight together
     const unsigned e0 = start[t], e1 = e0 + own[t];
@@ -526,7 +544,7 @@ __global__ __launch_bounds__((gather_threads<T, W>()), (gather_min_waves<T, W>()
 
     if (region_ok) {
         const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-        stage_flat<T, NT>(phi, ph, r0, S, N, lane, wave);
+        stage_groups<T, NT>(phi, ph, r0, S, N, lane, wave);
         __builtin_amdgcn_s_waitcnt(0);
     }
     __syncthreads();
@@ -631,10 +649,9 @@ struct TileLaunch {
     static void gat_wom(const TileArgs &a, const double *sh, const void *phi, const double *c, const KickArgs &k,
                         void *psi_out, hipStream_t st) {
         constexpr int H = O / 2;
-        const int SX = a.g.TX + a.g.RL + a.g.RR + 2 * H, SY = a.g.TY + a.g.RL + a.g.RR + 2 * H,
-                  SZ = a.g.TZ + a.g.RL + a.g.RR + 2 * H;
+        const StageGeom q = stage_geom<T>(a.g, H);
         // a region that is not staged (as large as the mesh) is read from global memory by the kernel
-        size_t shm = gather_staged<T>(a.g, H) ? sizeof(T) * (size_t)SX * SY * SZ : sizeof(T);
+        size_t shm = q.ok ? sizeof(T) * (size_t)q.s[0] * q.s[1] * q.s[2] : sizeof(T);
         auto kern = k_tile_gather<T, W, O, M>;
         (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
         kern<<<a.g.nt, gather_threads<T, W>(), shm, st>>>((const Entry<T> *)a.ent, a.start, a.cnt, a.g, (T)sh[0], (T)sh[1], (T)sh[2],

@@ -75,10 +75,11 @@ def test_float_deposit(W):
         got = S.deposit(psi)
         again = S.deposit(psi)
     ref = O.pm_scatter(O.lagrangian_positions(psi, L).astype(np.float64), res, L, W)
-    # FP32 weights summed as integers of 2^-31 (order-independent), rounded once to float: the
-    # difference to the FP64 sum is the FP32 evaluation of the weights, a few ulp per term
+    # FP32 weights summed as integers of 2^-31 (order-independent, so two deposits agree bit for bit)
+    # and rounded once to float; the difference to the FP64 deposit is the FP32 grid coordinate
+    # g = X / L * N, whose ulp (7.6e-6 at g ~ 64) enters each weight directly
     assert np.array_equal(got, again)
-    assert np.max(np.abs(got - ref)) < 2e-6 * max(1.0, np.max(ref))
+    assert np.max(np.abs(got - ref)) < 1e-5 * max(1.0, np.max(ref))
     assert got.sum() == pytest.approx(n**3, rel=1e-6)
 
 
