@@ -29,9 +29,11 @@ namespace {
 
 thread_local std::string g_create_error;
 
-enum KernelId { K_ZERO = 0, K_SCATTER, K_XF, K_YF, K_ZM, K_YI, K_XI, K_GATHER, K_GATHER_KICK, K_KICK, K_DRIFT, K_SORT, K_COUNT };
+enum KernelId { K_ZERO = 0, K_SCATTER, K_XF, K_YF, K_ZM, K_YI, K_XI, K_GATHER, K_GATHER_KICK, K_KICK, K_DRIFT, K_SORT,
+                K_SORT_EXACT, K_COUNT };
 const char *kernel_names[K_COUNT] = {"zero", "scatter", "fft_x_r2c", "fft_y_fwd", "fft_z_mult", "fft_y_inv",
-                                     "fft_x_c2r", "gather", "gather_kick", "kick_drift", "drift", "bin"};
+                                     "fft_x_c2r", "gather", "gather_kick", "kick_drift", "drift", "bin",
+                                     "bin_recount"};
 
 struct ProfEvent {
     int id;
@@ -59,7 +61,8 @@ struct dj_ctx {
     double fdc[3] = {0, 0, 0};
     TileOps tiles{};
     TileGeom tg{};
-    unsigned *tcnt = nullptr, *tstart = nullptr, *tcur = nullptr;
+    unsigned *tcnt = nullptr, *tstart = nullptr, *tgstart = nullptr, *tcur = nullptr, *tflag = nullptr;
+    bool tlists = false;   // tcur holds the counts of a completed sort (capacities for the next one)
     void *tent = nullptr;
     size_t tent_cap = 0;   // entries
     double *d_shifts = nullptr;  // (16 + 1, 3): the interlacing shifts, then one free slot
@@ -279,7 +282,7 @@ int aa_shifts(int aa, double dh, double (*s)[3]) {
 
 void free_all(dj_ctx *c) {
     void *bufs[] = {c->psi, c->mom, c->acc, c->psi_in, c->q1, c->mesh, c->meshfix, c->A, c->B, c->C, c->tw, c->tabs,
-                    c->tcnt, c->tstart, c->tcur, c->tent, c->d_shifts};
+                    c->tcnt, c->tstart, c->tgstart, c->tcur, c->tflag, c->tent, c->d_shifts};
     // teardown: failures here have no caller to report to
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -307,7 +310,8 @@ void set_bytes(dj_ctx *c) {
     c->bytes[K_KICK] = 3 * T * np * 5;
     c->bytes[K_DRIFT] = 3 * T * np * 3;
     // binning: read psi twice (count, fill), write one entry per (particle, tile)
-    c->bytes[K_SORT] = 2 * 3 * T * np + (double)c->tiles.entry_bytes * np;
+    c->bytes[K_SORT] = 3 * T * np + (double)c->tiles.entry_bytes * np;
+    c->bytes[K_SORT_EXACT] = 2 * 3 * T * np + (double)c->tiles.entry_bytes * np;
 }
 
 double fscale_dep(const dj_pm_config &cfg) { return std::ldexp(1.0, cfg.fixed_frac_bits ? cfg.fixed_frac_bits : 32); }
@@ -326,32 +330,54 @@ TileArgs tile_args(dj_ctx *c, const void *psi, const double *d_shifts, int ns) {
     a.g = c->tg;
     a.cnt = c->tcnt;
     a.start = c->tstart;
+    a.gstart = c->tgstart;
     a.cur = c->tcur;
+    a.flag = c->tflag;
     a.ent = c->tent;
     a.ent_cap = c->tent_cap;
     return a;
 }
 
-// counting sort of the particles by mesh tile for the given shifts (device array of ns x 3)
+// Sort of the particles by mesh tile for the given shifts (device array of ns x 3).  Tile lists are
+// reserved from the previous sort's counts plus slack (particles move less than a tile per step), so
+// one fill pass writes them; a list that outgrows its reserve (or a full entry buffer) raises a flag,
+// and then the sort is repeated exactly: counting pass, lists of the exact sizes, fill.
 dj_status tile_sort(dj_ctx *c, const void *psi, const double *d_shifts, int ns, int W) {
-    Prof p(c, K_SORT);
     TileArgs a = tile_args(c, psi, d_shifts, ns);
-    c->part.zero(c->tcnt, sizeof(unsigned) * 2 * c->tg.nt, c->st);
-    c->tiles.count(a, W, c->st);
-    unsigned total = 0;
-    HIPCHK(c, hipMemcpyAsync(&total, c->tstart + c->tg.nt, sizeof(unsigned), hipMemcpyDeviceToHost, c->st));
-    HIPCHK(c, hipStreamSynchronize(c->st));
-    if (total > c->tent_cap) {
-        if (c->tent) HIPCHK(c, hipFree(c->tent));
-        c->tent = nullptr;
-        size_t cap = (size_t)total + total / 4 + 1024;
-        HIPCHK(c, hipMalloc(&c->tent, cap * c->tiles.entry_bytes));
-        c->tent_cap = cap;
-        a.ent = c->tent;
-        a.ent_cap = c->tent_cap;
+    for (int attempt = 0; attempt < 3; attempt++) {
+        const bool exact = !c->tlists || attempt > 0;
+        {
+            Prof p(c, exact ? K_SORT_EXACT : K_SORT);
+            if (exact) {
+                c->part.zero(c->tcnt, sizeof(unsigned) * 2 * c->tg.nt, c->st);
+                c->tiles.count(a, W, c->st);
+                c->tiles.scan(a, c->tcnt, 0, c->st);
+            } else {
+                c->tiles.scan(a, c->tcur, 1, c->st);
+            }
+            c->tiles.fill(a, W, c->st);
+        }
+        unsigned hv[2] = {0, 0};  // overflow flag, entries reserved
+        HIPCHK(c, hipMemcpyAsync(&hv[0], c->tflag, sizeof(unsigned), hipMemcpyDeviceToHost, c->st));
+        HIPCHK(c, hipMemcpyAsync(&hv[1], c->tstart + c->tg.nt, sizeof(unsigned), hipMemcpyDeviceToHost, c->st));
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        if (hv[0] == 0 && hv[1] <= c->tent_cap) {
+            c->tlists = true;
+            return check_launch(c, "tile binning");
+        }
+        c->tlists = false;
+        if (hv[1] > c->tent_cap) {
+            // grow the entry buffer (exact sizes follow in the next attempt)
+            if (c->tent) HIPCHK(c, hipFree(c->tent));
+            c->tent = nullptr;
+            const size_t cap = (size_t)hv[1] + hv[1] / 4 + 1024;
+            HIPCHK(c, hipMalloc(&c->tent, cap * c->tiles.entry_bytes));
+            c->tent_cap = cap;
+            a.ent = c->tent;
+            a.ent_cap = c->tent_cap;
+        }
     }
-    c->tiles.fill(a, W, c->st);
-    return check_launch(c, "tile binning");
+    return fail(c, DJ_ERR_DEVICE, "tile binning: lists did not fit after an exact recount");
 }
 
 // ------------------------------------------------------------------------ one force evaluation
@@ -474,9 +500,14 @@ dj_status setup_tiles(dj_ctx *c) {
     HIPCHK(c, hipMalloc(&c->tcnt, sizeof(unsigned) * 2 * g.nt));
     HIPCHK(c, hipMalloc(&c->tcur, sizeof(unsigned) * 2 * g.nt));
     HIPCHK(c, hipMalloc(&c->tstart, sizeof(unsigned) * (g.nt + 1)));
+    HIPCHK(c, hipMalloc(&c->tgstart, sizeof(unsigned) * g.nt));
+    HIPCHK(c, hipMalloc(&c->tflag, sizeof(unsigned)));
+    c->tlists = false;
     HIPCHK(c, hipMalloc(&c->d_shifts, sizeof(double) * 17 * 3));
     HIPCHK(c, hipMemcpy(c->d_shifts, c->shifts, sizeof(double) * 16 * 3, hipMemcpyHostToDevice));
-    c->tent_cap = (size_t)(2.0 * (double)c->np) + 1024;
+    // own + ghost entries (about 2.1 per particle for 32 x 16 x 8 tiles with TSC and interlacing) plus
+    // the per-list slack of the reserved sizes
+    c->tent_cap = (size_t)(2.8 * (double)c->np) + 64 * (size_t)g.nt + 1024;
     HIPCHK(c, hipMalloc(&c->tent, c->tent_cap * c->tiles.entry_bytes));
     return DJ_OK;
 }

@@ -75,14 +75,18 @@ struct TileArgs {
     const double *shifts;  // device (ns, 3) interlacing shifts used by the binning
     int ns;
     TileGeom g;
-    unsigned *cnt;         // (2 nt) own then ghost counts
-    unsigned *start;       // (nt + 1) first entry of each tile
-    unsigned *cur;         // (2 nt) fill cursors
-    void *ent;             // entries (X, particle index), own then ghosts per tile
+    unsigned *cnt;         // (2 nt) own then ghost counts of the counting pass
+    unsigned *start;       // (nt + 1) first entry of each tile's own list; start[nt] = entries reserved
+    unsigned *gstart;      // (nt) first entry of each tile's ghost list (own capacity after start)
+    unsigned *cur;         // (2 nt) fill cursors: after the fill, the own and ghost counts
+    unsigned *flag;        // (1) set by the fill when a list outgrows its capacity or the buffer
+    void *ent;             // entries (X, particle index)
     size_t ent_cap;        // entries the buffer holds
 };
 struct TileOps {
-    void (*count)(const TileArgs &a, int W, hipStream_t st);  // counts + scan
+    void (*count)(const TileArgs &a, int W, hipStream_t st);  // exact counts into cnt
+    // list capacities from src (2 nt counts; may be cur), plus slack when slack != 0; zeroes cur and flag
+    void (*scan)(const TileArgs &a, const unsigned *src, int slack, hipStream_t st);
     void (*fill)(const TileArgs &a, int W, hipStream_t st);
     void (*deposit)(const TileArgs &a, int W, bool fixed, const double *shift, void *mesh, double fscale,
                     hipStream_t st);

@@ -25,6 +25,12 @@ template <typename T, int N> struct FftShape {
     static constexpr bool TWL = (PY::EF::LDS_ELEMS + N) * sizeof(cx<T>) <= 80 * 1024;
 };
 
+// The hardware hands workgroup b to XCD b % 8.  Logical block (b % 8) (nb / 8) + b / 8 puts runs of
+// consecutive logical blocks on one XCD, where they run close in time: neighbouring line groups
+// whose transposed loads and stores share 128-byte lines then meet in that XCD's L2 instead of
+// fetching each line twice from HBM.
+__device__ __forceinline__ int xcd_block(int b, int nb) { return (nb & 7) ? b : (b & 7) * (nb >> 3) + (b >> 3); }
+
 // the twiddle table for a workgroup: a copy in LDS, or the global table
 template <typename T, int N, bool TWL>
 __device__ __forceinline__ const cx<T> *stage_twiddles(const cx<T> *__restrict__ tw, cx<T> *twl, int tid, int nt) {
@@ -45,7 +51,7 @@ __global__ __launch_bounds__(256) void k_xf(const IN *__restrict__ mesh, cx<T> *
     using S = FftShape<T, N>;
     using P = typename S::P;
     __shared__ cx<T> lds[P::EX::LDS_ELEMS];
-    const int yb = blockIdx.x % (N / S::B), z = blockIdx.x / (N / S::B);
+    const int lb = xcd_block(blockIdx.x, gridDim.x), yb = lb % (N / S::B), z = lb / (N / S::B);
     if constexpr (P::EX::wave_ok()) {
         P::xf_load_w(mesh, lds, yb, z, threadIdx.x, fixed_scale);
         P::EX::wave_sync();
@@ -65,7 +71,7 @@ __global__ __launch_bounds__(256) void k_xi(const cx<T> *__restrict__ A, T *__re
     using S = FftShape<T, N>;
     using P = typename S::P;
     __shared__ cx<T> lds[P::EX::LDS_ELEMS];
-    const int yb = blockIdx.x % (N / S::B), z = blockIdx.x / (N / S::B);
+    const int lb = xcd_block(blockIdx.x, gridDim.x), yb = lb % (N / S::B), z = lb / (N / S::B);
     P::xi_load(A, lds, yb, z, threadIdx.x);
     __syncthreads();
     if constexpr (P::EX::wave_ok()) {
@@ -88,7 +94,7 @@ __global__ __launch_bounds__((FftShape<T, N>::NTY)) void k_yf(const cx<T> *__res
     using P = typename S::PY;
     __shared__ cx<T> lds[P::EF::LDS_ELEMS];
     __shared__ cx<T> twl[S::TWL ? N : 1];
-    const int zb = blockIdx.x % (N / S::B), kx = blockIdx.x / (N / S::B);
+    const int lb = xcd_block(blockIdx.x, gridDim.x), zb = lb % (N / S::B), kx = lb / (N / S::B);
     const cx<T> *t = stage_twiddles<T, N, S::TWL>(tw, twl, threadIdx.x, S::NTY);
     if constexpr (P::EF::wave_ok()) {
         P::y_load_a_w(A, lds, zb, kx, threadIdx.x);
@@ -110,7 +116,7 @@ __global__ __launch_bounds__((FftShape<T, N>::NTY)) void k_yi(const cx<T> *__res
     using P = typename S::PY;
     __shared__ cx<T> lds[P::EF::LDS_ELEMS];
     __shared__ cx<T> twl[S::TWL ? N : 1];
-    const int zb = blockIdx.x % (N / S::B), kx = blockIdx.x / (N / S::B);
+    const int lb = xcd_block(blockIdx.x, gridDim.x), zb = lb % (N / S::B), kx = lb / (N / S::B);
     const cx<T> *t = stage_twiddles<T, N, S::TWL>(tw, twl, threadIdx.x, S::NTY);
     P::y_load_b(Bm, lds, zb, kx, threadIdx.x);
     __syncthreads();

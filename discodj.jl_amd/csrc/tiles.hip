@@ -124,7 +124,8 @@ __device__ __forceinline__ int particle_tiles(const T X[3], const TileGeom &g, c
 template <typename T, int W, bool FILL>
 __global__ __launch_bounds__(256) void k_tile_bin(const T *__restrict__ psi, const T *__restrict__ q1, int n,
                                                   const double *__restrict__ shifts, int ns, T L, TileGeom g,
-                                                  unsigned *cnt, const unsigned *__restrict__ start, unsigned *cur,
+                                                  unsigned *cnt, const unsigned *__restrict__ start,
+                                                  const unsigned *__restrict__ gstart, unsigned *cur, unsigned *flag,
                                                   Entry<T> *__restrict__ ent, size_t ent_cap) {
     __shared__ unsigned keys[HASH], vals[HASH], base[HASH];
     for (int s = threadIdx.x; s < HASH; s += 256) { keys[s] = EMPTY; vals[s] = 0; }
@@ -195,30 +196,39 @@ __global__ __launch_bounds__(256) void k_tile_bin(const T *__restrict__ psi, con
     __syncthreads();
     if (!ok) return;
     for (int m = 0; m < nt; m++) {
-        unsigned t = tiles[m];
-        unsigned off = m > 0 ? cnt[t] : 0u;  // ghosts follow the tile's own particles
-        unsigned r = slot[m] >= 0 ? base[slot[m]] + (unsigned)rank[m] : atomicAdd(&cur[(m > 0 ? g.nt : 0) + t], 1u);
-        Entry<T> e;
-        e.x = X[0];
-        e.y = X[1];
-        e.z = X[2];
-        e.idx = (int)p;
-        const size_t at = (size_t)start[t] + off + r;
-        if (at < ent_cap) ent[at] = e;  // always true when the counts and the fill agree
+        const unsigned t = tiles[m];
+        const unsigned r = slot[m] >= 0 ? base[slot[m]] + (unsigned)rank[m] : atomicAdd(&cur[(m > 0 ? g.nt : 0) + t], 1u);
+        // own list [start[t], gstart[t]), ghost list [gstart[t], start[t + 1])
+        const unsigned lo = m > 0 ? gstart[t] : start[t], hi = m > 0 ? start[t + 1] : gstart[t];
+        const size_t at = (size_t)lo + r;
+        if (r < hi - lo && at < ent_cap) {
+            Entry<T> e;
+            e.x = X[0];
+            e.y = X[1];
+            e.z = X[2];
+            e.idx = (int)p;
+            ent[at] = e;
+        } else {
+            atomicOr(flag, 1u);  // capacity exceeded: the host repeats the sort with exact counts
+        }
     }
 }
 
-// start[t] = sum_{u<t} (own[u] + ghost[u]); start[nt] = total.  One workgroup of 1024 threads; each
-// thread owns a contiguous chunk of tiles whose counters it reads with independent (unrolled)
-// loads, then a block scan of the chunk sums.
-__global__ __launch_bounds__(1024) void k_tile_scan(const unsigned *__restrict__ cnt, int nt, unsigned *__restrict__ start,
-                                                    unsigned *__restrict__ cur) {
+// List capacities of every tile from counts src (own, then ghost; src may be the fill cursors of
+// the previous sort), with slack c/4 + 32 per list when slack != 0: start[t] = sum_{u<t} capacities,
+// gstart[t] = start[t] + own capacity, start[nt] = total reserved.  Zeroes the cursors and the
+// overflow flag.  One workgroup of 1024 threads; each thread owns a contiguous chunk of tiles whose
+// counts it reads with independent (unrolled) loads, then a block scan of the chunk sums.
+__device__ __forceinline__ unsigned list_cap(unsigned c, int slack) { return slack ? c + c / 4 + 32 : c; }
+
+__global__ __launch_bounds__(1024) void k_tile_scan(const unsigned *src, int nt, int slack, unsigned *__restrict__ start,
+                                                    unsigned *__restrict__ gstart, unsigned *cur, unsigned *flag) {
     __shared__ unsigned long long part[1024];
     const int tid = threadIdx.x, chunk = (nt + 1023) / 1024;
     const int a = min(nt, tid * chunk), b = min(nt, a + chunk);
     unsigned long long s = 0;
 #pragma unroll 16
-    for (int t = a; t < b; t++) s += (unsigned long long)cnt[t] + cnt[nt + t];
+    for (int t = a; t < b; t++) s += (unsigned long long)list_cap(src[t], slack) + list_cap(src[nt + t], slack);
     part[tid] = s;
     __syncthreads();
     for (int off = 1; off < 1024; off <<= 1) {
@@ -228,14 +238,20 @@ __global__ __launch_bounds__(1024) void k_tile_scan(const unsigned *__restrict__
         __syncthreads();
     }
     unsigned long long run = part[tid] - s;
+    // src may be cur: each tile's counts are read before this thread clears them
 #pragma unroll 16
     for (int t = a; t < b; t++) {
+        const unsigned co = list_cap(src[t], slack), cg = list_cap(src[nt + t], slack);
         start[t] = (unsigned)run;
-        run += (unsigned long long)cnt[t] + cnt[nt + t];
+        gstart[t] = (unsigned)(run + co);
+        run += (unsigned long long)co + cg;
         cur[t] = 0;
         cur[nt + t] = 0;
     }
-    if (tid == 1023) start[nt] = (unsigned)part[1023];
+    if (tid == 1023) {
+        start[nt] = (unsigned)min(part[1023], 0xffffffffull);
+        flag[0] = part[1023] > 0xffffffffull ? 1u : 0u;
+    }
 }
 
 // ------------------------------------------------------------------------------------ deposit
@@ -251,6 +267,7 @@ enum { ACC_T = 0, ACC_Q31 = 1, ACC_FIXED = 2 };
 
 template <typename T, int W, int ACC>
 __global__ __launch_bounds__(256) void k_tile_deposit(const Entry<T> *__restrict__ ent, const unsigned *__restrict__ start,
+                                                      const unsigned *__restrict__ gstart, const unsigned *__restrict__ cnt,
                                                       TileGeom g, T sx, T sy, T sz, T L, void *mesh, double fscale) {
     using Acc = typename std::conditional<ACC == ACC_T, T, unsigned long long>::type;
     extern __shared__ unsigned char smem[];
@@ -260,9 +277,10 @@ __global__ __launch_bounds__(256) void k_tile_deposit(const Entry<T> *__restrict
     const int tx = t % g.ntx, ty = (t / g.ntx) % g.nty, tz = t / (g.ntx * g.nty);
     const int ox = tx << g.lT[0], oy = ty << g.lT[1], oz = tz << g.lT[2];
     __syncthreads();
-    const unsigned e0 = start[t], e1 = start[t + 1];
-    for (unsigned e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
-        const Entry<T> en = ent[e];
+    // the tile's own list, then its ghost list
+    const unsigned n_own = cnt[t], n_all = n_own + cnt[g.nt + t], s_own = start[t], s_gh = gstart[t] - n_own;
+    for (unsigned i = threadIdx.x; i < n_all; i += blockDim.x) {
+        const Entry<T> en = ent[i < n_own ? s_own + i : s_gh + i];
         const T xs[3] = {en.x + sx, en.y + sy, en.z + sz};
         T w[3][W];
         int loc[3][W];
@@ -608,19 +626,22 @@ struct TileLaunch {
     template <int W>
     static void bin_w(const TileArgs &a, hipStream_t st) {
         dim3 grid((a.n + 15) / 16, (a.n + 3) / 4, (a.n + 3) / 4);
-        const unsigned *cnt = a.cnt;
         k_tile_bin<T, W, false><<<grid, 256, 0, st>>>((const T *)a.psi, (const T *)a.q1, a.n, a.shifts, a.ns, (T)a.L,
-                                                      a.g, a.cnt, a.start, a.cur, (Entry<T> *)a.ent, a.ent_cap);
-        k_tile_scan<<<1, 1024, 0, st>>>(cnt, a.g.nt, a.start, a.cur);
+                                                      a.g, a.cnt, a.start, a.gstart, a.cur, a.flag, (Entry<T> *)a.ent,
+                                                      a.ent_cap);
     }
     template <int W>
     static void fill_w(const TileArgs &a, hipStream_t st) {
         dim3 grid((a.n + 15) / 16, (a.n + 3) / 4, (a.n + 3) / 4);
         k_tile_bin<T, W, true><<<grid, 256, 0, st>>>((const T *)a.psi, (const T *)a.q1, a.n, a.shifts, a.ns, (T)a.L,
-                                                     a.g, a.cnt, a.start, a.cur, (Entry<T> *)a.ent, a.ent_cap);
+                                                     a.g, a.cnt, a.start, a.gstart, a.cur, a.flag, (Entry<T> *)a.ent,
+                                                     a.ent_cap);
     }
     static void count(const TileArgs &a, int W, hipStream_t st) {
         if (W == 2) bin_w<2>(a, st); else if (W == 3) bin_w<3>(a, st); else bin_w<4>(a, st);
+    }
+    static void scan(const TileArgs &a, const unsigned *src, int slack, hipStream_t st) {
+        k_tile_scan<<<1, 1024, 0, st>>>(src, a.g.nt, slack, a.start, a.gstart, a.cur, a.flag);
     }
     static void fill(const TileArgs &a, int W, hipStream_t st) {
         if (W == 2) fill_w<2>(a, st); else if (W == 3) fill_w<3>(a, st); else fill_w<4>(a, st);
@@ -630,8 +651,8 @@ struct TileLaunch {
         size_t shm = (size_t)a.g.TX * a.g.TY * a.g.TZ * (ACC == ACC_T ? sizeof(T) : 8);
         auto kern = k_tile_deposit<T, W, ACC>;
         (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
-        kern<<<a.g.nt, 256, shm, st>>>((const Entry<T> *)a.ent, a.start, a.g, (T)sh[0], (T)sh[1], (T)sh[2], (T)a.L,
-                                       mesh, fscale);
+        kern<<<a.g.nt, 256, shm, st>>>((const Entry<T> *)a.ent, a.start, a.gstart, a.cur, a.g, (T)sh[0], (T)sh[1],
+                                       (T)sh[2], (T)a.L, mesh, fscale);
     }
     template <int W>
     static void dep_sel(const TileArgs &a, bool fixed, const double *sh, void *mesh, double fscale, hipStream_t st) {
@@ -654,7 +675,7 @@ struct TileLaunch {
         size_t shm = q.ok ? sizeof(T) * (size_t)q.s[0] * q.s[1] * q.s[2] : sizeof(T);
         auto kern = k_tile_gather<T, W, O, M>;
         (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
-        kern<<<a.g.nt, gather_threads<T, W>(), shm, st>>>((const Entry<T> *)a.ent, a.start, a.cnt, a.g, (T)sh[0], (T)sh[1], (T)sh[2],
+        kern<<<a.g.nt, gather_threads<T, W>(), shm, st>>>((const Entry<T> *)a.ent, a.start, a.cur, a.g, (T)sh[0], (T)sh[1], (T)sh[2],
                                         (T)a.L, (const T *)phi, (T)c[0], (T)c[1], (T)c[2], (const T *)a.psi,
                                         (T *)psi_out, (T *)a.mom, (T *)a.acc, (size_t)a.n * a.n * a.n, (T)k.alpha,
                                         (T)k.beta, (T)k.dd2, (T)k.dd1_next, k.has_next, k.nshifts);
@@ -683,6 +704,7 @@ struct TileLaunch {
 
 template <typename T> void tile_ops(TileOps &ops) {
     ops.count = TileLaunch<T>::count;
+    ops.scan = TileLaunch<T>::scan;
     ops.fill = TileLaunch<T>::fill;
     ops.deposit = TileLaunch<T>::deposit;
     ops.gather = TileLaunch<T>::gather;
