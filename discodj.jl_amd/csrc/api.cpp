@@ -339,7 +339,7 @@ TileArgs tile_args(dj_ctx *c, const void *psi, const double *d_shifts, int ns) {
 }
 
 // Sort of the particles by mesh tile for the given shifts (device array of ns x 3).  Tile lists are
-// reserved from the previous sort's counts plus slack (particles move less than a tile per step), so
+// reserved from the previous sort's counts with room to double (particles move less than a tile per step), so
 // one fill pass writes them; a list that outgrows its reserve (or a full entry buffer) raises a flag,
 // and then the sort is repeated exactly: counting pass, lists of the exact sizes, fill.
 dj_status tile_sort(dj_ctx *c, const void *psi, const double *d_shifts, int ns, int W) {
@@ -370,7 +370,7 @@ dj_status tile_sort(dj_ctx *c, const void *psi, const double *d_shifts, int ns, 
             // grow the entry buffer (exact sizes follow in the next attempt)
             if (c->tent) HIPCHK(c, hipFree(c->tent));
             c->tent = nullptr;
-            const size_t cap = (size_t)hv[1] + hv[1] / 4 + 1024;
+            const size_t cap = (size_t)hv[1] + hv[1] / 8 + 1024;
             HIPCHK(c, hipMalloc(&c->tent, cap * c->tiles.entry_bytes));
             c->tent_cap = cap;
             a.ent = c->tent;
@@ -505,9 +505,9 @@ dj_status setup_tiles(dj_ctx *c) {
     c->tlists = false;
     HIPCHK(c, hipMalloc(&c->d_shifts, sizeof(double) * 17 * 3));
     HIPCHK(c, hipMemcpy(c->d_shifts, c->shifts, sizeof(double) * 16 * 3, hipMemcpyHostToDevice));
-    // own + ghost entries (about 2.1 per particle for 32 x 16 x 8 tiles with TSC and interlacing) plus
-    // the per-list slack of the reserved sizes
-    c->tent_cap = (size_t)(2.8 * (double)c->np) + 64 * (size_t)g.nt + 1024;
+    // own + ghost entries: about 2.1 per particle for 32 x 16 x 8 tiles with TSC and interlacing, twice
+    // that plus 64 per list once the lists are reserved from a previous sort (grown on demand)
+    c->tent_cap = (size_t)(2.5 * (double)c->np) + 1024;
     HIPCHK(c, hipMalloc(&c->tent, c->tent_cap * c->tiles.entry_bytes));
     return DJ_OK;
 }

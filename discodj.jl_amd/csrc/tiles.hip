@@ -215,11 +215,12 @@ __global__ __launch_bounds__(256) void k_tile_bin(const T *__restrict__ psi, con
 }
 
 // List capacities of every tile from counts src (own, then ghost; src may be the fill cursors of
-// the previous sort), with slack c/4 + 32 per list when slack != 0: start[t] = sum_{u<t} capacities,
+// the previous sort), reserving 2 c + 64 per list when slack != 0 (with some 2^18 tiles at 512^3,
+// some list almost surely grows by more than a quarter between steps; memory is plentiful): start[t] = sum_{u<t} capacities,
 // gstart[t] = start[t] + own capacity, start[nt] = total reserved.  Zeroes the cursors and the
 // overflow flag.  One workgroup of 1024 threads; each thread owns a contiguous chunk of tiles whose
 // counts it reads with independent (unrolled) loads, then a block scan of the chunk sums.
-__device__ __forceinline__ unsigned list_cap(unsigned c, int slack) { return slack ? c + c / 4 + 32 : c; }
+__device__ __forceinline__ unsigned list_cap(unsigned c, int slack) { return slack ? 2 * c + 64 : c; }
 
 __global__ __launch_bounds__(1024) void k_tile_scan(const unsigned *src, int nt, int slack, unsigned *__restrict__ start,
                                                     unsigned *__restrict__ gstart, unsigned *cur, unsigned *flag) {
