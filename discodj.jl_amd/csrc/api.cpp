@@ -62,6 +62,7 @@ struct dj_ctx {
     TileOps tiles{};
     TileGeom tg{};
     unsigned *tcnt = nullptr, *tstart = nullptr, *tgstart = nullptr, *tcur = nullptr, *tflag = nullptr;
+    unsigned long long *tpart = nullptr;  // block sums of the tile-list scan
     bool tlists = false;   // tcur holds the counts of a completed sort (capacities for the next one)
     void *tent = nullptr;
     size_t tent_cap = 0;   // entries
@@ -282,7 +283,7 @@ int aa_shifts(int aa, double dh, double (*s)[3]) {
 
 void free_all(dj_ctx *c) {
     void *bufs[] = {c->psi, c->mom, c->acc, c->psi_in, c->q1, c->mesh, c->meshfix, c->A, c->B, c->C, c->tw, c->tabs,
-                    c->tcnt, c->tstart, c->tgstart, c->tcur, c->tflag, c->tent, c->d_shifts};
+                    c->tcnt, c->tstart, c->tgstart, c->tcur, c->tflag, c->tpart, c->tent, c->d_shifts};
     // teardown: failures here have no caller to report to
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -333,6 +334,7 @@ TileArgs tile_args(dj_ctx *c, const void *psi, const double *d_shifts, int ns) {
     a.gstart = c->tgstart;
     a.cur = c->tcur;
     a.flag = c->tflag;
+    a.scan_part = c->tpart;
     a.ent = c->tent;
     a.ent_cap = c->tent_cap;
     return a;
@@ -502,6 +504,8 @@ dj_status setup_tiles(dj_ctx *c) {
     HIPCHK(c, hipMalloc(&c->tstart, sizeof(unsigned) * (g.nt + 1)));
     HIPCHK(c, hipMalloc(&c->tgstart, sizeof(unsigned) * g.nt));
     HIPCHK(c, hipMalloc(&c->tflag, sizeof(unsigned)));
+    if (g.nt > (1 << 20)) return fail(c, DJ_ERR_UNSUPPORTED, "more than 2^20 mesh tiles");
+    HIPCHK(c, hipMalloc(&c->tpart, sizeof(unsigned long long) * (g.nt / 1024 + 2)));
     c->tlists = false;
     HIPCHK(c, hipMalloc(&c->d_shifts, sizeof(double) * 17 * 3));
     HIPCHK(c, hipMemcpy(c->d_shifts, c->shifts, sizeof(double) * 16 * 3, hipMemcpyHostToDevice));

@@ -80,6 +80,7 @@ struct TileArgs {
     unsigned *gstart;      // (nt) first entry of each tile's ghost list (own capacity after start)
     unsigned *cur;         // (2 nt) fill cursors: after the fill, the own and ghost counts
     unsigned *flag;        // (1) set by the fill when a list outgrows its capacity or the buffer
+    unsigned long long *scan_part;  // (nt / 1024 + 1) block sums of the list scan
     void *ent;             // entries (X, particle index)
     size_t ent_cap;        // entries the buffer holds
 };

@@ -127,7 +127,7 @@ __global__ __launch_bounds__(256) void k_tile_bin(const T *__restrict__ psi, con
                                                   unsigned *cnt, const unsigned *__restrict__ start,
                                                   const unsigned *__restrict__ gstart, unsigned *cur, unsigned *flag,
                                                   Entry<T> *__restrict__ ent, size_t ent_cap) {
-    __shared__ unsigned keys[HASH], vals[HASH], base[HASH];
+    __shared__ unsigned keys[HASH], vals[HASH], base[HASH], lend[HASH];
     for (int s = threadIdx.x; s < HASH; s += 256) { keys[s] = EMPTY; vals[s] = 0; }
     __syncthreads();
     // 16 x 4 x 4 Lagrangian block of particles per workgroup
@@ -178,13 +178,21 @@ __global__ __launch_bounds__(256) void k_tile_bin(const T *__restrict__ psi, con
         }
     }
     __syncthreads();
-    // one global atomic per distinct (tile, own/ghost) of the workgroup
+    // one global atomic per distinct (tile, own/ghost) of the workgroup; the fill also records the
+    // group's first entry and its list's end (own list [start[t], gstart[t]), ghost list
+    // [gstart[t], start[t + 1]))
     for (int s = threadIdx.x; s < HASH; s += 256) {
         unsigned key = keys[s];
         if (key == EMPTY) continue;
         unsigned t = key >> 1, gh = key & 1u;
-        if (!FILL) atomicAdd(&cnt[gh * g.nt + t], vals[s]);
-        else base[s] = atomicAdd(&cur[gh * g.nt + t], vals[s]);
+        if (!FILL) {
+            atomicAdd(&cnt[gh * g.nt + t], vals[s]);
+        } else {
+            const unsigned b = atomicAdd(&cur[gh * g.nt + t], vals[s]);
+            const unsigned lo = gh ? gstart[t] : start[t];
+            base[s] = lo + b;
+            lend[s] = gh ? start[t + 1] : gstart[t];
+        }
     }
     if (!FILL) {
         // overflow pairs (table full) count directly
@@ -197,11 +205,16 @@ __global__ __launch_bounds__(256) void k_tile_bin(const T *__restrict__ psi, con
     if (!ok) return;
     for (int m = 0; m < nt; m++) {
         const unsigned t = tiles[m];
-        const unsigned r = slot[m] >= 0 ? base[slot[m]] + (unsigned)rank[m] : atomicAdd(&cur[(m > 0 ? g.nt : 0) + t], 1u);
-        // own list [start[t], gstart[t]), ghost list [gstart[t], start[t + 1])
-        const unsigned lo = m > 0 ? gstart[t] : start[t], hi = m > 0 ? start[t + 1] : gstart[t];
-        const size_t at = (size_t)lo + r;
-        if (r < hi - lo && at < ent_cap) {
+        unsigned at, end;
+        if (slot[m] >= 0) {
+            at = base[slot[m]] + (unsigned)rank[m];
+            end = lend[slot[m]];
+        } else {  // pair that did not fit the workgroup's hash table
+            const unsigned r = atomicAdd(&cur[(m > 0 ? g.nt : 0) + t], 1u);
+            at = (m > 0 ? gstart[t] : start[t]) + r;
+            end = m > 0 ? start[t + 1] : gstart[t];
+        }
+        if (at < end && at < ent_cap) {
             Entry<T> e;
             e.x = X[0];
             e.y = X[1];
@@ -216,42 +229,87 @@ __global__ __launch_bounds__(256) void k_tile_bin(const T *__restrict__ psi, con
 
 // List capacities of every tile from counts src (own, then ghost; src may be the fill cursors of
 // the previous sort), reserving 2 c + 64 per list when slack != 0 (with some 2^18 tiles at 512^3,
-// some list almost surely grows by more than a quarter between steps; memory is plentiful): start[t] = sum_{u<t} capacities,
-// gstart[t] = start[t] + own capacity, start[nt] = total reserved.  Zeroes the cursors and the
-// overflow flag.  One workgroup of 1024 threads; each thread owns a contiguous chunk of tiles whose
-// counts it reads with independent (unrolled) loads, then a block scan of the chunk sums.
+// some list almost surely grows by more than a quarter between steps; memory is plentiful):
+// start[t] = sum_{u<t} capacities, gstart[t] = start[t] + own capacity, start[nt] = total reserved.
+// Three parallel kernels: per-block sums, a scan of the block sums, per-block scan and write-out
+// (which also clears the cursors, after reading them when src is cur, and the overflow flag).
 __device__ __forceinline__ unsigned list_cap(unsigned c, int slack) { return slack ? 2 * c + 64 : c; }
 
-__global__ __launch_bounds__(1024) void k_tile_scan(const unsigned *src, int nt, int slack, unsigned *__restrict__ start,
-                                                    unsigned *__restrict__ gstart, unsigned *cur, unsigned *flag) {
-    __shared__ unsigned long long part[1024];
-    const int tid = threadIdx.x, chunk = (nt + 1023) / 1024;
-    const int a = min(nt, tid * chunk), b = min(nt, a + chunk);
+constexpr int SCAN_TILES = 1024;  // tiles per block of the scan (4 per thread)
+
+__global__ __launch_bounds__(256) void k_scan_sums(const unsigned *src, int nt, int slack, unsigned long long *part) {
+    __shared__ unsigned long long red[256];
+    const int t0 = blockIdx.x * SCAN_TILES;
     unsigned long long s = 0;
-#pragma unroll 16
-    for (int t = a; t < b; t++) s += (unsigned long long)list_cap(src[t], slack) + list_cap(src[nt + t], slack);
-    part[tid] = s;
+#pragma unroll
+    for (int k = 0; k < SCAN_TILES / 256; k++) {
+        const int t = t0 + k * 256 + threadIdx.x;
+        if (t < nt) s += (unsigned long long)list_cap(src[t], slack) + list_cap(src[nt + t], slack);
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// exclusive scan of nb <= 1024 block sums in place; part[nb] = total
+__global__ __launch_bounds__(1024) void k_scan_top(unsigned long long *part, int nb, unsigned *start, int nt,
+                                                   unsigned *flag) {
+    __shared__ unsigned long long v[1024];
+    const int i = threadIdx.x;
+    const unsigned long long x = i < nb ? part[i] : 0ull;
+    v[i] = x;
     __syncthreads();
     for (int off = 1; off < 1024; off <<= 1) {
-        unsigned long long v = tid >= off ? part[tid - off] : 0ull;
+        const unsigned long long y = i >= off ? v[i - off] : 0ull;
         __syncthreads();
-        part[tid] += v;
+        v[i] += y;
         __syncthreads();
     }
-    unsigned long long run = part[tid] - s;
-    // src may be cur: each tile's counts are read before this thread clears them
-#pragma unroll 16
-    for (int t = a; t < b; t++) {
-        const unsigned co = list_cap(src[t], slack), cg = list_cap(src[nt + t], slack);
+    if (i < nb) part[i] = v[i] - x;
+    if (i == 1023) {
+        start[nt] = (unsigned)min(v[1023], 0xffffffffull);
+        flag[0] = v[1023] > 0xffffffffull ? 1u : 0u;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_scan_write(const unsigned *src, int nt, int slack,
+                                                    const unsigned long long *__restrict__ part, unsigned *__restrict__ start,
+                                                    unsigned *__restrict__ gstart, unsigned *cur) {
+    constexpr int PER = SCAN_TILES / 256;  // consecutive tiles per thread
+    __shared__ unsigned long long v[256];
+    const int t0 = blockIdx.x * SCAN_TILES + threadIdx.x * PER;
+    unsigned co[PER], cg[PER];
+    unsigned long long s = 0;
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const int t = t0 + k;
+        co[k] = t < nt ? list_cap(src[t], slack) : 0u;
+        cg[k] = t < nt ? list_cap(src[nt + t], slack) : 0u;
+        s += (unsigned long long)co[k] + cg[k];
+    }
+    v[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const unsigned long long y = threadIdx.x >= off ? v[threadIdx.x - off] : 0ull;
+        __syncthreads();
+        v[threadIdx.x] += y;
+        __syncthreads();
+    }
+    unsigned long long run = part[blockIdx.x] + v[threadIdx.x] - s;
+    // src may be cur: this thread read its tiles' counts above, before clearing them here
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const int t = t0 + k;
+        if (t >= nt) break;
         start[t] = (unsigned)run;
-        gstart[t] = (unsigned)(run + co);
-        run += (unsigned long long)co + cg;
+        gstart[t] = (unsigned)(run + co[k]);
+        run += (unsigned long long)co[k] + cg[k];
         cur[t] = 0;
         cur[nt + t] = 0;
-    }
-    if (tid == 1023) {
-        start[nt] = (unsigned)min(part[1023], 0xffffffffull);
-        flag[0] = part[1023] > 0xffffffffull ? 1u : 0u;
     }
 }
 
@@ -642,7 +700,10 @@ struct TileLaunch {
         if (W == 2) bin_w<2>(a, st); else if (W == 3) bin_w<3>(a, st); else bin_w<4>(a, st);
     }
     static void scan(const TileArgs &a, const unsigned *src, int slack, hipStream_t st) {
-        k_tile_scan<<<1, 1024, 0, st>>>(src, a.g.nt, slack, a.start, a.gstart, a.cur, a.flag);
+        const int nb = (a.g.nt + SCAN_TILES - 1) / SCAN_TILES;  // <= 1024 (nt <= 2^20)
+        k_scan_sums<<<nb, 256, 0, st>>>(src, a.g.nt, slack, a.scan_part);
+        k_scan_top<<<1, 1024, 0, st>>>(a.scan_part, nb, a.start, a.g.nt, a.flag);
+        k_scan_write<<<nb, 256, 0, st>>>(src, a.g.nt, slack, a.scan_part, a.start, a.gstart, a.cur);
     }
     static void fill(const TileArgs &a, int W, hipStream_t st) {
         if (W == 2) fill_w<2>(a, st); else if (W == 3) fill_w<3>(a, st); else fill_w<4>(a, st);
