@@ -114,6 +114,14 @@ def main():
     roof = {"kernel": dom, "bound": "hbm", "achieved": kernels[dom]["achieved_GBs"], "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": kernels[dom]["frac"], "traffic": None,
             "algorithmic_bytes_per_launch": int(kernels[dom]["algorithmic_GB"] * 1e9)}
+    # HBM bytes per launch of the same kernel, measured by rocprofv3 FETCH_SIZE / WRITE_SIZE passes over
+    # this workload and scaled by a copy kernel of known size (profiles/pm_step_c3_counters.md)
+    tpath = os.path.join(ROOT, "profiles", "pm_step_c3_traffic.json")
+    if not args.small and os.path.exists(tpath):
+        tk = json.load(open(tpath))["kernels"].get(dom)
+        if tk:
+            roof["traffic"] = int(tk["traffic_bytes"])
+            roof["traffic_source"] = "profiles/pm_step_c3_counters.md (" + tk["rocprof_kernel"] + ")"
 
     accuracy = None
     if not args.no_accuracy:
