@@ -30,10 +30,10 @@ namespace {
 thread_local std::string g_create_error;
 
 enum KernelId { K_ZERO = 0, K_SCATTER, K_XF, K_YF, K_ZM, K_YI, K_XI, K_GATHER, K_GATHER_KICK, K_KICK, K_DRIFT, K_SORT,
-                K_SORT_EXACT, K_COUNT };
+                K_SORT_EXACT, K_RESAMPLE, K_COUNT };
 const char *kernel_names[K_COUNT] = {"zero", "scatter", "fft_x_r2c", "fft_y_fwd", "fft_z_mult", "fft_y_inv",
                                      "fft_x_c2r", "gather", "gather_kick", "kick_drift", "drift", "bin",
-                                     "bin_recount"};
+                                     "bin_recount", "resample"};
 
 struct ProfEvent {
     int id;
@@ -67,6 +67,17 @@ struct dj_ctx {
     void *tent = nullptr;
     size_t tent_cap = 0;   // entries
     double *d_shifts = nullptr;  // (16 + 1, 3): the interlacing shifts, then one free slot
+    // sheet resampling (n_resample > 1): particle-grid FFT ops and buffers
+    ResampleOps rs{};
+    FftOps fftn{};
+    void *rs_tw = nullptr;                      // twiddles of the particle grid, exp(-2 pi i k / n)
+    void *rs_psi = nullptr;                     // (3, n^3) wrapped psi (linear) or one component (Fourier)
+    void *rs_hat = nullptr;                     // (3) half spectra of psi (Fourier)
+    void *rs_A = nullptr, *rs_B = nullptr;      // particle-grid FFT work spectra
+    void *rs_xr = nullptr;                      // (3, n^3) positions of one offset's point set
+    void *rs_phase = nullptr;                   // px (n), py (n), pz (n/2+1) of the current offset
+    int n_offsets = 0;
+    double offsets[64][3];
     std::string err;
     bool prof = false;
     std::vector<ProfEvent> pending;
@@ -283,7 +294,8 @@ int aa_shifts(int aa, double dh, double (*s)[3]) {
 
 void free_all(dj_ctx *c) {
     void *bufs[] = {c->psi, c->mom, c->acc, c->psi_in, c->q1, c->mesh, c->meshfix, c->A, c->B, c->C, c->tw, c->tabs,
-                    c->tcnt, c->tstart, c->tgstart, c->tcur, c->tflag, c->tpart, c->tent, c->d_shifts};
+                    c->tcnt, c->tstart, c->tgstart, c->tcur, c->tflag, c->tpart, c->tent, c->d_shifts,
+                    c->rs_tw, c->rs_psi, c->rs_hat, c->rs_A, c->rs_B, c->rs_xr, c->rs_phase};
     // teardown: failures here have no caller to report to
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -313,6 +325,8 @@ void set_bytes(dj_ctx *c) {
     // binning: read psi twice (count, fill), write one entry per (particle, tile)
     c->bytes[K_SORT] = 3 * T * np + (double)c->tiles.entry_bytes * np;
     c->bytes[K_SORT_EXACT] = 2 * 3 * T * np + (double)c->tiles.entry_bytes * np;
+    // sheet resampling of one pass: read psi once, then per offset read and write the mesh once
+    c->bytes[K_RESAMPLE] = 3 * T * np + (double)c->n_offsets * 2 * T * M;
 }
 
 double fscale_dep(const dj_pm_config &cfg) { return std::ldexp(1.0, cfg.fixed_frac_bits ? cfg.fixed_frac_bits : 32); }
@@ -385,6 +399,9 @@ dj_status tile_sort(dj_ctx *c, const void *psi, const double *d_shifts, int ns, 
 // ------------------------------------------------------------------------ one force evaluation
 enum { FORCE_ACC = 0, FORCE_KICK = 1 };
 
+template <typename T>
+dj_status resample_deposit(dj_ctx *c, const void *psi, const double *sh, void *dens, bool fixed, double fscale);
+
 dj_status force(dj_ctx *c, const void *psi, int mode, const KickArgs &kick) {
     const dj_pm_config &cfg = c->cfg;
     const bool fixed = cfg.deposit == DJ_DEPOSIT_FIXED64;
@@ -415,6 +432,12 @@ dj_status force(dj_ctx *c, const void *psi, int mode, const KickArgs &kick) {
         pa.shift[2] = c->shifts[s][2];
         TileArgs ta = tile_args(c, psi, c->d_shifts, ns);
         { Prof p(c, K_SCATTER); c->tiles.deposit(ta, cfg.worder, fixed, c->shifts[s], dens, fscale_dep(cfg), c->st); }
+        if (cfg.n_resample > 1) {
+            Prof p(c, K_RESAMPLE);
+            dj_status rr = c->tsize == 8 ? resample_deposit<double>(c, psi, c->shifts[s], dens, fixed, fscale_dep(cfg))
+                                         : resample_deposit<float>(c, psi, c->shifts[s], dens, fixed, fscale_dep(cfg));
+            if (rr) return rr;
+        }
         { Prof p(c, K_XF); c->fft.xf(dens, fixed, c->A, c->tw, fscale, c->st); }
         { Prof p(c, K_YF); c->fft.yf(c->A, c->B, c->tw, c->st); }
         if (!ik) {
@@ -455,7 +478,110 @@ dj_status setup_ops(dj_ctx *c) {
                                                " is not a built mesh size (16..1024, 2^a 3^b)");
     part_ops<T>(c->part);
     tile_ops<T>(c->tiles);
+    resample_ops<T>(c->rs);
     return upload_tables<T>(c);
+}
+
+// Buffers of sheet resampling and the offsets d = (a, b, c) / n_resample, (0,0,0) excluded, in the
+// order of pm.jl:233-234.  Fourier resampling transforms on the particle grid with the library's
+// FFT passes (n must be a built FFT size).
+template <typename T>
+dj_status setup_resampling(dj_ctx *c) {
+    const int nr = c->cfg.n_resample, n = c->n;
+    c->n_offsets = 0;
+    for (int a = 0; a < nr; a++)
+        for (int b = 0; b < nr; b++)
+            for (int d = 0; d < nr; d++) {
+                if (a == 0 && b == 0 && d == 0) continue;
+                c->offsets[c->n_offsets][0] = (double)a / nr;
+                c->offsets[c->n_offsets][1] = (double)b / nr;
+                c->offsets[c->n_offsets][2] = (double)d / nr;
+                c->n_offsets++;
+            }
+    const size_t n3 = (size_t)c->n3, hc = (size_t)n * n * (n / 2 + 1);
+    HIPCHK(c, hipMalloc(&c->rs_xr, 3 * n3 * sizeof(T)));
+    if (c->cfg.resampling == 1) {
+        HIPCHK(c, hipMalloc(&c->rs_psi, 4 * n3 * sizeof(T)));  // 3 wrapped components + 1 shifted
+        return DJ_OK;
+    }
+    if (!fft_dispatch<T>(n, c->fftn))
+        return fail(c, DJ_ERR_UNSUPPORTED, "Fourier sheet resampling needs n_part in {16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512, 768, 1024}");
+    std::vector<cx<T>> tw(n);
+    for (int k = 0; k < n; k++) {
+        const double a = -2.0 * M_PI * (double)k / (double)n;
+        tw[k] = {(T)std::cos(a), (T)std::sin(a)};
+    }
+    HIPCHK(c, hipMalloc(&c->rs_tw, sizeof(cx<T>) * n));
+    HIPCHK(c, hipMemcpy(c->rs_tw, tw.data(), sizeof(cx<T>) * n, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMalloc(&c->rs_psi, n3 * sizeof(T)));
+    HIPCHK(c, hipMalloc(&c->rs_hat, 3 * hc * sizeof(cx<T>)));
+    HIPCHK(c, hipMalloc(&c->rs_A, hc * sizeof(cx<T>)));
+    HIPCHK(c, hipMalloc(&c->rs_B, hc * sizeof(cx<T>)));
+    // phase factors of every offset, px (n), py (n), pz (n/2+1) each, as pm.jl:452-454 builds them in T:
+    // exp(i (T(2 pi) m) d) with m = (m < n/2 ? m : m - n) / n on the full axes, m / n on the half axis
+    const int H = n / 2 + 1, per = 2 * n + H;
+    std::vector<cx<T>> ph((size_t)per * c->n_offsets);
+    const T tp = (T)(2 * M_PI);
+    for (int o = 0; o < c->n_offsets; o++) {
+        cx<T> *p = ph.data() + (size_t)o * per;
+        for (int a = 0; a < 2; a++)
+            for (int m = 0; m < n; m++) {
+                const T mf = (T)((double)(m < n / 2 ? m : m - n) / n), th = tp * mf * (T)c->offsets[o][a];
+                p[a * n + m] = {(T)std::cos(th), (T)std::sin(th)};
+            }
+        for (int m = 0; m < H; m++) {
+            const T mh = (T)((double)m / n), th = tp * mh * (T)c->offsets[o][2];
+            p[2 * n + m] = {(T)std::cos(th), (T)std::sin(th)};
+        }
+    }
+    HIPCHK(c, hipMalloc(&c->rs_phase, ph.size() * sizeof(cx<T>)));
+    HIPCHK(c, hipMemcpy(c->rs_phase, ph.data(), ph.size() * sizeof(cx<T>), hipMemcpyHostToDevice));
+    return DJ_OK;
+}
+
+// The extra point sets of one interlacing pass (shift sh) deposited into dens (pm.jl:471-483 with
+// _resample_prepare! / _resample_offset! for Fourier, _spawn for linear).
+template <typename T>
+dj_status resample_deposit(dj_ctx *c, const void *psi, const double *sh, void *dens, bool fixed, double fscale) {
+    const int n = c->n, H = n / 2 + 1;
+    const double L = c->cfg.boxsize;
+    const size_t n3 = (size_t)c->n3, hc = (size_t)n * n * H;
+    T *xr = (T *)c->rs_xr;
+    if (c->cfg.resampling == 1) {
+        T *w = (T *)c->rs_psi;
+        for (int comp = 0; comp < 3; comp++)
+            c->rs.wrap(psi, c->q1, n, comp, sh[comp], L, false, w + comp * n3, c->st);
+        for (int o = 0; o < c->n_offsets; o++) {
+            const double *d = c->offsets[o];
+            for (int comp = 0; comp < 3; comp++) {
+                c->rs.trilinear(w + comp * n3, n, d, w + 3 * n3, c->st);
+                c->rs.place(w + 3 * n3, c->q1, n, comp, (double)(T)(d[comp] * L / n), L, false, xr, c->st);
+            }
+            c->rs.scatter(xr, n, c->cfg.worder, dens, c->res, L, fixed, fscale, c->st);
+        }
+        return check_launch(c, "linear sheet resampling");
+    }
+    cx<T> *hat = (cx<T> *)c->rs_hat;
+    for (int comp = 0; comp < 3; comp++) {
+        c->rs.wrap(psi, c->q1, n, comp, sh[comp], L, true, c->rs_psi, c->st);
+        c->fftn.xf(c->rs_psi, false, c->rs_A, c->rs_tw, 1.0, c->st);
+        c->fftn.yf(c->rs_A, c->rs_B, c->rs_tw, c->st);
+        c->fftn.zm(ZM_NONE, true, false, c->rs_B, hat + comp * hc, c->rs_tw, c->kmult, c->st);
+    }
+    const double scale = (double)((T)1 / ((T)n * (T)n * (T)n));
+    for (int o = 0; o < c->n_offsets; o++) {
+        const double *d = c->offsets[o];
+        const cx<T> *px = (const cx<T> *)c->rs_phase + (size_t)o * (2 * n + H), *py = px + n, *pz = px + 2 * n;
+        for (int comp = 0; comp < 3; comp++) {
+            c->rs.phase(hat + comp * hc, c->rs_B, n, px, py, pz, scale, c->st);
+            c->fftn.zm(ZM_NONE, false, true, c->rs_B, c->rs_B, c->rs_tw, c->kmult, c->st);
+            c->fftn.yi(c->rs_B, c->rs_A, c->rs_tw, c->st);
+            c->fftn.xi(c->rs_A, c->rs_psi, c->rs_tw, c->st);
+            c->rs.place(c->rs_psi, c->q1, n, comp, (double)(T)(d[comp] * L / n), L, true, xr, c->st);
+        }
+        c->rs.scatter(xr, n, c->cfg.worder, dens, c->res, L, fixed, fscale, c->st);
+    }
+    return check_launch(c, "Fourier sheet resampling");
 }
 
 // tile shape (FP32: 32 x 16 x 16 cells, FP64: 16^3) and the stencil reach over the shifts
@@ -535,7 +661,7 @@ dj_status validate(const dj_pm_config &cfg, std::string &msg) {
     if (cfg.precision != 32 && cfg.precision != 64) { msg = "precision must be 32 or 64"; return DJ_ERR_INVALID_CONFIG; }
     if (cfg.deposit != DJ_DEPOSIT_FLOAT && cfg.deposit != DJ_DEPOSIT_FIXED64) { msg = "unknown deposit mode"; return DJ_ERR_INVALID_CONFIG; }
     if (cfg.fixed_frac_bits < 0 || cfg.fixed_frac_bits > 40) { msg = "fixed_frac_bits must be in 0..40"; return DJ_ERR_INVALID_CONFIG; }
-    if (cfg.n_resample > 1) { msg = "sheet resampling (n_resample > 1) is not built into this library"; return DJ_ERR_UNSUPPORTED; }
+    if (cfg.n_resample > 4) { msg = "n_resample above 4 (63 offsets) is not built into this library"; return DJ_ERR_UNSUPPORTED; }
     if (cfg.n_part > 1024) { msg = "n_part above 1024 is not supported"; return DJ_ERR_UNSUPPORTED; }
     return DJ_OK;
 }
@@ -591,6 +717,10 @@ dj_status dj_create(const dj_pm_config *cfg, dj_ctx **out) {
     if (r) return bail(r);
     r = setup_tiles(c);
     if (r) return bail(r);
+    if (cfg->n_resample > 1) {
+        r = c->tsize == 8 ? setup_resampling<double>(c) : setup_resampling<float>(c);
+        if (r) return bail(r);
+    }
     size_t pbytes = (size_t)c->n3 * c->tsize, cbytes = c->HC * 2 * c->tsize;
     void **bufs[] = {&c->psi, &c->mom, &c->acc};
     for (void **b : bufs) {

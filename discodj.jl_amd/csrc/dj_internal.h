@@ -97,6 +97,25 @@ struct TileOps {
 };
 template <typename T> void tile_ops(TileOps &ops);
 
+// sheet resampling (resample.hip): the steps of one offset's extra point set
+struct ResampleOps {
+    // psi_c = mod((psi + q)_c + s_c - q_c + L/2, L) - L/2 of every particle, natural or transposed layout
+    void (*wrap)(const void *psi, const void *q1, int n, int c, double sc, double L, bool transposed, void *out,
+                 hipStream_t st);
+    // out = in * px py pz * scale on the transposed field's half spectrum
+    void (*phase)(const void *in, void *out, int n, const void *px, const void *py, const void *pz, double scale,
+                  hipStream_t st);
+    // periodic trilinear shift by d (grid cells)
+    void (*trilinear)(const void *f, int n, const double *d, void *out, hipStream_t st);
+    // Xr_c = mod(q_c + npsi_c + shift + L, L)
+    void (*place)(const void *npsi, const void *q1, int n, int c, double shift, double L, bool transposed, void *xr,
+                  hipStream_t st);
+    // unit-mass deposit of positions (3, n^3) with device atomics (float, or int64 fixed point)
+    void (*scatter)(const void *X, int n, int W, void *mesh, int res, double L, bool fixed, double fscale,
+                    hipStream_t st);
+};
+template <typename T> void resample_ops(ResampleOps &ops);
+
 // power spectrum binning of an x-halved B-layout spectrum (analysis.hip)
 template <typename T>
 void pk_bin(const void *B, int N, double L, int W, int nbins, double scale, double *sum_p, double *sum_w,

@@ -1,4 +1,4 @@
-"""run_nbody on the MI355X library (NR/src/nbody/run_nbody.jl:124-172, 180-208).
+"""run_nbody and bullfrog_ics on the MI355X library (NR/src/nbody/run_nbody.jl:124-172, 180-225).
 
 The host computes the FP64 stepper coefficients once; the particle state stays on the device for the
 whole loop (dj_run), with callbacks only at the steps callback_when selects.
@@ -81,3 +81,25 @@ def nbody_ics_lpt(c, psi1, psi2, a):
     """EdS 2LPT initial conditions Psi = D psi1 + D^2 psi2, Pi = psi1 + 2 D psi2 (run_nbody.jl:200-206)."""
     D = c.Dplus(a)
     return (0.0 + D * psi1) + (D * D) * psi2, (0.0 + 1.0 * psi1) + (2 * D) * psi2
+
+
+# the discreteness-suppressing defaults of bullfrog_ics (run_nbody.jl:219-220)
+BULLFROG_IC_SETTINGS = dict(time_var="D", stepper="bullfrog", antialias=1, grad_kernel_order=0,
+                            laplace_kernel_order=0, worder=4, deconvolve=True, n_resample=2,
+                            resampling_method="fourier")
+
+
+def bullfrog_ics(c, psi1, a_ini, *, boxsize, res_pm, precision=None, **settings):
+    """-> (Psi, Pi): one BullFrog step from a = 0 to a_ini starting from 1LPT (Psi = D(0) psi1,
+    Pi = psi1), PCS with deconvolution, ik gradient, -1/k^2, interlacing and 2^3 Fourier sheet
+    resampling unless settings override them (run_nbody.jl:217-225).  psi1 is (3,n,n,n)."""
+    s = dict(BULLFROG_IC_SETTINGS)
+    s.update(settings)
+    psi1 = np.asarray(psi1)
+    if precision is None:
+        precision = 64 if psi1.dtype == np.float64 else 32
+    T = np.float64 if precision == 64 else np.float32
+    psi1 = psi1.astype(T)
+    psi, P, _ = run_nbody(c, T(c.Dplus(0.0)) * psi1, psi1.copy(), boxsize=boxsize, a_ini=0.0, a_end=a_ini,
+                          n_steps=1, res_pm=res_pm, return_displacement=True, precision=precision, **s)
+    return psi, P / T(c.Fplus(a_ini))

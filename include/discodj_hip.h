@@ -47,7 +47,9 @@ typedef int32_t dj_status;
 
 /* PMConfig (pm.jl:35-46) plus the device-side choices.  Field meanings and accepted values are the
  * reference's: worder 2 (CIC) / 3 (TSC) / 4 (PCS); antialias -1, 0, 1, 2, 3; grad_order 0 (ik) or
- * 2/4/6 (finite differences); lap_order 0 (-1/k^2) or 2/4/6; resampling 0 = fourier, 1 = linear. */
+ * 2/4/6 (finite differences); lap_order 0 (-1/k^2) or 2/4/6; resampling 0 = fourier, 1 = linear.
+ * Sheet resampling is built for n_resample 1..4 (DJ_ERR_UNSUPPORTED above); the Fourier form needs
+ * n_part in {16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512, 768, 1024}. */
 typedef struct dj_pm_config {
     int32_t res_pm;
     int32_t n_part;

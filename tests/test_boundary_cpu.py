@@ -26,7 +26,7 @@ def test_library_exports_every_header_symbol():
 @pytest.mark.parametrize("kw, code", [
     (dict(worder=5), -1), (dict(grad_order=3), -1), (dict(lap_order=1), -1), (dict(antialias=4), -1),
     (dict(antialias=1, lean=1), -1), (dict(grad_order=0, lean=1), -1), (dict(precision=16), -1),
-    (dict(n_resample=2), -3), (dict(res_pm=40), None)])
+    (dict(n_resample=5), -3), (dict(resampling=2), -1), (dict(n_resample=0), -1), (dict(res_pm=40), None)])
 def test_invalid_configs_are_rejected_before_any_device_work(kw, code):
     base = dict(res_pm=32, n_part=16, boxsize=100.0, worder=2, deconvolve=0, antialias=0, grad_order=4,
                 lap_order=0, n_resample=1, resampling=0, precision=32, deposit=0, fixed_frac_bits=0,

@@ -23,7 +23,7 @@ def _cfg(name):
                       resampling=kw["resampling_method"])
 
 
-NO_RESAMPLE = [c for c in G.CONFIGS if G.kwargs(c)["n_resample"] == 1 and G.kwargs(c)["ic_method"] == "lpt"]
+RESAMPLE = [c for c in G.CONFIGS if G.kwargs(c)["n_resample"] > 1]
 
 
 # ------------------------------------------------------------------------------------------- FFT
@@ -84,24 +84,30 @@ def test_float_deposit(W):
 
 
 # ------------------------------------------------------------------ PM force vs golden (FP64 device)
-@pytest.mark.parametrize("name", NO_RESAMPLE)
+@pytest.mark.parametrize("name", G.CONFIGS)
 def test_acc_ini_fp64_golden(name):
     acc = D.pm_acceleration(G.particles("ics/psi_ini"), _cfg(name), precision=64)
     assert G.rel(acc, G.particles(f"{name}/acc_ini")) < 1e-12
 
 
-@pytest.mark.parametrize("name", NO_RESAMPLE)
+@pytest.mark.parametrize("name", G.CONFIGS)
 def test_run_nbody_fp64_golden(name):
     kw = G.kwargs(name)
     c = D.Cosmology.named("Planck18EEBAOSN")
     tv = kw["time_var"]
     tv = np.array(tv, np.float64) if isinstance(tv, list) else tv
-    psi, P, _ = D.run_nbody(c, G.particles("ics/psi_ini"), G.particles("ics/pi_ini"), boxsize=G.BOXSIZE,
+    if kw["ic_method"] == "bullfrog":
+        # nbody_parity_tests.jl:52-61: ICs from one BullFrog step with 2^3 Fourier sheet resampling
+        p0, m0 = D.bullfrog_ics(c, G.particles("ics/psi1"), kw["a_ini"], boxsize=G.BOXSIZE, res_pm=kw["res_pm"])
+    else:
+        p0, m0 = G.particles("ics/psi_ini"), G.particles("ics/pi_ini")
+    psi, P, _ = D.run_nbody(c, p0, m0, boxsize=G.BOXSIZE,
                             a_ini=kw["a_ini"], a_end=kw["a_end"], n_steps=kw["n_steps"], res_pm=kw["res_pm"],
                             time_var=tv, stepper=kw["stepper"], antialias=kw["antialias"],
                             grad_kernel_order=kw["grad_kernel_order"],
                             laplace_kernel_order=kw["laplace_kernel_order"], worder=kw["worder"],
-                            deconvolve=kw["deconvolve"], return_displacement=True, precision=64)
+                            deconvolve=kw["deconvolve"], n_resample=kw["n_resample"],
+                            resampling_method=kw["resampling_method"], return_displacement=True, precision=64)
     assert G.rel(psi, G.particles(f"{name}/psi_out")) < 1e-11
     assert G.rel(P, G.particles(f"{name}/p_out")) < 1e-11
 
@@ -124,7 +130,27 @@ def test_tile_pipeline_vs_oracle(W, order, prec):
     assert G.rel(acc, ref) < (1e-12 if prec == 64 else 2e-5)
 
 
-@pytest.mark.parametrize("name", NO_RESAMPLE)
+@pytest.mark.parametrize("n, nr, method, W, grad", [
+    (32, 2, "fourier", 4, 0), (32, 3, "fourier", 3, 4), (24, 2, "fourier", 2, 6), (32, 2, "linear", 3, 4),
+    (24, 3, "linear", 4, 0)])
+@pytest.mark.parametrize("prec", [32, 64])
+def test_resampling_vs_oracle(n, nr, method, W, grad, prec):
+    """Sheet resampling (nr^3 - 1 extra point sets per interlacing pass) on meshes of 2n cells, against
+    the oracle's FP64 solver-form force; the transforms on the particle grid run through the library's
+    FFT passes (n = 24: radix-3 sizes)."""
+    res, L = 2 * n, 100.0
+    dt = np.float32 if prec == 32 else np.float64
+    psi = _random_psi(n, L, 2.0, 7 * nr + W, dt)
+    cfg = D.PMConfig(res_pm=res, n_part=n, boxsize=L, worder=W, grad_order=grad, antialias=1,
+                     deconvolve=grad == 0, n_resample=nr, resampling=method)
+    acc = D.pm_acceleration(psi, cfg, precision=prec)
+    ocfg = O.pm_config(res, n, L, worder=W, grad_order=grad, antialias=1, deconvolve=grad == 0, n_resample=nr,
+                       resampling=method)
+    ref = O.pm_acceleration(psi.astype(np.float64), ocfg, "solver")
+    assert G.rel(acc, ref) < (1e-12 if prec == 64 else 5e-5)
+
+
+@pytest.mark.parametrize("name", G.CONFIGS)
 def test_acc_ini_fp32(name):
     # FP32 mesh / FFT / forces against the FP64 golden acceleration
     acc = D.pm_acceleration(G.particles("ics/psi_ini").astype(np.float32), _cfg(name), precision=32)
