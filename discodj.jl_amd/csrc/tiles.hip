@@ -316,82 +316,137 @@ __global__ __launch_bounds__(256) void k_scan_write(const unsigned *src, int nt,
 // ------------------------------------------------------------------------------------ deposit
 // Accumulators of a tile in LDS:
 //   ACC_T      T atomics (FP64 meshes)
-//   ACC_Q31    FP32 meshes: each weight w <= 1 enters as the integer round(w 2^31) (exact in 32 bits),
+//   ACC_Q31    FP32 meshes: each weight w <= 1 enters as the integer trunc(w 2^31) (exact in 32 bits),
 //              summed in 64-bit integers and rounded once to T at the tile store.  Integer LDS atomics
 //              run at about five times the rate of float ones on gfx950, the sum is independent of
 //              the order of the additions (the deposit is reproducible bit for bit), and its error
-//              (2^-32 per term, then one rounding) is below that of an FP32 running sum.
+//              (below 2^-31 per term, then one rounding) is below that of an FP32 running sum.
 //   ACC_FIXED  the fixed-point mode: llrint(w 2^frac) in 64-bit integers, stored as integers
 enum { ACC_T = 0, ACC_Q31 = 1, ACC_FIXED = 2 };
 
-template <typename T, int W, int ACC>
+template <typename T, int ACC, typename Acc>
+__device__ __forceinline__ void dep_add(Acc *a, T wx, T wy, T wz, T wyz, double fscale) {
+    if constexpr (ACC == ACC_FIXED) {
+        // the oracle's product order, (wx wy) wz, then the fixed-point scale in FP64
+        const T wv = wx * wy * wz;
+        atomicAdd(a, (unsigned long long)__double2ll_rn((double)wv * fscale));
+    } else if constexpr (ACC == ACC_Q31) {
+        atomicAdd(a, (unsigned long long)(unsigned)(wx * wyz));  // wx carries the 2^31
+    } else {
+        atomicAdd(a, wx * wy * wz);
+    }
+}
+
+// Padded accumulator (PAD, every axis of the mesh holds at least two tiles): rows of SXD = TX + 2 (W-1)
+// cells, the tile's x cells at [W-1, W-1+TX), for the TY TZ rows of the tile plus one discard row.  An
+// entry's stencil starts at local x cell l0 in [-(W-1), TX) whenever it overlaps the tile, so its W x
+// cells are always in the row (cells outside the tile land in the pads), and rows outside the tile are
+// redirected to the discard row: every entry makes W^3 unmasked atomics at immediate offsets from
+// W^2 row addresses.  Pads and the discard row are not stored; their cells belong to neighbour tiles,
+// which receive the same particles through their ghost lists.  With a single tile along an axis the
+// stencil wraps onto the tile itself, and the masked form (!PAD) wraps each cell.
+template <typename T, int W, int ACC, bool PAD>
 __global__ __launch_bounds__(256) void k_tile_deposit(const Entry<T> *__restrict__ ent, const unsigned *__restrict__ start,
                                                       const unsigned *__restrict__ gstart, const unsigned *__restrict__ cnt,
                                                       TileGeom g, T sx, T sy, T sz, T L, void *mesh, double fscale) {
     using Acc = typename std::conditional<ACC == ACC_T, T, unsigned long long>::type;
     extern __shared__ unsigned char smem[];
     Acc *acc = (Acc *)smem;
+    constexpr int P = W - 1;
     const int t = blockIdx.x, TV = g.TX * g.TY * g.TZ;
-    for (int c = threadIdx.x; c < TV; c += blockDim.x) acc[c] = 0;
+    const int SXD = PAD ? g.TX + 2 * P : g.TX, ROWS = g.TY * g.TZ, CELLS = PAD ? SXD * (ROWS + 1) : TV;
+    for (int c = threadIdx.x; c < CELLS; c += blockDim.x) acc[c] = 0;
     const int tx = t % g.ntx, ty = (t / g.ntx) % g.nty, tz = t / (g.ntx * g.nty);
     const int ox = tx << g.lT[0], oy = ty << g.lT[1], oz = tz << g.lT[2];
     __syncthreads();
     // the tile's own list, then its ghost list
     const unsigned n_own = cnt[t], n_all = n_own + cnt[g.nt + t], s_own = start[t], s_gh = gstart[t] - n_own;
+    const int o3[3] = {ox, oy, oz}, T3[3] = {g.TX, g.TY, g.TZ};
     for (unsigned i = threadIdx.x; i < n_all; i += blockDim.x) {
         const Entry<T> en = ent[i < n_own ? s_own + i : s_gh + i];
         const T xs[3] = {en.x + sx, en.y + sy, en.z + sz};
         T w[3][W];
-        int loc[3][W];
-        const int o3[3] = {ox, oy, oz}, T3[3] = {g.TX, g.TY, g.TZ};
+        int b0[3];
 #pragma unroll
         for (int a = 0; a < 3; a++) {
-            T gg = tjmod(xs[a], L) / L * (T)g.N;
-            int b = twbase<W>(gg);
+            const T gg = tjmod(xs[a], L) / L * (T)g.N;
+            const int b = twbase<W>(gg);
 #pragma unroll
-            for (int o = 0; o < W; o++) {
-                int bo = b + twlo<W>() + o;
-                w[a][o] = twk<W>(fabs(gg - (T)bo));
-                int l = wrapn(bo, g.N) - o3[a];
-                loc[a][o] = (l >= 0 && l < T3[a]) ? l : -1;
-            }
+            for (int o = 0; o < W; o++) w[a][o] = twk<W>(fabs(gg - (T)(b + twlo<W>() + o)));
+            b0[a] = b + twlo<W>();
         }
         if constexpr (ACC == ACC_Q31) {
             // x weights scaled by 2^31 (exact); one w_y w_z product per row
 #pragma unroll
             for (int ix = 0; ix < W; ix++) w[0][ix] *= (T)2147483648.0f;
         }
+        if constexpr (PAD) {
+            // first stencil cell relative to the tile, nearest periodic image (N >= 2 T per axis)
+            int l0[3];
 #pragma unroll
-        for (int iz = 0; iz < W; iz++) {
-            if (loc[2][iz] < 0) continue;
+            for (int a = 0; a < 3; a++) {
+                int l = b0[a] - o3[a];
+                l = l >= (g.N >> 1) ? l - g.N : (l < -(g.N >> 1) ? l + g.N : l);
+                l0[a] = l;
+            }
+            // x: start inside the padded row or discard the whole entry (it overlaps this tile only
+            // for another interlacing shift)
+            const bool okx = l0[0] >= -P && l0[0] < g.TX;
+            const int xo = okx ? l0[0] + P : 0;
+            // row offsets (cells) per y and z stencil cell, out-of-tile ones pushed past the discard row
+            constexpr int BIG = 1 << 28;
+            const int SZ = SXD * g.TY, DISCARD = SXD * ROWS;
+            int ly[W], lz[W];
 #pragma unroll
-            for (int iy = 0; iy < W; iy++) {
-                if (loc[1][iy] < 0) continue;
-                const int row = g.TX * (loc[1][iy] + g.TY * loc[2][iz]);
-                const T wyz = w[1][iy] * w[2][iz];
+            for (int o = 0; o < W; o++) {
+                const int y = l0[1] + o, z = l0[2] + o;
+                ly[o] = (unsigned)y < (unsigned)g.TY ? __mul24(y, SXD) : BIG;
+                lz[o] = (okx && (unsigned)z < (unsigned)g.TZ) ? __mul24(z, SZ) : BIG;
+            }
 #pragma unroll
-                for (int ix = 0; ix < W; ix++) {
-                    if (loc[0][ix] < 0) continue;
-                    if constexpr (ACC == ACC_FIXED) {
-                        // the oracle's product order, (wx wy) wz, then the fixed-point scale in FP64
-                        const T wv = w[0][ix] * w[1][iy] * w[2][iz];
-                        atomicAdd(&acc[row + loc[0][ix]], (unsigned long long)__double2ll_rn((double)wv * fscale));
-                    } else if constexpr (ACC == ACC_Q31) {
-                        atomicAdd(&acc[row + loc[0][ix]], (unsigned long long)(unsigned)rintf(w[0][ix] * wyz));
-                    } else {
-                        const T wv = w[0][ix] * w[1][iy] * w[2][iz];
-                        atomicAdd(&acc[row + loc[0][ix]], wv);
+            for (int iz = 0; iz < W; iz++)
+#pragma unroll
+                for (int iy = 0; iy < W; iy++) {
+                    Acc *r = acc + (xo + min(ly[iy] + lz[iz], DISCARD));
+                    const T wyz = w[1][iy] * w[2][iz];
+#pragma unroll
+                    for (int ix = 0; ix < W; ix++) dep_add<T, ACC>(r + ix, w[0][ix], w[1][iy], w[2][iz], wyz, fscale);
+                }
+        } else {
+            int loc[3][W];
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+#pragma unroll
+                for (int o = 0; o < W; o++) {
+                    const int l = wrapn(b0[a] + o, g.N) - o3[a];
+                    loc[a][o] = (l >= 0 && l < T3[a]) ? l : -1;
+                }
+#pragma unroll
+            for (int iz = 0; iz < W; iz++) {
+                if (loc[2][iz] < 0) continue;
+#pragma unroll
+                for (int iy = 0; iy < W; iy++) {
+                    if (loc[1][iy] < 0) continue;
+                    const int row = g.TX * (loc[1][iy] + g.TY * loc[2][iz]);
+                    const T wyz = w[1][iy] * w[2][iz];
+#pragma unroll
+                    for (int ix = 0; ix < W; ix++) {
+                        if (loc[0][ix] < 0) continue;
+                        dep_add<T, ACC>(&acc[row + loc[0][ix]], w[0][ix], w[1][iy], w[2][iz], wyz, fscale);
                     }
                 }
             }
         }
     }
     __syncthreads();
-    for (int c = threadIdx.x; c < TV; c += blockDim.x) {
-        const int lx = c & (g.TX - 1), ly = (c >> g.lT[0]) & (g.TY - 1), lz = c >> (g.lT[0] + g.lT[1]);
-        const size_t gi = (size_t)(ox + lx) + (size_t)g.N * ((size_t)(oy + ly) + (size_t)g.N * (oz + lz));
-        if constexpr (ACC == ACC_Q31) ((T *)mesh)[gi] = (T)((double)acc[c] * 0x1p-31);
-        else ((Acc *)mesh)[gi] = acc[c];
+    // one x row of the tile per TX threads (the block size is a multiple of TX); cell indices < N^3 <= 2^30
+    const unsigned N = (unsigned)g.N, lx = threadIdx.x & (g.TX - 1);
+    for (int r = threadIdx.x >> g.lT[0]; r < ROWS; r += blockDim.x >> g.lT[0]) {
+        const unsigned ly = r & (g.TY - 1), lz = r >> g.lT[1];
+        const unsigned gi = (ox + lx) + N * ((oy + ly) + N * (oz + lz));
+        const Acc v = acc[(PAD ? lx + P : lx) + SXD * r];
+        if constexpr (ACC == ACC_Q31) ((T *)mesh)[gi] = (T)((double)v * 0x1p-31);
+        else ((Acc *)mesh)[gi] = v;
     }
 }
 
@@ -710,11 +765,14 @@ struct TileLaunch {
     }
     template <int W, int ACC>
     static void dep_w(const TileArgs &a, const double *sh, void *mesh, double fscale, hipStream_t st) {
-        size_t shm = (size_t)a.g.TX * a.g.TY * a.g.TZ * (ACC == ACC_T ? sizeof(T) : 8);
-        auto kern = k_tile_deposit<T, W, ACC>;
+        const TileGeom &g = a.g;
+        const bool pad = g.ntx >= 2 && g.nty >= 2 && g.ntz >= 2;
+        const size_t cells = pad ? (size_t)(g.TX + 2 * (W - 1)) * (g.TY * g.TZ + 1) : (size_t)g.TX * g.TY * g.TZ;
+        const size_t shm = cells * (ACC == ACC_T ? sizeof(T) : 8);
+        auto kern = pad ? k_tile_deposit<T, W, ACC, true> : k_tile_deposit<T, W, ACC, false>;
         (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
-        kern<<<a.g.nt, 256, shm, st>>>((const Entry<T> *)a.ent, a.start, a.gstart, a.cur, a.g, (T)sh[0], (T)sh[1],
-                                       (T)sh[2], (T)a.L, mesh, fscale);
+        kern<<<g.nt, 256, shm, st>>>((const Entry<T> *)a.ent, a.start, a.gstart, a.cur, g, (T)sh[0], (T)sh[1],
+                                     (T)sh[2], (T)a.L, mesh, fscale);
     }
     template <int W>
     static void dep_sel(const TileArgs &a, bool fixed, const double *sh, void *mesh, double fscale, hipStream_t st) {
