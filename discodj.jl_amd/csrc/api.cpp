@@ -78,6 +78,13 @@ struct dj_ctx {
     void *rs_phase = nullptr;                   // px (n), py (n), pz (n/2+1) of the current offset
     int n_offsets = 0;
     double offsets[64][3];
+    // reverse mode: work buffers (allocated on first use) and the step checkpoints of dj_run_checkpointed
+    AdjointOps adj{};
+    void *vb_psi1 = nullptr, *vb_abar = nullptr, *vb_psibar = nullptr, *vb_pibar = nullptr;
+    void *vb_fbar = nullptr, *vb_pbar = nullptr;
+    void *ckpt = nullptr;  // (steps, 2, 3 n^3): Psi and Pi before each step
+    int ck_steps = 0;
+    std::vector<double> ck_dd1, ck_alpha, ck_beta, ck_dd2;
     std::string err;
     bool prof = false;
     std::vector<ProfEvent> pending;
@@ -295,7 +302,8 @@ int aa_shifts(int aa, double dh, double (*s)[3]) {
 void free_all(dj_ctx *c) {
     void *bufs[] = {c->psi, c->mom, c->acc, c->psi_in, c->q1, c->mesh, c->meshfix, c->A, c->B, c->C, c->tw, c->tabs,
                     c->tcnt, c->tstart, c->tgstart, c->tcur, c->tflag, c->tpart, c->tent, c->d_shifts,
-                    c->rs_tw, c->rs_psi, c->rs_hat, c->rs_A, c->rs_B, c->rs_xr, c->rs_phase};
+                    c->rs_tw, c->rs_psi, c->rs_hat, c->rs_A, c->rs_B, c->rs_xr, c->rs_phase,
+                    c->vb_psi1, c->vb_abar, c->vb_psibar, c->vb_pibar, c->vb_fbar, c->vb_pbar, c->ckpt};
     // teardown: failures here have no caller to report to
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -479,6 +487,7 @@ dj_status setup_ops(dj_ctx *c) {
     part_ops<T>(c->part);
     tile_ops<T>(c->tiles);
     resample_ops<T>(c->rs);
+    adjoint_ops<T>(c->adj);
     return upload_tables<T>(c);
 }
 
@@ -498,7 +507,7 @@ dj_status setup_resampling(dj_ctx *c) {
                 c->offsets[c->n_offsets][2] = (double)d / nr;
                 c->n_offsets++;
             }
-    const size_t n3 = (size_t)c->n3, hc = (size_t)n * n * (n / 2 + 1);
+    const size_t n3 = (size_t)c->np, hc = (size_t)n * n * (n / 2 + 1);
     HIPCHK(c, hipMalloc(&c->rs_xr, 3 * n3 * sizeof(T)));
     if (c->cfg.resampling == 1) {
         HIPCHK(c, hipMalloc(&c->rs_psi, 4 * n3 * sizeof(T)));  // 3 wrapped components + 1 shifted
@@ -545,7 +554,7 @@ template <typename T>
 dj_status resample_deposit(dj_ctx *c, const void *psi, const double *sh, void *dens, bool fixed, double fscale) {
     const int n = c->n, H = n / 2 + 1;
     const double L = c->cfg.boxsize;
-    const size_t n3 = (size_t)c->n3, hc = (size_t)n * n * H;
+    const size_t n3 = (size_t)c->np, hc = (size_t)n * n * H;
     T *xr = (T *)c->rs_xr;
     if (c->cfg.resampling == 1) {
         T *w = (T *)c->rs_psi;
@@ -900,6 +909,190 @@ dj_status dj_ics_lpt(dj_ctx *c, uint32_t seed, const double *white, const double
     int e = ics_lpt_device(n, c->cfg.boxsize, order, white, white ? nullptr : field.data(), k_tab, pk_tab, ntab, D,
                            c->cfg.precision, c->psi, c->mom, c->st);
     if (e) return hip_fail(c, (hipError_t)e, "dj_ics_lpt");
+    return DJ_OK;
+}
+
+// ------------------------------------------------------------------------ reverse mode
+dj_status adj_supported(dj_ctx *c) {
+    if (c->cfg.grad_order == 0)
+        return fail(c, DJ_ERR_UNSUPPORTED, "the adjoint is built for finite-difference gradients (grad_order 2/4/6)");
+    if (c->cfg.n_resample > 1) return fail(c, DJ_ERR_UNSUPPORTED, "the adjoint is built without sheet resampling");
+    return DJ_OK;
+}
+
+dj_status adj_buffers(dj_ctx *c) {
+    const size_t pb = (size_t)c->n3 * c->tsize, mb = c->M * c->tsize;
+    void **pbufs[] = {&c->vb_psi1, &c->vb_abar, &c->vb_psibar, &c->vb_pibar};
+    for (void **b : pbufs)
+        if (!*b) HIPCHK(c, hipMalloc(b, pb));
+    if (!c->vb_fbar) HIPCHK(c, hipMalloc(&c->vb_fbar, 3 * mb));
+    if (!c->vb_pbar) HIPCHK(c, hipMalloc(&c->vb_pbar, mb));
+    return DJ_OK;
+}
+
+// psibar += (d acc / d psi)^T abar at psi (device arrays), for the averaged interlacing passes: per pass the
+// potential phi is recomputed with the forward kernels, abar / ns is deposited to Fbar, phibar = sum_a D_a
+// Fbar_a goes through the same Poisson operator (self-adjoint), and one gather adds the weight-gradient terms.
+dj_status vjp_force(dj_ctx *c, const void *psi, const void *abar, void *psibar) {
+    const dj_pm_config &cfg = c->cfg;
+    const bool fixed = cfg.deposit == DJ_DEPOSIT_FIXED64;
+    const int ns = c->nshifts, N = c->res;
+    void *dens = fixed ? c->meshfix : c->mesh;
+    const double fscale = std::ldexp(1.0, -(cfg.fixed_frac_bits ? cfg.fixed_frac_bits : 32));
+    dj_status r = tile_sort(c, psi, c->d_shifts, ns, cfg.worder);
+    if (r) return r;
+    for (int s = 0; s < ns; s++) {
+        TileArgs ta = tile_args(c, psi, c->d_shifts, ns);
+        c->tiles.deposit(ta, cfg.worder, fixed, c->shifts[s], dens, fscale_dep(cfg), c->st);
+        c->fft.xf(dens, fixed, c->A, c->tw, fscale, c->st);
+        c->fft.yf(c->A, c->B, c->tw, c->st);
+        c->fft.zm(ZM_POT, true, true, c->B, c->B, c->tw, c->kmult, c->st);
+        c->fft.yi(c->B, c->A, c->tw, c->st);
+        c->fft.xi(c->A, c->mesh, c->tw, c->st);
+        HIPCHK(c, hipMemsetAsync(c->vb_fbar, 0, 3 * c->M * c->tsize, c->st));
+        c->adj.scatter3(psi, c->q1, c->n, N, cfg.boxsize, c->shifts[s], abar, 1.0 / ns, cfg.worder, c->vb_fbar, c->st);
+        c->adj.divergence(c->vb_fbar, N, c->fdc, c->vb_pbar, c->st);
+        c->fft.xf(c->vb_pbar, false, c->A, c->tw, 1.0, c->st);
+        c->fft.yf(c->A, c->B, c->tw, c->st);
+        c->fft.zm(ZM_POT, true, true, c->B, c->B, c->tw, c->kmult, c->st);
+        c->fft.yi(c->B, c->A, c->tw, c->st);
+        c->fft.xi(c->A, c->vb_pbar, c->tw, c->st);
+        c->adj.gather_grad(psi, c->q1, c->n, N, cfg.boxsize, c->shifts[s], c->vb_pbar, c->mesh, abar, 1.0 / ns,
+                           c->fdc, cfg.worder, psibar, c->st);
+        r = check_launch(c, "force adjoint");
+        if (r) return r;
+    }
+    return DJ_OK;
+}
+
+dj_status dj_vjp_acceleration(dj_ctx *c, const void *psi, const void *acc_bar, void *psi_bar_out) {
+    if (!c || !acc_bar || !psi_bar_out) return fail(c, DJ_ERR_INVALID_ARGUMENT, "null argument");
+    dj_status r = adj_supported(c);
+    if (r) return r;
+    HIPCHK(c, hipSetDevice(c->dev));
+    r = adj_buffers(c);
+    if (r) return r;
+    const size_t pb = (size_t)c->n3 * c->tsize;
+    const void *src = c->psi;
+    if (psi) {
+        HIPCHK(c, hipMemcpyAsync(c->vb_psi1, psi, pb, hipMemcpyHostToDevice, c->st));
+        src = c->vb_psi1;
+    }
+    HIPCHK(c, hipMemcpyAsync(c->vb_abar, acc_bar, pb, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemsetAsync(c->vb_psibar, 0, pb, c->st));
+    r = vjp_force(c, src, c->vb_abar, c->vb_psibar);
+    if (r) return r;
+    HIPCHK(c, hipMemcpyAsync(psi_bar_out, c->vb_psibar, pb, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return DJ_OK;
+}
+
+dj_status dj_vjp_deposit(dj_ctx *c, const void *psi, const void *mesh_bar, void *psi_bar_out) {
+    if (!c || !mesh_bar || !psi_bar_out) return fail(c, DJ_ERR_INVALID_ARGUMENT, "null argument");
+    HIPCHK(c, hipSetDevice(c->dev));
+    dj_status r = adj_buffers(c);
+    if (r) return r;
+    const size_t pb = (size_t)c->n3 * c->tsize;
+    const void *src = c->psi;
+    if (psi) {
+        HIPCHK(c, hipMemcpyAsync(c->vb_psi1, psi, pb, hipMemcpyHostToDevice, c->st));
+        src = c->vb_psi1;
+    }
+    HIPCHK(c, hipMemcpyAsync(c->vb_pbar, mesh_bar, c->M * c->tsize, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemsetAsync(c->vb_psibar, 0, pb, c->st));
+    const double zero[3] = {0, 0, 0};
+    c->adj.gather_grad(src, c->q1, c->n, c->res, c->cfg.boxsize, zero, c->vb_pbar, nullptr, nullptr, 0.0, c->fdc,
+                       c->cfg.worder, c->vb_psibar, c->st);
+    r = check_launch(c, "deposit adjoint");
+    if (r) return r;
+    HIPCHK(c, hipMemcpyAsync(psi_bar_out, c->vb_psibar, pb, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return DJ_OK;
+}
+
+dj_status dj_run_checkpointed(dj_ctx *c, int32_t n_steps, const double *dd1, const double *alpha, const double *beta,
+                              const double *dd2) {
+    if (!c || n_steps < 1 || !dd1 || !alpha || !beta || !dd2)
+        return fail(c, DJ_ERR_INVALID_ARGUMENT, "invalid run arguments");
+    HIPCHK(c, hipSetDevice(c->dev));
+    const size_t pb = (size_t)c->n3 * c->tsize;
+    if (c->ckpt && c->ck_steps < n_steps) {
+        HIPCHK(c, hipFree(c->ckpt));
+        c->ckpt = nullptr;
+    }
+    if (!c->ckpt) HIPCHK(c, hipMalloc(&c->ckpt, 2 * pb * (size_t)n_steps));
+    c->ck_steps = n_steps;
+    c->ck_dd1.assign(dd1, dd1 + n_steps);
+    c->ck_alpha.assign(alpha, alpha + n_steps);
+    c->ck_beta.assign(beta, beta + n_steps);
+    c->ck_dd2.assign(dd2, dd2 + n_steps);
+    for (int s = 0; s < n_steps; s++) {
+        char *ck = (char *)c->ckpt + 2 * pb * (size_t)s;
+        HIPCHK(c, hipMemcpyAsync(ck, c->psi, pb, hipMemcpyDeviceToDevice, c->st));
+        HIPCHK(c, hipMemcpyAsync(ck + pb, c->mom, pb, hipMemcpyDeviceToDevice, c->st));
+        c->part.drift(c->psi, c->mom, c->n3, dd1[s], c->st);
+        KickArgs k{alpha[s], beta[s], dd2[s], 0.0, 0, c->nshifts};
+        dj_status r = force(c, c->psi, FORCE_KICK, k);
+        if (r) return r;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    drain_prof(c);
+    return check_launch(c, "checkpointed run");
+}
+
+dj_status dj_vjp_run(dj_ctx *c, void *psi_bar, void *pi_bar) {
+    if (!c || !psi_bar || !pi_bar) return fail(c, DJ_ERR_INVALID_ARGUMENT, "null argument");
+    if (!c->ckpt || c->ck_steps < 1) return fail(c, DJ_ERR_INVALID_ARGUMENT, "no checkpointed run to reverse");
+    dj_status r = adj_supported(c);
+    if (r) return r;
+    HIPCHK(c, hipSetDevice(c->dev));
+    r = adj_buffers(c);
+    if (r) return r;
+    const size_t pb = (size_t)c->n3 * c->tsize, np3 = (size_t)c->n3;
+    HIPCHK(c, hipMemcpyAsync(c->vb_psibar, psi_bar, pb, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->vb_pibar, pi_bar, pb, hipMemcpyHostToDevice, c->st));
+    for (int s = c->ck_steps - 1; s >= 0; s--) {
+        const char *ck = (const char *)c->ckpt + 2 * pb * (size_t)s;
+        // the drifted positions of the step's force, recomputed from its checkpoint
+        HIPCHK(c, hipMemcpyAsync(c->vb_psi1, ck, pb, hipMemcpyDeviceToDevice, c->st));
+        c->part.drift(c->vb_psi1, ck + pb, c->n3, c->ck_dd1[s], c->st);
+        c->adj.kick(c->vb_psibar, c->vb_pibar, c->vb_abar, np3, c->ck_alpha[s], c->ck_beta[s], c->ck_dd2[s], c->st);
+        r = vjp_force(c, c->vb_psi1, c->vb_abar, c->vb_psibar);
+        if (r) return r;
+        c->adj.drift(c->vb_psibar, c->vb_pibar, np3, c->ck_dd1[s], c->st);
+    }
+    r = check_launch(c, "run adjoint");
+    if (r) return r;
+    HIPCHK(c, hipMemcpyAsync(psi_bar, c->vb_psibar, pb, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(pi_bar, c->vb_pibar, pb, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return DJ_OK;
+}
+
+dj_status dj_vjp_ics_lpt(dj_ctx *c, uint32_t seed, const double *white, const double *k_tab, const double *pk_tab,
+                         int32_t ntab, double D, int32_t order, const void *psi_bar, const void *pi_bar,
+                         double *white_bar_out) {
+    if (!c || !k_tab || !pk_tab || ntab < 2 || (order != 1 && order != 2) || !psi_bar || !pi_bar || !white_bar_out)
+        return fail(c, DJ_ERR_INVALID_ARGUMENT, "invalid IC adjoint arguments");
+    const int n = c->n;
+    FftOps probe{};
+    if (n % 2 || !fft_dispatch<double>(n, probe) || (order == 2 && !fft_dispatch<double>(3 * n / 2, probe)))
+        return fail(c, DJ_ERR_UNSUPPORTED, "n_part " + std::to_string(n) + " has no built IC transform size");
+    HIPCHK(c, hipSetDevice(c->dev));
+    dj_status r = adj_buffers(c);
+    if (r) return r;
+    const size_t pb = (size_t)c->n3 * c->tsize;
+    HIPCHK(c, hipMemcpyAsync(c->vb_psibar, psi_bar, pb, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->vb_pibar, pi_bar, pb, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    std::vector<double> field;
+    if (!white) {
+        field.resize(2 * (size_t)n * n * (n / 2 + 1));
+        ngenic_field(seed, n, field.data());
+    }
+    int e = ics_lpt_vjp_device(n, c->cfg.boxsize, order, white, white ? nullptr : field.data(), k_tab, pk_tab, ntab,
+                               D, c->cfg.precision, c->vb_psibar, c->vb_pibar, white_bar_out, c->st);
+    if (e) return hip_fail(c, (hipError_t)e, "dj_vjp_ics_lpt");
     return DJ_OK;
 }
 

@@ -116,6 +116,25 @@ struct ResampleOps {
 };
 template <typename T> void resample_ops(ResampleOps &ops);
 
+// reverse mode of the force and the DKD step (adjoint.hip); particle arrays (3, n^3), meshes N^3
+struct AdjointOps {
+    // Fbar_a += sum_p W(X_p - c) abar_p,a s   (3 meshes)
+    void (*scatter3)(const void *psi, const void *q1, int n, int N, double L, const double *sh, const void *abar,
+                     double s, int W, void *fbar, hipStream_t st);
+    // phibar = sum_a D_a Fbar_a (FD coefficients c[3])
+    void (*divergence)(const void *fbar, int N, const double *c, void *phibar, hipStream_t st);
+    // psibar += N/L sum_c grad W(X - c) (rhobar_c + sum_a abar_a s F_a(c)), F = -D phi; rhobar, phi nullable
+    void (*gather_grad)(const void *psi, const void *q1, int n, int N, double L, const double *sh, const void *rhobar,
+                        const void *phi, const void *abar, double s, const double *c, int W, void *psibar,
+                        hipStream_t st);
+    // kick and second drift backward: Pibar_t = Pibar + dd2 Psibar; abar = beta Pibar_t; Pibar = alpha Pibar_t
+    void (*kick)(const void *psibar, void *pibar, void *abar, size_t n, double alpha, double beta, double dd2,
+                 hipStream_t st);
+    // first drift backward: Pibar += dd1 Psibar
+    void (*drift)(const void *psibar, void *pibar, size_t n, double dd1, hipStream_t st);
+};
+template <typename T> void adjoint_ops(AdjointOps &ops);
+
 // power spectrum binning of an x-halved B-layout spectrum (analysis.hip)
 template <typename T>
 void pk_bin(const void *B, int N, double L, int W, int nbins, double scale, double *sum_p, double *sum_w,
@@ -127,5 +146,8 @@ void ngenic_field(uint32_t seed, int res, double *out);
 int ics_lpt_device(int n, double L, int order, const double *white_host, const double *field_host,
                    const double *kt, const double *pt, int ntab, double D, int prec, void *psi_out,
                    void *pi_out, hipStream_t st);
+int ics_lpt_vjp_device(int n, double L, int order, const double *white_host, const double *field_host,
+                       const double *kt, const double *pt, int ntab, double D, int prec, const void *psibar,
+                       const void *pibar, double *wbar_host, hipStream_t st);
 
 }  // namespace dj

@@ -70,9 +70,10 @@ __device__ __forceinline__ double kdir(const BIdx &b, int c, int N, double dk) {
     return m == N / 2 ? 0.0 : sgn_full(m, N) * dk;
 }
 
-enum { IC_PSI1 = 0, IC_DERIV = 1, IC_PSI2 = 2 };
+enum { IC_PSI1 = 0, IC_DERIV = 1, IC_PSI2 = 2, IC_PSI2_ADJ = 3 };
 
-// out = mult(k) in:  PSI1: -(i k_c) ;  DERIV: i k_c ;  PSI2: (-1/k^2) (i k_c)
+// out = mult(k) in:  PSI1: -(i k_c) ;  DERIV: i k_c ;  PSI2: (-1/k^2) (i k_c) ;  PSI2_ADJ: the complex
+// conjugate of PSI2's multiplier, (1/k^2) (i k_c)  (the adjoint of a complex multiplier is its conjugate)
 template <int MODE>
 __global__ void k_ic_mult(const cx<double> *in, cx<double> *out, int N, double L, int c) {
     const int H = N / 2 + 1;
@@ -84,11 +85,11 @@ __global__ void k_ic_mult(const cx<double> *in, cx<double> *out, int N, double L
     cx<double> v = in[e];
     cx<double> r = {-g * v.y, g * v.x};  // (i g) v
     if constexpr (MODE == IC_PSI1) r = {-r.x, -r.y};
-    if constexpr (MODE == IC_PSI2) {
+    if constexpr (MODE == IC_PSI2 || MODE == IC_PSI2_ADJ) {
         double kx = b.kx * dk, ky = sgn_full(b.ky, N) * dk, kz = sgn_full(b.kz, N) * dk;
         double k2 = (kx * kx + ky * ky) + kz * kz;
         double il = -1.0 / (e == 0 ? 1.0 : k2);
-        r = scale(r, il);
+        r = scale(r, MODE == IC_PSI2 ? il : -il);
     }
     out[e] = r;
 }
@@ -143,7 +144,59 @@ __global__ void k_ic_combine(const double *p1, const double *p2, T *psi, T *pi, 
     pi[i] = (T)b;
 }
 
+// Adjoint helpers.  On the x-halved spectrum the r2c / c2r pair is adjoint up to the half-axis weights
+// w(kx) = 1 at kx = 0 and kx = N/2, 2 otherwise (nlpt_core.jl:682-713): for y = c2r(Y) / M,
+// Ybar = w r2c(ybar) / M; for Y = r2c(x), xbar = c2r(Ybar / w).
+__global__ void k_ic_halfw(cx<double> *B, int N, int divide, double s) {
+    const int H = N / 2 + 1;
+    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x, tot = (size_t)N * H * N;
+    if (e >= tot) return;
+    const int kx = (int)((e / N) % H);
+    const double w = (kx == 0 || kx == N / 2) ? 1.0 : 2.0;
+    B[e] = scale(B[e], divide ? s / w : s * w);
+}
+__global__ void k_ic_axpy(cx<double> *out, const cx<double> *in, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = {out[i].x + in[i].x, out[i].y + in[i].y};
+}
+// Abar = mubar * (a + b) (b may be null), or -mubar * a when neg
+__global__ void k_ic_partner(double *out, const double *mubar, const double *a, const double *b, size_t n, int neg) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double p = b ? a[i] + b[i] : a[i];
+    out[i] = neg ? -(mubar[i] * p) : mubar[i] * p;
+}
+// (Psibar, Pibar) -> (psi1bar, psi2bar) of Psi = D psi1 + D^2 psi2, Pi = psi1 + 2 D psi2
+template <typename T>
+__global__ void k_ic_combine_adj(const T *psibar, const T *pibar, double *p1b, double *p2b, size_t n3, double D, int order) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n3) return;
+    const double a = (double)psibar[i], b = (double)pibar[i];
+    p1b[i] = D * a + b;
+    if (order >= 2) p2b[i] = (D * D) * a + (2 * D) * b;
+}
+
 static unsigned g1(size_t n) { return (unsigned)((n + 255) / 256); }
+
+// N-GenIC field (z halved, (n, n, n/2+1)) -> B layout (x halved) by Hermitian symmetry, times n^(3/2)
+static void field_to_blayout(int n, const double *field_host, cx<double> *hb) {
+    const int Hn = n / 2 + 1;
+    const double sc = std::pow((double)n, 1.5);
+    const cx<double> *f = (const cx<double> *)field_host;
+#pragma omp parallel for collapse(2)
+    for (int ky = 0; ky < n; ky++)
+        for (int kx = 0; kx < Hn; kx++)
+            for (int kz = 0; kz < n; kz++) {
+                cx<double> v;
+                if (kz <= n / 2) {
+                    v = f[(size_t)kx + n * ((size_t)ky + (size_t)n * kz)];
+                } else {
+                    int mx = (n - kx) % n, my = (n - ky) % n, mz = n - kz;
+                    v = conj(f[(size_t)mx + n * ((size_t)my + (size_t)n * mz)]);
+                }
+                hb[(size_t)kz + (size_t)n * (kx + (size_t)Hn * ky)] = scale(v, sc);
+            }
+}
 
 // host orchestration; returns a HIP error code (0 = success)
 int ics_lpt_device(int n, double L, int order, const double *white_host, const double *field_host,
@@ -196,23 +249,8 @@ int ics_lpt_device(int n, double L, int order, const double *white_host, const d
     if (white_host) {
         err = hipMemcpy(w, white_host, M * 8, hipMemcpyHostToDevice);
     } else {
-        // N-GenIC field (z halved) -> B layout (x halved) by Hermitian symmetry, times n^(3/2)
         std::vector<cx<double>> hb(HC);
-        const double sc = std::pow((double)n, 1.5);
-        const cx<double> *f = (const cx<double> *)field_host;
-#pragma omp parallel for collapse(2)
-        for (int ky = 0; ky < n; ky++)
-            for (int kx = 0; kx < Hn; kx++)
-                for (int kz = 0; kz < n; kz++) {
-                    cx<double> v;
-                    if (kz <= n / 2) {
-                        v = f[(size_t)kx + n * ((size_t)ky + (size_t)n * kz)];
-                    } else {
-                        int mx = (n - kx) % n, my = (n - ky) % n, mz = n - kz;
-                        v = conj(f[(size_t)mx + n * ((size_t)my + (size_t)n * mz)]);
-                    }
-                    hb[(size_t)kz + (size_t)n * (kx + (size_t)Hn * ky)] = scale(v, sc);
-                }
+        field_to_blayout(n, field_host, hb.data());
         err = hipMemcpy(Bm, hb.data(), HC * 16, hipMemcpyHostToDevice);
         if (err == hipSuccess) {
             inv(fn, Bm, A, w, twn);
@@ -271,6 +309,157 @@ int ics_lpt_device(int n, double L, int order, const double *white_host, const d
     else
         k_ic_combine<float><<<g1(3 * M), 256, 0, st>>>(p1, p2, (float *)psi_out, (float *)pi_out, 3 * M, D, order);
     err = hipGetLastError();
+    if (err == hipSuccess) err = hipStreamSynchronize(st);
+done:
+    (void)hipStreamSynchronize(st);
+    for (void *p : allocs) (void)hipFree(p);
+    return (int)err;
+}
+
+// Reverse mode of ics_lpt_device: (Psibar, Pibar) of the ICs (device, precision prec) -> the white-noise
+// cotangent wbar (host, n^3 FP64).  The forward fields are recomputed (phi(k), the nine A_ik on the ext
+// grid, three at a time); each linear step is transposed with the adjoint rules above, and the 2LPT
+// source mu is a quadratic form whose cotangent per field is mubar times its partner in mu.
+int ics_lpt_vjp_device(int n, double L, int order, const double *white_host, const double *field_host,
+                       const double *kt_h, const double *pt_h, int ntab, double D, int prec, const void *psibar,
+                       const void *pibar, double *wbar_host, hipStream_t st) {
+    FftOps fn{}, fe{};
+    const int ext = 3 * n / 2;
+    if (!fft_dispatch<double>(n, fn)) return (int)hipErrorInvalidValue;
+    if (order >= 2 && !fft_dispatch<double>(ext, fe)) return (int)hipErrorInvalidValue;
+    const int Hn = n / 2 + 1, He = ext / 2 + 1;
+    const size_t M = (size_t)n * n * n, HC = (size_t)n * Hn * n, ME = (size_t)ext * ext * ext,
+                 HE = (size_t)ext * He * ext;
+    std::vector<void *> allocs;
+    hipError_t err = hipSuccess;
+    auto alloc = [&](size_t bytes) -> void * {
+        void *p = nullptr;
+        if (err == hipSuccess) err = hipMalloc(&p, bytes);
+        if (p) allocs.push_back(p);
+        return p;
+    };
+    auto twiddles = [&](int N) -> void * {
+        std::vector<cx<double>> tw(N);
+        for (int k = 0; k < N; k++) {
+            double a = -2.0 * M_PI * (double)k / (double)N;
+            tw[k] = {std::cos(a), std::sin(a)};
+        }
+        void *d = alloc(sizeof(cx<double>) * N);
+        if (d) err = hipMemcpy(d, tw.data(), sizeof(cx<double>) * N, hipMemcpyHostToDevice);
+        return d;
+    };
+    void *twn = twiddles(n);
+    cx<double> *A = (cx<double> *)alloc(HC * 16), *Bm = (cx<double> *)alloc(HC * 16), *F = (cx<double> *)alloc(HC * 16),
+               *Fb = (cx<double> *)alloc(HC * 16), *T1 = (cx<double> *)alloc(HC * 16);
+    double *w = (double *)alloc(M * 8), *p1b = (double *)alloc(3 * M * 8), *p2b = (double *)alloc(3 * M * 8);
+    double *kt = (double *)alloc(ntab * 8), *pt = (double *)alloc(ntab * 8);
+    if (err == hipSuccess) err = hipMemcpy(kt, kt_h, ntab * 8, hipMemcpyHostToDevice);
+    if (err == hipSuccess) err = hipMemcpy(pt, pt_h, ntab * 8, hipMemcpyHostToDevice);
+    const KMult<double> none{};
+    auto fwd = [&](FftOps &f, const double *real, cx<double> *a, cx<double> *b, void *tw) {
+        f.xf(real, false, a, tw, 1.0, st);
+        f.yf(a, b, tw, st);
+        f.zm(ZM_NONE, true, false, b, b, tw, &none, st);
+    };
+    auto inv = [&](FftOps &f, cx<double> *b, cx<double> *a, double *real, void *tw) {  // destroys b
+        f.zm(ZM_NONE, false, true, b, b, tw, &none, st);
+        f.yi(b, a, tw, st);
+        f.xi(a, real, tw, st);
+    };
+    if (err != hipSuccess) goto done;
+    if (white_host) {
+        err = hipMemcpy(w, white_host, M * 8, hipMemcpyHostToDevice);
+    } else {
+        std::vector<cx<double>> hb(HC);
+        field_to_blayout(n, field_host, hb.data());
+        err = hipMemcpy(Bm, hb.data(), HC * 16, hipMemcpyHostToDevice);
+        if (err == hipSuccess) {
+            inv(fn, Bm, A, w, twn);
+            k_ic_scale_real<<<g1(M), 256, 0, st>>>(w, M, 1.0 / (double)M);
+        }
+    }
+    if (err != hipSuccess) goto done;
+    // forward phi(k), then the state cotangents split into psi1bar, psi2bar
+    fwd(fn, w, A, F, twn);
+    k_ic_scale<<<g1(HC), 256, 0, st>>>(F, n, L, kt, pt, ntab);
+    if (prec == 64)
+        k_ic_combine_adj<double><<<g1(3 * M), 256, 0, st>>>((const double *)psibar, (const double *)pibar, p1b, p2b,
+                                                          3 * M, D, order);
+    else
+        k_ic_combine_adj<float><<<g1(3 * M), 256, 0, st>>>((const float *)psibar, (const float *)pibar, p1b, p2b,
+                                                         3 * M, D, order);
+    (void)hipMemsetAsync(Fb, 0, HC * 16, st);
+    // psi1_c = c2r(-(i k_c) phi) / M:  phibar += (i k_c) w r2c(psi1bar_c) / M
+    for (int c = 0; c < 3; c++) {
+        fwd(fn, p1b + c * M, A, Bm, twn);
+        k_ic_halfw<<<g1(HC), 256, 0, st>>>(Bm, n, 0, 1.0 / (double)M);
+        k_ic_mult<IC_DERIV><<<g1(HC), 256, 0, st>>>(Bm, T1, n, L, c);
+        k_ic_axpy<<<g1(HC), 256, 0, st>>>(Fb, T1, HC);
+    }
+    if (order >= 2) {
+        void *twe = twiddles(ext);
+        cx<double> *Ae = (cx<double> *)alloc(HE * 16), *Be = (cx<double> *)alloc(HE * 16), *Pe = (cx<double> *)alloc(HE * 16);
+        double *mub = (double *)alloc(ME * 8), *ab = (double *)alloc(ME * 8);
+        double *fld[3] = {(double *)alloc(ME * 8), (double *)alloc(ME * 8), (double *)alloc(ME * 8)};
+        cx<double> *fLb = (cx<double> *)alloc(HC * 16);
+        if (err != hipSuccess) goto done;
+        const double pad_s = std::pow((double)ext / n, 3), crop_s = std::pow((double)n / ext, 3);
+        // psi2_c = c2r((-1/k^2)(i k_c) fL) / M:  fLbar = sum_c (1/k^2)(i k_c) w r2c(psi2bar_c) / M
+        (void)hipMemsetAsync(fLb, 0, HC * 16, st);
+        for (int c = 0; c < 3; c++) {
+            fwd(fn, p2b + c * M, A, Bm, twn);
+            k_ic_halfw<<<g1(HC), 256, 0, st>>>(Bm, n, 0, 1.0 / (double)M);
+            k_ic_mult<IC_PSI2_ADJ><<<g1(HC), 256, 0, st>>>(Bm, T1, n, L, c);
+            k_ic_axpy<<<g1(HC), 256, 0, st>>>(fLb, T1, HC);
+        }
+        // fL = crop(r2c_ext(-3/7 mu)) crop_s:  mubar = -3/7 c2r_ext(pad(fLbar) crop_s / w)
+        k_ic_pad<<<g1(HE), 256, 0, st>>>(fLb, Be, n, ext, crop_s);
+        k_ic_halfw<<<g1(HE), 256, 0, st>>>(Be, ext, 1, 1.0);
+        inv(fe, Be, Ae, mub, twe);
+        k_ic_scale_real<<<g1(ME), 256, 0, st>>>(mub, ME, ((3.0 - 2) / 2 - 2.0) / ((2 + 1.5) * 1.0));
+        // A_ik = c2r_ext((i k_k) pad(-(i k_i) phi) pad_s) / ME
+        auto deriv = [&](int i, int k, double *out) {
+            k_ic_mult<IC_PSI1><<<g1(HC), 256, 0, st>>>(F, Bm, n, L, i);
+            k_ic_pad<<<g1(HE), 256, 0, st>>>(Bm, Pe, n, ext, pad_s);
+            k_ic_mult<IC_DERIV><<<g1(HE), 256, 0, st>>>(Pe, Be, ext, L, k);
+            inv(fe, Be, Ae, out, twe);
+            k_ic_scale_real<<<g1(ME), 256, 0, st>>>(out, ME, 1.0 / (double)ME);
+        };
+        // Abar_ik -> phibar: w r2c_ext(Abar) / ME, conj(i k_k), crop pad_s, conj(-(i k_i))
+        auto deriv_adj = [&](int i, int k, double *abar) {
+            fwd(fe, abar, Ae, Be, twe);
+            k_ic_halfw<<<g1(HE), 256, 0, st>>>(Be, ext, 0, 1.0 / (double)ME);
+            k_ic_mult<IC_PSI1><<<g1(HE), 256, 0, st>>>(Be, Pe, ext, L, k);
+            k_ic_crop<<<g1(HC), 256, 0, st>>>(Pe, Bm, n, ext, pad_s);
+            k_ic_mult<IC_DERIV><<<g1(HC), 256, 0, st>>>(Bm, T1, n, L, i);
+            k_ic_axpy<<<g1(HC), 256, 0, st>>>(Fb, T1, HC);
+        };
+        // mu = A11 A22 + A11 A33 + A22 A33 - A12 A21 - A13 A31 - A23 A32
+        deriv(0, 0, fld[0]);
+        deriv(1, 1, fld[1]);
+        deriv(2, 2, fld[2]);
+        for (int d = 0; d < 3; d++) {
+            const double *o1 = fld[(d + 1) % 3], *o2 = fld[(d + 2) % 3];
+            k_ic_partner<<<g1(ME), 256, 0, st>>>(ab, mub, o1, o2, ME, 0);
+            deriv_adj(d, d, ab);
+        }
+        const int pairs[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+        for (auto &pr : pairs) {
+            const int i = pr[0], k = pr[1];
+            deriv(i, k, fld[0]);
+            deriv(k, i, fld[1]);
+            k_ic_partner<<<g1(ME), 256, 0, st>>>(ab, mub, fld[1], nullptr, ME, 1);
+            deriv_adj(i, k, ab);
+            k_ic_partner<<<g1(ME), 256, 0, st>>>(ab, mub, fld[0], nullptr, ME, 1);
+            deriv_adj(k, i, ab);
+        }
+    }
+    // phi = r2c(w) s(k):  wbar = c2r(phibar s / w)
+    k_ic_scale<<<g1(HC), 256, 0, st>>>(Fb, n, L, kt, pt, ntab);
+    k_ic_halfw<<<g1(HC), 256, 0, st>>>(Fb, n, 1, 1.0);
+    inv(fn, Fb, A, w, twn);
+    err = hipGetLastError();
+    if (err == hipSuccess) err = hipMemcpyAsync(wbar_host, w, M * 8, hipMemcpyDeviceToHost, st);
     if (err == hipSuccess) err = hipStreamSynchronize(st);
 done:
     (void)hipStreamSynchronize(st);

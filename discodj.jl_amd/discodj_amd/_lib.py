@@ -65,6 +65,12 @@ _SIGS = {
                                       C.c_void_p]),
     "dj_ngenic_field": (C.c_int32, [C.c_uint32, C.c_int32, C.c_void_p]),
     "dj_ranlxd1_uniforms": (C.c_int32, [C.c_uint32, C.c_int32, C.c_void_p]),
+    "dj_vjp_acceleration": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "dj_vjp_deposit": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "dj_run_checkpointed": (C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "dj_vjp_run": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "dj_vjp_ics_lpt": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32,
+                                   C.c_double, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
 }
 
 _lib = None

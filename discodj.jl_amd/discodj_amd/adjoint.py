@@ -1,0 +1,54 @@
+"""Gradient of a density loss with respect to the white-noise IC field, by reverse mode on the device.
+
+The forward chain is the path of run_nbody: white noise -> phi(k) -> 2LPT ICs (dj_ics_lpt) -> n_steps DKD
+steps (dj_run_checkpointed keeps the state before every step) -> the deposit of the final positions on the
+PM mesh, delta = mesh / rho_bar - 1 with rho_bar = n^3 / N^3.  For L = sum (delta - target)^2 the reverse
+pass is: deltabar = 2 (delta - target) -> Psibar of the deposit (dj_vjp_deposit) -> back through the steps,
+recomputing each force from its checkpoint (dj_vjp_run) -> back through the ICs (dj_vjp_ics_lpt).  The
+reference has no adjoint of its PM path (NR/README.md:169); the pieces follow its CIC and FFT rrules
+(NR/src/field/sheet_deposit.jl:118-132, NR/src/lpt/nlpt_core.jl:682-713).
+"""
+import numpy as np
+
+from .cosmology import linear_power_spectrum, stepper_coefficients
+from .pm import PMConfig, PMSolverHIP
+
+
+def density_loss_gradient(cosmo, white, target, *, boxsize, res_pm, a_ini=0.05, a_end=1.0, n_steps=10,
+                          stepper="bullfrog", time_var="D", order=2, worder=2, grad_kernel_order=4,
+                          laplace_kernel_order=0, antialias=0, deconvolve=False, precision=32, k_tab=None,
+                          pk_tab=None, solver=None):
+    """-> (loss, white_bar, delta): L = sum (delta_final - target)^2 on the res_pm mesh and its gradient with
+    respect to the (n,n,n) real white noise [k,j,i] of the ICs (FP64 array).  The momentum variable of the
+    steps is Pi (bullfrog, fastpm); the symplectic stepper's velocity scaling is applied and reversed."""
+    white = np.ascontiguousarray(white, np.float64)
+    n = white.shape[0]
+    if k_tab is None:
+        k_tab, pk_tab = linear_power_spectrum(cosmo)
+    cfg = PMConfig(res_pm=res_pm, n_part=n, boxsize=float(boxsize), worder=worder, deconvolve=deconvolve,
+                   antialias=antialias, grad_order=grad_kernel_order, lap_order=laplace_kernel_order)
+    co = stepper_coefficients(cosmo, stepper, a_ini, a_end, n_steps, time_var)
+    D = cosmo.Dplus(a_ini)
+    own = solver is None
+    S = solver or PMSolverHIP(cfg, precision=precision)
+    try:
+        S.ics_lpt(k_tab, pk_tab, D, order=order, white=white)
+        fp = cosmo.Fplus(a_ini) if stepper == "symplectic" else 1.0  # run_nbody.jl:104
+        if fp != 1.0:
+            psi0, pi0 = S.get_state()
+            S.set_state(psi0, (pi0.astype(np.float64) * fp).astype(S.dtype))
+        S.run_checkpointed(co["ddrift1"], co["alpha"], co["beta"], co["ddrift2"])
+        rho = n**3 / res_pm**3
+        delta = S.deposit().astype(np.float64) / rho - 1.0
+        r = delta - np.asarray(target, np.float64)
+        loss = float(np.sum(r * r))
+        psi_bar = S.vjp_deposit(None, (2.0 / rho) * r)
+        pi_bar = np.zeros_like(psi_bar)
+        psi_bar, pi_bar = S.vjp_run(psi_bar, pi_bar)
+        if fp != 1.0:
+            pi_bar = (pi_bar.astype(np.float64) * fp).astype(S.dtype)
+        white_bar = S.vjp_ics_lpt(k_tab, pk_tab, D, psi_bar, pi_bar, order=order, white=white)
+    finally:
+        if own:
+            S.close()
+    return loss, white_bar, delta

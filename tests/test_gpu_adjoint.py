@@ -1,0 +1,165 @@
+"""Reverse mode of the PM path against central finite differences of the forward map.
+
+The checker is the directional derivative: <ybar, (f(x + eps v) - f(x - eps v)) / 2 eps> must equal
+<vjp(ybar), v>.  Each adjoint is checked against finite differences of the device forward and of the
+oracle's forward (the CPU restatement, pinned by the golden vectors).  FP64 throughout; the assignment
+weights are C^1 for TSC and C^2 for PCS, and CIC is piecewise linear, so its tolerance allows a rare
+cell-face crossing inside the difference step.
+"""
+import numpy as np
+import pytest
+
+import golden as G
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+D = pytest.importorskip("discodj_amd")
+
+COSMO = (0.2621, 0.049, 0.6766, 0.8102, 0.9665)
+
+
+def _dot(a, b):
+    return float(np.sum(np.asarray(a, np.float64) * np.asarray(b, np.float64)))
+
+
+def _rel(a, b):
+    return abs(a - b) / max(abs(a), abs(b), 1e-300)
+
+
+def _psi(n, L, seed, scale=2.0):
+    return np.random.default_rng(seed).normal(scale=scale * L / 100.0, size=(3, n, n, n))
+
+
+@pytest.mark.parametrize("W", [2, 3, 4])
+@pytest.mark.parametrize("order", [2, 4, 6])
+@pytest.mark.parametrize("aa", [0, 1])
+def test_vjp_acceleration_fd(W, order, aa):
+    n, res, L = 16, 32, 100.0
+    rng = np.random.default_rng(10 * W + order + aa)
+    psi, abar, v = _psi(n, L, W + 3 * order + aa), rng.normal(size=(3, n, n, n)), rng.normal(size=(3, n, n, n))
+    cfg = D.PMConfig(res_pm=res, n_part=n, boxsize=L, worder=W, grad_order=order, lap_order=order if W > 2 else 0,
+                     antialias=aa)
+    eps = 1e-5
+    with D.PMSolverHIP(cfg, precision=64) as S:
+        g = _dot(S.vjp_acceleration(psi, abar), v)
+        fd = (_dot(abar, S.pm_acceleration(psi + eps * v)) - _dot(abar, S.pm_acceleration(psi - eps * v))) / (2 * eps)
+    ocfg = O.pm_config(res, n, L, worder=W, grad_order=order, lap_order=order if W > 2 else 0, antialias=aa)
+    fdo = (_dot(abar, O.pm_acceleration(psi + eps * v, ocfg, "solver")) -
+           _dot(abar, O.pm_acceleration(psi - eps * v, ocfg, "solver"))) / (2 * eps)
+    tol = 1e-4 if W == 2 else 1e-6
+    assert _rel(g, fd) < tol, (g, fd)
+    assert _rel(g, fdo) < tol, (g, fdo)
+
+
+@pytest.mark.parametrize("W", [2, 3, 4])
+def test_vjp_deposit_fd(W):
+    n, res, L = 16, 32, 100.0
+    rng = np.random.default_rng(W)
+    psi, mbar, v = _psi(n, L, 7 + W), rng.normal(size=(res, res, res)), rng.normal(size=(3, n, n, n))
+    eps = 1e-5
+    with D.PMSolverHIP(D.PMConfig(res_pm=res, n_part=n, boxsize=L, worder=W), precision=64) as S:
+        g = _dot(S.vjp_deposit(psi, mbar), v)
+        fd = (_dot(mbar, S.deposit(psi + eps * v)) - _dot(mbar, S.deposit(psi - eps * v))) / (2 * eps)
+    dep = lambda p: O.pm_scatter(O.lagrangian_positions(p, L), res, L, W)  # noqa: E731
+    fdo = (_dot(mbar, dep(psi + eps * v)) - _dot(mbar, dep(psi - eps * v))) / (2 * eps)
+    tol = 1e-4 if W == 2 else 1e-7
+    assert _rel(g, fd) < tol, (g, fd)
+    assert _rel(g, fdo) < tol, (g, fdo)
+
+
+@pytest.mark.parametrize("stepper, W, aa", [("bullfrog", 3, 0), ("fastpm", 4, 1), ("symplectic", 3, 1)])
+def test_vjp_run_fd(stepper, W, aa):
+    """Three DKD steps: <Psibar, dPsi_f> + <Pibar, dPi_f> along (v, u) against <(Psibar_0, Pibar_0), (v, u)>."""
+    n, res, L, steps = 16, 32, 100.0, 3
+    rng = np.random.default_rng(5)
+    psi0 = _psi(n, L, 11, 1.0)
+    pi0 = rng.normal(scale=2.0, size=(3, n, n, n))
+    pb, mb = rng.normal(size=(3, n, n, n)), rng.normal(size=(3, n, n, n))
+    v, u = rng.normal(size=(3, n, n, n)), rng.normal(size=(3, n, n, n))
+    c = D.Cosmology.named("Planck18EEBAOSN")
+    co = D.stepper_coefficients(c, stepper, 0.05, 0.5, steps, "D")
+    coef = [co["ddrift1"], co["alpha"], co["beta"], co["ddrift2"]]
+    cfg = D.PMConfig(res_pm=res, n_part=n, boxsize=L, worder=W, grad_order=4, antialias=aa)
+    eps = 1e-6
+    with D.PMSolverHIP(cfg, precision=64) as S:
+        S.set_state(psi0, pi0)
+        S.run_checkpointed(*coef)
+        g0, g1 = S.vjp_run(pb, mb)
+
+        def f(e):
+            S.set_state(psi0 + e * v, pi0 + e * u)
+            S.run(*coef)
+            p, m = S.get_state()
+            return _dot(pb, p) + _dot(mb, m)
+
+        fd = (f(eps) - f(-eps)) / (2 * eps)
+    g = _dot(g0, v) + _dot(g1, u)
+    assert _rel(g, fd) < 1e-6, (g, fd)
+
+
+@pytest.mark.parametrize("order", [1, 2])
+@pytest.mark.parametrize("n", [16, 32])
+def test_vjp_ics_fd(order, n):
+    """2LPT is quadratic in the white noise, so the central difference is exact up to rounding."""
+    L, a = 2.0 * n, 0.05
+    rng = np.random.default_rng(n + order)
+    white = O.ngenic_wnoise_real(42, n)
+    v = rng.normal(size=(n, n, n))
+    pb, mb = rng.normal(size=(3, n, n, n)), rng.normal(size=(3, n, n, n))
+    oc = O.Cosmology(*COSMO)
+    k, pk = O.linear_power_spectrum(oc)
+    Dg = oc.Dplus(a)
+    eps = 1e-3
+    with D.PMSolverHIP(D.PMConfig(res_pm=2 * n, n_part=n, boxsize=L), precision=64) as S:
+        wb = S.vjp_ics_lpt(k, pk, Dg, pb, mb, order=order, white=white)
+
+        def f(e):
+            S.ics_lpt(k, pk, Dg, order=order, white=white + e * v)
+            p, m = S.get_state()
+            return _dot(pb, p) + _dot(mb, m)
+
+        fd = (f(eps) - f(-eps)) / (2 * eps)
+
+    def fo(e):
+        fphi = O.white_noise_to_fphi(O.ic_operator(n, L, k, pk), white + e * v)
+        psi1, psi2 = O.compute_core2(fphi, L)
+        if order == 1:
+            psi2 = np.zeros_like(psi2)
+        p, m = O.nbody_ics_lpt2(oc, psi1, psi2, a)
+        return _dot(pb, p) + _dot(mb, m)
+
+    fdo = (fo(eps) - fo(-eps)) / (2 * eps)
+    g = _dot(wb, v)
+    assert _rel(g, fd) < 1e-9, (g, fd)
+    assert _rel(g, fdo) < 1e-9, (g, fdo)
+
+
+def test_density_loss_gradient_directions():
+    """The differentiation workload's check at reduced size: the gradient of sum (delta - target)^2 with
+    respect to the white noise through 2LPT and 10 BullFrog CIC steps (grad order 4), against central
+    differences of the whole pipeline along 32 random directions (FP64), and the FP32 gradient against
+    the FP64 one."""
+    n, L = 16, 64.0
+    c = D.Cosmology.named("Planck18EEBAOSN")
+    white = O.ngenic_wnoise_real(7, n)
+    target = np.random.default_rng(3).normal(scale=0.5, size=(n, n, n))
+    kw = dict(boxsize=L, res_pm=n, n_steps=10, stepper="bullfrog", order=2, worder=2, grad_kernel_order=4)
+    cfg = D.PMConfig(res_pm=n, n_part=n, boxsize=L, worder=2, grad_order=4)
+    with D.PMSolverHIP(cfg, precision=64) as S:
+        loss, wb, _ = D.density_loss_gradient(c, white, target, precision=64, solver=S, **kw)
+        rng = np.random.default_rng(11)
+        eps = 1e-6
+        bad = []
+        for i in range(32):
+            v = rng.normal(size=(n, n, n))
+            lp = D.density_loss_gradient(c, white + eps * v, target, precision=64, solver=S, **kw)[0]
+            lm = D.density_loss_gradient(c, white - eps * v, target, precision=64, solver=S, **kw)[0]
+            fd, g = (lp - lm) / (2 * eps), _dot(wb, v)
+            if _rel(g, fd) > 1e-4:
+                bad.append((i, g, fd))
+    assert loss > 0
+    assert len(bad) <= 1, bad  # a CIC cell-face crossing inside the step may spoil one direction
+    with D.PMSolverHIP(cfg, precision=32) as S32:
+        _, wb32, _ = D.density_loss_gradient(c, white, target, precision=32, solver=S32, **kw)
+    assert G.rel(wb32, wb) < 2e-3
