@@ -30,10 +30,10 @@ namespace {
 thread_local std::string g_create_error;
 
 enum KernelId { K_ZERO = 0, K_SCATTER, K_XF, K_YF, K_ZM, K_YI, K_XI, K_GATHER, K_GATHER_KICK, K_KICK, K_DRIFT, K_SORT,
-                K_SORT_EXACT, K_RESAMPLE, K_COUNT };
+                K_SORT_EXACT, K_RESAMPLE, K_NN, K_COUNT };
 const char *kernel_names[K_COUNT] = {"zero", "scatter", "fft_x_r2c", "fft_y_fwd", "fft_z_mult", "fft_y_inv",
                                      "fft_x_c2r", "gather", "gather_kick", "kick_drift", "drift", "bin",
-                                     "bin_recount", "resample"};
+                                     "bin_recount", "resample", "nn_correction"};
 
 struct ProfEvent {
     int id;
@@ -85,6 +85,15 @@ struct dj_ctx {
     void *ckpt = nullptr;  // (steps, 2, 3 n^3): Psi and Pi before each step
     int ck_steps = 0;
     std::vector<double> ck_dd1, ck_alpha, ck_beta, ck_dd2;
+    // learned correction: FP32 master parameters, Adam moments, BF16 lane-ordered operands, activations
+    NnOps nn{};
+    bool nn_on = false, nn_train = false;
+    double nn_a = 1.0, nn_oscale = 1.0;
+    std::vector<double> nn_sched;
+    void *nn_p = nullptr, *nn_g = nullptr, *nn_m = nullptr, *nn_v = nullptr, *nn_mask = nullptr;
+    void *nn_wf = nullptr, *nn_wb = nullptr;  // (layers, 27, 64) x 8 bytes, forward / transposed
+    void *nn_t[10] = {};                      // z1 z2 z3 h1 h2 h3 and four gradient tensors (N^3 x 16 BF16)
+    void *nn_dphi = nullptr, *nn_dens = nullptr, *nn_tgt = nullptr, *nn_sum = nullptr;
     std::string err;
     bool prof = false;
     std::vector<ProfEvent> pending;
@@ -303,7 +312,10 @@ void free_all(dj_ctx *c) {
     void *bufs[] = {c->psi, c->mom, c->acc, c->psi_in, c->q1, c->mesh, c->meshfix, c->A, c->B, c->C, c->tw, c->tabs,
                     c->tcnt, c->tstart, c->tgstart, c->tcur, c->tflag, c->tpart, c->tent, c->d_shifts,
                     c->rs_tw, c->rs_psi, c->rs_hat, c->rs_A, c->rs_B, c->rs_xr, c->rs_phase,
-                    c->vb_psi1, c->vb_abar, c->vb_psibar, c->vb_pibar, c->vb_fbar, c->vb_pbar, c->ckpt};
+                    c->vb_psi1, c->vb_abar, c->vb_psibar, c->vb_pibar, c->vb_fbar, c->vb_pbar, c->ckpt,
+                    c->nn_p, c->nn_g, c->nn_m, c->nn_v, c->nn_mask, c->nn_wf, c->nn_wb, c->nn_t[0], c->nn_t[1],
+                    c->nn_t[2], c->nn_t[3], c->nn_t[4], c->nn_t[5], c->nn_t[6], c->nn_t[7], c->nn_t[8], c->nn_t[9],
+                    c->nn_dphi, c->nn_dens, c->nn_tgt, c->nn_sum};
     // teardown: failures here have no caller to report to
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -333,6 +345,9 @@ void set_bytes(dj_ctx *c) {
     // binning: read psi twice (count, fill), write one entry per (particle, tile)
     c->bytes[K_SORT] = 3 * T * np + (double)c->tiles.entry_bytes * np;
     c->bytes[K_SORT_EXACT] = 2 * 3 * T * np + (double)c->tiles.entry_bytes * np;
+    // learned correction of one pass: read the density, write and read three 16-channel BF16 hidden
+    // tensors (the residual layers read their input twice), write the potential correction
+    c->bytes[K_NN] = (4.0 + 32 + 64 + 64 + 32 + 4) * M;
     // sheet resampling of one pass: read psi once, then per offset read and write the mesh once
     c->bytes[K_RESAMPLE] = 3 * T * np + (double)c->n_offsets * 2 * T * M;
 }
@@ -409,6 +424,7 @@ enum { FORCE_ACC = 0, FORCE_KICK = 1 };
 
 template <typename T>
 dj_status resample_deposit(dj_ctx *c, const void *psi, const double *sh, void *dens, bool fixed, double fscale);
+void nn_forward_dev(dj_ctx *c, const void *dens, double a);
 
 dj_status force(dj_ctx *c, const void *psi, int mode, const KickArgs &kick) {
     const dj_pm_config &cfg = c->cfg;
@@ -446,12 +462,19 @@ dj_status force(dj_ctx *c, const void *psi, int mode, const KickArgs &kick) {
                                          : resample_deposit<float>(c, psi, c->shifts[s], dens, fixed, fscale_dep(cfg));
             if (rr) return rr;
         }
+        if (c->nn_on) {
+            // the learned potential correction of this pass, from the density before the transform
+            Prof p(c, K_NN);
+            if (c->nn_train) HIPCHK(c, hipMemcpyAsync(c->nn_dens, dens, c->M * 4, hipMemcpyDeviceToDevice, c->st));
+            nn_forward_dev(c, dens, c->nn_a);
+        }
         { Prof p(c, K_XF); c->fft.xf(dens, fixed, c->A, c->tw, fscale, c->st); }
         { Prof p(c, K_YF); c->fft.yf(c->A, c->B, c->tw, c->st); }
         if (!ik) {
             { Prof p(c, K_ZM); c->fft.zm(ZM_POT, true, true, c->B, c->B, c->tw, c->kmult, c->st); }
             { Prof p(c, K_YI); c->fft.yi(c->B, c->A, c->tw, c->st); }
             { Prof p(c, K_XI); c->fft.xi(c->A, c->mesh, c->tw, c->st); }
+            if (c->nn_on) c->nn.axpy(c->mesh, c->nn_dphi, c->M, 1.0, c->st);
             const bool last = s == ns - 1;
             // first pass stores acc, later passes add to it; the last one of a step kicks and drifts
             const int gmode = (mode == FORCE_KICK && last) ? GATHER_KICK : (s == 0 ? GATHER_ASSIGN : GATHER_ACC);
@@ -488,6 +511,7 @@ dj_status setup_ops(dj_ctx *c) {
     tile_ops<T>(c->tiles);
     resample_ops<T>(c->rs);
     adjoint_ops<T>(c->adj);
+    nn_ops(c->nn);
     return upload_tables<T>(c);
 }
 
@@ -840,6 +864,7 @@ dj_status dj_run(dj_ctx *c, int32_t n_steps, const double *dd1, const double *al
     if (n_steps == 0) return DJ_OK;
     { Prof p(c, K_DRIFT); c->part.drift(c->psi, c->mom, c->n3, dd1[0], c->st); }
     for (int s = 0; s < n_steps; s++) {
+        if (c->nn_on && (size_t)s < c->nn_sched.size()) c->nn_a = c->nn_sched[s];
         const bool stop = wants(s + 1);
         const bool has_next = s + 1 < n_steps && !stop;
         KickArgs k{alpha[s], beta[s], dd2[s], has_next ? dd1[s + 1] : 0.0, has_next ? 1 : 0, c->nshifts};
@@ -913,7 +938,9 @@ dj_status dj_ics_lpt(dj_ctx *c, uint32_t seed, const double *white, const double
 }
 
 // ------------------------------------------------------------------------ reverse mode
+namespace {
 dj_status adj_supported(dj_ctx *c) {
+    if (c->nn_on) return fail(c, DJ_ERR_UNSUPPORTED, "the adjoint is built without the learned correction");
     if (c->cfg.grad_order == 0)
         return fail(c, DJ_ERR_UNSUPPORTED, "the adjoint is built for finite-difference gradients (grad_order 2/4/6)");
     if (c->cfg.n_resample > 1) return fail(c, DJ_ERR_UNSUPPORTED, "the adjoint is built without sheet resampling");
@@ -964,6 +991,8 @@ dj_status vjp_force(dj_ctx *c, const void *psi, const void *abar, void *psibar) 
     }
     return DJ_OK;
 }
+
+}  // namespace
 
 dj_status dj_vjp_acceleration(dj_ctx *c, const void *psi, const void *acc_bar, void *psi_bar_out) {
     if (!c || !acc_bar || !psi_bar_out) return fail(c, DJ_ERR_INVALID_ARGUMENT, "null argument");
@@ -1094,6 +1123,211 @@ dj_status dj_vjp_ics_lpt(dj_ctx *c, uint32_t seed, const double *white, const do
                                D, c->cfg.precision, c->vb_psibar, c->vb_pibar, white_bar_out, c->st);
     if (e) return hip_fail(c, (hipError_t)e, "dj_vjp_ics_lpt");
     return DJ_OK;
+}
+
+// ------------------------------------------------------------------------ learned correction
+namespace {
+enum { NN_L = NN_PARAMS_PER_LAYER, NN_W = 27 * 16 * 16 };
+
+static float *nn_layer(void *p, int l) { return (float *)p + (size_t)l * NN_L; }
+static void *nn_op(void *w, int l) { return (char *)w + (size_t)l * 27 * 64 * 8; }
+
+dj_status nn_supported(dj_ctx *c) {
+    if (c->cfg.precision != 32 || c->cfg.deposit != DJ_DEPOSIT_FLOAT)
+        return fail(c, DJ_ERR_UNSUPPORTED, "the learned correction is built for FP32 meshes with the float deposit");
+    if (c->cfg.grad_order == 0)
+        return fail(c, DJ_ERR_UNSUPPORTED, "the learned correction is built for finite-difference gradients");
+    if (c->res % 64) return fail(c, DJ_ERR_UNSUPPORTED, "the learned correction needs res_pm to be a multiple of 64");
+    return DJ_OK;
+}
+
+dj_status nn_alloc(dj_ctx *c) {
+    if (c->nn_p) return DJ_OK;
+    const size_t pb = sizeof(float) * NN_LAYERS * NN_L, tb = c->M * 16 * 2;
+    void **pv[] = {&c->nn_p, &c->nn_g, &c->nn_m, &c->nn_v, &c->nn_mask};
+    for (void **b : pv) HIPCHK(c, hipMalloc(b, pb));
+    HIPCHK(c, hipMalloc(&c->nn_wf, (size_t)NN_LAYERS * 27 * 64 * 8));
+    HIPCHK(c, hipMalloc(&c->nn_wb, (size_t)NN_LAYERS * 27 * 64 * 8));
+    for (void *&t : c->nn_t) HIPCHK(c, hipMalloc(&t, tb));
+    HIPCHK(c, hipMalloc(&c->nn_dphi, c->M * 4));
+    HIPCHK(c, hipMalloc(&c->nn_dens, c->M * 4));
+    HIPCHK(c, hipMalloc(&c->nn_tgt, (size_t)c->n3 * 4));
+    HIPCHK(c, hipMalloc(&c->nn_sum, sizeof(double)));
+    // the real parameters: layer 1 has inputs 0, 1; layer 4 has output 0
+    std::vector<float> mask((size_t)NN_LAYERS * NN_L, 0.0f);
+    for (int l = 0; l < NN_LAYERS; l++)
+        for (int t = 0; t < 27; t++)
+            for (int co = 0; co < 16; co++)
+                for (int ci = 0; ci < 16; ci++)
+                    if ((l != 0 || ci < 2) && (l != 3 || co == 0)) mask[(size_t)l * NN_L + (t * 16 + co) * 16 + ci] = 1;
+    for (int l = 0; l < NN_LAYERS; l++)
+        for (int co = 0; co < 16; co++)
+            if (l != 3 || co == 0) mask[(size_t)l * NN_L + NN_W + co] = 1;
+    HIPCHK(c, hipMemcpy(c->nn_mask, mask.data(), pb, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemset(c->nn_m, 0, pb));
+    HIPCHK(c, hipMemset(c->nn_v, 0, pb));
+    return DJ_OK;
+}
+
+void nn_swizzle_all(dj_ctx *c) {
+    for (int l = 0; l < NN_LAYERS; l++) c->nn.swizzle(nn_layer(c->nn_p, l), nn_op(c->nn_wf, l), nn_op(c->nn_wb, l), c->st);
+}
+
+// the net on a density mesh (float, unnormalised deposit) into nn_dphi; pre-activations kept when training
+void nn_forward_dev(dj_ctx *c, const void *dens, double a) {
+    const int N = c->res;
+    const double inv_rho = (double)c->M / (double)c->np;
+    void **t = c->nn_t;
+    const bool keep = c->nn_train;
+    c->nn.conv(1, 0, N, nullptr, dens, inv_rho, a, nn_op(c->nn_wf, 0), nn_layer(c->nn_p, 0) + NN_W, nullptr, nullptr,
+               keep ? t[0] : nullptr, t[3], nullptr, 0.0, c->st);
+    c->nn.conv(0, 0, N, t[3], nullptr, 0.0, 0.0, nn_op(c->nn_wf, 1), nn_layer(c->nn_p, 1) + NN_W, t[3], nullptr,
+               keep ? t[1] : nullptr, t[4], nullptr, 0.0, c->st);
+    c->nn.conv(0, 0, N, t[4], nullptr, 0.0, 0.0, nn_op(c->nn_wf, 2), nn_layer(c->nn_p, 2) + NN_W, t[4], nullptr,
+               keep ? t[2] : nullptr, t[5], nullptr, 0.0, c->st);
+    c->nn.conv(0, 1, N, t[5], nullptr, 0.0, 0.0, nn_op(c->nn_wf, 3), nn_layer(c->nn_p, 3) + NN_W, nullptr, nullptr,
+               nullptr, nullptr, c->nn_dphi, c->nn_oscale, c->st);
+}
+
+// parameter gradients into nn_g from gphi = dL / d(dphi) (float mesh), after nn_forward_dev with nn_train set
+dj_status nn_backward_dev(dj_ctx *c, const void *gphi, const void *dens, double a) {
+    const int N = c->res;
+    const double inv_rho = (double)c->M / (double)c->np, os = c->nn_oscale;
+    void **t = c->nn_t;
+    HIPCHK(c, hipMemsetAsync(c->nn_g, 0, sizeof(float) * NN_LAYERS * NN_L, c->st));
+    float *g3 = nn_layer(c->nn_g, 3), *g2 = nn_layer(c->nn_g, 2), *g1 = nn_layer(c->nn_g, 1), *g0 = nn_layer(c->nn_g, 0);
+    // layer 4: dphi = os (W4 * h3 + b4)
+    c->nn.wgrad(0, 1, N, t[5], nullptr, 0.0, 0.0, nullptr, gphi, os, g3, g3 + NN_W, c->st);
+    c->nn.conv(2, 2, N, nullptr, gphi, os, 0.0, nn_op(c->nn_wb, 3), nullptr, nullptr, t[2], t[7], t[6], nullptr, 0.0,
+               c->st);  // gh3 (t6), gz3 (t7)
+    // layer 3: h3 = h2 + relu(z3)
+    c->nn.wgrad(0, 0, N, t[4], nullptr, 0.0, 0.0, t[7], nullptr, 0.0, g2, g2 + NN_W, c->st);
+    c->nn.conv(0, 2, N, t[7], nullptr, 0.0, 0.0, nn_op(c->nn_wb, 2), nullptr, t[6], t[1], t[9], t[8], nullptr, 0.0,
+               c->st);  // gh2 (t8), gz2 (t9)
+    // layer 2: h2 = h1 + relu(z2)
+    c->nn.wgrad(0, 0, N, t[3], nullptr, 0.0, 0.0, t[9], nullptr, 0.0, g1, g1 + NN_W, c->st);
+    c->nn.conv(0, 2, N, t[9], nullptr, 0.0, 0.0, nn_op(c->nn_wb, 1), nullptr, t[8], t[0], t[7], t[6], nullptr, 0.0,
+               c->st);  // gh1 (t6), gz1 (t7)
+    // layer 1: h1 = relu(z1), input (log(1 + delta), a)
+    c->nn.wgrad(1, 0, N, nullptr, dens, inv_rho, a, t[7], nullptr, 0.0, g0, g0 + NN_W, c->st);
+    return check_launch(c, "learned correction backward");
+}
+
+}  // namespace
+
+dj_status dj_nn_set_params(dj_ctx *c, const float *params) {
+    if (!c || !params) return fail(c, DJ_ERR_INVALID_ARGUMENT, "null argument");
+    dj_status r = nn_supported(c);
+    if (r) return r;
+    HIPCHK(c, hipSetDevice(c->dev));
+    r = nn_alloc(c);
+    if (r) return r;
+    HIPCHK(c, hipMemcpy(c->nn_p, params, sizeof(float) * NN_LAYERS * NN_L, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemset(c->nn_m, 0, sizeof(float) * NN_LAYERS * NN_L));
+    HIPCHK(c, hipMemset(c->nn_v, 0, sizeof(float) * NN_LAYERS * NN_L));
+    nn_swizzle_all(c);
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return check_launch(c, "learned correction parameters");
+}
+
+dj_status dj_nn_get_params(dj_ctx *c, float *params) {
+    if (!c || !params) return fail(c, DJ_ERR_INVALID_ARGUMENT, "null argument");
+    if (!c->nn_p) return fail(c, DJ_ERR_INVALID_ARGUMENT, "no parameters set");
+    HIPCHK(c, hipSetDevice(c->dev));
+    HIPCHK(c, hipMemcpy(params, c->nn_p, sizeof(float) * NN_LAYERS * NN_L, hipMemcpyDeviceToHost));
+    return DJ_OK;
+}
+
+dj_status dj_nn_enable(dj_ctx *c, int32_t on, double out_scale) {
+    if (!c) return fail(c, DJ_ERR_INVALID_ARGUMENT, "null context");
+    if (on && !c->nn_p) return fail(c, DJ_ERR_INVALID_ARGUMENT, "no parameters set");
+    c->nn_on = on != 0;
+    c->nn_oscale = out_scale;
+    return DJ_OK;
+}
+
+dj_status dj_nn_set_schedule(dj_ctx *c, int32_t n, const double *a_force) {
+    if (!c || n < 0 || (n > 0 && !a_force)) return fail(c, DJ_ERR_INVALID_ARGUMENT, "invalid schedule");
+    c->nn_sched.assign(a_force, a_force + n);
+    if (n > 0) c->nn_a = a_force[0];
+    return DJ_OK;
+}
+
+dj_status dj_nn_forward(dj_ctx *c, const float *dens, double a, float *dphi_out) {
+    if (!c || !dens || !dphi_out) return fail(c, DJ_ERR_INVALID_ARGUMENT, "null argument");
+    if (!c->nn_p) return fail(c, DJ_ERR_INVALID_ARGUMENT, "no parameters set");
+    HIPCHK(c, hipSetDevice(c->dev));
+    HIPCHK(c, hipMemcpyAsync(c->nn_dens, dens, c->M * 4, hipMemcpyHostToDevice, c->st));
+    const bool tr = c->nn_train;
+    c->nn_train = true;
+    nn_forward_dev(c, c->nn_dens, a);
+    c->nn_train = tr;
+    dj_status r = check_launch(c, "learned correction");
+    if (r) return r;
+    HIPCHK(c, hipMemcpyAsync(dphi_out, c->nn_dphi, c->M * 4, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return DJ_OK;
+}
+
+dj_status dj_nn_backward(dj_ctx *c, const float *dens, double a, const float *gphi, float *grad_out) {
+    if (!c || !dens || !gphi || !grad_out) return fail(c, DJ_ERR_INVALID_ARGUMENT, "null argument");
+    if (!c->nn_p) return fail(c, DJ_ERR_INVALID_ARGUMENT, "no parameters set");
+    HIPCHK(c, hipSetDevice(c->dev));
+    HIPCHK(c, hipMemcpyAsync(c->nn_dens, dens, c->M * 4, hipMemcpyHostToDevice, c->st));
+    const bool tr = c->nn_train;
+    c->nn_train = true;
+    nn_forward_dev(c, c->nn_dens, a);
+    c->nn_train = tr;
+    HIPCHK(c, hipMemcpyAsync(c->nn_dphi, gphi, c->M * 4, hipMemcpyHostToDevice, c->st));
+    dj_status r = nn_backward_dev(c, c->nn_dphi, c->nn_dens, a);
+    if (r) return r;
+    HIPCHK(c, hipMemcpyAsync(grad_out, c->nn_g, sizeof(float) * NN_LAYERS * NN_L, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return DJ_OK;
+}
+
+dj_status dj_nn_train_step(dj_ctx *c, const void *psi, const void *acc_target, double a, double weight, double lr,
+                           int32_t step, double *loss_out) {
+    if (!c || !psi || !acc_target || step < 1) return fail(c, DJ_ERR_INVALID_ARGUMENT, "invalid training arguments");
+    if (!c->nn_p) return fail(c, DJ_ERR_INVALID_ARGUMENT, "no parameters set");
+    if (c->nshifts != 1) return fail(c, DJ_ERR_UNSUPPORTED, "training is built for antialias = 0 (one pass)");
+    HIPCHK(c, hipSetDevice(c->dev));
+    dj_status r = adj_buffers(c);
+    if (r) return r;
+    const size_t pb = (size_t)c->n3 * 4;
+    HIPCHK(c, hipMemcpyAsync(c->vb_psi1, psi, pb, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->nn_tgt, acc_target, pb, hipMemcpyHostToDevice, c->st));
+    // forward: the corrected acceleration of the snapshot
+    const bool on = c->nn_on;
+    c->nn_on = true;
+    c->nn_train = true;
+    c->nn_a = a;
+    KickArgs k{};
+    k.nshifts = c->nshifts;
+    r = force(c, c->vb_psi1, FORCE_ACC, k);
+    c->nn_train = false;
+    c->nn_on = on;
+    if (r) return r;
+    // loss = weight x mean over components of (acc - target)^2, abar = dL / d acc
+    HIPCHK(c, hipMemsetAsync(c->nn_sum, 0, sizeof(double), c->st));
+    c->nn.residual(c->acc, c->nn_tgt, c->vb_abar, (size_t)c->n3, 2.0 * weight / (double)c->n3, c->nn_sum, c->st);
+    // dL / dphi: the gather's transpose (deposit of abar) and the FD stencil's transpose
+    const double zero[3] = {0, 0, 0};
+    HIPCHK(c, hipMemsetAsync(c->vb_fbar, 0, 3 * c->M * 4, c->st));
+    c->adj.scatter3(c->vb_psi1, c->q1, c->n, c->res, c->cfg.boxsize, zero, c->vb_abar, 1.0, c->cfg.worder, c->vb_fbar,
+                    c->st);
+    c->adj.divergence(c->vb_fbar, c->res, c->fdc, c->vb_pbar, c->st);
+    r = nn_backward_dev(c, c->vb_pbar, c->nn_dens, a);
+    if (r) return r;
+    const double b1 = 0.9, b2 = 0.999;
+    c->nn.adam(c->nn_p, c->nn_g, c->nn_m, c->nn_v, c->nn_mask, NN_LAYERS * NN_L, lr, b1, b2, 1e-8,
+               1.0 - std::pow(b1, step), 1.0 - std::pow(b2, step), c->st);
+    nn_swizzle_all(c);
+    double sum = 0;
+    HIPCHK(c, hipMemcpyAsync(&sum, c->nn_sum, sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    if (loss_out) *loss_out = weight * sum / (double)c->n3;
+    return check_launch(c, "learned correction training step");
 }
 
 dj_status dj_power_spectrum(dj_ctx *c, const void *psi, int32_t W, int32_t nbins, double *k_out, double *pk_out,

@@ -135,6 +135,24 @@ struct AdjointOps {
 };
 template <typename T> void adjoint_ops(AdjointOps &ops);
 
+// learned correction (nn.hip): FP32 fields, BF16 channel-last tensors t[v * 16 + c]
+enum { NN_PARAMS_PER_LAYER = 27 * 16 * 16 + 16, NN_LAYERS = 4 };
+struct NnOps {
+    // one 3x3x3 convolution layer (in: 0 tensor, 1 density field, 2 gradient field; epi: 0 hidden, 1 out, 2 bwd)
+    void (*conv)(int in, int epi, int N, const void *hin, const void *fin, double inv_rho, double cval, const void *wsw,
+                 const void *bias, const void *res, const void *zmask, void *out_z, void *out_h, void *out_f,
+                 double oscale, hipStream_t st);
+    // dW[27][16][16] += sum_v gz(v) x(v + off)^T, db += sum_v gz(v)  (ga: gz is a 1-channel field gf * gscale)
+    void (*wgrad)(int in, int ga, int N, const void *hin, const void *fin, double inv_rho, double cval, const void *gz,
+                  const void *gf, double gscale, void *dW, void *db, hipStream_t st);
+    void (*swizzle)(const void *W, void *fwd, void *bwd, hipStream_t st);
+    void (*adam)(void *p, const void *g, void *m, void *v, const void *mask, int n, double lr, double b1, double b2,
+                 double eps, double c1, double c2, hipStream_t st);
+    void (*axpy)(void *out, const void *in, size_t n, double s, hipStream_t st);
+    void (*residual)(const void *acc, const void *tgt, void *abar, size_t n, double gs, void *sum, hipStream_t st);
+};
+void nn_ops(NnOps &ops);
+
 // power spectrum binning of an x-halved B-layout spectrum (analysis.hip)
 template <typename T>
 void pk_bin(const void *B, int N, double L, int W, int nbins, double scale, double *sum_p, double *sum_w,

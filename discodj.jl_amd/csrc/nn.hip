@@ -1,0 +1,299 @@
+// Learned force correction: a residual 3-D convolution net on the PM mesh, BF16 weights and
+// activations, FP32 accumulation on MFMA (v_mfma_f32_16x16x16bf16_1k).  There is no counterpart in the
+// reference; the net adds a potential correction to the PM potential before the gradient and the gather.
+//
+//   x0 = (log(1 + max(delta, -0.999)), a)            delta = mesh / rho_bar - 1, a = scale factor
+//   z1 = W1 * x0 + b1,  h1 = relu(z1)
+//   z2 = W2 * h1 + b2,  h2 = h1 + relu(z2)
+//   z3 = W3 * h2 + b3,  h3 = h2 + relu(z3)
+//   dphi = s_out (W4 * h3 + b4)                       (*: periodic 3x3x3 convolution, 16 channels)
+//
+// Tensors are channel-last: t[v * 16 + c], v = x + N (y + N z), BF16 (round to nearest even).  Every
+// layer is stored as W[27][16][16] (tap, out, in) padded with zeros (layer 1 has 2 inputs, layer 4 one
+// output).  One wave computes 16 output channels of 16 voxels of an x row: per tap one MFMA with
+// A = the tap's weights (16 out x 16 in) and B = the 16 voxels' input channels; the transposed
+// convolution of the backward pass is the same kernel with the flipped, transposed taps.
+#include <hip/hip_runtime.h>
+
+#include "dj_internal.h"
+
+namespace dj {
+
+typedef short s4v __attribute__((ext_vector_type(4)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+// FP32 -> BF16 round to nearest even (finite values); BF16 -> FP32 exact
+__device__ __forceinline__ unsigned short f2bf(float x) {
+    unsigned u = __float_as_uint(x);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(unsigned short b) { return __uint_as_float((unsigned)b << 16); }
+
+__device__ __forceinline__ float nn_input(float dens, float inv_rho) {
+    const float d = dens * inv_rho - 1.0f;
+    return log1pf(fmaxf(d, -0.999f));
+}
+
+__device__ __forceinline__ int nn_wrap(int v, int N) { return v < 0 ? v + N : (v >= N ? v - N : v); }
+
+enum { NN_IN_H = 0, NN_IN_FIELD = 1, NN_IN_GRAD = 2 };  // B: 16-channel tensor / density field / 1-channel float
+enum { NN_EPI_HIDDEN = 0, NN_EPI_OUT = 1, NN_EPI_BWD = 2 };
+
+// B operand of one tap: channels 4 (l/16) .. +3 of voxel v
+template <int IN>
+__device__ __forceinline__ s4v nn_bop(const unsigned short *hin, const float *fin, float inv_rho, float cval, size_t v,
+                                      int cg) {
+    s4v b = {0, 0, 0, 0};
+    if constexpr (IN == NN_IN_H) {
+        b = *(const s4v *)(hin + v * 16 + cg);
+    } else if (cg == 0) {
+        // density field: the net's input transform; gradient field: scaled by inv_rho
+        const float f = IN == NN_IN_FIELD ? nn_input(fin[v], inv_rho) : fin[v] * inv_rho;
+        b[0] = (short)f2bf(f);
+        b[1] = (short)f2bf(cval);
+    }
+    return b;
+}
+
+// one wave per 16 voxels x 16 output channels; grid covers N^3 / 16 waves, 4 waves per block
+template <int IN, int EPI>
+__global__ __launch_bounds__(256) void k_nn_conv(int N, const unsigned short *__restrict__ hin, const float *__restrict__ fin,
+                                                 float inv_rho, float cval, const s4v *__restrict__ wsw,
+                                                 const float *__restrict__ bias, const unsigned short *__restrict__ res,
+                                                 const unsigned short *__restrict__ zmask, unsigned short *__restrict__ out_z,
+                                                 unsigned short *__restrict__ out_h, float *__restrict__ out_f, float oscale) {
+    const int lane = threadIdx.x & 63;
+    const size_t w = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int rows = N / 16;
+    if (w >= (size_t)rows * N * N) return;
+    const int x0 = (int)(w % rows) * 16, y = (int)((w / rows) % N), z = (int)(w / ((size_t)rows * N));
+    const int vx = x0 + (lane & 15), cg = 4 * (lane >> 4);
+    f4v acc = {0, 0, 0, 0};
+#pragma unroll
+    for (int dz = -1; dz <= 1; dz++)
+#pragma unroll
+        for (int dy = -1; dy <= 1; dy++) {
+            const size_t rowb = (size_t)N * ((size_t)nn_wrap(y + dy, N) + (size_t)N * nn_wrap(z + dz, N));
+#pragma unroll
+            for (int dx = -1; dx <= 1; dx++) {
+                const int t = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
+                const s4v a = wsw[t * 64 + lane];
+                const s4v b = nn_bop<IN>(hin, fin, inv_rho, cval, rowb + nn_wrap(vx + dx, N), cg);
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+            }
+        }
+    // lane holds D[cg + j][lane & 15]: output channels cg..cg+3 of voxel vx
+    const size_t v = (size_t)vx + (size_t)N * ((size_t)y + (size_t)N * z);
+    if constexpr (EPI == NN_EPI_OUT) {
+        if (cg == 0) out_f[v] = oscale * (acc[0] + bias[0]);
+    } else {
+        s4v zo, ho;
+        s4v r = {0, 0, 0, 0}, m = {0, 0, 0, 0};
+        if (res) r = *(const s4v *)(res + v * 16 + cg);
+        if constexpr (EPI == NN_EPI_BWD) m = *(const s4v *)(zmask + v * 16 + cg);
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const float rv = bf2f((unsigned short)r[j]);
+            if constexpr (EPI == NN_EPI_HIDDEN) {
+                const float zz = acc[j] + bias[cg + j];
+                zo[j] = (short)f2bf(zz);
+                ho[j] = (short)f2bf(rv + fmaxf(zz, 0.0f));
+            } else {
+                const float g = rv + acc[j];
+                ho[j] = (short)f2bf(g);
+                zo[j] = (short)f2bf(bf2f((unsigned short)m[j]) > 0.0f ? g : 0.0f);
+            }
+        }
+        if (out_z) *(s4v *)(out_z + v * 16 + cg) = zo;  // pre-activations: kept for training only
+        *(s4v *)(out_h + v * 16 + cg) = ho;
+    }
+}
+
+// Weight gradient dW[t][co][ci] += sum_v gz(v, co) x(v + off(t), ci) and db[co] += sum_v gz(v, co).
+// A block takes rows of 64 voxels (x chunks), stages gz (64 x 16) and the 9 neighbouring input rows
+// (66 x 16 each, periodic) in LDS, and each of its 4 waves accumulates the MFMAs of taps w, w+4, ...
+// over the voxels (K = 16 voxels per MFMA) in registers; the sums go to global memory once per block.
+template <int IN, int GA>
+__global__ __launch_bounds__(256) void k_nn_wgrad(int N, const unsigned short *__restrict__ hin, const float *__restrict__ fin,
+                                                  float inv_rho, float cval, const unsigned short *__restrict__ gz,
+                                                  const float *__restrict__ gf, float gscale, float *__restrict__ dW,
+                                                  float *__restrict__ db) {
+    __shared__ unsigned short sA[64 * 16];
+    __shared__ unsigned short sB[9 * 66 * 16];
+    __shared__ float sdb[16];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int chunks = N / 64, nrows = chunks * N * N;
+    if (threadIdx.x < 16) sdb[threadIdx.x] = 0;
+    f4v acc[7];
+#pragma unroll
+    for (int i = 0; i < 7; i++) acc[i] = {0, 0, 0, 0};
+    for (int r = blockIdx.x; r < nrows; r += gridDim.x) {
+        const int x0 = (r % chunks) * 64, y = (r / chunks) % N, z = r / (chunks * N);
+        __syncthreads();  // previous row's LDS reads are done
+        // gz of the row: 64 voxels x 16 channels (GA: a 1-channel float field in channel 0, scaled)
+        for (int e = threadIdx.x; e < 64 * 16; e += 256) {
+            const int vx = e >> 4, c = e & 15;
+            const size_t v = (size_t)(x0 + vx) + (size_t)N * ((size_t)y + (size_t)N * z);
+            unsigned short g;
+            if constexpr (GA) g = c == 0 ? f2bf(gscale * gf[v]) : (unsigned short)0;
+            else g = gz[v * 16 + c];
+            sA[e] = g;
+        }
+        // the input rows (y + dy, z + dz), x from x0 - 1 to x0 + 64
+        for (int e = threadIdx.x; e < 9 * 66 * 16; e += 256) {
+            const int c = e & 15, xi = (e >> 4) % 66, rr = (e >> 4) / 66;
+            const int yy = nn_wrap(y + rr % 3 - 1, N), zz = nn_wrap(z + rr / 3 - 1, N), xx = nn_wrap(x0 + xi - 1, N);
+            const size_t v = (size_t)xx + (size_t)N * ((size_t)yy + (size_t)N * zz);
+            unsigned short h;
+            if constexpr (IN == NN_IN_H) h = hin[v * 16 + c];
+            else h = c == 0 ? f2bf(nn_input(fin[v], inv_rho)) : (c == 1 ? f2bf(cval) : (unsigned short)0);
+            sB[e] = h;
+        }
+        __syncthreads();
+        if (threadIdx.x < 64) {  // bias: one channel per 4 threads, 16 voxels each
+            const int c = threadIdx.x & 15, q = threadIdx.x >> 4;
+            float s = 0;
+            for (int vx = q * 16; vx < q * 16 + 16; vx++) s += bf2f(sA[vx * 16 + c]);
+            atomicAdd(&sdb[c], s);
+        }
+        const int cg = 4 * (lane >> 4), cl = lane & 15;
+#pragma unroll
+        for (int kc = 0; kc < 4; kc++) {
+            s4v a;
+#pragma unroll
+            for (int j = 0; j < 4; j++) a[j] = (short)sA[(kc * 16 + cg + j) * 16 + cl];
+#pragma unroll
+            for (int i = 0; i < 7; i++) {
+                const int t = wave + 4 * i;
+                if (t < 27) {
+                    const int dx = t % 3 - 1, rr = t / 3;  // rr = (dz + 1) 3 + (dy + 1)
+                    s4v b;
+#pragma unroll
+                    for (int j = 0; j < 4; j++) b[j] = (short)sB[(rr * 66 + kc * 16 + cg + j + dx + 1) * 16 + cl];
+                    acc[i] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[i], 0, 0, 0);
+                }
+            }
+        }
+    }
+    // D[co = cg + j][ci = lane & 15] of tap t
+    const int cg = 4 * (lane >> 4), cl = lane & 15;
+#pragma unroll
+    for (int i = 0; i < 7; i++) {
+        const int t = wave + 4 * i;
+        if (t < 27) {
+#pragma unroll
+            for (int j = 0; j < 4; j++) atomicAdd(&dW[(t * 16 + cg + j) * 16 + cl], acc[i][j]);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 16) atomicAdd(&db[threadIdx.x], sdb[threadIdx.x]);
+}
+
+// lane-ordered BF16 A operands of the forward (W[t][co][ci] at lane (co, ci / 4)) and of the
+// transposed convolution (W[26 - t][co][ci] at lane (ci, co / 4)) of one layer
+__global__ void k_nn_swizzle(const float *__restrict__ W, s4v *__restrict__ fwd, s4v *__restrict__ bwd) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 27 * 64) return;
+    const int t = i / 64, l = i % 64;
+    s4v f, b;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        f[j] = (short)f2bf(W[(t * 16 + (l & 15)) * 16 + 4 * (l >> 4) + j]);
+        b[j] = (short)f2bf(W[((26 - t) * 16 + 4 * (l >> 4) + j) * 16 + (l & 15)]);
+    }
+    fwd[i] = f;
+    bwd[i] = b;
+}
+
+// Adam (Kingma & Ba 2015) on the FP32 master parameters; mask zeroes the padding
+__global__ void k_nn_adam(float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m, float *__restrict__ v,
+                          const float *__restrict__ mask, int n, float lr, float b1, float b2, float eps, float c1,
+                          float c2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i] * mask[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi, vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// out += s in  (the correction into the potential); r = acc - target and its squared sum
+__global__ void k_nn_axpy(float *__restrict__ out, const float *__restrict__ in, size_t n, float s) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] += s * in[i];
+}
+__global__ void k_nn_residual(const float *__restrict__ acc, const float *__restrict__ tgt, float *__restrict__ abar, size_t n,
+                              float gs, double *__restrict__ sum) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double s = 0;
+    if (i < n) {
+        const float r = acc[i] - tgt[i];
+        abar[i] = gs * r;
+        s = (double)r * r;
+    }
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o);
+    if ((threadIdx.x & 63) == 0) atomicAdd(sum, s);
+}
+
+static unsigned ngrid(size_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+struct NnLaunch {
+    static void conv(int in, int epi, int N, const void *hin, const void *fin, double inv_rho, double cval,
+                     const void *wsw, const void *bias, const void *res, const void *zmask, void *out_z, void *out_h,
+                     void *out_f, double oscale, hipStream_t st) {
+        const size_t waves = (size_t)N * N * N / 16;
+        const dim3 g(ngrid(waves, 4));
+#define NN_C(I, E)                                                                                                   \
+    k_nn_conv<I, E><<<g, 256, 0, st>>>(N, (const unsigned short *)hin, (const float *)fin, (float)inv_rho, (float)cval, \
+                                       (const s4v *)wsw, (const float *)bias, (const unsigned short *)res,              \
+                                       (const unsigned short *)zmask, (unsigned short *)out_z, (unsigned short *)out_h, \
+                                       (float *)out_f, (float)oscale)
+        if (in == NN_IN_FIELD && epi == NN_EPI_HIDDEN) NN_C(NN_IN_FIELD, NN_EPI_HIDDEN);
+        else if (in == NN_IN_H && epi == NN_EPI_HIDDEN) NN_C(NN_IN_H, NN_EPI_HIDDEN);
+        else if (in == NN_IN_H && epi == NN_EPI_OUT) NN_C(NN_IN_H, NN_EPI_OUT);
+        else if (in == NN_IN_GRAD && epi == NN_EPI_BWD) NN_C(NN_IN_GRAD, NN_EPI_BWD);
+        else NN_C(NN_IN_H, NN_EPI_BWD);
+#undef NN_C
+    }
+    static void wgrad(int in, int ga, int N, const void *hin, const void *fin, double inv_rho, double cval, const void *gz,
+                      const void *gf, double gscale, void *dW, void *db, hipStream_t st) {
+        const int nrows = (N / 64) * N * N, blocks = nrows < 2048 ? nrows : 2048;
+#define NN_W(I, G)                                                                                                     \
+    k_nn_wgrad<I, G><<<blocks, 256, 0, st>>>(N, (const unsigned short *)hin, (const float *)fin, (float)inv_rho,        \
+                                             (float)cval, (const unsigned short *)gz, (const float *)gf, (float)gscale, \
+                                             (float *)dW, (float *)db)
+        if (in == NN_IN_FIELD) NN_W(NN_IN_FIELD, 0);
+        else if (ga) NN_W(NN_IN_H, 1);
+        else NN_W(NN_IN_H, 0);
+#undef NN_W
+    }
+    static void swizzle(const void *W, void *fwd, void *bwd, hipStream_t st) {
+        k_nn_swizzle<<<ngrid(27 * 64, 256), 256, 0, st>>>((const float *)W, (s4v *)fwd, (s4v *)bwd);
+    }
+    static void adam(void *p, const void *g, void *m, void *v, const void *mask, int n, double lr, double b1, double b2,
+                     double eps, double c1, double c2, hipStream_t st) {
+        k_nn_adam<<<ngrid(n, 256), 256, 0, st>>>((float *)p, (const float *)g, (float *)m, (float *)v,
+                                                 (const float *)mask, n, (float)lr, (float)b1, (float)b2, (float)eps,
+                                                 (float)c1, (float)c2);
+    }
+    static void axpy(void *out, const void *in, size_t n, double s, hipStream_t st) {
+        k_nn_axpy<<<ngrid(n, 256), 256, 0, st>>>((float *)out, (const float *)in, n, (float)s);
+    }
+    static void residual(const void *acc, const void *tgt, void *abar, size_t n, double gs, void *sum, hipStream_t st) {
+        k_nn_residual<<<ngrid(n, 256), 256, 0, st>>>((const float *)acc, (const float *)tgt, (float *)abar, n, (float)gs,
+                                                    (double *)sum);
+    }
+};
+
+void nn_ops(NnOps &ops) {
+    ops.conv = NnLaunch::conv;
+    ops.wgrad = NnLaunch::wgrad;
+    ops.swizzle = NnLaunch::swizzle;
+    ops.adam = NnLaunch::adam;
+    ops.axpy = NnLaunch::axpy;
+    ops.residual = NnLaunch::residual;
+}
+
+}  // namespace dj

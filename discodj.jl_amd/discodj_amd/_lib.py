@@ -71,7 +71,16 @@ _SIGS = {
     "dj_vjp_run": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "dj_vjp_ics_lpt": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32,
                                    C.c_double, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "dj_nn_set_params": (C.c_int32, [C.c_void_p, C.c_void_p]),
+    "dj_nn_get_params": (C.c_int32, [C.c_void_p, C.c_void_p]),
+    "dj_nn_enable": (C.c_int32, [C.c_void_p, C.c_int32, C.c_double]),
+    "dj_nn_set_schedule": (C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p]),
+    "dj_nn_forward": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p]),
+    "dj_nn_backward": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p]),
+    "dj_nn_train_step": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_double,
+                                     C.c_int32, C.POINTER(C.c_double)]),
 }
+NN_PARAMS = 4 * (27 * 16 * 16 + 16)
 
 _lib = None
 

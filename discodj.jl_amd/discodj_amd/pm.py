@@ -150,6 +150,49 @@ class PMSolverHIP:
     def check_finite(self):
         self._chk(self._L.dj_check_finite(self._ctx))
 
+    # learned correction -----------------------------------------------------------------------
+    def nn_set_params(self, params):
+        p = np.ascontiguousarray(params, np.float32)
+        assert p.size == _lib.NN_PARAMS
+        self._chk(self._L.dj_nn_set_params(self._ctx, p.ctypes.data))
+
+    def nn_get_params(self):
+        p = np.empty(_lib.NN_PARAMS, np.float32)
+        self._chk(self._L.dj_nn_get_params(self._ctx, p.ctypes.data))
+        return p
+
+    def nn_enable(self, on=True, out_scale=1.0):
+        self._chk(self._L.dj_nn_enable(self._ctx, int(on), float(out_scale)))
+
+    def nn_set_schedule(self, a_force):
+        a = np.ascontiguousarray(a_force, np.float64)
+        self._chk(self._L.dj_nn_set_schedule(self._ctx, len(a), a.ctypes.data))
+
+    def nn_forward(self, dens, a):
+        """The potential correction (N,N,N) of a density mesh (the unnormalised deposit) at scale factor a."""
+        N = self.cfg.res_pm
+        d = np.ascontiguousarray(dens, np.float32).reshape(N, N, N)
+        out = np.empty((N, N, N), np.float32)
+        self._chk(self._L.dj_nn_forward(self._ctx, d.ctypes.data, float(a), out.ctypes.data))
+        return out
+
+    def nn_backward(self, dens, a, gphi):
+        """Parameter gradient of <gphi, nn_forward(dens, a)>."""
+        N = self.cfg.res_pm
+        d = np.ascontiguousarray(dens, np.float32).reshape(N, N, N)
+        g = np.ascontiguousarray(gphi, np.float32).reshape(N, N, N)
+        out = np.empty(_lib.NN_PARAMS, np.float32)
+        self._chk(self._L.dj_nn_backward(self._ctx, d.ctypes.data, float(a), g.ctypes.data, out.ctypes.data))
+        return out
+
+    def nn_train_step(self, psi, acc_target, a, lr, step, weight=1.0):
+        """One Adam step on weight x the mean squared force error of the corrected PM at snapshot psi;
+        -> the loss before the step."""
+        loss = C.c_double()
+        self._chk(self._L.dj_nn_train_step(self._ctx, self._arr(psi).ctypes.data, self._arr(acc_target).ctypes.data,
+                                           float(a), float(weight), float(lr), int(step), C.byref(loss)))
+        return loss.value
+
     # reverse mode -----------------------------------------------------------------------------
     def vjp_acceleration(self, psi, acc_bar):
         """(d acc / d psi)^T acc_bar of pm_acceleration at psi (None: the solver's state)."""

@@ -161,6 +161,28 @@ dj_status dj_vjp_ics_lpt(dj_ctx *ctx, uint32_t seed, const double *white, const 
                          int32_t ntab, double D, int32_t order, const void *psi_bar, const void *pi_bar,
                          double *white_bar_out);
 
+/* Learned force correction (no counterpart in the reference; for callers that run the corrected PM of
+ * the learned-correction workload).  A residual 3-D convolution net on the mesh, BF16 weights and
+ * activations with FP32 accumulation on MFMA, adds s_out NN(log(1 + delta), a) to the potential of every
+ * force pass.  Built for FP32 contexts with the float deposit, finite-difference gradients and res_pm a
+ * multiple of 64.  Parameters: DJ_NN_PARAMS FP32 values, 4 layers of W[27][16][16] (tap (dz,dy,dx)
+ * z-major, out, in; layer 1 uses inputs 0, 1 and layer 4 output 0, the rest is zero) then b[16]. */
+#define DJ_NN_PARAMS (4 * (27 * 16 * 16 + 16))
+dj_status dj_nn_set_params(dj_ctx *ctx, const float *params);  /* also resets the optimiser state */
+dj_status dj_nn_get_params(dj_ctx *ctx, float *params);
+/* on: add the correction, scaled by out_scale, in every force evaluation */
+dj_status dj_nn_enable(dj_ctx *ctx, int32_t on, double out_scale);
+/* the net's time input per step of dj_run (n values: the scale factor at each step's force) */
+dj_status dj_nn_set_schedule(dj_ctx *ctx, int32_t n, const double *a_force);
+/* the correction (N^3 floats) of a density mesh (N^3 floats, the unnormalised deposit) at scale factor a */
+dj_status dj_nn_forward(dj_ctx *ctx, const float *dens, double a, float *dphi_out);
+/* parameter gradient (DJ_NN_PARAMS floats) of <gphi, dphi(dens, a)> */
+dj_status dj_nn_backward(dj_ctx *ctx, const float *dens, double a, const float *gphi, float *grad_out);
+/* one Adam step (learning rate lr, step >= 1) on L = weight mean((acc(psi) - acc_target)^2) of the corrected
+ * force at the snapshot psi (antialias 0); loss_out receives L before the update */
+dj_status dj_nn_train_step(dj_ctx *ctx, const void *psi, const void *acc_target, double a, double weight, double lr,
+                           int32_t step, double *loss_out);
+
 /* NaN/Inf scan of the particle state (failure detection); DJ_ERR_NONFINITE if any */
 dj_status dj_check_finite(dj_ctx *ctx);
 

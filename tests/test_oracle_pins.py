@@ -250,3 +250,41 @@ def test_callback_steps():
     O.run_nbody(c, psi, psi / 0.05, O.pm_config(res_pm=16, n_part=n, boxsize=100.0), 0.05, 0.5, 10,
                 lean=True, callback=lambda k, a: seen.append(k))
     assert seen == list(range(11))
+
+
+# ------------------------------------------------------------------ learned-correction oracle
+def test_nn_oracle_bf16_rounding_known_answers():
+    import nn_ref as R
+    # 1 + 2^-8 is a tie between 1 and 1 + 2^-7: nearest even is 1; 1 + 3 2^-9 rounds up; sign and
+    # exact values are kept
+    x = np.array([1 + 2.0**-8, 1 + 3 * 2.0**-9, -(1 + 2.0**-8), 3.0, 1 + 2.0**-7 + 2.0**-8], np.float32)
+    assert list(R.bf16(x)) == [1.0, 1 + 2.0**-7, -1.0, 3.0, 1 + 2.0**-6]
+
+
+def test_nn_oracle_gradient_is_the_derivative_of_its_forward():
+    """With the roundings off, the oracle's parameter gradient equals central differences of its forward
+    (which pins the chain rule through the residual blocks, the relu masks and the transposed taps)."""
+    import nn_ref as R
+    N, n, a, s = 8, 4, 0.5, 0.3
+    rng = np.random.default_rng(0)
+    p = np.zeros(R.N_PARAMS, np.float32)
+    for l, (co, ci) in enumerate([(16, 2), (16, 16), (16, 16), (1, 16)]):
+        W = np.zeros((27, 16, 16), np.float32)
+        W[:, :co, :ci] = rng.normal(scale=0.3, size=(27, co, ci))
+        p[l * R.PER_LAYER:l * R.PER_LAYER + 27 * 256] = W.reshape(-1)
+        p[l * R.PER_LAYER + 27 * 256:l * R.PER_LAYER + 27 * 256 + co] = rng.normal(scale=0.1, size=co)
+    p = p.astype(np.float64)
+    dens = rng.lognormal(size=(N, N, N)) * (n / N) ** 3
+    g = rng.normal(size=(N, N, N))
+    out, cache = R.forward(p, dens, (N / n) ** 3, a, s, rnd=R.exact)
+    grad = R.backward(cache, g, s)
+    idx = [0, 27 * 256 + 3, R.PER_LAYER + 5 * 256 + 17, 2 * R.PER_LAYER + 13 * 256 + 40, 3 * R.PER_LAYER + 26 * 16 + 7,
+           3 * R.PER_LAYER + 27 * 256]
+    for i in idx:
+        eps = 1e-6
+        pp, pm = p.copy(), p.copy()
+        pp[i] += eps
+        pm[i] -= eps
+        fd = (np.sum(g * R.forward(pp, dens, (N / n) ** 3, a, s, rnd=R.exact)[0]) -
+              np.sum(g * R.forward(pm, dens, (N / n) ** 3, a, s, rnd=R.exact)[0])) / (2 * eps)
+        assert abs(fd - grad[i]) <= 1e-6 * max(1.0, abs(fd)), (i, fd, grad[i])
