@@ -17,6 +17,8 @@ Reported beside the throughput:
                  cores: one interlacing pass of the same force, the step counted as two passes
 
 usage: python bench.py [--gpus 1] [--steps 19] [--warmup 1] [--no-accuracy] [--no-cpu] [--small]
+       python bench.py --workload gradient|learned      (the differentiation and learned-correction
+                                                        workloads at 256^3; one JSON line each)
 """
 import argparse
 import json
@@ -43,6 +45,7 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--small", action="store_true", help="128^3 particles on a 256^3 mesh (quick check)")
+    ap.add_argument("--workload", choices=["pm", "gradient", "learned"], default="pm")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy kernel three times (the reference point that scales "
                          "the FETCH_SIZE / WRITE_SIZE counters of a profiling run to bytes)")
@@ -57,6 +60,10 @@ def main():
         torch.cuda.synchronize()
         log("copy calibration: 3 launches of a 1 GiB read + 1 GiB write elementwise kernel")
         return 0
+    if args.workload == "gradient":
+        return bench_gradient(args)
+    if args.workload == "learned":
+        return bench_learned(args)
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every box of this build has one MI355X; --gpus {args.gpus} "
               f"is not run (the multi-GPU slab/pencil decomposition is a design, not a build)", file=sys.stderr)
@@ -170,6 +177,92 @@ def main():
                       "particles": n**3, "mesh": N**3, "boxsize": L, "stepper": "fastpm", "n_steps": total},
            "roofline": roof, "kernels": kernels, "accuracy": accuracy, "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
+    return 0
+
+
+def bench_gradient(args):
+    """Gradient of sum (delta - target)^2 on the 256^3 mesh with respect to the 256^3 white noise through
+    2LPT and 10 BullFrog CIC steps (grad order 4), FP32: seconds per gradient, split into the forward run
+    with step checkpoints and the reverse pass."""
+    import numpy as np
+
+    import discodj_amd as D
+
+    n, L = (64, 125.0) if args.small else (256, 500.0)
+    c = D.compute_timetables(D.Cosmology(**COSMO))
+    k_tab, pk_tab = D.linear_power_spectrum(c)
+    white = np.random.default_rng(42).normal(size=(n, n, n))
+    target = np.zeros((n, n, n))
+    co = D.stepper_coefficients(c, "bullfrog", 0.05, 1.0, 10, "D")
+    cfg = D.PMConfig(res_pm=n, n_part=n, boxsize=L, worder=2, grad_order=4)
+    with D.PMSolverHIP(cfg, precision=32) as S:
+        times = []
+        for rep in range(3):
+            S.synchronize()
+            t0 = time.perf_counter()
+            S.ics_lpt(k_tab, pk_tab, c.Dplus(0.05), order=2, white=white)
+            S.run_checkpointed(co["ddrift1"], co["alpha"], co["beta"], co["ddrift2"])
+            rho = n**3 / n**3
+            delta = S.deposit().astype(np.float64) / rho - 1.0
+            S.synchronize()
+            t1 = time.perf_counter()
+            pb = S.vjp_deposit(None, 2.0 * (delta - target) / rho)
+            pb, mb = S.vjp_run(pb, np.zeros_like(pb))
+            wb = S.vjp_ics_lpt(k_tab, pk_tab, c.Dplus(0.05), pb, mb, order=2, white=white)
+            S.synchronize()
+            t2 = time.perf_counter()
+            times.append((t1 - t0, t2 - t1))
+            log(f"gradient {rep}: forward {t1 - t0:.3f}s, reverse {t2 - t1:.3f}s, |wbar| {np.linalg.norm(wb):.3e}")
+    fw, bw = np.median([t[0] for t in times]), np.median([t[1] for t in times])
+    print(json.dumps({"metric": "gradients/s of the density loss w.r.t. the white noise (2LPT + 10 BullFrog steps)",
+                      "value": 1.0 / (fw + bw), "unit": "gradients/s", "forward_s": fw, "reverse_s": bw,
+                      "dtype": "f32", "data": "synthetic", "n_gpus": 1,
+                      "config": {"workload": f"{n}^3 particles, {n}^3 mesh, L={L}, CIC, grad order 4, BullFrog 10",
+                                 "median_of": 3}}), flush=True)
+    return 0
+
+
+def bench_learned(args):
+    """Step time of the plain PM and of the PM with the learned correction (untrained weights: the
+    kernels' cost does not depend on their values) at 256^3 particles on a 256^3 mesh, BullFrog 10 steps."""
+    import numpy as np
+
+    import discodj_amd as D
+
+    n, L = (64, 125.0) if args.small else (256, 500.0)
+    c = D.compute_timetables(D.Cosmology(**COSMO))
+    k_tab, pk_tab = D.linear_power_spectrum(c)
+    co = D.stepper_coefficients(c, "bullfrog", 0.05, 1.0, 10, "D")
+    coef = [co["ddrift1"], co["alpha"], co["beta"], co["ddrift2"]]
+    cfg = D.PMConfig(res_pm=n, n_part=n, boxsize=L, worder=2, grad_order=4)
+    out = {}
+    with D.PMSolverHIP(cfg, precision=32) as S:
+        S.nn_set_params(D.learned.init_params(0))
+        S.nn_set_schedule(D.learned.force_scale_factors(c, co))
+        for mode in ("plain", "learned", "plain", "learned"):
+            S.nn_enable(mode == "learned", 1.0)
+            S.ics_lpt(k_tab, pk_tab, c.Dplus(0.05), order=2, seed=42)
+            S.synchronize()
+            S.set_profiling(True)
+            S.reset_stats()
+            t0 = time.perf_counter()
+            S.run(*coef)
+            S.synchronize()
+            out[mode] = (time.perf_counter() - t0) / 10
+            stats = S.kernel_stats()
+            S.set_profiling(False)
+            log(f"{mode}: {out[mode] * 1e3:.2f} ms/step")
+        nn = stats.get("nn_correction", {})
+    nn_ms = nn["ms_total"] / nn["calls"] if nn.get("calls") else None
+    flops = 4 * 27 * 2 * 16 * 16 * 16 * (n**3 / 16)  # MFMA work of the four layers, padding included
+    print(json.dumps({"metric": "step time of the learned-correction PM relative to plain PM",
+                      "value": out["learned"] / out["plain"], "unit": "ratio", "higher_is_better": False,
+                      "ms_per_step_plain": out["plain"] * 1e3, "ms_per_step_learned": out["learned"] * 1e3,
+                      "nn_ms_per_call": nn_ms,
+                      "nn_mfma_tflops": (flops / (nn_ms * 1e-3) / 1e12) if nn_ms else None,
+                      "dtype": "bf16 net / f32 PM", "data": "synthetic", "n_gpus": 1,
+                      "config": {"workload": f"{n}^3 particles, {n}^3 mesh, L={L}, CIC, grad order 4, BullFrog 10"}}),
+          flush=True)
     return 0
 
 
