@@ -56,57 +56,119 @@ __device__ __forceinline__ s4v nn_bop(const unsigned short *hin, const float *fi
     return b;
 }
 
-// one wave per 16 voxels x 16 output channels; grid covers N^3 / 16 waves, 4 waves per block
-template <int IN, int EPI>
-__global__ __launch_bounds__(256) void k_nn_conv(int N, const unsigned short *__restrict__ hin, const float *__restrict__ fin,
-                                                 float inv_rho, float cval, const s4v *__restrict__ wsw,
-                                                 const float *__restrict__ bias, const unsigned short *__restrict__ res,
-                                                 const unsigned short *__restrict__ zmask, unsigned short *__restrict__ out_z,
-                                                 unsigned short *__restrict__ out_h, float *__restrict__ out_f, float oscale) {
-    const int lane = threadIdx.x & 63;
-    const size_t w = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int rows = N / 16;
-    if (w >= (size_t)rows * N * N) return;
-    const int x0 = (int)(w % rows) * 16, y = (int)((w / rows) % N), z = (int)(w / ((size_t)rows * N));
-    const int vx = x0 + (lane & 15), cg = 4 * (lane >> 4);
-    f4v acc = {0, 0, 0, 0};
+// One block marches along z over a 64 (x) x 8 (y) footprint, 16 planes: the input planes z-1, z, z+1
+// (with a one-voxel x-y halo, 66 x 10 voxels) sit in a three-slot LDS ring, so each input voxel is
+// read from memory about 1.3 times instead of once per tap.  Two blocks share a CU (77 KB of LDS
+// each), so one stages while the other computes.  Wave w computes y rows 2w, 2w+1 of the plane (8
+// segments of 16 voxels), per tap one MFMA per segment with B read from LDS (8 bytes per lane,
+// conflict-free) and A (the tap's weights) from LDS.
+constexpr int NN_TX = 64, NN_TY = 8, NN_TZ = 16, NN_PX = NN_TX + 2, NN_PY = NN_TY + 2, NN_PV = NN_PX * NN_PY;
+constexpr int NN_CHUNKS = NN_PV * 2;                       // 16-byte chunks of a 16-channel plane
+constexpr int NN_PER = (NN_CHUNKS + 255) / 256;            // chunks per thread
+
+// stage input plane zz (wrapped) of the block's footprint into an LDS slot as 16-channel BF16; a field
+// input becomes (bf16(input), bf16(cval), 0, ...)
+template <int IN>
+__device__ __forceinline__ void nn_plane_stage(unsigned short *slot, const unsigned short *hin, const float *fin, int N,
+                                               int x0, int y0, int zz, float inv_rho, float cval) {
+    const size_t zb = (size_t)N * N * nn_wrap(zz, N);
+    if constexpr (IN == NN_IN_H) {
 #pragma unroll
-    for (int dz = -1; dz <= 1; dz++)
-#pragma unroll
-        for (int dy = -1; dy <= 1; dy++) {
-            const size_t rowb = (size_t)N * ((size_t)nn_wrap(y + dy, N) + (size_t)N * nn_wrap(z + dz, N));
-#pragma unroll
-            for (int dx = -1; dx <= 1; dx++) {
-                const int t = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
-                const s4v a = wsw[t * 64 + lane];
-                const s4v b = nn_bop<IN>(hin, fin, inv_rho, cval, rowb + nn_wrap(vx + dx, N), cg);
-                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+        for (int k = 0; k < NN_PER; k++) {
+            const int ch = threadIdx.x + 256 * k;
+            if (ch < NN_CHUNKS) {
+                const int vox = ch >> 1, xi = vox % NN_PX, yi = vox / NN_PX;
+                const size_t v = zb + (size_t)nn_wrap(x0 - 1 + xi, N) + (size_t)N * nn_wrap(y0 - 1 + yi, N);
+                *(uint4 *)(slot + ch * 8) = *(const uint4 *)(hin + v * 16 + (ch & 1) * 8);
             }
         }
-    // lane holds D[cg + j][lane & 15]: output channels cg..cg+3 of voxel vx
-    const size_t v = (size_t)vx + (size_t)N * ((size_t)y + (size_t)N * z);
-    if constexpr (EPI == NN_EPI_OUT) {
-        if (cg == 0) out_f[v] = oscale * (acc[0] + bias[0]);
     } else {
-        s4v zo, ho;
-        s4v r = {0, 0, 0, 0}, m = {0, 0, 0, 0};
-        if (res) r = *(const s4v *)(res + v * 16 + cg);
-        if constexpr (EPI == NN_EPI_BWD) m = *(const s4v *)(zmask + v * 16 + cg);
+        const unsigned c1 = (unsigned)f2bf(cval) << 16;
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const float rv = bf2f((unsigned short)r[j]);
-            if constexpr (EPI == NN_EPI_HIDDEN) {
-                const float zz = acc[j] + bias[cg + j];
-                zo[j] = (short)f2bf(zz);
-                ho[j] = (short)f2bf(rv + fmaxf(zz, 0.0f));
-            } else {
-                const float g = rv + acc[j];
-                ho[j] = (short)f2bf(g);
-                zo[j] = (short)f2bf(bf2f((unsigned short)m[j]) > 0.0f ? g : 0.0f);
+        for (int k = 0; k < 3; k++) {
+            const int vox = threadIdx.x + 256 * k;
+            if (vox < NN_PV) {
+                const int xi = vox % NN_PX, yi = vox / NN_PX;
+                const float fv = fin[zb + (size_t)nn_wrap(x0 - 1 + xi, N) + (size_t)N * nn_wrap(y0 - 1 + yi, N)];
+                const float x = IN == NN_IN_FIELD ? nn_input(fv, inv_rho) : fv * inv_rho;
+                *(uint4 *)(slot + vox * 16) = make_uint4((unsigned)f2bf(x) | c1, 0u, 0u, 0u);
+                *(uint4 *)(slot + vox * 16 + 8) = make_uint4(0u, 0u, 0u, 0u);
             }
         }
-        if (out_z) *(s4v *)(out_z + v * 16 + cg) = zo;  // pre-activations: kept for training only
-        *(s4v *)(out_h + v * 16 + cg) = ho;
+    }
+}
+
+template <int IN, int EPI>
+__global__ __launch_bounds__(256, 2) void k_nn_conv(int N, const unsigned short *__restrict__ hin,
+                                                    const float *__restrict__ fin, float inv_rho, float cval,
+                                                    const s4v *__restrict__ wsw, const float *__restrict__ bias,
+                                                    const unsigned short *__restrict__ res,
+                                                    const unsigned short *__restrict__ zmask,
+                                                    unsigned short *__restrict__ out_z, unsigned short *__restrict__ out_h,
+                                                    float *__restrict__ out_f, float oscale) {
+    extern __shared__ unsigned char nn_smem[];
+    s4v *wl = (s4v *)nn_smem;                                           // 27 x 64 lane-ordered A operands
+    unsigned short *ring = (unsigned short *)(nn_smem + 27 * 64 * 8);   // 3 x NN_PV x 16 BF16
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, cg = 4 * (lane >> 4), lx = lane & 15;
+    const int nbx = N / NN_TX, nby = N / NN_TY;
+    const int x0 = (blockIdx.x % nbx) * NN_TX, y0 = ((blockIdx.x / nbx) % nby) * NN_TY,
+              z0 = (blockIdx.x / (nbx * nby)) * NN_TZ;
+    for (int i = threadIdx.x; i < 27 * 64; i += 256) wl[i] = wsw[i];
+    auto slot = [&](int z) { return ring + (size_t)(((z % 3) + 3) % 3) * NN_PV * 16; };
+    nn_plane_stage<IN>(slot(z0 - 1), hin, fin, N, x0, y0, z0 - 1, inv_rho, cval);
+    nn_plane_stage<IN>(slot(z0), hin, fin, N, x0, y0, z0, inv_rho, cval);
+    for (int z = z0; z < z0 + NN_TZ; z++) {
+        nn_plane_stage<IN>(slot(z + 1), hin, fin, N, x0, y0, z + 1, inv_rho, cval);
+        __syncthreads();  // planes z-1, z, z+1 (and the weights) in LDS
+        f4v acc[8];
+#pragma unroll
+        for (int sg = 0; sg < 8; sg++) acc[sg] = {0, 0, 0, 0};
+#pragma unroll 1
+        for (int dz = -1; dz <= 1; dz++) {
+            const unsigned short *sh = slot(z + dz);
+#pragma unroll
+            for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+                for (int dx = -1; dx <= 1; dx++) {
+                    const s4v a = wl[((dz + 1) * 9 + (dy + 1) * 3 + (dx + 1)) * 64 + lane];
+#pragma unroll
+                    for (int sg = 0; sg < 8; sg++) {
+                        const int yi = 2 * wave + (sg >> 2) + dy + 1, xi = (sg & 3) * 16 + lx + dx + 1;
+                        const s4v b = *(const s4v *)(sh + (yi * NN_PX + xi) * 16 + cg);
+                        acc[sg] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[sg], 0, 0, 0);
+                    }
+                }
+        }
+        // epilogue: lane holds D[cg + j][lx] of each segment: channels cg..cg+3 of one voxel
+#pragma unroll
+        for (int sg = 0; sg < 8; sg++) {
+            const int y = y0 + 2 * wave + (sg >> 2), x = x0 + (sg & 3) * 16 + lx;
+            const size_t v = (size_t)x + (size_t)N * ((size_t)y + (size_t)N * z);
+            if constexpr (EPI == NN_EPI_OUT) {
+                if (cg == 0) out_f[v] = oscale * (acc[sg][0] + bias[0]);
+            } else {
+                s4v zo, ho;
+                s4v rr = {0, 0, 0, 0}, m = {0, 0, 0, 0};
+                if (res) rr = *(const s4v *)(res + v * 16 + cg);
+                if constexpr (EPI == NN_EPI_BWD) m = *(const s4v *)(zmask + v * 16 + cg);
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const float rv = bf2f((unsigned short)rr[j]);
+                    if constexpr (EPI == NN_EPI_HIDDEN) {
+                        const float zz = acc[sg][j] + bias[cg + j];
+                        zo[j] = (short)f2bf(zz);
+                        ho[j] = (short)f2bf(rv + fmaxf(zz, 0.0f));
+                    } else {
+                        const float g = rv + acc[sg][j];
+                        ho[j] = (short)f2bf(g);
+                        zo[j] = (short)f2bf(bf2f((unsigned short)m[j]) > 0.0f ? g : 0.0f);
+                    }
+                }
+                if (out_z) *(s4v *)(out_z + v * 16 + cg) = zo;  // pre-activations: kept for training only
+                *(s4v *)(out_h + v * 16 + cg) = ho;
+            }
+        }
+        __syncthreads();  // slot (z - 1) is overwritten by plane z + 2 next
     }
 }
 
@@ -243,13 +305,15 @@ struct NnLaunch {
     static void conv(int in, int epi, int N, const void *hin, const void *fin, double inv_rho, double cval,
                      const void *wsw, const void *bias, const void *res, const void *zmask, void *out_z, void *out_h,
                      void *out_f, double oscale, hipStream_t st) {
-        const size_t waves = (size_t)N * N * N / 16;
-        const dim3 g(ngrid(waves, 4));
+        const dim3 g((N / NN_TX) * (N / NN_TY) * (N / NN_TZ));
+        const size_t shm = (size_t)27 * 64 * 8 + (size_t)3 * NN_PV * 16 * 2;
 #define NN_C(I, E)                                                                                                   \
-    k_nn_conv<I, E><<<g, 256, 0, st>>>(N, (const unsigned short *)hin, (const float *)fin, (float)inv_rho, (float)cval, \
+    do {                                                                                                             \
+    (void)hipFuncSetAttribute((const void *)k_nn_conv<I, E>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);   \
+    k_nn_conv<I, E><<<g, 256, shm, st>>>(N, (const unsigned short *)hin, (const float *)fin, (float)inv_rho, (float)cval, \
                                        (const s4v *)wsw, (const float *)bias, (const unsigned short *)res,              \
                                        (const unsigned short *)zmask, (unsigned short *)out_z, (unsigned short *)out_h, \
-                                       (float *)out_f, (float)oscale)
+                                       (float *)out_f, (float)oscale); } while (0)
         if (in == NN_IN_FIELD && epi == NN_EPI_HIDDEN) NN_C(NN_IN_FIELD, NN_EPI_HIDDEN);
         else if (in == NN_IN_H && epi == NN_EPI_HIDDEN) NN_C(NN_IN_H, NN_EPI_HIDDEN);
         else if (in == NN_IN_H && epi == NN_EPI_OUT) NN_C(NN_IN_H, NN_EPI_OUT);
