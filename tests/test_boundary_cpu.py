@@ -152,3 +152,26 @@ def test_fft_poisson_emulated_vs_oracle(code):
     K[0, 0, 0] = 0
     ref = O.brfft3(F * K)
     assert G.rel(out, ref) < 1e-13
+
+
+def test_snapshot_roundtrip_and_layout(tmp_path):
+    from discodj_amd import snapshots as SN
+    from discodj_amd.nbody import add_q
+    rng = np.random.default_rng(0)
+    n, L = 8, 50.0
+    psi = rng.normal(size=(3, n, n, n)).astype(np.float32)
+    P = rng.normal(size=(3, n, n, n)).astype(np.float32)
+    # remove_q inverts add_q (the X that run_nbody's callback receives)
+    assert np.allclose(SN.remove_q(add_q(psi, L), L), psi, atol=1e-5)
+    w = SN.SnapshotWriter(str(tmp_path), [0.5, 1.0], L)
+    assert w.when(3, 0.5) and w.when(9, 1.0) and not w.when(4, 0.6)
+    w(3, 0.5, add_q(psi, L), P)
+    w(4, 0.6, add_q(psi, L), P)  # not a target: nothing written
+    assert len(w.written) == 1
+    s = SN.read_snapshot(w.written[0])
+    assert s["a"] == 0.5 and s["step"] == 3 and s["n"] == n and s["boxsize"] == L
+    assert np.allclose(s["psi"], psi, atol=1e-5) and np.array_equal(s["P"], P)
+    # the reference's column-major (n,n,n,3) [i,j,k,c] array has the same memory layout
+    ref = np.asfortranarray(s["psi"].transpose(3, 2, 1, 0))
+    assert ref[2, 5, 7, 1] == s["psi"][1, 7, 5, 2]
+    assert np.array_equal(ref.ravel(order="F"), s["psi"].ravel(order="C"))

@@ -240,3 +240,31 @@ def test_fp32_run_vs_fp64_oracle_c1_shape():
                             a_end=1.0, n_steps=10, res_pm=res, antialias=1, return_displacement=True)
     rms = np.sqrt(np.mean((got.astype(np.float64) - ref) ** 2)) / (L / n)
     assert rms < 1e-3
+
+
+def test_snapshots_through_the_step_callback(tmp_path):
+    """run_nbody's callback writes snapshots at chosen step times; the last equals the returned state
+    and a run restarted from a middle snapshot reaches the same final state."""
+    from discodj_amd import snapshots as SN
+    kw = G.kwargs("fastpm_D_tsc")
+    c = D.Cosmology.named("Planck18EEBAOSN")
+    co = D.stepper_coefficients(c, kw["stepper"], kw["a_ini"], kw["a_end"], kw["n_steps"], "D")
+    targets = [co["all_a"][3], co["all_a"][-1]]
+    w = SN.SnapshotWriter(str(tmp_path), targets, G.BOXSIZE)
+    run = dict(boxsize=G.BOXSIZE, res_pm=kw["res_pm"], stepper=kw["stepper"], worder=kw["worder"],
+               grad_kernel_order=kw["grad_kernel_order"], return_displacement=True, precision=64)
+    psi, P, _ = D.run_nbody(c, G.particles("ics/psi_ini"), G.particles("ics/pi_ini"), a_ini=kw["a_ini"],
+                            a_end=kw["a_end"], n_steps=kw["n_steps"], step_callback=w, callback_when=w.when, **run)
+    assert len(w.written) == 2
+    last = SN.read_snapshot(w.written[-1])
+    assert np.allclose(last["psi"], psi, rtol=0, atol=1e-12) and np.array_equal(last["P"], P)
+    mid = SN.read_snapshot(w.written[0])
+    assert mid["step"] == 3
+    # restart: canonical P back to Pi, the remaining steps on the same grid
+    co2 = {k: np.asarray(v)[3:] for k, v in co.items() if k != "all_a"}
+    with D.PMSolverHIP(D.PMConfig(res_pm=kw["res_pm"], n_part=G.N_PART, boxsize=G.BOXSIZE, worder=kw["worder"],
+                                  grad_order=kw["grad_kernel_order"]), precision=64) as S:
+        S.set_state(mid["psi"], mid["P"] / c.Fplus(mid["a"]))
+        S.run(co2["ddrift1"], co2["alpha"], co2["beta"], co2["ddrift2"])
+        psi2, _ = S.get_state()
+    assert G.rel(psi2, psi) < 1e-12
