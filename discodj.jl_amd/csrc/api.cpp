@@ -94,6 +94,8 @@ struct dj_ctx {
     void *nn_wf = nullptr, *nn_wb = nullptr;  // (layers, 27, 64) x 8 bytes, forward / transposed
     void *nn_t[10] = {};                      // z1 z2 z3 h1 h2 h3 and four gradient tensors (N^3 x 16 BF16)
     void *nn_dphi = nullptr, *nn_dens = nullptr, *nn_tgt = nullptr, *nn_sum = nullptr;
+    void *acc_list = nullptr;  // (entries, 3): the acceleration between interlacing passes, in tile-list order
+    size_t acc_list_cap = 0;
     std::string err;
     bool prof = false;
     std::vector<ProfEvent> pending;
@@ -315,7 +317,7 @@ void free_all(dj_ctx *c) {
                     c->vb_psi1, c->vb_abar, c->vb_psibar, c->vb_pibar, c->vb_fbar, c->vb_pbar, c->ckpt,
                     c->nn_p, c->nn_g, c->nn_m, c->nn_v, c->nn_mask, c->nn_wf, c->nn_wb, c->nn_t[0], c->nn_t[1],
                     c->nn_t[2], c->nn_t[3], c->nn_t[4], c->nn_t[5], c->nn_t[6], c->nn_t[7], c->nn_t[8], c->nn_t[9],
-                    c->nn_dphi, c->nn_dens, c->nn_tgt, c->nn_sum};
+                    c->nn_dphi, c->nn_dens, c->nn_tgt, c->nn_sum, c->acc_list};
     // teardown: failures here have no caller to report to
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -450,11 +452,24 @@ dj_status force(dj_ctx *c, const void *psi, int mode, const KickArgs &kick) {
     dj_status rs = tile_sort(c, psi, c->d_shifts, ns, cfg.worder);
     if (rs) return rs;
     (void)mesh_bytes;
+    // in a step the passes hand the acceleration over in tile-list order: both walk the same lists, so
+    // the hand-over is read and written contiguously instead of at the particles' positions
+    const bool lacc = mode == FORCE_KICK && ns > 1 && !ik;
+    if (lacc && c->acc_list_cap < c->tent_cap) {
+        if (c->acc_list) HIPCHK(c, hipFree(c->acc_list));
+        c->acc_list = nullptr;
+        HIPCHK(c, hipMalloc(&c->acc_list, 3 * c->tent_cap * c->tsize));
+        c->acc_list_cap = c->tent_cap;
+    }
     for (int s = 0; s < ns; s++) {
         pa.shift[0] = c->shifts[s][0];
         pa.shift[1] = c->shifts[s][1];
         pa.shift[2] = c->shifts[s][2];
         TileArgs ta = tile_args(c, psi, c->d_shifts, ns);
+        if (lacc) {
+            ta.acc = c->acc_list;
+            ta.acc_by_entry = 1;
+        }
         { Prof p(c, K_SCATTER); c->tiles.deposit(ta, cfg.worder, fixed, c->shifts[s], dens, fscale_dep(cfg), c->st); }
         if (cfg.n_resample > 1) {
             Prof p(c, K_RESAMPLE);

@@ -83,6 +83,7 @@ struct TileArgs {
     unsigned long long *scan_part;  // (nt / 1024 + 1) block sums of the list scan
     void *ent;             // entries (X, particle index)
     size_t ent_cap;        // entries the buffer holds
+    int acc_by_entry;      // the gather's acc is (ent_cap, 3) in list order (between interlacing passes)
 };
 struct TileOps {
     void (*count)(const TileArgs &a, int W, hipStream_t st);  // exact counts into cnt

@@ -567,7 +567,7 @@ template <typename T, int W> constexpr int gather_min_waves() { return sizeof(T)
 template <typename T, int MODE>
 __device__ __forceinline__ void gather_fetch(const Entry<T> *__restrict__ ent, unsigned e, const T *psi_in,
                                              const T *__restrict__ mom, const T *__restrict__ acc, size_t np, int ns,
-                                             Entry<T> &en, T (&m3)[3], T (&p3)[3], T (&a3)[3]) {
+                                             int lacc, Entry<T> &en, T (&m3)[3], T (&p3)[3], T (&a3)[3]) {
     en = ent[e];
     const size_t p = (size_t)en.idx;
     if constexpr (MODE == GATHER_KICK) {
@@ -578,8 +578,9 @@ __device__ __forceinline__ void gather_fetch(const Entry<T> *__restrict__ ent, u
             p3[c] = psi_in[p + c * np];
         }
     } else if constexpr (MODE == GATHER_ACC) {
+        const size_t ai = lacc ? 3 * (size_t)e : p, as = lacc ? 1 : np;
 #pragma unroll
-        for (int c = 0; c < 3; c++) a3[c] = acc[p + c * np];
+        for (int c = 0; c < 3; c++) a3[c] = acc[ai + c * as];
     }
 }
 
@@ -654,7 +655,7 @@ __global__ __launch_bounds__((gather_threads<T, W>()), (gather_min_waves<T, W>()
                                                       const unsigned *__restrict__ own, TileGeom g, T sx, T sy, T sz,
                                                       T L, const T *__restrict__ phi, T c1, T c2, T c3, const T *psi_in,
                                                       T *psi_out, T *__restrict__ mom, T *__restrict__ acc, size_t np,
-                                                      T alpha, T beta, T dd2, T dd1n, int has_next, int ns) {
+                                                      T alpha, T beta, T dd2, T dd1n, int has_next, int ns, int lacc) {
     constexpr int H = ORDER / 2, Q = W + 2 * H, NT = gather_threads<T, W>();
     extern __shared__ unsigned char smem[];
     T *ph = (T *)smem;
@@ -672,7 +673,7 @@ __global__ __launch_bounds__((gather_threads<T, W>()), (gather_min_waves<T, W>()
     unsigned e = e0 + threadIdx.x;
     Entry<T> en{};
     T m3[3] = {0, 0, 0}, p3[3] = {0, 0, 0}, a3[3] = {0, 0, 0};
-    if (e < e1) gather_fetch<T, MODE>(ent, e, psi_in, mom, acc, np, ns, en, m3, p3, a3);
+    if (e < e1) gather_fetch<T, MODE>(ent, e, psi_in, mom, acc, np, ns, lacc, en, m3, p3, a3);
 
     if (region_ok) {
         const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -711,17 +712,19 @@ __global__ __launch_bounds__((gather_threads<T, W>()), (gather_min_waves<T, W>()
         }
         const size_t p = (size_t)en.idx;
         const T f3[3] = {s1, s2, s3};
+        // acc of this particle: (3, Np) in particle order, or (entries, 3) in list order between the passes
+        const size_t ai = lacc ? 3 * (size_t)e : p, as = lacc ? 1 : np;
         if constexpr (MODE == GATHER_ACC) {
 #pragma unroll
-            for (int c = 0; c < 3; c++) acc[p + c * np] = a3[c] + f3[c];
+            for (int c = 0; c < 3; c++) acc[ai + c * as] = a3[c] + f3[c];
         } else if constexpr (MODE == GATHER_ASSIGN) {
 #pragma unroll
-            for (int c = 0; c < 3; c++) acc[p + c * np] = f3[c];
+            for (int c = 0; c < 3; c++) acc[ai + c * as] = f3[c];
         } else {
 #pragma unroll
             for (int c = 0; c < 3; c++) {
                 T ac = f3[c];
-                if (ns > 1) ac = (acc[p + c * np] + ac) / (T)ns;
+                if (ns > 1) ac = (acc[ai + c * as] + ac) / (T)ns;
                 T m = alpha * m3[c] + beta * ac;
                 T s = p3[c] + dd2 * m;
                 if (has_next) s = s + dd1n * m;
@@ -730,7 +733,7 @@ __global__ __launch_bounds__((gather_threads<T, W>()), (gather_min_waves<T, W>()
             }
         }
         e += NT;
-        if (e < e1) gather_fetch<T, MODE>(ent, e, psi_in, mom, acc, np, ns, en, m3, p3, a3);
+        if (e < e1) gather_fetch<T, MODE>(ent, e, psi_in, mom, acc, np, ns, lacc, en, m3, p3, a3);
     }
 }
 
@@ -798,7 +801,7 @@ struct TileLaunch {
         kern<<<a.g.nt, gather_threads<T, W>(), shm, st>>>((const Entry<T> *)a.ent, a.start, a.cur, a.g, (T)sh[0], (T)sh[1], (T)sh[2],
                                         (T)a.L, (const T *)phi, (T)c[0], (T)c[1], (T)c[2], (const T *)a.psi,
                                         (T *)psi_out, (T *)a.mom, (T *)a.acc, (size_t)a.n * a.n * a.n, (T)k.alpha,
-                                        (T)k.beta, (T)k.dd2, (T)k.dd1_next, k.has_next, k.nshifts);
+                                        (T)k.beta, (T)k.dd2, (T)k.dd1_next, k.has_next, k.nshifts, a.acc_by_entry);
     }
     template <int W, int O>
     static void gat_wo(const TileArgs &a, const double *sh, const void *phi, const double *c, int mode,
