@@ -46,10 +46,26 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--small", action="store_true", help="128^3 particles on a 256^3 mesh (quick check)")
     ap.add_argument("--workload", choices=["pm", "gradient", "learned"], default="pm")
+    ap.add_argument("--copy-rate", action="store_true", help="only time a 1 GiB read + 1 GiB write copy kernel")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy kernel three times (the reference point that scales "
                          "the FETCH_SIZE / WRITE_SIZE counters of a profiling run to bytes)")
     args = ap.parse_args()
+    if args.copy_rate:
+        import torch
+
+        a = torch.ones(2**28, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            torch.neg(a, out=b)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(20):
+            torch.neg(a, out=b)
+        e1.record()
+        e1.synchronize()
+        print(json.dumps({"copy_GBs": round(20 * 2 * 2**30 / (e0.elapsed_time(e1) * 1e-3) / 1e9, 1)}), flush=True)
+        return 0
     if args.copy_calibration:
         import torch
 
@@ -129,6 +145,18 @@ def main():
         if tk:
             roof["traffic"] = int(tk["traffic_bytes"])
             roof["traffic_source"] = "profiles/pm_step_c3_counters.md (" + tk["rocprof_kernel"] + ")"
+
+    # a plain streaming kernel of known size (1 GiB read + 1 GiB written, torch.neg) on the same device,
+    # in a child process (torch brings its own HIP runtime): the bandwidth a kernel of this chip reaches
+    # without any computation, beside the 8 TB/s peak
+    import subprocess
+
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--copy-rate"], capture_output=True, text=True,
+                       timeout=300)
+    if r.returncode == 0 and r.stdout.strip():
+        roof["copy_GBs"] = json.loads(r.stdout.strip().splitlines()[-1])["copy_GBs"]
+        roof["frac_of_copy"] = round(roof["achieved"] / roof["copy_GBs"], 4)
+        log(f"streaming copy {roof['copy_GBs']} GB/s")
 
     accuracy = None
     if not args.no_accuracy:
