@@ -991,8 +991,12 @@ dj_status vjp_force(dj_ctx *c, const void *psi, const void *abar, void *psibar) 
         c->fft.zm(ZM_POT, true, true, c->B, c->B, c->tw, c->kmult, c->st);
         c->fft.yi(c->B, c->A, c->tw, c->st);
         c->fft.xi(c->A, c->mesh, c->tw, c->st);
-        HIPCHK(c, hipMemsetAsync(c->vb_fbar, 0, 3 * c->M * c->tsize, c->st));
-        c->adj.scatter3(psi, c->q1, c->n, N, cfg.boxsize, c->shifts[s], abar, 1.0 / ns, cfg.worder, c->vb_fbar, c->st);
+        // Fbar = G^T abar / ns: the tiled vector deposit, or device atomics on a single-tile axis
+        if (!c->tiles.deposit3(ta, cfg.worder, c->shifts[s], abar, 1.0 / ns, c->vb_fbar, c->st)) {
+            HIPCHK(c, hipMemsetAsync(c->vb_fbar, 0, 3 * c->M * c->tsize, c->st));
+            c->adj.scatter3(psi, c->q1, c->n, N, cfg.boxsize, c->shifts[s], abar, 1.0 / ns, cfg.worder, c->vb_fbar,
+                            c->st);
+        }
         c->adj.divergence(c->vb_fbar, N, c->fdc, c->vb_pbar, c->st);
         c->fft.xf(c->vb_pbar, false, c->A, c->tw, 1.0, c->st);
         c->fft.yf(c->A, c->B, c->tw, c->st);
@@ -1032,7 +1036,7 @@ dj_status dj_vjp_acceleration(dj_ctx *c, const void *psi, const void *acc_bar, v
 }
 
 dj_status dj_vjp_deposit(dj_ctx *c, const void *psi, const void *mesh_bar, void *psi_bar_out) {
-    if (!c || !mesh_bar || !psi_bar_out) return fail(c, DJ_ERR_INVALID_ARGUMENT, "null argument");
+    if (!c || !mesh_bar) return fail(c, DJ_ERR_INVALID_ARGUMENT, "null argument");
     HIPCHK(c, hipSetDevice(c->dev));
     dj_status r = adj_buffers(c);
     if (r) return r;
@@ -1049,7 +1053,11 @@ dj_status dj_vjp_deposit(dj_ctx *c, const void *psi, const void *mesh_bar, void 
                        c->cfg.worder, c->vb_psibar, c->st);
     r = check_launch(c, "deposit adjoint");
     if (r) return r;
-    HIPCHK(c, hipMemcpyAsync(psi_bar_out, c->vb_psibar, pb, hipMemcpyDeviceToHost, c->st));
+    if (psi_bar_out) {
+        HIPCHK(c, hipMemcpyAsync(psi_bar_out, c->vb_psibar, pb, hipMemcpyDeviceToHost, c->st));
+    } else {
+        HIPCHK(c, hipMemsetAsync(c->vb_pibar, 0, pb, c->st));  // resident cotangents (Psibar, Pibar = 0)
+    }
     HIPCHK(c, hipStreamSynchronize(c->st));
     return DJ_OK;
 }
@@ -1085,7 +1093,7 @@ dj_status dj_run_checkpointed(dj_ctx *c, int32_t n_steps, const double *dd1, con
 }
 
 dj_status dj_vjp_run(dj_ctx *c, void *psi_bar, void *pi_bar) {
-    if (!c || !psi_bar || !pi_bar) return fail(c, DJ_ERR_INVALID_ARGUMENT, "null argument");
+    if (!c || (!psi_bar) != (!pi_bar)) return fail(c, DJ_ERR_INVALID_ARGUMENT, "psi_bar and pi_bar: both or neither");
     if (!c->ckpt || c->ck_steps < 1) return fail(c, DJ_ERR_INVALID_ARGUMENT, "no checkpointed run to reverse");
     dj_status r = adj_supported(c);
     if (r) return r;
@@ -1093,8 +1101,10 @@ dj_status dj_vjp_run(dj_ctx *c, void *psi_bar, void *pi_bar) {
     r = adj_buffers(c);
     if (r) return r;
     const size_t pb = (size_t)c->n3 * c->tsize, np3 = (size_t)c->n3;
-    HIPCHK(c, hipMemcpyAsync(c->vb_psibar, psi_bar, pb, hipMemcpyHostToDevice, c->st));
-    HIPCHK(c, hipMemcpyAsync(c->vb_pibar, pi_bar, pb, hipMemcpyHostToDevice, c->st));
+    if (psi_bar) {
+        HIPCHK(c, hipMemcpyAsync(c->vb_psibar, psi_bar, pb, hipMemcpyHostToDevice, c->st));
+        HIPCHK(c, hipMemcpyAsync(c->vb_pibar, pi_bar, pb, hipMemcpyHostToDevice, c->st));
+    }
     for (int s = c->ck_steps - 1; s >= 0; s--) {
         const char *ck = (const char *)c->ckpt + 2 * pb * (size_t)s;
         // the drifted positions of the step's force, recomputed from its checkpoint
@@ -1107,8 +1117,10 @@ dj_status dj_vjp_run(dj_ctx *c, void *psi_bar, void *pi_bar) {
     }
     r = check_launch(c, "run adjoint");
     if (r) return r;
-    HIPCHK(c, hipMemcpyAsync(psi_bar, c->vb_psibar, pb, hipMemcpyDeviceToHost, c->st));
-    HIPCHK(c, hipMemcpyAsync(pi_bar, c->vb_pibar, pb, hipMemcpyDeviceToHost, c->st));
+    if (psi_bar) {
+        HIPCHK(c, hipMemcpyAsync(psi_bar, c->vb_psibar, pb, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(c, hipMemcpyAsync(pi_bar, c->vb_pibar, pb, hipMemcpyDeviceToHost, c->st));
+    }
     HIPCHK(c, hipStreamSynchronize(c->st));
     return DJ_OK;
 }
@@ -1116,7 +1128,8 @@ dj_status dj_vjp_run(dj_ctx *c, void *psi_bar, void *pi_bar) {
 dj_status dj_vjp_ics_lpt(dj_ctx *c, uint32_t seed, const double *white, const double *k_tab, const double *pk_tab,
                          int32_t ntab, double D, int32_t order, const void *psi_bar, const void *pi_bar,
                          double *white_bar_out) {
-    if (!c || !k_tab || !pk_tab || ntab < 2 || (order != 1 && order != 2) || !psi_bar || !pi_bar || !white_bar_out)
+    if (!c || !k_tab || !pk_tab || ntab < 2 || (order != 1 && order != 2) || (!psi_bar) != (!pi_bar) ||
+        !white_bar_out)
         return fail(c, DJ_ERR_INVALID_ARGUMENT, "invalid IC adjoint arguments");
     const int n = c->n;
     FftOps probe{};
@@ -1126,8 +1139,10 @@ dj_status dj_vjp_ics_lpt(dj_ctx *c, uint32_t seed, const double *white, const do
     dj_status r = adj_buffers(c);
     if (r) return r;
     const size_t pb = (size_t)c->n3 * c->tsize;
-    HIPCHK(c, hipMemcpyAsync(c->vb_psibar, psi_bar, pb, hipMemcpyHostToDevice, c->st));
-    HIPCHK(c, hipMemcpyAsync(c->vb_pibar, pi_bar, pb, hipMemcpyHostToDevice, c->st));
+    if (psi_bar) {
+        HIPCHK(c, hipMemcpyAsync(c->vb_psibar, psi_bar, pb, hipMemcpyHostToDevice, c->st));
+        HIPCHK(c, hipMemcpyAsync(c->vb_pibar, pi_bar, pb, hipMemcpyHostToDevice, c->st));
+    }
     HIPCHK(c, hipStreamSynchronize(c->st));
     std::vector<double> field;
     if (!white) {

@@ -450,6 +450,79 @@ __global__ __launch_bounds__(256) void k_tile_deposit(const Entry<T> *__restrict
     }
 }
 
+// Vector deposit (reverse mode): out_c(cell) = sum_p W(X_p + s - cell) vals[p + c np] scale for c = 0, 1, 2,
+// per tile in three padded LDS accumulators of T (the layout of the padded deposit above), stored as the
+// tile's cells: no device atomics, no zero fill.  Every axis of the mesh holds at least two tiles.
+template <typename T, int W>
+__global__ __launch_bounds__(256) void k_tile_deposit3(const Entry<T> *__restrict__ ent, const unsigned *__restrict__ start,
+                                                       const unsigned *__restrict__ gstart, const unsigned *__restrict__ cnt,
+                                                       TileGeom g, T sx, T sy, T sz, T L, const T *__restrict__ vals,
+                                                       size_t np, T vscale, T *__restrict__ out) {
+    extern __shared__ unsigned char smem[];
+    T *acc = (T *)smem;
+    constexpr int P = W - 1;
+    const int t = blockIdx.x;
+    const int SXD = g.TX + 2 * P, ROWS = g.TY * g.TZ, CELLS = SXD * (ROWS + 1);
+    const int SZ = SXD * g.TY, DISCARD = SXD * ROWS;
+    for (int c = threadIdx.x; c < 3 * CELLS; c += blockDim.x) acc[c] = 0;
+    const int tx = t % g.ntx, ty = (t / g.ntx) % g.nty, tz = t / (g.ntx * g.nty);
+    const int o3[3] = {tx << g.lT[0], ty << g.lT[1], tz << g.lT[2]};
+    __syncthreads();
+    const unsigned n_own = cnt[t], n_all = n_own + cnt[g.nt + t], s_own = start[t], s_gh = gstart[t] - n_own;
+    for (unsigned i = threadIdx.x; i < n_all; i += blockDim.x) {
+        const Entry<T> en = ent[i < n_own ? s_own + i : s_gh + i];
+        const size_t p = (size_t)en.idx;
+        const T v0 = vals[p] * vscale, v1 = vals[p + np] * vscale, v2 = vals[p + 2 * np] * vscale;
+        const T xs[3] = {en.x + sx, en.y + sy, en.z + sz};
+        T w[3][W];
+        int l0[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            const T gg = tjmod(xs[a], L) / L * (T)g.N;
+            const int b = twbase<W>(gg);
+#pragma unroll
+            for (int o = 0; o < W; o++) w[a][o] = twk<W>(fabs(gg - (T)(b + twlo<W>() + o)));
+            int l = b + twlo<W>() - o3[a];
+            l0[a] = l >= (g.N >> 1) ? l - g.N : (l < -(g.N >> 1) ? l + g.N : l);
+        }
+        const bool okx = l0[0] >= -P && l0[0] < g.TX;
+        const int xo = okx ? l0[0] + P : 0;
+        constexpr int BIG = 1 << 28;
+        int ly[W], lz[W];
+#pragma unroll
+        for (int o = 0; o < W; o++) {
+            const int y = l0[1] + o, z = l0[2] + o;
+            ly[o] = (unsigned)y < (unsigned)g.TY ? __mul24(y, SXD) : BIG;
+            lz[o] = (okx && (unsigned)z < (unsigned)g.TZ) ? __mul24(z, SZ) : BIG;
+        }
+#pragma unroll
+        for (int iz = 0; iz < W; iz++)
+#pragma unroll
+            for (int iy = 0; iy < W; iy++) {
+                T *r = acc + (xo + min(ly[iy] + lz[iz], DISCARD));
+                const T wyz = w[1][iy] * w[2][iz];
+#pragma unroll
+                for (int ix = 0; ix < W; ix++) {
+                    const T wv = w[0][ix] * wyz;
+                    unsafeAtomicAdd(r + ix, wv * v0);
+                    unsafeAtomicAdd(r + CELLS + ix, wv * v1);
+                    unsafeAtomicAdd(r + 2 * CELLS + ix, wv * v2);
+                }
+            }
+    }
+    __syncthreads();
+    const unsigned N = (unsigned)g.N, lx = threadIdx.x & (g.TX - 1);
+    const size_t M = (size_t)N * N * N;
+    for (int r = threadIdx.x >> g.lT[0]; r < ROWS; r += blockDim.x >> g.lT[0]) {
+        const unsigned ly = r & (g.TY - 1), lz = r >> g.lT[1];
+        const unsigned gi = (o3[0] + lx) + N * ((o3[1] + ly) + N * (o3[2] + lz));
+        const int li = (int)lx + P + SXD * r;
+        out[gi] = acc[li];
+        out[gi + M] = acc[li + CELLS];
+        out[gi + 2 * M] = acc[li + 2 * CELLS];
+    }
+}
+
 // ------------------------------------------------------------------------------------ gather
 // -(c1 d1 + c2 d2 + c3 d3) of a line of potential values v[0 .. W+2H) at stencil point o (index o + H):
 // the reference's finite-difference gradient at one cell (global-memory path)
@@ -783,6 +856,26 @@ struct TileLaunch {
         if (fixed) dep_w<W, ACC_FIXED>(a, sh, mesh, fscale, st);
         else dep_w<W, FLOAT_ACC>(a, sh, mesh, fscale, st);
     }
+    template <int W>
+    static void dep3_w(const TileArgs &a, const double *sh, const void *vals, double scale, void *out, hipStream_t st) {
+        const TileGeom &g = a.g;
+        const size_t shm = 3 * (size_t)(g.TX + 2 * (W - 1)) * (g.TY * g.TZ + 1) * sizeof(T);
+        (void)hipFuncSetAttribute((const void *)k_tile_deposit3<T, W>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)shm);
+        k_tile_deposit3<T, W><<<g.nt, 256, shm, st>>>((const Entry<T> *)a.ent, a.start, a.gstart, a.cur, g, (T)sh[0],
+                                                      (T)sh[1], (T)sh[2], (T)a.L, (const T *)vals,
+                                                      (size_t)a.n * a.n * a.n, (T)scale, (T *)out);
+    }
+    static bool deposit3(const TileArgs &a, int W, const double *sh, const void *vals, double scale, void *out,
+                         hipStream_t st) {
+        const TileGeom &g = a.g;
+        if (g.ntx < 2 || g.nty < 2 || g.ntz < 2) return false;
+        if ((size_t)3 * (g.TX + 2 * (W - 1)) * (g.TY * g.TZ + 1) * sizeof(T) > 160 * 1024) return false;
+        if (W == 2) dep3_w<2>(a, sh, vals, scale, out, st);
+        else if (W == 3) dep3_w<3>(a, sh, vals, scale, out, st);
+        else dep3_w<4>(a, sh, vals, scale, out, st);
+        return true;
+    }
     static void deposit(const TileArgs &a, int W, bool fixed, const double *sh, void *mesh, double fscale,
                         hipStream_t st) {
         if (W == 2) dep_sel<2>(a, fixed, sh, mesh, fscale, st);
@@ -830,6 +923,7 @@ template <typename T> void tile_ops(TileOps &ops) {
     ops.scan = TileLaunch<T>::scan;
     ops.fill = TileLaunch<T>::fill;
     ops.deposit = TileLaunch<T>::deposit;
+    ops.deposit3 = TileLaunch<T>::deposit3;
     ops.gather = TileLaunch<T>::gather;
     ops.entry_bytes = sizeof(Entry<T>);
 }

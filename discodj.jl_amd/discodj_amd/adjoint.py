@@ -42,12 +42,16 @@ def density_loss_gradient(cosmo, white, target, *, boxsize, res_pm, a_ini=0.05, 
         delta = S.deposit().astype(np.float64) / rho - 1.0
         r = delta - np.asarray(target, np.float64)
         loss = float(np.sum(r * r))
-        psi_bar = S.vjp_deposit(None, (2.0 / rho) * r)
-        pi_bar = np.zeros_like(psi_bar)
-        psi_bar, pi_bar = S.vjp_run(psi_bar, pi_bar)
-        if fp != 1.0:
+        if fp == 1.0:
+            # cotangents stay on the device from the deposit through the steps to the ICs
+            S.vjp_deposit(None, (2.0 / rho) * r, resident=True)
+            S.vjp_run()
+            white_bar = S.vjp_ics_lpt(k_tab, pk_tab, D, order=order, white=white)
+        else:
+            psi_bar = S.vjp_deposit(None, (2.0 / rho) * r)
+            psi_bar, pi_bar = S.vjp_run(psi_bar, np.zeros_like(psi_bar))
             pi_bar = (pi_bar.astype(np.float64) * fp).astype(S.dtype)
-        white_bar = S.vjp_ics_lpt(k_tab, pk_tab, D, psi_bar, pi_bar, order=order, white=white)
+            white_bar = S.vjp_ics_lpt(k_tab, pk_tab, D, psi_bar, pi_bar, order=order, white=white)
     finally:
         if own:
             S.close()

@@ -203,14 +203,15 @@ class PMSolverHIP:
                                               out.ctypes.data))
         return out
 
-    def vjp_deposit(self, psi, mesh_bar):
-        """(d mesh / d psi)^T mesh_bar of the unshifted deposit at psi (None: the solver's state)."""
-        out = np.empty(self.shape, self.dtype)
+    def vjp_deposit(self, psi, mesh_bar, resident=False):
+        """(d mesh / d psi)^T mesh_bar of the unshifted deposit at psi (None: the solver's state); resident:
+        keep it on the device as the cotangents that vjp_run / vjp_ics_lpt take with None."""
+        out = None if resident else np.empty(self.shape, self.dtype)
         src = None if psi is None else self._arr(psi)
         N = self.cfg.res_pm
         mb = np.ascontiguousarray(mesh_bar, self.dtype).reshape(N, N, N)
         self._chk(self._L.dj_vjp_deposit(self._ctx, None if src is None else src.ctypes.data, mb.ctypes.data,
-                                         out.ctypes.data))
+                                         None if out is None else out.ctypes.data))
         return out
 
     def run_checkpointed(self, dd1, alpha, beta, dd2):
@@ -218,22 +219,28 @@ class PMSolverHIP:
         arrs = [np.ascontiguousarray(a, np.float64) for a in (dd1, alpha, beta, dd2)]
         self._chk(self._L.dj_run_checkpointed(self._ctx, len(arrs[0]), *(a.ctypes.data for a in arrs)))
 
-    def vjp_run(self, psi_bar, pi_bar):
-        """Cotangents of the state before the last run_checkpointed, given those of the state after it."""
+    def vjp_run(self, psi_bar=None, pi_bar=None):
+        """Cotangents of the state before the last run_checkpointed, given those of the state after it
+        (both None: the resident cotangents, which stay on the device; returns None then)."""
+        if psi_bar is None and pi_bar is None:
+            self._chk(self._L.dj_vjp_run(self._ctx, None, None))
+            return None
         pb, mb = self._arr(psi_bar).copy(), self._arr(pi_bar).copy()
         self._chk(self._L.dj_vjp_run(self._ctx, pb.ctypes.data, mb.ctypes.data))
         return pb, mb
 
-    def vjp_ics_lpt(self, k_tab, pk_tab, D, psi_bar, pi_bar, order=2, seed=42, white=None):
-        """Cotangent (n,n,n) FP64 of the real white noise of ics_lpt, given those of the ICs (Psi, Pi)."""
+    def vjp_ics_lpt(self, k_tab, pk_tab, D, psi_bar=None, pi_bar=None, order=2, seed=42, white=None):
+        """Cotangent (n,n,n) FP64 of the real white noise of ics_lpt, given those of the ICs (Psi, Pi)
+        (both None: the resident cotangents)."""
         kt = np.ascontiguousarray(k_tab, np.float64)
         pt = np.ascontiguousarray(pk_tab, np.float64)
         w = None if white is None else np.ascontiguousarray(white, np.float64)
-        pb, mb = self._arr(psi_bar), self._arr(pi_bar)
+        res = psi_bar is None and pi_bar is None
+        pb, mb = (None, None) if res else (self._arr(psi_bar), self._arr(pi_bar))
         out = np.empty((self.n,) * 3, np.float64)
         self._chk(self._L.dj_vjp_ics_lpt(self._ctx, seed, None if w is None else w.ctypes.data, kt.ctypes.data,
-                                         pt.ctypes.data, len(kt), float(D), order, pb.ctypes.data, mb.ctypes.data,
-                                         out.ctypes.data))
+                                         pt.ctypes.data, len(kt), float(D), order, None if res else pb.ctypes.data,
+                                         None if res else mb.ctypes.data, out.ctypes.data))
         return out
 
     # measurement ------------------------------------------------------------------------------

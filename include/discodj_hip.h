@@ -146,17 +146,20 @@ dj_status dj_ranlxd1_uniforms(uint32_t seed, int32_t n, double *out);
 
 /* psi_bar_out = (d acc / d psi)^T acc_bar of dj_acceleration at psi (NULL: the context's Psi) */
 dj_status dj_vjp_acceleration(dj_ctx *ctx, const void *psi, const void *acc_bar, void *psi_bar_out);
-/* psi_bar_out = (d mesh / d psi)^T mesh_bar of dj_deposit (no shift) at psi (NULL: the context's Psi) */
+/* psi_bar_out = (d mesh / d psi)^T mesh_bar of dj_deposit (no shift) at psi (NULL: the context's Psi).
+ * psi_bar_out NULL keeps the result on the device as the context's resident cotangents (Psibar, Pibar = 0),
+ * which dj_vjp_run and dj_vjp_ics_lpt take when their cotangent pointers are NULL. */
 dj_status dj_vjp_deposit(dj_ctx *ctx, const void *psi, const void *mesh_bar, void *psi_bar_out);
 /* dj_run's n_steps DKD steps from the current state, keeping (Psi, Pi) before each step on the device
  * (2 x 3 n^3 values per step) for dj_vjp_run; the final state is the context's state as after dj_run */
 dj_status dj_run_checkpointed(dj_ctx *ctx, int32_t n_steps, const double *dd1, const double *alpha,
                               const double *beta, const double *dd2);
 /* reverse of the last dj_run_checkpointed: (psi_bar, pi_bar), the cotangents of the final (Psi, Pi), are
- * replaced by those of the initial state; each step's force is recomputed from its checkpoint */
+ * replaced by those of the initial state; each step's force is recomputed from its checkpoint.  Both NULL:
+ * the context's resident cotangents, left resident */
 dj_status dj_vjp_run(dj_ctx *ctx, void *psi_bar, void *pi_bar);
 /* reverse of dj_ics_lpt (same seed / white, tables, D, order): white_bar_out (n^3 FP64) = the cotangent of
- * the real white-noise field given psi_bar, pi_bar of the ICs */
+ * the real white-noise field given psi_bar, pi_bar of the ICs (both NULL: the resident cotangents) */
 dj_status dj_vjp_ics_lpt(dj_ctx *ctx, uint32_t seed, const double *white, const double *k_tab, const double *pk_tab,
                          int32_t ntab, double D, int32_t order, const void *psi_bar, const void *pi_bar,
                          double *white_bar_out);
