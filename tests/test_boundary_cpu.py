@@ -175,3 +175,19 @@ def test_snapshot_roundtrip_and_layout(tmp_path):
     ref = np.asfortranarray(s["psi"].transpose(3, 2, 1, 0))
     assert ref[2, 5, 7, 1] == s["psi"][1, 7, 5, 2]
     assert np.array_equal(ref.ravel(order="F"), s["psi"].ravel(order="C"))
+
+
+def test_missing_library_fails_loudly():
+    """No CPU fallback: without libdiscodj_hip.so every entry point raises with build instructions."""
+    import subprocess
+    import sys
+    code = ("import sys; sys.path.insert(0, %r)\n"
+            "import discodj_amd._lib as L\n"
+            "L.LIB_PATH = '/nonexistent/libdiscodj_hip.so'; L._lib = None\n"
+            "import discodj_amd as D\n"
+            "try:\n"
+            "    D.PMSolverHIP(D.PMConfig(res_pm=32, n_part=16, boxsize=100.0))\n"
+            "except RuntimeError as e:\n"
+            "    print('raised:', e)\n") % os.path.dirname(os.path.dirname(os.path.abspath(D.__file__)))
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+    assert "raised:" in r.stdout and "build it" in r.stdout, r.stdout + r.stderr
