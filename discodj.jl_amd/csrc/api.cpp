@@ -92,6 +92,7 @@ struct dj_ctx {
     std::vector<double> nn_sched;
     void *nn_p = nullptr, *nn_g = nullptr, *nn_m = nullptr, *nn_v = nullptr, *nn_mask = nullptr;
     void *nn_wf = nullptr, *nn_wb = nullptr;  // (layers, 27, 64) x 8 bytes, forward / transposed
+    void *nn_pf = nullptr, *nn_pb = nullptr;  // (layers, 4, 64) x 8 bytes, packed for two-channel inputs
     void *nn_t[10] = {};                      // z1 z2 z3 h1 h2 h3 and four gradient tensors (N^3 x 16 BF16)
     void *nn_dphi = nullptr, *nn_dens = nullptr, *nn_tgt = nullptr, *nn_sum = nullptr;
     void *acc_list = nullptr;  // (entries, 3): the acceleration between interlacing passes, in tile-list order
@@ -315,7 +316,8 @@ void free_all(dj_ctx *c) {
                     c->tcnt, c->tstart, c->tgstart, c->tcur, c->tflag, c->tpart, c->tent, c->d_shifts,
                     c->rs_tw, c->rs_psi, c->rs_hat, c->rs_A, c->rs_B, c->rs_xr, c->rs_phase,
                     c->vb_psi1, c->vb_abar, c->vb_psibar, c->vb_pibar, c->vb_fbar, c->vb_pbar, c->ckpt,
-                    c->nn_p, c->nn_g, c->nn_m, c->nn_v, c->nn_mask, c->nn_wf, c->nn_wb, c->nn_t[0], c->nn_t[1],
+                    c->nn_p, c->nn_g, c->nn_m, c->nn_v, c->nn_mask, c->nn_wf, c->nn_wb, c->nn_pf, c->nn_pb,
+                    c->nn_t[0], c->nn_t[1],
                     c->nn_t[2], c->nn_t[3], c->nn_t[4], c->nn_t[5], c->nn_t[6], c->nn_t[7], c->nn_t[8], c->nn_t[9],
                     c->nn_dphi, c->nn_dens, c->nn_tgt, c->nn_sum, c->acc_list};
     // teardown: failures here have no caller to report to
@@ -1161,6 +1163,7 @@ enum { NN_L = NN_PARAMS_PER_LAYER, NN_W = 27 * 16 * 16 };
 
 static float *nn_layer(void *p, int l) { return (float *)p + (size_t)l * NN_L; }
 static void *nn_op(void *w, int l) { return (char *)w + (size_t)l * 27 * 64 * 8; }
+static void *nn_pop(void *w, int l) { return (char *)w + (size_t)l * 4 * 64 * 8; }
 
 dj_status nn_supported(dj_ctx *c) {
     if (c->cfg.precision != 32 || c->cfg.deposit != DJ_DEPOSIT_FLOAT)
@@ -1178,6 +1181,8 @@ dj_status nn_alloc(dj_ctx *c) {
     for (void **b : pv) HIPCHK(c, hipMalloc(b, pb));
     HIPCHK(c, hipMalloc(&c->nn_wf, (size_t)NN_LAYERS * 27 * 64 * 8));
     HIPCHK(c, hipMalloc(&c->nn_wb, (size_t)NN_LAYERS * 27 * 64 * 8));
+    HIPCHK(c, hipMalloc(&c->nn_pf, (size_t)NN_LAYERS * 4 * 64 * 8));
+    HIPCHK(c, hipMalloc(&c->nn_pb, (size_t)NN_LAYERS * 4 * 64 * 8));
     for (void *&t : c->nn_t) HIPCHK(c, hipMalloc(&t, tb));
     HIPCHK(c, hipMalloc(&c->nn_dphi, c->M * 4));
     HIPCHK(c, hipMalloc(&c->nn_dens, c->M * 4));
@@ -1200,7 +1205,9 @@ dj_status nn_alloc(dj_ctx *c) {
 }
 
 void nn_swizzle_all(dj_ctx *c) {
-    for (int l = 0; l < NN_LAYERS; l++) c->nn.swizzle(nn_layer(c->nn_p, l), nn_op(c->nn_wf, l), nn_op(c->nn_wb, l), c->st);
+    for (int l = 0; l < NN_LAYERS; l++)
+        c->nn.swizzle(nn_layer(c->nn_p, l), nn_op(c->nn_wf, l), nn_op(c->nn_wb, l), nn_pop(c->nn_pf, l),
+                      nn_pop(c->nn_pb, l), c->st);
 }
 
 // the net on a density mesh (float, unnormalised deposit) into nn_dphi; pre-activations kept when training
@@ -1209,7 +1216,7 @@ void nn_forward_dev(dj_ctx *c, const void *dens, double a) {
     const double inv_rho = (double)c->M / (double)c->np;
     void **t = c->nn_t;
     const bool keep = c->nn_train;
-    c->nn.conv(1, 0, N, nullptr, dens, inv_rho, a, nn_op(c->nn_wf, 0), nn_layer(c->nn_p, 0) + NN_W, nullptr, nullptr,
+    c->nn.conv(1, 0, N, nullptr, dens, inv_rho, a, nn_pop(c->nn_pf, 0), nn_layer(c->nn_p, 0) + NN_W, nullptr, nullptr,
                keep ? t[0] : nullptr, t[3], nullptr, 0.0, c->st);
     c->nn.conv(0, 0, N, t[3], nullptr, 0.0, 0.0, nn_op(c->nn_wf, 1), nn_layer(c->nn_p, 1) + NN_W, t[3], nullptr,
                keep ? t[1] : nullptr, t[4], nullptr, 0.0, c->st);
@@ -1228,7 +1235,7 @@ dj_status nn_backward_dev(dj_ctx *c, const void *gphi, const void *dens, double 
     float *g3 = nn_layer(c->nn_g, 3), *g2 = nn_layer(c->nn_g, 2), *g1 = nn_layer(c->nn_g, 1), *g0 = nn_layer(c->nn_g, 0);
     // layer 4: dphi = os (W4 * h3 + b4)
     c->nn.wgrad(0, 1, N, t[5], nullptr, 0.0, 0.0, nullptr, gphi, os, g3, g3 + NN_W, c->st);
-    c->nn.conv(2, 2, N, nullptr, gphi, os, 0.0, nn_op(c->nn_wb, 3), nullptr, nullptr, t[2], t[7], t[6], nullptr, 0.0,
+    c->nn.conv(2, 2, N, nullptr, gphi, os, 0.0, nn_pop(c->nn_pb, 3), nullptr, nullptr, t[2], t[7], t[6], nullptr, 0.0,
                c->st);  // gh3 (t6), gz3 (t7)
     // layer 3: h3 = h2 + relu(z3)
     c->nn.wgrad(0, 0, N, t[4], nullptr, 0.0, 0.0, t[7], nullptr, 0.0, g2, g2 + NN_W, c->st);

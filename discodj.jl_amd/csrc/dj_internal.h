@@ -150,7 +150,8 @@ struct NnOps {
     // dW[27][16][16] += sum_v gz(v) x(v + off)^T, db += sum_v gz(v)  (ga: gz is a 1-channel field gf * gscale)
     void (*wgrad)(int in, int ga, int N, const void *hin, const void *fin, double inv_rho, double cval, const void *gz,
                   const void *gf, double gscale, void *dW, void *db, hipStream_t st);
-    void (*swizzle)(const void *W, void *fwd, void *bwd, hipStream_t st);
+    // lane-ordered BF16 operands of a layer: per tap (fwd, bwd) and packed for two-channel inputs
+    void (*swizzle)(const void *W, void *fwd, void *bwd, void *pfwd, void *pbwd, hipStream_t st);
     void (*adam)(void *p, const void *g, void *m, void *v, const void *mask, int n, double lr, double b1, double b2,
                  double eps, double c1, double c2, hipStream_t st);
     void (*axpy)(void *out, const void *in, size_t n, double s, hipStream_t st);

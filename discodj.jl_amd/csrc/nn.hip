@@ -83,6 +83,7 @@ __device__ __forceinline__ void nn_plane_stage(unsigned short *slot, const unsig
             }
         }
     } else {
+        // a field input has two channels, (input, cval): one 4-byte pair per voxel
         const unsigned c1 = (unsigned)f2bf(cval) << 16;
 #pragma unroll
         for (int k = 0; k < 3; k++) {
@@ -91,8 +92,7 @@ __device__ __forceinline__ void nn_plane_stage(unsigned short *slot, const unsig
                 const int xi = vox % NN_PX, yi = vox / NN_PX;
                 const float fv = fin[zb + (size_t)nn_wrap(x0 - 1 + xi, N) + (size_t)N * nn_wrap(y0 - 1 + yi, N)];
                 const float x = IN == NN_IN_FIELD ? nn_input(fv, inv_rho) : fv * inv_rho;
-                *(uint4 *)(slot + vox * 16) = make_uint4((unsigned)f2bf(x) | c1, 0u, 0u, 0u);
-                *(uint4 *)(slot + vox * 16 + 8) = make_uint4(0u, 0u, 0u, 0u);
+                ((unsigned *)slot)[vox] = (unsigned)f2bf(x) | c1;
             }
         }
     }
@@ -113,7 +113,7 @@ __global__ __launch_bounds__(256, 2) void k_nn_conv(int N, const unsigned short 
     const int nbx = N / NN_TX, nby = N / NN_TY;
     const int x0 = (blockIdx.x % nbx) * NN_TX, y0 = ((blockIdx.x / nbx) % nby) * NN_TY,
               z0 = (blockIdx.x / (nbx * nby)) * NN_TZ;
-    for (int i = threadIdx.x; i < 27 * 64; i += 256) wl[i] = wsw[i];
+    for (int i = threadIdx.x; i < (IN == NN_IN_H ? 27 : 4) * 64; i += 256) wl[i] = wsw[i];
     auto slot = [&](int z) { return ring + (size_t)(((z % 3) + 3) % 3) * NN_PV * 16; };
     nn_plane_stage<IN>(slot(z0 - 1), hin, fin, N, x0, y0, z0 - 1, inv_rho, cval);
     nn_plane_stage<IN>(slot(z0), hin, fin, N, x0, y0, z0, inv_rho, cval);
@@ -123,6 +123,7 @@ __global__ __launch_bounds__(256, 2) void k_nn_conv(int N, const unsigned short 
         f4v acc[8];
 #pragma unroll
         for (int sg = 0; sg < 8; sg++) acc[sg] = {0, 0, 0, 0};
+        if constexpr (IN == NN_IN_H) {
 #pragma unroll 1
         for (int dz = -1; dz <= 1; dz++) {
             const unsigned short *sh = slot(z + dz);
@@ -139,6 +140,25 @@ __global__ __launch_bounds__(256, 2) void k_nn_conv(int N, const unsigned short 
                     }
                 }
         }
+        } else {
+            // two input channels: one MFMA takes 8 taps x 2 channels (K = 16), 4 MFMAs for the 27 taps
+            // (the 5 extra taps have zero weights); lane k-group 4 (lane / 16) + j reads taps
+            // 8 q + 2 (lane / 16) + j / 2, channel j % 2
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const s4v a = wl[q * 64 + lane];
+                const int t1 = min(8 * q + 2 * (lane >> 4), 26), t2 = min(t1 + 1, 26);
+                const unsigned *s1 = (const unsigned *)slot(z + t1 / 9 - 1), *s2 = (const unsigned *)slot(z + t2 / 9 - 1);
+                const int d1 = ((t1 / 3) % 3) * NN_PX + t1 % 3, d2 = ((t2 / 3) % 3) * NN_PX + t2 % 3;
+#pragma unroll
+                for (int sg = 0; sg < 8; sg++) {
+                    const int v0 = (2 * wave + (sg >> 2)) * NN_PX + (sg & 3) * 16 + lx;  // the (-1,-1) corner cell
+                    const unsigned u1 = s1[v0 + d1], u2 = s2[v0 + d2];
+                    const s4v b = {(short)(u1 & 0xffffu), (short)(u1 >> 16), (short)(u2 & 0xffffu), (short)(u2 >> 16)};
+                    acc[sg] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[sg], 0, 0, 0);
+                }
+            }
+        }
         // epilogue: lane holds D[cg + j][lx] of each segment: channels cg..cg+3 of one voxel
 #pragma unroll
         for (int sg = 0; sg < 8; sg++) {
@@ -149,7 +169,14 @@ __global__ __launch_bounds__(256, 2) void k_nn_conv(int N, const unsigned short 
             } else {
                 s4v zo, ho;
                 s4v rr = {0, 0, 0, 0}, m = {0, 0, 0, 0};
-                if (res) rr = *(const s4v *)(res + v * 16 + cg);
+                if constexpr (IN == NN_IN_H) {
+                    if (res == hin)  // the residual is the layer's input: its centre cell is in the LDS ring
+                        rr = *(const s4v *)(slot(z) + ((2 * wave + (sg >> 2) + 1) * NN_PX + (sg & 3) * 16 + lx + 1) * 16 + cg);
+                    else if (res)
+                        rr = *(const s4v *)(res + v * 16 + cg);
+                } else if (res) {
+                    rr = *(const s4v *)(res + v * 16 + cg);
+                }
                 if constexpr (EPI == NN_EPI_BWD) m = *(const s4v *)(zmask + v * 16 + cg);
 #pragma unroll
                 for (int j = 0; j < 4; j++) {
@@ -254,6 +281,24 @@ __global__ __launch_bounds__(256) void k_nn_wgrad(int N, const unsigned short *_
 
 // lane-ordered BF16 A operands of the forward (W[t][co][ci] at lane (co, ci / 4)) and of the
 // transposed convolution (W[26 - t][co][ci] at lane (ci, co / 4)) of one layer
+// packed A operands of a two-channel input (8 taps x 2 channels per K = 16): forward A[co][k] =
+// W[t][co][ch], transposed A[ci][k] = W[26 - t][ch][ci], t = 8 q + 2 (k / 4) + (k % 4) / 2, ch = k % 2,
+// zero for t >= 27
+__global__ void k_nn_swizzle_packed(const float *__restrict__ W, s4v *__restrict__ fwd, s4v *__restrict__ bwd) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 4 * 64) return;
+    const int q = i / 64, l = i % 64, row = l & 15;
+    s4v f, b;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int t = 8 * q + 2 * (l >> 4) + j / 2, ch = j % 2;
+        f[j] = (short)(t < 27 ? f2bf(W[(t * 16 + row) * 16 + ch]) : 0);
+        b[j] = (short)(t < 27 ? f2bf(W[((26 - t) * 16 + ch) * 16 + row]) : 0);
+    }
+    fwd[i] = f;
+    bwd[i] = b;
+}
+
 __global__ void k_nn_swizzle(const float *__restrict__ W, s4v *__restrict__ fwd, s4v *__restrict__ bwd) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= 27 * 64) return;
@@ -333,8 +378,9 @@ struct NnLaunch {
         else NN_W(NN_IN_H, 0);
 #undef NN_W
     }
-    static void swizzle(const void *W, void *fwd, void *bwd, hipStream_t st) {
+    static void swizzle(const void *W, void *fwd, void *bwd, void *pfwd, void *pbwd, hipStream_t st) {
         k_nn_swizzle<<<ngrid(27 * 64, 256), 256, 0, st>>>((const float *)W, (s4v *)fwd, (s4v *)bwd);
+        k_nn_swizzle_packed<<<1, 256, 0, st>>>((const float *)W, (s4v *)pfwd, (s4v *)pbwd);
     }
     static void adam(void *p, const void *g, void *m, void *v, const void *mask, int n, double lr, double b1, double b2,
                      double eps, double c1, double c2, hipStream_t st) {
