@@ -574,8 +574,8 @@ __device__ __forceinline__ void gather_sums(const LD &ld, const T (&w)[3][W], T 
         }
 }
 
-// The same sums with the finite difference and the interpolation along the line folded into one
-// filter: sum_o w[o] fd(v, o + H) = sum_q u[q] v[q] with u[o+H+t] -= w[o] c_t, u[o+H-t] += w[o] c_t.
+// The finite difference and the interpolation along a line folded into one filter:
+// sum_o w[o] fd(v, o + H) = sum_q u[q] v[q] with u[o+H+t] -= w[o] c_t, u[o+H-t] += w[o] c_t.
 // A line then costs Q fused multiply-adds instead of W finite differences; the value equals the
 // per-cell form up to rounding (the LDS path).
 template <typename T, int W, int ORDER>
@@ -593,41 +593,60 @@ __device__ __forceinline__ void fd_filter(const T (&w)[W], T c1, T c2, T c3, T (
         }
 }
 
+// The filtered sums with every staged cell read once.  F_x, F_y and F_z read the union of an x
+// box (Q x W x W cells), a y box (W x Q x W) and a z box (W x W x Q), which share the W^3 core.
+// The loop runs over the Q x slices: each slice's W x W core block feeds F_x (weighted by the
+// x filter's tap), and the W core slices also read their y and z columns for F_y and F_z.  That is
+// Q W^2 + 4 H W^2 reads per particle (189 for TSC at order 6) instead of 3 Q W^2 (243).
 template <typename T, int W, int ORDER, typename LD>
-__device__ __forceinline__ void gather_sums_filtered(const LD &ld, const T (&w)[3][W], T c1, T c2, T c3, T &s1, T &s2,
-                                                     T &s3) {
+__device__ __forceinline__ void gather_sums_union(const LD &ld, const T (&w)[3][W], T c1, T c2, T c3, T &s1, T &s2,
+                                                  T &s3) {
     constexpr int H = ORDER / 2, Q = W + 2 * H;
-    T u[Q];
-    fd_filter<T, W, ORDER>(w[0], c1, c2, c3, u);
+    T u1[Q], u2[Q], u3[Q], wyz[W][W];
+    fd_filter<T, W, ORDER>(w[0], c1, c2, c3, u1);
+    fd_filter<T, W, ORDER>(w[1], c1, c2, c3, u2);
+    fd_filter<T, W, ORDER>(w[2], c1, c2, c3, u3);
 #pragma unroll
     for (int oz = 0; oz < W; oz++)
 #pragma unroll
-        for (int oy = 0; oy < W; oy++) {
-            T l = 0;
+        for (int oy = 0; oy < W; oy++) wyz[oz][oy] = w[1][oy] * w[2][oz];
 #pragma unroll
-            for (int q = 0; q < Q; q++) l = fma(u[q], ld(q, oy + H, oz + H), l);
-            s1 = fma(w[1][oy] * w[2][oz], l, s1);
+    for (int x = 0; x < Q; x++) {
+        T core[W][W];  // [oz][oy]
+#pragma unroll
+        for (int oz = 0; oz < W; oz++)
+#pragma unroll
+            for (int oy = 0; oy < W; oy++) core[oz][oy] = ld(x, oy + H, oz + H);
+        T b = 0;
+#pragma unroll
+        for (int oz = 0; oz < W; oz++)
+#pragma unroll
+            for (int oy = 0; oy < W; oy++) b = fma(wyz[oz][oy], core[oz][oy], b);
+        s1 = fma(u1[x], b, s1);
+        if (x >= H && x < H + W) {
+            const int ox = x - H;
+            // F_y: y columns at (x, ., oz + H); F_z: z columns at (x, oy + H, .)
+            T ty = 0, tz = 0;
+#pragma unroll
+            for (int oz = 0; oz < W; oz++) {
+                T l = 0;
+#pragma unroll
+                for (int q = 0; q < Q; q++)
+                    l = fma(u2[q], (q >= H && q < H + W) ? core[oz][q - H] : ld(x, q, oz + H), l);
+                ty = fma(w[2][oz], l, ty);
+            }
+#pragma unroll
+            for (int oy = 0; oy < W; oy++) {
+                T l = 0;
+#pragma unroll
+                for (int q = 0; q < Q; q++)
+                    l = fma(u3[q], (q >= H && q < H + W) ? core[q - H][oy] : ld(x, oy + H, q), l);
+                tz = fma(w[1][oy], l, tz);
+            }
+            s2 = fma(w[0][ox], ty, s2);
+            s3 = fma(w[0][ox], tz, s3);
         }
-    fd_filter<T, W, ORDER>(w[1], c1, c2, c3, u);
-#pragma unroll
-    for (int oz = 0; oz < W; oz++)
-#pragma unroll
-        for (int ox = 0; ox < W; ox++) {
-            T l = 0;
-#pragma unroll
-            for (int q = 0; q < Q; q++) l = fma(u[q], ld(ox + H, q, oz + H), l);
-            s2 = fma(w[0][ox] * w[2][oz], l, s2);
-        }
-    fd_filter<T, W, ORDER>(w[2], c1, c2, c3, u);
-#pragma unroll
-    for (int oy = 0; oy < W; oy++)
-#pragma unroll
-        for (int ox = 0; ox < W; ox++) {
-            T l = 0;
-#pragma unroll
-            for (int q = 0; q < Q; q++) l = fma(u[q], ld(ox + H, oy + H, q), l);
-            s3 = fma(w[0][ox] * w[1][oy], l, s3);
-        }
+    }
 }
 
 // workgroup size: FP32 tiles hold ~512 particles, one per thread; FP64 PCS needs 256 threads to keep
@@ -674,17 +693,37 @@ struct StageGeom {
     bool ok;
 };
 template <typename T>
-__host__ __device__ __forceinline__ StageGeom stage_geom(const TileGeom &g, int H) {
+__host__ __device__ __forceinline__ StageGeom stage_geom_of(int TX, int TY, int TZ, int RL, int RS, int N, int H) {
     constexpr int CPG = 16 / (int)sizeof(T);
     StageGeom q;
-    q.xl = ((g.RL + H + CPG - 1) / CPG) * CPG;
-    q.s[0] = ((q.xl + g.TX + g.RS + H + CPG - 1) / CPG) * CPG;
-    q.s[1] = g.TY + g.RL + g.RS + 2 * H;
-    q.s[2] = g.TZ + g.RL + g.RS + 2 * H;
+    q.xl = ((RL + H + CPG - 1) / CPG) * CPG;
+    q.s[0] = ((q.xl + TX + RS + H + CPG - 1) / CPG) * CPG;
+    q.s[1] = TY + RL + RS + 2 * H;
+    q.s[2] = TZ + RL + RS + 2 * H;
     q.lo[0] = q.xl;
-    q.lo[1] = q.lo[2] = g.RL + H;
-    q.ok = q.s[0] < g.N && q.s[1] < g.N && q.s[2] < g.N && g.N % CPG == 0;
+    q.lo[1] = q.lo[2] = RL + H;
+    q.ok = q.s[0] < N && q.s[1] < N && q.s[2] < N && N % CPG == 0;
     return q;
+}
+template <typename T>
+__host__ __device__ __forceinline__ StageGeom stage_geom(const TileGeom &g, int H) {
+    return stage_geom_of<T>(g.TX, g.TY, g.TZ, g.RL, g.RS, g.N, H);
+}
+
+// The FP32 tiles of 32 x 16 x 8 cells with the reach of the interlacing shifts 0 and half a cell
+// (antialias -1, 0, 1, 2): the staged region's extents and strides are then compile-time constants,
+// and every LDS read of the gather is one base register plus an immediate offset.  A smaller reach
+// (antialias 0) stages a region that contains the one it needs.
+constexpr int FIXED_TX = 32, FIXED_TY = 16, FIXED_TZ = 8;
+template <int W> constexpr int fixed_rl() { return W == 2 ? 0 : 1; }
+template <int W> constexpr int fixed_rs() { return W == 4 ? 3 : 2; }
+template <int W> constexpr bool fixed_geom_fits(const TileGeom &g) {
+    return g.TX == FIXED_TX && g.TY == FIXED_TY && g.TZ == FIXED_TZ && g.RL <= fixed_rl<W>() && g.RS <= fixed_rs<W>();
+}
+template <typename T, int W, bool FG>
+__host__ __device__ __forceinline__ StageGeom gather_geom(const TileGeom &g, int H) {
+    if constexpr (FG) return stage_geom_of<T>(FIXED_TX, FIXED_TY, FIXED_TZ, fixed_rl<W>(), fixed_rs<W>(), g.N, H);
+    else return stage_geom<T>(g, H);
 }
 
 typedef __attribute__((address_space(3))) void lds_void_t;
@@ -707,6 +746,30 @@ __device__ __forceinline__ void stage_groups(const T *phi, T *ph, const int (&r0
     int ly = row % S[1], lz = row / S[1];
     const int step = NW * 64, dcg = step % GR, drow = step / GR;
     unsigned *dst0 = (unsigned *)ph;
+    if (r0[0] >= 0 && r0[0] + S[0] <= N && r0[1] >= 0 && r0[1] + S[1] <= N && r0[2] >= 0 && r0[2] + S[2] <= N) {
+        // the region does not wrap: the lane's source offset advances by the same increments as its
+        // (group, row, plane), with one carry from group to row and one from row to plane (the row
+        // step drow splits into dly < S1 rows and dlz planes)
+        const int dly = drow % S[1], dlz = drow / S[1];
+        const unsigned NN = (unsigned)N * (unsigned)N;
+        const unsigned inc = (unsigned)(dcg * CPG) + (unsigned)dly * (unsigned)N + (unsigned)dlz * NN;
+        const unsigned carry_x = (unsigned)N - (unsigned)(GR * CPG), carry_y = NN - (unsigned)S[1] * (unsigned)N;
+        unsigned off = (unsigned)(r0[0] + cg * CPG) + (unsigned)N * (unsigned)(r0[1] + ly) +
+                       NN * (unsigned)(r0[2] + lz);  // < N^3 <= 2^30 cells
+        for (int c = wave; c * 64 < total; c += NW) {
+            if (f < total)
+                __builtin_amdgcn_global_load_lds((glb_void_t *)(phi + off), (lds_void_t *)(dst0 + (size_t)c * 256), 16,
+                                                 0, 0);
+            f += step;
+            off += inc;
+            cg += dcg;
+            int dr = dly;
+            if (cg >= GR) { cg -= GR; dr++; off += carry_x; }
+            ly += dr;
+            if (ly >= S[1]) { ly -= S[1]; off += carry_y; }
+        }
+        return;
+    }
     for (int c = wave; c * 64 < total; c += NW) {
         if (f < total) {
             const unsigned gx = (unsigned)wrapn(r0[0] + cg * CPG, N), gy = (unsigned)wrapn(r0[1] + ly, N),
@@ -723,7 +786,7 @@ __device__ __forceinline__ void stage_groups(const T *phi, T *ph, const int (&r0
     }
 }
 
-template <typename T, int W, int ORDER, int MODE>
+template <typename T, int W, int ORDER, int MODE, bool FG>
 __global__ __launch_bounds__((gather_threads<T, W>()), (gather_min_waves<T, W>())) void k_tile_gather(const Entry<T> *__restrict__ ent, const unsigned *__restrict__ start,
                                                       const unsigned *__restrict__ own, TileGeom g, T sx, T sy, T sz,
                                                       T L, const T *__restrict__ phi, T c1, T c2, T c3, const T *psi_in,
@@ -734,9 +797,10 @@ __global__ __launch_bounds__((gather_threads<T, W>()), (gather_min_waves<T, W>()
     T *ph = (T *)smem;
     const int t = xcd_tile(blockIdx.x, g.nt);
     const int tx = t % g.ntx, ty = (t / g.ntx) % g.nty, tz = t / (g.ntx * g.nty);
-    // staged region: tile + stencil reach + FD halo H (stage_geom)
-    const StageGeom q = stage_geom<T>(g, H);
-    const int r0[3] = {(tx << g.lT[0]) - q.lo[0], (ty << g.lT[1]) - q.lo[1], (tz << g.lT[2]) - q.lo[2]};
+    // staged region: tile + stencil reach + FD halo H (gather_geom)
+    const StageGeom q = gather_geom<T, W, FG>(g, H);
+    const int lT[3] = {FG ? 5 : g.lT[0], FG ? 4 : g.lT[1], FG ? 3 : g.lT[2]};
+    const int r0[3] = {(tx << lT[0]) - q.lo[0], (ty << lT[1]) - q.lo[1], (tz << lT[2]) - q.lo[2]};
     const int S[3] = {q.s[0], q.s[1], q.s[2]};
     const int N = g.N, SX = S[0], SXY = S[0] * S[1];
     const bool region_ok = q.ok;
@@ -775,7 +839,7 @@ __global__ __launch_bounds__((gather_threads<T, W>()), (gather_min_waves<T, W>()
         if (staged) {
             const int base = lb[0] + SX * lb[1] + SXY * lb[2];
             auto ld = [&](int x, int y, int z) { return ph[base + x + SX * y + SXY * z]; };
-            gather_sums_filtered<T, W, ORDER>(ld, w, c1, c2, c3, s1, s2, s3);
+            gather_sums_union<T, W, ORDER>(ld, w, c1, c2, c3, s1, s2, s3);
         } else {
             auto ld = [&](int x, int y, int z) {
                 return phi[(size_t)wrapn(c0[0] + x, N) +
@@ -882,19 +946,27 @@ struct TileLaunch {
         else if (W == 3) dep_sel<3>(a, fixed, sh, mesh, fscale, st);
         else dep_sel<4>(a, fixed, sh, mesh, fscale, st);
     }
-    template <int W, int O, int M>
-    static void gat_wom(const TileArgs &a, const double *sh, const void *phi, const double *c, const KickArgs &k,
-                        void *psi_out, hipStream_t st) {
+    template <int W, int O, int M, bool FG>
+    static void gat_womf(const TileArgs &a, const double *sh, const void *phi, const double *c, const KickArgs &k,
+                         void *psi_out, hipStream_t st) {
         constexpr int H = O / 2;
-        const StageGeom q = stage_geom<T>(a.g, H);
+        const StageGeom q = gather_geom<T, W, FG>(a.g, H);
         // a region that is not staged (as large as the mesh) is read from global memory by the kernel
         size_t shm = q.ok ? sizeof(T) * (size_t)q.s[0] * q.s[1] * q.s[2] : sizeof(T);
-        auto kern = k_tile_gather<T, W, O, M>;
+        auto kern = k_tile_gather<T, W, O, M, FG>;
         (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
         kern<<<a.g.nt, gather_threads<T, W>(), shm, st>>>((const Entry<T> *)a.ent, a.start, a.cur, a.g, (T)sh[0], (T)sh[1], (T)sh[2],
                                         (T)a.L, (const T *)phi, (T)c[0], (T)c[1], (T)c[2], (const T *)a.psi,
                                         (T *)psi_out, (T *)a.mom, (T *)a.acc, (size_t)a.n * a.n * a.n, (T)k.alpha,
                                         (T)k.beta, (T)k.dd2, (T)k.dd1_next, k.has_next, k.nshifts, a.acc_by_entry);
+    }
+    template <int W, int O, int M>
+    static void gat_wom(const TileArgs &a, const double *sh, const void *phi, const double *c, const KickArgs &k,
+                        void *psi_out, hipStream_t st) {
+        if constexpr (sizeof(T) == 4) {
+            if (fixed_geom_fits<W>(a.g)) return gat_womf<W, O, M, true>(a, sh, phi, c, k, psi_out, st);
+        }
+        gat_womf<W, O, M, false>(a, sh, phi, c, k, psi_out, st);
     }
     template <int W, int O>
     static void gat_wo(const TileArgs &a, const double *sh, const void *phi, const double *c, int mode,

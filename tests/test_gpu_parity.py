@@ -130,6 +130,20 @@ def test_tile_pipeline_vs_oracle(W, order, prec):
     assert G.rel(acc, ref) < (1e-12 if prec == 64 else 2e-5)
 
 
+@pytest.mark.parametrize("W, order, aa", [(3, 6, 1), (3, 6, 0), (3, 6, 3), (2, 4, 1), (4, 2, 2), (4, 6, -1)])
+def test_tile_pipeline_large_mesh(W, order, aa):
+    """A 128^3 mesh, where most tiles' staged regions do not wrap around the box (the gather's
+    incremental staging addresses), for shift sets whose reach fits the compile-time staging geometry
+    of the FP32 tiles (antialias -1, 0, 1, 2) and one that does not (antialias 3: quarter-cell shifts)."""
+    n, res, L = 64, 128, 200.0
+    psi = _random_psi(n, L, 4.0, 7 + 10 * W + order + aa, np.float32)
+    cfg = D.PMConfig(res_pm=res, n_part=n, boxsize=L, worder=W, grad_order=order, lap_order=order, antialias=aa)
+    acc = D.pm_acceleration(psi, cfg, precision=32)
+    ocfg = O.pm_config(res, n, L, worder=W, grad_order=order, lap_order=order, antialias=aa)
+    ref = O.pm_acceleration(psi.astype(np.float64), ocfg, "solver")
+    assert G.rel(acc, ref) < 2e-5
+
+
 @pytest.mark.parametrize("n, nr, method, W, grad", [
     (32, 2, "fourier", 4, 0), (32, 3, "fourier", 3, 4), (24, 2, "fourier", 2, 6), (32, 2, "linear", 3, 4),
     (24, 3, "linear", 4, 0)])
