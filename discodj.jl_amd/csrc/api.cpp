@@ -406,6 +406,21 @@ dj_status tile_sort(dj_ctx *c, const void *psi, const double *d_shifts, int ns, 
         HIPCHK(c, hipStreamSynchronize(c->st));
         if (hv[0] == 0 && hv[1] <= c->tent_cap) {
             c->tlists = true;
+            // After an exact sort hv[1] is the number of entries E; the next sort reserves 2 E + 128 nt.
+            // Grow the buffer now (keeping this sort's lists), so that the next fill does not overflow it
+            // and repeat the sort.
+            const size_t next = 2 * (size_t)hv[1] + 128 * (size_t)c->tg.nt;
+            if (exact && next > c->tent_cap) {
+                const size_t cap = next + next / 8 + 1024;
+                void *grown = nullptr;
+                HIPCHK(c, hipMalloc(&grown, cap * c->tiles.entry_bytes));
+                HIPCHK(c, hipMemcpyAsync(grown, c->tent, (size_t)hv[1] * c->tiles.entry_bytes,
+                                         hipMemcpyDeviceToDevice, c->st));
+                HIPCHK(c, hipStreamSynchronize(c->st));
+                HIPCHK(c, hipFree(c->tent));
+                c->tent = grown;
+                c->tent_cap = cap;
+            }
             return check_launch(c, "tile binning");
         }
         c->tlists = false;

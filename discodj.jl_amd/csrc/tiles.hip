@@ -202,7 +202,7 @@ __global__ __launch_bounds__(256) void k_tile_bin(const T *__restrict__ psi, con
         return;
     }
     __syncthreads();
-    if (!ok) return;
+    bool over = false;
     for (int m = 0; m < nt; m++) {
         const unsigned t = tiles[m];
         unsigned at, end;
@@ -222,9 +222,12 @@ __global__ __launch_bounds__(256) void k_tile_bin(const T *__restrict__ psi, con
             e.idx = (int)p;
             ent[at] = e;
         } else {
-            atomicOr(flag, 1u);  // capacity exceeded: the host repeats the sort with exact counts
+            over = true;
         }
     }
+    // capacity exceeded: the host repeats the sort with exact counts (one flag atomic per wave; a
+    // buffer that is much too small would otherwise have every lane hit the one flag word)
+    if (__any(over) && __lane_id() == 0) atomicOr(flag, 1u);
 }
 
 // List capacities of every tile from counts src (own, then ghost; src may be the fill cursors of
