@@ -684,6 +684,29 @@ __device__ __forceinline__ void gather_fetch(const Entry<T> *__restrict__ ent, u
 // neighbouring tiles share is fetched into that XCD's L2 once.
 __device__ __forceinline__ int xcd_tile(int b, int nt) { return (nt & 7) ? b : (b >> 3) + (b & 7) * (nt >> 3); }
 
+// Gather order: within each XCD's run of workgroups, tiles go brick by brick, 2 x 4 x 8 tiles
+// (64^3 cells for 32 x 16 x 8 tiles, about as many workgroups as an XCD runs at once), x fastest
+// inside a brick.  The staged regions of a tile reach its neighbours along every axis; in a brick
+// the neighbours along z are in flight together too, and their shared halo planes are fetched into
+// the XCD's L2 once instead of once per tile.  Tile counts that the bricks do not divide keep the
+// row order.
+constexpr int BRICK_X = 2, BRICK_Y = 4, BRICK_Z = 8;
+__device__ __forceinline__ void gather_tile(int b, const TileGeom &g, int &tx, int &ty, int &tz) {
+    const int loc = xcd_tile(b, g.nt);
+    if (g.ntx % BRICK_X || g.nty % BRICK_Y || g.ntz % BRICK_Z) {
+        tx = loc % g.ntx;
+        ty = (loc / g.ntx) % g.nty;
+        tz = loc / (g.ntx * g.nty);
+        return;
+    }
+    constexpr int BV = BRICK_X * BRICK_Y * BRICK_Z;
+    const int w = loc % BV, br = loc / BV;
+    const int nbx = g.ntx / BRICK_X, nby = g.nty / BRICK_Y;
+    tx = (br % nbx) * BRICK_X + w % BRICK_X;
+    ty = ((br / nbx) % nby) * BRICK_Y + (w / BRICK_X) % BRICK_Y;
+    tz = (br / (nbx * nby)) * BRICK_Z + w / (BRICK_X * BRICK_Y);
+}
+
 // Staged region of a tile's gather, shared by the kernel and its launcher: cells
 // [tile + RL + H left, tile + RS + H right] along each axis, the x extent widened to whole 16-byte
 // groups (4 floats, 2 doubles) that start at a multiple of the group width, so that the region is
@@ -798,8 +821,9 @@ __global__ __launch_bounds__((gather_threads<T, W>()), (gather_min_waves<T, W>()
     constexpr int H = ORDER / 2, Q = W + 2 * H, NT = gather_threads<T, W>();
     extern __shared__ unsigned char smem[];
     T *ph = (T *)smem;
-    const int t = xcd_tile(blockIdx.x, g.nt);
-    const int tx = t % g.ntx, ty = (t / g.ntx) % g.nty, tz = t / (g.ntx * g.nty);
+    int tx, ty, tz;
+    gather_tile(blockIdx.x, g, tx, ty, tz);
+    const int t = tx + g.ntx * (ty + g.nty * tz);
     // staged region: tile + stencil reach + FD halo H (gather_geom)
     const StageGeom q = gather_geom<T, W, FG>(g, H);
     const int lT[3] = {FG ? 5 : g.lT[0], FG ? 4 : g.lT[1], FG ? 3 : g.lT[2]};
