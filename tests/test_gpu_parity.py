@@ -40,6 +40,20 @@ def test_fft3d_vs_numpy(N, prec):
     assert np.max(np.abs(back - x)) < tol * np.max(np.abs(x))
 
 
+def test_fft3d_1024_fp32_vs_fp64():
+    """The 1024^3 mesh (its FP32 x passes run 16 lines per workgroup) against the FP64 device transform
+    of the same field, and the FP32 round trip (numpy's FFT of 1024^3 would take minutes)."""
+    N = 1024
+    x = np.random.default_rng(5).normal(size=(N, N, N)).astype(np.float32)
+    ref = D.fft3d(x.astype(np.float64), -1)
+    got = D.fft3d(x, -1)
+    scale = np.max(np.abs(ref))
+    assert np.max(np.abs(got - ref)) < 5e-6 * scale
+    del ref
+    back = D.fft3d(got, +1)
+    assert np.max(np.abs(back / N**3 - x)) < 5e-6 * np.max(np.abs(x))
+
+
 # -------------------------------------------------------------------------------- mass assignment
 def _random_psi(n, L, scale, seed, dtype):
     rng = np.random.default_rng(seed)
