@@ -282,7 +282,9 @@ def bench_learned(args):
             log(f"{mode}: {out[mode] * 1e3:.2f} ms/step")
         nn = stats.get("nn_correction", {})
     nn_ms = nn["ms_total"] / nn["calls"] if nn.get("calls") else None
-    flops = 4 * 27 * 2 * 16 * 16 * 16 * (n**3 / 16)  # MFMA work of the four layers, padding included
+    # MFMA work per 16 voxels: 27 16x16x16 products in each of the two 16-channel layers, 4 in the
+    # packed two-channel first layer and in the one-channel last layer (8 taps x 2 channels each)
+    flops = (2 * 27 + 2 * 4) * 2 * 16 * 16 * 16 * (n**3 / 16)
     print(json.dumps({"metric": "step time of the learned-correction PM relative to plain PM",
                       "value": out["learned"] / out["plain"], "unit": "ratio", "higher_is_better": False,
                       "ms_per_step_plain": out["plain"] * 1e3, "ms_per_step_learned": out["learned"] * 1e3,
