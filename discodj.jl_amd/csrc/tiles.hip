@@ -64,6 +64,23 @@ template <int W, typename T> __device__ __forceinline__ T twk(T d) {
     }
 }
 
+// The weight of stencil cell o (0 <= o < W) at d = |g - (b + lo + o)|, b the base cell of g.  Each
+// cell's d stays in one piece of the window, so only that piece is evaluated: TSC (b = round(g),
+// |g - b| <= 1/2) has d <= 1/2 at the centre (inner piece) and 1/2 <= d <= 3/2 at the sides (outer
+// piece; at d = 1/2 both pieces give 1/2 and at d = 3/2 the outer one gives the reference's 0); CIC
+// (b = floor(g)) has d <= 1, where the weight is 1 - d.  The values are the same bits as twk's.
+template <int W, typename T> __device__ __forceinline__ T twk_at(int o, T d) {
+    if constexpr (W == 3) {
+        if (o == 1) return (T)0.75 - d * d;
+        const T t = (T)1.5 - d;
+        return (T)0.5 * (t * t);
+    } else if constexpr (W == 2) {
+        return (T)1 - d;
+    } else {
+        return twk<W>(d);
+    }
+}
+
 // ------------------------------------------------------------------------------------ binning
 constexpr int HASH = 512;          // LDS hash slots per workgroup (keys: tile * 2 + ghost)
 constexpr unsigned EMPTY = 0xffffffffu;
@@ -375,7 +392,7 @@ __global__ __launch_bounds__(256) void k_tile_deposit(const Entry<T> *__restrict
             const T gg = tjmod(xs[a], L) / L * (T)g.N;
             const int b = twbase<W>(gg);
 #pragma unroll
-            for (int o = 0; o < W; o++) w[a][o] = twk<W>(fabs(gg - (T)(b + twlo<W>() + o)));
+            for (int o = 0; o < W; o++) w[a][o] = twk_at<W>(o, fabs(gg - (T)(b + twlo<W>() + o)));
             b0[a] = b + twlo<W>();
         }
         if constexpr (ACC == ACC_Q31) {
@@ -484,7 +501,7 @@ __global__ __launch_bounds__(256) void k_tile_deposit3(const Entry<T> *__restric
             const T gg = tjmod(xs[a], L) / L * (T)g.N;
             const int b = twbase<W>(gg);
 #pragma unroll
-            for (int o = 0; o < W; o++) w[a][o] = twk<W>(fabs(gg - (T)(b + twlo<W>() + o)));
+            for (int o = 0; o < W; o++) w[a][o] = twk_at<W>(o, fabs(gg - (T)(b + twlo<W>() + o)));
             int l = b + twlo<W>() - o3[a];
             l0[a] = l >= (g.N >> 1) ? l - g.N : (l < -(g.N >> 1) ? l + g.N : l);
         }
@@ -857,7 +874,7 @@ __global__ __launch_bounds__((gather_threads<T, W>()), (gather_min_waves<T, W>()
             const T gg = tjmod(xe[a] + xsh[a], L) / L * (T)N;
             const int b = twbase<W>(gg);
 #pragma unroll
-            for (int o = 0; o < W; o++) w[a][o] = twk<W>(fabs(gg - (T)(b + twlo<W>() + o)));
+            for (int o = 0; o < W; o++) w[a][o] = twk_at<W>(o, fabs(gg - (T)(b + twlo<W>() + o)));
             c0[a] = b + twlo<W>() - H;  // first cell of the stencil's FD neighbourhood (unwrapped)
             lb[a] = wrapn(c0[a] - r0[a], N);
             staged &= lb[a] + Q <= S[a];
