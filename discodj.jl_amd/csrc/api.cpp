@@ -55,6 +55,7 @@ struct dj_ctx {
     void *psi = nullptr, *mom = nullptr, *acc = nullptr, *psi_in = nullptr, *q1 = nullptr;
     void *mesh = nullptr, *meshfix = nullptr, *A = nullptr, *B = nullptr, *C = nullptr, *tw = nullptr;
     void *tabs = nullptr;
+    void *kgain = nullptr;  // fitted transfer function of the Poisson kernel (dj_set_kgain)
     alignas(16) unsigned char kmult[sizeof(KMult<double>)];
     double shifts[16][3];
     int nshifts = 1;
@@ -319,7 +320,7 @@ void free_all(dj_ctx *c) {
                     c->nn_p, c->nn_g, c->nn_m, c->nn_v, c->nn_mask, c->nn_wf, c->nn_wb, c->nn_pf, c->nn_pb,
                     c->nn_t[0], c->nn_t[1],
                     c->nn_t[2], c->nn_t[3], c->nn_t[4], c->nn_t[5], c->nn_t[6], c->nn_t[7], c->nn_t[8], c->nn_t[9],
-                    c->nn_dphi, c->nn_dens, c->nn_tgt, c->nn_sum, c->acc_list};
+                    c->nn_dphi, c->nn_dens, c->nn_tgt, c->nn_sum, c->acc_list, c->kgain};
     // teardown: failures here have no caller to report to
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -1296,6 +1297,44 @@ dj_status dj_nn_enable(dj_ctx *c, int32_t on, double out_scale) {
     c->nn_on = on != 0;
     c->nn_oscale = out_scale;
     return DJ_OK;
+}
+
+extern "C++" {
+namespace {
+template <typename T>
+dj_status set_kgain_t(dj_ctx *c, const double *gain, int32_t nbins, double kmax) {
+    KMult<T> K;
+    std::memcpy(&K, c->kmult, sizeof(K));
+    if (!gain || nbins == 0) {
+        K.kg = nullptr;
+        K.kg_n = 0;
+    } else {
+        std::vector<T> g(gain, gain + nbins);
+        if (c->kgain) HIPCHK(c, hipFree(c->kgain));
+        c->kgain = nullptr;
+        HIPCHK(c, hipMalloc(&c->kgain, sizeof(T) * nbins));
+        HIPCHK(c, hipMemcpy(c->kgain, g.data(), sizeof(T) * nbins, hipMemcpyHostToDevice));
+        K.kg = (const T *)c->kgain;
+        K.kg_n = nbins;
+        K.kg_inv_dk = (T)((nbins - 1) / kmax);
+    }
+    std::memcpy(c->kmult, &K, sizeof(K));
+    return DJ_OK;
+}
+}  // namespace
+}  // extern "C++"
+
+dj_status dj_set_kgain(dj_ctx *c, const double *gain, int32_t nbins, double kmax) {
+    if (!c) return fail(c, DJ_ERR_INVALID_ARGUMENT, "null context");
+    if (gain && nbins != 0 && (nbins < 2 || nbins > 65536 || !(kmax > 0)))
+        return fail(c, DJ_ERR_INVALID_ARGUMENT, "transfer function: nbins in [2, 65536] and kmax > 0");
+    if (gain)
+        for (int i = 0; i < nbins; i++)
+            if (!std::isfinite(gain[i])) return fail(c, DJ_ERR_INVALID_ARGUMENT, "transfer function not finite");
+    HIPCHK(c, hipSetDevice(c->dev));
+    // the kernels of earlier calls may still read the old table
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return c->tsize == 8 ? set_kgain_t<double>(c, gain, nbins, kmax) : set_kgain_t<float>(c, gain, nbins, kmax);
 }
 
 dj_status dj_nn_set_schedule(dj_ctx *c, int32_t n, const double *a_force) {

@@ -33,6 +33,11 @@ struct KMult {
     const cx<T> *gf, *gh;      // gradient kernels (ik path) full / half
     T scale;                   // 1 / (rho nsets N^3), rho = n^3/N^3, built in FP64 and rounded once
     int lap_order;
+    // optional isotropic transfer function g(|k|) of the kernel (a fitted correction): kg_n values
+    // on a uniform grid |k| = i / kg_inv_dk, linear in between, the last value beyond; null = none
+    const T *kg;
+    T kg_inv_dk;
+    int kg_n;
 
     DJ_HD T kphi(int kx, int ky, int kz) const {
         if (kx == 0 && ky == 0 && kz == 0) return (T)0;
@@ -46,6 +51,16 @@ struct KMult {
         }
         T v = il;
         if (mf) v = il * (mh[kx] * mf[ky] * mf[kz]);
+        if (kg) {
+            const T u = sqrt(k2h[kx] + k2f[ky] + k2f[kz]) * kg_inv_dk;
+            const int i = (int)u;
+            if (i >= kg_n - 1) {
+                v = v * kg[kg_n - 1];
+            } else {
+                const T f = u - (T)i;
+                v = v * (kg[i] + f * (kg[i + 1] - kg[i]));
+            }
+        }
         return v * scale;
     }
 };

@@ -53,6 +53,7 @@ _SIGS = {
                            C.c_void_p, STEP_CB, C.c_void_p]),
     "dj_deposit": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "dj_check_finite": (C.c_int32, [C.c_void_p]),
+    "dj_set_kgain": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32, C.c_double]),
     "dj_synchronize": (C.c_int32, [C.c_void_p]),
     "dj_set_profiling": (C.c_int32, [C.c_void_p, C.c_int32]),
     "dj_reset_stats": (C.c_int32, [C.c_void_p]),

@@ -1,5 +1,9 @@
 """Learned force correction of the PM step: training data, training and evaluation on the device.
 
+Two learned parts, used alone or together: a fitted transfer function g(|k|) of the Poisson kernel
+(fit_transfer: P(k) of the coarse run matched to the fine-mesh run's), and a residual convolution net
+that adds a potential correction (train: force matched to the fine-mesh force at snapshots).
+
 The corrected force is that of the PM solver on an N^3 mesh with a residual convolution net's potential
 correction (dj_nn_*): it is trained on pairs of runs the method generates itself, the same particles and
 ICs on the N^3 mesh and on a finer mesh (4N)^3, by minimising the mean squared difference between the
@@ -41,7 +45,7 @@ class Workload:
     """Particles n^3 in a box L, the coarse mesh N and the fine mesh ratio*N, a stepper from a_ini to a_end."""
 
     def __init__(self, cosmo, n, N, L, ratio=4, n_steps=10, a_ini=0.05, a_end=1.0, stepper="bullfrog",
-                 worder=2, grad_order=4):
+                 worder=2, grad_order=4, deconvolve=False):
         self.cosmo, self.n, self.N, self.L, self.ratio = cosmo, n, N, float(L), ratio
         self.co = stepper_coefficients(cosmo, stepper, a_ini, a_end, n_steps, "D")
         self.coef = [self.co["ddrift1"], self.co["alpha"], self.co["beta"], self.co["ddrift2"]]
@@ -49,7 +53,9 @@ class Workload:
         self.D_ini = cosmo.Dplus(a_ini)
         self.k_tab, self.pk_tab = linear_power_spectrum(cosmo)
         kw = dict(n_part=n, boxsize=self.L, worder=worder, grad_order=grad_order)
-        self.cfg_lo = PMConfig(res_pm=N, **kw)
+        # deconvolve: the coarse solve divides by the mass-assignment window (pm.jl:150-151), and the net
+        # learns the residual; the fine-mesh reference stays plain PM
+        self.cfg_lo = PMConfig(res_pm=N, deconvolve=deconvolve, **kw)
         self.cfg_hi = PMConfig(res_pm=ratio * N, **kw)
 
     def run(self, S, seed, want=None, cb=None):
@@ -117,19 +123,61 @@ def force_errors(work, data, params, out_scale):
     return np.array(e_plain), np.array(e_nn)
 
 
-def power_spectrum_errors(work, seed, params, out_scale):
-    """max |P/P_fine - 1| for kf <= k <= k_Nyq/2 of the coarse mesh at a_end, for the plain and the
-    corrected coarse runs of seed (all P(k) measured with the coarse mesh's CIC estimator)."""
+def fine_states(work, seeds):
+    """Displacements at a_end of the fine-mesh runs of seeds."""
+    out = []
     with PMSolverHIP(work.cfg_hi, precision=32) as S:
-        work.run(S, seed)
-        psi_hi, _ = S.get_state()
+        for seed in seeds:
+            work.run(S, seed)
+            out.append(S.get_state()[0])
+    return out
+
+
+def fit_transfer(work, seeds, iters=8, power=0.5, psi_fine=None, log=None):
+    """Transfer function g(|k|) of the coarse Poisson kernel fitted on the device so that the coarse run's
+    P(k) at a_end matches the fine-mesh run's for the training seeds (all P(k) with the coarse mesh's CIC
+    estimator, linear bins of width kf up to k_Nyq/2): fixed-point updates g <- g (P_fine / P_coarse)^power
+    averaged over the seeds, on the grid |k| = i kf, held at its last value beyond k_Nyq/2.
+    -> (g, kmax) for PMSolverHIP.set_kgain."""
+    psi_fine = psi_fine if psi_fine is not None else fine_states(work, seeds)
     with PMSolverHIP(work.cfg_lo, precision=32) as S:
-        _, p_hi, nm = S.power_spectrum(psi_hi, W=2)
+        spectra = [S.power_spectrum(p, W=2) for p in psi_fine]
+        k, nm = spectra[0][0], spectra[0][2]
+        ok = nm > 0
+        kf = 2 * np.pi / work.L
+        nb = int(np.ceil(k[ok][-1] / kf)) + 1
+        kmax = (nb - 1) * kf
+        grid = np.arange(nb) * kf
+        g = np.ones(nb)
+        for it in range(iters):
+            S.set_kgain(g, kmax)
+            ratio = np.zeros(ok.sum())
+            for seed, (_, p_hi, _) in zip(seeds, spectra):
+                work.run(S, seed)
+                _, p_lo, _ = S.power_spectrum(W=2)
+                ratio += p_hi[ok] / p_lo[ok] / len(seeds)
+            if log:
+                log(f"transfer fit {it}: max |P_coarse / P_fine - 1| = {np.max(np.abs(1 / ratio - 1)):.4f}")
+            g = g * np.interp(grid, k[ok], ratio ** power)
+    return g, kmax
+
+
+def power_spectrum_errors(work, seed, params=None, out_scale=None, kgain=None, psi_fine=None):
+    """max |P/P_fine - 1| for kf <= k <= k_Nyq/2 of the coarse mesh at a_end, for the plain and the
+    corrected coarse runs of seed (all P(k) measured with the coarse mesh's CIC estimator).  The
+    correction is the net (params, out_scale), the transfer function kgain = (g, kmax), or both."""
+    if psi_fine is None:
+        psi_fine = fine_states(work, [seed])[0]
+    with PMSolverHIP(work.cfg_lo, precision=32) as S:
+        _, p_hi, nm = S.power_spectrum(psi_fine, W=2)
         work.run(S, seed)
         _, p_plain, _ = S.power_spectrum(W=2)
-        S.nn_set_params(params)
-        S.nn_enable(True, out_scale)
-        S.nn_set_schedule(work.a_force)
+        if params is not None:
+            S.nn_set_params(params)
+            S.nn_enable(True, out_scale)
+            S.nn_set_schedule(work.a_force)
+        if kgain is not None:
+            S.set_kgain(*kgain)
         work.run(S, seed)
         k, p_nn, _ = S.power_spectrum(W=2)
     ok = nm > 0

@@ -150,6 +150,15 @@ class PMSolverHIP:
     def check_finite(self):
         self._chk(self._L.dj_check_finite(self._ctx))
 
+    def set_kgain(self, gain, kmax=None):
+        """Multiply the Poisson kernel by the transfer function g(|k|) given at len(gain) points
+        |k| = i kmax / (len - 1) (linear in between, the last value beyond; dj_set_kgain); None removes it."""
+        if gain is None:
+            self._chk(self._L.dj_set_kgain(self._ctx, None, 0, 1.0))
+            return
+        g = np.ascontiguousarray(gain, np.float64)
+        self._chk(self._L.dj_set_kgain(self._ctx, g.ctypes.data, len(g), float(kmax)))
+
     # learned correction -----------------------------------------------------------------------
     def nn_set_params(self, params):
         p = np.ascontiguousarray(params, np.float32)

@@ -186,6 +186,12 @@ dj_status dj_nn_backward(dj_ctx *ctx, const float *dens, double a, const float *
 dj_status dj_nn_train_step(dj_ctx *ctx, const void *psi, const void *acc_target, double a, double weight, double lr,
                            int32_t step, double *loss_out);
 
+/* Fitted transfer function of the Poisson kernel (no counterpart in the reference; part of the learned
+ * correction): every force pass multiplies the kernel by g(|k|), with g given at nbins >= 2 points
+ * |k| = i kmax / (nbins - 1), i < nbins (k in h/Mpc), linear in between and g[nbins - 1] beyond kmax.
+ * gain NULL (or nbins 0) removes it.  Works with every gradient and deconvolution option. */
+dj_status dj_set_kgain(dj_ctx *ctx, const double *gain, int32_t nbins, double kmax);
+
 /* NaN/Inf scan of the particle state (failure detection); DJ_ERR_NONFINITE if any */
 dj_status dj_check_finite(dj_ctx *ctx);
 

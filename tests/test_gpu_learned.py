@@ -1,5 +1,6 @@
 """Learned force correction: the device net (MFMA BF16 convolutions) against the numpy oracle, its
-parameter gradients against the oracle's, and what training gains over plain PM."""
+parameter gradients against the oracle's, and what training gains over plain PM; the fitted transfer
+function of the Poisson kernel (identity, linearity, argument checks, held-out P(k) after the fit)."""
 import numpy as np
 import pytest
 
@@ -62,3 +63,59 @@ def test_training_reduces_force_error():
     pk_plain, pk_nn, _ = D.learned.power_spectrum_errors(work, 3, params, scale)
     print(f"max |P/P_fine - 1| to k_Nyq/2: plain {pk_plain:.4f}, corrected {pk_nn:.4f}")
     assert pk_nn < pk_plain + 0.02
+
+
+# ------------------------------------------------------------------ fitted transfer function
+def _psi(n, L, seed):
+    return np.random.default_rng(seed).normal(scale=2.0, size=(3, n, n, n)).astype(np.float32)
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_transfer_function_identity_and_linearity(prec):
+    """g = 1 leaves the force as it is; g = 2 doubles it (every later step is linear and the factor is a
+    power of two); a g rising beyond kmax / 2 changes it.  FP32 to the bit (its deposit sums integers,
+    the same in every call); FP64 to 1e-12 (its LDS float atomics sum in a varying order)."""
+    n, N, L = 32, 64, 100.0
+    dt = np.float32 if prec == 32 else np.float64
+    psi = _psi(n, L, 4).astype(dt)
+    cfg = D.PMConfig(res_pm=N, n_part=n, boxsize=L, worder=3, grad_order=4, antialias=1)
+    kmax = np.pi * N / L
+
+    def same(a, b):
+        return np.array_equal(a, b) if prec == 32 else np.max(np.abs(a - b)) <= 1e-12 * np.max(np.abs(b))
+
+    with D.PMSolverHIP(cfg, precision=prec) as S:
+        plain = S.pm_acceleration(psi).copy()
+        S.set_kgain(np.ones(17), kmax)
+        assert same(S.pm_acceleration(psi), plain)
+        S.set_kgain(np.full(5, 2.0), kmax)
+        assert same(S.pm_acceleration(psi), 2 * plain)
+        S.set_kgain(None)
+        assert same(S.pm_acceleration(psi), plain)
+        # beyond kmax the last value holds: g = 1 up to kmax / 2, rising to 3 at kmax
+        S.set_kgain(np.array([1.0, 1.0, 3.0]), kmax)
+        hi = S.pm_acceleration(psi)
+        assert not np.allclose(hi, plain) and np.all(np.isfinite(hi))
+
+
+def test_transfer_function_rejects_bad_tables():
+    with D.PMSolverHIP(D.PMConfig(res_pm=32, n_part=16, boxsize=50.0), precision=32) as S:
+        for g, kmax in (([1.0], 1.0), ([1.0, 2.0], 0.0), ([1.0, np.nan], 1.0)):
+            with pytest.raises(Exception):
+                S.set_kgain(np.array(g), kmax)
+
+
+def test_fitted_transfer_function_matches_fine_mesh_power():
+    """The transfer function fitted on two seeds (64^3 particles, 64^3 mesh with window deconvolution,
+    against the 256^3-mesh run) brings the held-out seed's P(k) within 2 % of the fine mesh's to
+    k_Nyq/2 (plain PM: 10 %)."""
+    c = D.compute_timetables(D.Cosmology(Omega_c=0.2621, Omega_b=0.049, h=0.6766, sigma8=0.8102, n_s=0.9665))
+    work = D.learned.Workload(c, n=64, N=64, L=128.0, ratio=4, n_steps=10, deconvolve=True)
+    fine = D.learned.fine_states(work, [1, 2, 3])
+    g, kmax = D.learned.fit_transfer(work, [1, 2], psi_fine=fine[:2])
+    pk_plain, pk_corr, _ = D.learned.power_spectrum_errors(work, 3, kgain=(g, kmax), psi_fine=fine[2])
+    plain_work = D.learned.Workload(c, n=64, N=64, L=128.0, ratio=4, n_steps=10)
+    pk_undeconvolved, _, _ = D.learned.power_spectrum_errors(plain_work, 3, psi_fine=fine[2])
+    print(f"max |P/P_fine - 1| to k_Nyq/2: plain {pk_undeconvolved:.4f}, deconvolved {pk_plain:.4f}, "
+          f"fitted transfer function {pk_corr:.4f}")
+    assert pk_corr < 0.02 < pk_undeconvolved
