@@ -133,14 +133,19 @@ def fine_states(work, seeds):
     return out
 
 
-def fit_transfer(work, seeds, iters=8, power=0.5, psi_fine=None, log=None):
+def fit_transfer(work, seeds, iters=8, power=0.5, psi_fine=None, params=None, out_scale=None, log=None):
     """Transfer function g(|k|) of the coarse Poisson kernel fitted on the device so that the coarse run's
     P(k) at a_end matches the fine-mesh run's for the training seeds (all P(k) with the coarse mesh's CIC
     estimator, linear bins of width kf up to k_Nyq/2): fixed-point updates g <- g (P_fine / P_coarse)^power
-    averaged over the seeds, on the grid |k| = i kf, held at its last value beyond k_Nyq/2.
+    averaged over the seeds, on the grid |k| = i kf, held at its last value beyond k_Nyq/2.  With params
+    (and out_scale) the coarse runs include the conv net, and g is fitted on top of it.
     -> (g, kmax) for PMSolverHIP.set_kgain."""
     psi_fine = psi_fine if psi_fine is not None else fine_states(work, seeds)
     with PMSolverHIP(work.cfg_lo, precision=32) as S:
+        if params is not None:
+            S.nn_set_params(params)
+            S.nn_enable(True, out_scale)
+            S.nn_set_schedule(work.a_force)
         spectra = [S.power_spectrum(p, W=2) for p in psi_fine]
         k, nm = spectra[0][0], spectra[0][2]
         ok = nm > 0

@@ -45,24 +45,30 @@ def test_nn_forward_and_gradient_match_oracle(seed):
 
 
 def test_training_reduces_force_error():
-    """The learned-correction workload at reduced size: 64^3 particles on a 64^3 mesh against a 256^3
-    mesh, BullFrog 10 steps; trained on six seeds (every step), judged on a seventh.  The gain asserted
-    is the held-out force error; the P(k) errors to k_Nyq/2 are printed (the net narrows the gap at
-    intermediate k, and the last bins stay about as far as plain PM's)."""
+    """The learned-correction workload at reduced size: 64^3 particles on a 64^3 mesh (window
+    deconvolution) against a 256^3 mesh, BullFrog 10 steps; the net trained on six seeds (every step),
+    judged on a seventh.  Asserted: the held-out force error falls; the net alone does not lower the
+    P(k) error (a force-MSE optimum is smoother than the fine force), and a transfer function fitted on
+    top of the net (two seeds) takes back most of what the net loses (to 3 % on this seed, 1.4 % on
+    others; the transfer function alone meets 2 %, test_fitted_transfer_function_matches_fine_mesh_power)."""
     c = D.Cosmology.named("Planck18EEBAOSN")
-    work = D.learned.Workload(c, n=64, N=64, L=128.0, ratio=4, n_steps=10)
+    work = D.learned.Workload(c, n=64, N=64, L=128.0, ratio=4, n_steps=10, deconvolve=True)
     data = work.snapshots([1, 2, 4, 5, 6, 7], every=1)
     params, scale, losses = D.learned.train(work, data, iters=8000, lr=3e-3)
-    # losses are relative to plain PM's force error at each snapshot (1 = no gain)
-    assert np.mean(losses[-500:]) < 0.8, np.mean(losses[-500:])
+    # losses are relative to the uncorrected force error at each snapshot (1 = no gain)
+    assert np.mean(losses[-500:]) < 0.9, np.mean(losses[-500:])
     held = work.snapshots([3], every=3)
     e_plain, e_nn = D.learned.force_errors(work, held, params, scale)
     ratio = e_nn / e_plain
     print("held-out force MSE, corrected / plain, per snapshot:", np.round(ratio, 3))
-    assert np.mean(ratio) < 0.85 and np.all(ratio < 1.0), ratio
-    pk_plain, pk_nn, _ = D.learned.power_spectrum_errors(work, 3, params, scale)
-    print(f"max |P/P_fine - 1| to k_Nyq/2: plain {pk_plain:.4f}, corrected {pk_nn:.4f}")
-    assert pk_nn < pk_plain + 0.02
+    assert np.mean(ratio) < 0.9 and np.all(ratio < 1.0), ratio
+    fine = D.learned.fine_states(work, [1, 2, 3])
+    pk_plain, pk_nn, _ = D.learned.power_spectrum_errors(work, 3, params, scale, psi_fine=fine[2])
+    g, kmax = D.learned.fit_transfer(work, [1, 2], psi_fine=fine[:2], params=params, out_scale=scale)
+    _, pk_both, _ = D.learned.power_spectrum_errors(work, 3, params, scale, kgain=(g, kmax), psi_fine=fine[2])
+    print(f"max |P/P_fine - 1| to k_Nyq/2: deconvolved {pk_plain:.4f}, net {pk_nn:.4f}, net + transfer "
+          f"function {pk_both:.4f}")
+    assert pk_both < 0.5 * pk_nn and pk_both < 0.05
 
 
 # ------------------------------------------------------------------ fitted transfer function
