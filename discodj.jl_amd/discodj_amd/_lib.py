@@ -1,7 +1,7 @@
 """ctypes binding of libdiscodj_hip.so (the C ABI of include/discodj_hip.h).
 
-The library is built in-tree by __graft_entry__.build() (or `make -C discodj.jl_amd/csrc`).  There
-is no fallback: if the library is missing, loading fails with an error that says how to build it.
+The library is built in-tree by `make -C discodj.jl_amd/csrc` (the repository's build step runs the
+same make).  There is no fallback: if the library is missing, loading fails with an error that says how to build it.
 """
 import ctypes as C
 import os
@@ -97,8 +97,7 @@ def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
-            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; "
-                               f"g.build()'` or `make -C discodj.jl_amd/csrc`")
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `make -C discodj.jl_amd/csrc`")
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in _SIGS.items():
             f = getattr(L, name)

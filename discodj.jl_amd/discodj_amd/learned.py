@@ -8,7 +8,7 @@ The corrected force is that of the PM solver on an N^3 mesh with a residual conv
 correction (dj_nn_*): it is trained on pairs of runs the method generates itself, the same particles and
 ICs on the N^3 mesh and on a finer mesh (4N)^3, by minimising the mean squared difference between the
 corrected N^3 force and the fine-mesh force at snapshots of the fine run.  Plain PM stays the fallback
-(nn_enable(False)) and the judge of the correction's P(k) error.  There is no counterpart in the reference.
+(nn_enable(False)) and the baseline of the correction's P(k) error.  There is no counterpart in the reference.
 """
 import numpy as np
 

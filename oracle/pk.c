@@ -1,7 +1,7 @@
 /*
  * Oracle (test infrastructure only): power spectra of a mesh.
  *   evaluate_power_spectrum ........ NR/src/analysis/power_spectrum.jl:25-79
- *   linear-bin, window-deconvolved P(k) used to judge the PM path's accuracy, following the
+ *   linear-bin, window-deconvolved P(k) that measures the PM path's accuracy, following the
  *   study estimator studies/eft_shell_crossing/src/measure.jl:403-447 (mode weights 1 on the kz = 0
  *   and kz = N/2 planes and 2 elsewhere, |W(k)|^2 = prod sinc(k dx / 2)^(2W) divided out).
  */

@@ -47,7 +47,7 @@ def test_nn_forward_and_gradient_match_oracle(seed):
 def test_training_reduces_force_error():
     """The learned-correction workload at reduced size: 64^3 particles on a 64^3 mesh (window
     deconvolution) against a 256^3 mesh, BullFrog 10 steps; the net trained on six seeds (every step),
-    judged on a seventh.  Asserted: the held-out force error falls; the net alone does not lower the
+    evaluated on a seventh.  Asserted: the held-out force error falls; the net alone does not lower the
     P(k) error (a force-MSE optimum is smoother than the fine force), and a transfer function fitted on
     top of the net (two seeds) takes back most of what the net loses (to 3 % on this seed, 1.4 % on
     others; the transfer function alone meets 2 %, test_fitted_transfer_function_matches_fine_mesh_power)."""
