@@ -21,6 +21,30 @@ struct alignas(2 * sizeof(T)) cx {
 };
 
 template <typename T> DJ_HD cx<T> mk(T a, T b) { return cx<T>{a, b}; }
+
+// Accesses of the transforms' contiguous (read-once, write-once) sides, with non-temporal hints in
+// the HIP build so that these lines do not displace the transposed chunks that neighbouring blocks
+// still have to merge in L2; plain accesses in the host emulation.
+template <typename T> DJ_HD cx<T> ld_stream(const cx<T> *p) {
+#if defined(__HIPCC__)
+    typedef T V __attribute__((ext_vector_type(2)));
+    const V v = __builtin_nontemporal_load((const V *)p);
+    return cx<T>{v.x, v.y};
+#else
+    return *p;
+#endif
+}
+template <typename T> DJ_HD void st_stream(cx<T> *p, cx<T> a) {
+#if defined(__HIPCC__)
+    typedef T V __attribute__((ext_vector_type(2)));
+    V v;
+    v.x = a.x;
+    v.y = a.y;
+    __builtin_nontemporal_store(v, (V *)p);
+#else
+    *p = a;
+#endif
+}
 template <typename T> DJ_HD cx<T> operator+(cx<T> a, cx<T> b) { return {a.x + b.x, a.y + b.y}; }
 template <typename T> DJ_HD cx<T> operator-(cx<T> a, cx<T> b) { return {a.x - b.x, a.y - b.y}; }
 template <typename T> DJ_HD cx<T> scale(cx<T> a, T s) { return {a.x * s, a.y * s}; }

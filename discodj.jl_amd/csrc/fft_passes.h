@@ -279,6 +279,8 @@ struct Passes {
                 const size_t g = base + (size_t)N * l + 2 * m;
                 if constexpr (sizeof(IN) == 8 && !std::is_floating_point<IN>::value)
                     return cx<T>{(T)((double)mesh[g] * fixed_scale), (T)((double)mesh[g + 1] * fixed_scale)};
+                else if constexpr (std::is_same<IN, T>::value)
+                    return ld_stream((const cx<T> *)(mesh + g));  // g even: a pair of reals
                 else
                     return cx<T>{(T)mesh[g], (T)mesh[g + 1]};
             },
@@ -320,19 +322,19 @@ struct Passes {
             [&](int i, cx<T> v) {
                 const int l = i / NC, m = i - l * NC;
                 const size_t g = base + (size_t)N * l + 2 * m;
-                mesh[g] = v.x;
-                mesh[g + 1] = v.y;
+                st_stream((cx<T> *)(mesh + g), v);  // g even: a pair of reals
             });
     }
     DJ_HD static void y_load_a_w(const cx<T> *A, cx<T> *lds, int zb, int kx, int tid) {
         copy_wave<cx<T>, EF::LW * N>(
-            tid >> 6, tid & 63, [&](int i) { const int l = i / N, y = i - l * N; return A[a_at(y, kx, zb * B + l)]; },
+            tid >> 6, tid & 63,
+            [&](int i) { const int l = i / N, y = i - l * N; return A[a_at(y, kx, zb * B + l)]; },
             [&](int i, cx<T> v) { const int l = i / N, y = i - l * N; lds[EF::idx(l, y)] = v; });
     }
     DJ_HD static void y_store_a_w(const cx<T> *lds, cx<T> *A, int zb, int kx, int tid) {
         copy_wave<cx<T>, EF::LW * N>(
             tid >> 6, tid & 63, [&](int i) { const int l = i / N, y = i - l * N; return lds[EF::idx(l, y)]; },
-            [&](int i, cx<T> v) { const int l = i / N, y = i - l * N; A[a_at(y, kx, zb * B + l)] = v; });
+            [&](int i, cx<T> v) { const int l = i / N, y = i - l * N; st_stream(&A[a_at(y, kx, zb * B + l)], v); });
     }
     DJ_HD static void z_load_w(const cx<T> *Bm, cx<T> *lds, int b, int tid) {
         const cx<T> *src = Bm + (size_t)N * B * b;
