@@ -358,13 +358,13 @@ __device__ __forceinline__ void dep_add(Acc *a, T wx, T wy, T wz, T wyz, double 
 }
 
 // Padded accumulator (PAD, every axis of the mesh holds at least two tiles): rows of SXD = TX + 2 (W-1)
-// cells, the tile's x cells at [W-1, W-1+TX), for the TY TZ rows of the tile plus one discard row.  An
-// entry's stencil starts at local x cell l0 in [-(W-1), TX) whenever it overlaps the tile, so its W x
-// cells are always in the row (cells outside the tile land in the pads), and rows outside the tile are
-// redirected to the discard row: every entry makes W^3 unmasked atomics at immediate offsets from
-// W^2 row addresses.  Pads and the discard row are not stored; their cells belong to neighbour tiles,
-// which receive the same particles through their ghost lists.  With a single tile along an axis the
-// stencil wraps onto the tile itself, and the masked form (!PAD) wraps each cell.
+// cells, the tile's x cells at [W-1, W-1+TX), for the TY TZ rows of the tile.  An entry's stencil starts at local x cell l0 in
+// [-(W-1), TX) whenever it overlaps the tile, so its W x cells are always in the row (cells outside
+// the tile land in the pads) and each row makes W unmasked atomics at immediate offsets from one row
+// address.  Rows outside the tile are skipped: a ghost entry's stencil is mostly outside, and those
+// lanes sit out the row's atomics.  Pads are not stored; their cells belong to neighbour tiles, which
+// receive the same particles through their ghost lists.  With a single tile along an axis the stencil
+// wraps onto the tile itself, and the masked form (!PAD) wraps each cell.
 template <typename T, int W, int ACC, bool PAD>
 __global__ __launch_bounds__(256) void k_tile_deposit(const Entry<T> *__restrict__ ent, const unsigned *__restrict__ start,
                                                       const unsigned *__restrict__ gstart, const unsigned *__restrict__ cnt,
@@ -374,7 +374,7 @@ __global__ __launch_bounds__(256) void k_tile_deposit(const Entry<T> *__restrict
     Acc *acc = (Acc *)smem;
     constexpr int P = W - 1;
     const int t = blockIdx.x, TV = g.TX * g.TY * g.TZ;
-    const int SXD = PAD ? g.TX + 2 * P : g.TX, ROWS = g.TY * g.TZ, CELLS = PAD ? SXD * (ROWS + 1) : TV;
+    const int SXD = PAD ? g.TX + 2 * P : g.TX, ROWS = g.TY * g.TZ, CELLS = PAD ? SXD * ROWS : TV;
     for (int c = threadIdx.x; c < CELLS; c += blockDim.x) acc[c] = 0;
     const int tx = t % g.ntx, ty = (t / g.ntx) % g.nty, tz = t / (g.ntx * g.nty);
     const int ox = tx << g.lT[0], oy = ty << g.lT[1], oz = tz << g.lT[2];
@@ -413,9 +413,9 @@ __global__ __launch_bounds__(256) void k_tile_deposit(const Entry<T> *__restrict
             // for another interlacing shift)
             const bool okx = l0[0] >= -P && l0[0] < g.TX;
             const int xo = okx ? l0[0] + P : 0;
-            // row offsets (cells) per y and z stencil cell, out-of-tile ones pushed past the discard row
+            // row offsets (cells) per y and z stencil cell, out-of-tile ones pushed past the last row
             constexpr int BIG = 1 << 28;
-            const int SZ = SXD * g.TY, DISCARD = SXD * ROWS;
+            const int SZ = SXD * g.TY, END = SXD * ROWS;
             int ly[W], lz[W];
 #pragma unroll
             for (int o = 0; o < W; o++) {
@@ -423,11 +423,15 @@ __global__ __launch_bounds__(256) void k_tile_deposit(const Entry<T> *__restrict
                 ly[o] = (unsigned)y < (unsigned)g.TY ? __mul24(y, SXD) : BIG;
                 lz[o] = (okx && (unsigned)z < (unsigned)g.TZ) ? __mul24(z, SZ) : BIG;
             }
+            // rows outside the tile are skipped (their lanes sit out the row's atomics); sending them to a
+            // shared discard row instead made every lane's atomics meet on one row
 #pragma unroll
             for (int iz = 0; iz < W; iz++)
 #pragma unroll
                 for (int iy = 0; iy < W; iy++) {
-                    Acc *r = acc + (xo + min(ly[iy] + lz[iz], DISCARD));
+                    const int ro = ly[iy] + lz[iz];
+                    if (ro >= END) continue;
+                    Acc *r = acc + (xo + ro);
                     const T wyz = w[1][iy] * w[2][iz];
 #pragma unroll
                     for (int ix = 0; ix < W; ix++) dep_add<T, ACC>(r + ix, w[0][ix], w[1][iy], w[2][iz], wyz, fscale);
@@ -482,8 +486,8 @@ __global__ __launch_bounds__(256) void k_tile_deposit3(const Entry<T> *__restric
     T *acc = (T *)smem;
     constexpr int P = W - 1;
     const int t = blockIdx.x;
-    const int SXD = g.TX + 2 * P, ROWS = g.TY * g.TZ, CELLS = SXD * (ROWS + 1);
-    const int SZ = SXD * g.TY, DISCARD = SXD * ROWS;
+    const int SXD = g.TX + 2 * P, ROWS = g.TY * g.TZ, CELLS = SXD * ROWS;
+    const int SZ = SXD * g.TY, END = SXD * ROWS;
     for (int c = threadIdx.x; c < 3 * CELLS; c += blockDim.x) acc[c] = 0;
     const int tx = t % g.ntx, ty = (t / g.ntx) % g.nty, tz = t / (g.ntx * g.nty);
     const int o3[3] = {tx << g.lT[0], ty << g.lT[1], tz << g.lT[2]};
@@ -519,7 +523,9 @@ __global__ __launch_bounds__(256) void k_tile_deposit3(const Entry<T> *__restric
         for (int iz = 0; iz < W; iz++)
 #pragma unroll
             for (int iy = 0; iy < W; iy++) {
-                T *r = acc + (xo + min(ly[iy] + lz[iz], DISCARD));
+                const int ro = ly[iy] + lz[iz];
+                if (ro >= END) continue;  // a row outside the tile
+                T *r = acc + (xo + ro);
                 const T wyz = w[1][iy] * w[2][iz];
 #pragma unroll
                 for (int ix = 0; ix < W; ix++) {
@@ -951,7 +957,7 @@ struct TileLaunch {
     static void dep_w(const TileArgs &a, const double *sh, void *mesh, double fscale, hipStream_t st) {
         const TileGeom &g = a.g;
         const bool pad = g.ntx >= 2 && g.nty >= 2 && g.ntz >= 2;
-        const size_t cells = pad ? (size_t)(g.TX + 2 * (W - 1)) * (g.TY * g.TZ + 1) : (size_t)g.TX * g.TY * g.TZ;
+        const size_t cells = pad ? (size_t)(g.TX + 2 * (W - 1)) * (g.TY * g.TZ) : (size_t)g.TX * g.TY * g.TZ;
         const size_t shm = cells * (ACC == ACC_T ? sizeof(T) : 8);
         auto kern = pad ? k_tile_deposit<T, W, ACC, true> : k_tile_deposit<T, W, ACC, false>;
         (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
@@ -967,7 +973,7 @@ struct TileLaunch {
     template <int W>
     static void dep3_w(const TileArgs &a, const double *sh, const void *vals, double scale, void *out, hipStream_t st) {
         const TileGeom &g = a.g;
-        const size_t shm = 3 * (size_t)(g.TX + 2 * (W - 1)) * (g.TY * g.TZ + 1) * sizeof(T);
+        const size_t shm = 3 * (size_t)(g.TX + 2 * (W - 1)) * (g.TY * g.TZ) * sizeof(T);
         (void)hipFuncSetAttribute((const void *)k_tile_deposit3<T, W>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)shm);
         k_tile_deposit3<T, W><<<g.nt, 256, shm, st>>>((const Entry<T> *)a.ent, a.start, a.gstart, a.cur, g, (T)sh[0],
@@ -978,7 +984,7 @@ struct TileLaunch {
                          hipStream_t st) {
         const TileGeom &g = a.g;
         if (g.ntx < 2 || g.nty < 2 || g.ntz < 2) return false;
-        if ((size_t)3 * (g.TX + 2 * (W - 1)) * (g.TY * g.TZ + 1) * sizeof(T) > 160 * 1024) return false;
+        if ((size_t)3 * (g.TX + 2 * (W - 1)) * (g.TY * g.TZ) * sizeof(T) > 160 * 1024) return false;
         if (W == 2) dep3_w<2>(a, sh, vals, scale, out, st);
         else if (W == 3) dep3_w<3>(a, sh, vals, scale, out, st);
         else dep3_w<4>(a, sh, vals, scale, out, st);
