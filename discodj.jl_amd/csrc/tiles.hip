@@ -27,13 +27,19 @@ template <> struct alignas(16) Entry<double> { double x, y, z; int idx; int pad;
 // Julia's float mod(x, L) for L > 0.  On [-L, 2L) it is computed without fmod: there fmod returns x,
 // x - L (exact, Sterbenz) or x (for -L < x < 0, then shifted by L), and every branch below gives
 // the same bits as the general rule (rem, then + L for a negative remainder, +0 for a zero one).
-template <typename T> __device__ __forceinline__ T tjmod(T x, T L) {
-    if (x >= (T)0 && x < L) return x + (T)0;  // + 0 turns -0 into +0, as copysign(0, L) does
-    if (x >= L && x < (T)2 * L) return x - L;
-    if (x < (T)0 && x >= -L) return x + L;
+// The three in-range cases are computed with the same operations and selected without branches;
+// only positions outside [-L, 2L) take the general path.
+template <typename T> __device__ __noinline__ T tjmod_general(T x, T L) {
     T r = fmod(x, L);
     if (r == (T)0) return copysign(r, L);
     if ((r > (T)0) != (L > (T)0)) return r + L;
+    return r;
+}
+template <typename T> __device__ __forceinline__ T tjmod(T x, T L) {
+    T r = x + (T)0;            // [0, L): + 0 turns -0 into +0, as copysign(0, L) does
+    if (x >= L) r = x - L;     // [L, 2L)
+    if (x < (T)0) r = x + L;   // [-L, 0)
+    if (__builtin_expect(x < -L || x >= (T)2 * L, 0)) r = tjmod_general(x, L);
     return r;
 }
 // a mod n for a in [-n, 2n)
