@@ -81,8 +81,8 @@ def main():
     if args.workload == "learned":
         return bench_learned(args)
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: every box of this build has one MI355X; --gpus {args.gpus} "
-              f"is not run (the multi-GPU slab/pencil decomposition is a design, not a build)", file=sys.stderr)
+        print(f"bench.py measures one GPU (one MI355X); --gpus {args.gpus} is not run: the multi-GPU slab/pencil "
+              f"decomposition of the mesh is designed, not implemented", file=sys.stderr)
         return 2
 
     import numpy as np
