@@ -127,13 +127,11 @@ struct Emu {
             }
         for (int z = 0; z < N; z++)
             for (int yb = 0; yb < N / B; yb++) {
-                for (int t = 0; t < NT; t++) P::xi_load(A, lds.data(), yb, z, t);
+                for (int t = 0; t < NT; t++) P::xi_load_pre(A, lds.data(), tw, yb, z, t);
                 if constexpr (WAVE) {
-                    for (int t = 0; t < NT; t++) P::xi_pre_w(lds.data(), tw, t);
                     emu_engine_wave<EX, +1, 2>(lds, tw);
                     for (int t = 0; t < NT; t++) P::xi_store_w(lds.data(), mesh, yb, z, t);
                 } else {
-                    for (int t = 0; t < NT; t++) P::xi_pre(lds.data(), tw, t);
                     emu_engine<EX, +1, 2>(lds, tw);
                     for (int t = 0; t < NT; t++) P::xi_store(lds.data(), mesh, yb, z, t);
                 }

@@ -72,16 +72,12 @@ __global__ __launch_bounds__(256) void k_xi(const cx<T> *__restrict__ A, T *__re
     using P = typename S::P;
     __shared__ cx<T> lds[P::EX::LDS_ELEMS];
     const int lb = xcd_block(blockIdx.x, gridDim.x), yb = lb % (N / S::B), z = lb / (N / S::B);
-    P::xi_load(A, lds, yb, z, threadIdx.x);
+    P::xi_load_pre(A, lds, tw, yb, z, threadIdx.x);
     __syncthreads();
     if constexpr (P::EX::wave_ok()) {
-        P::xi_pre_w(lds, tw, threadIdx.x);
-        P::EX::wave_sync();
         P::EX::template run_wave<+1, 2>(lds, tw, threadIdx.x);
         P::xi_store_w(lds, mesh, yb, z, threadIdx.x);
     } else {
-        P::xi_pre(lds, tw, threadIdx.x);
-        __syncthreads();
         P::EX::template run<+1, 2>(lds, tw, threadIdx.x);
         P::xi_store(lds, mesh, yb, z, threadIdx.x);
     }

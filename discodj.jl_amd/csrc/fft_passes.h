@@ -151,36 +151,38 @@ struct Passes {
     }
 
     // ------------------------------------------------------------------ XI: A -> mesh
-    DJ_HD static void xi_load(const cx<T> *A, cx<T> *lds, int yb, int z, int tid) {
-        copy_batched<cx<T>, B * H, NT>(
-            tid,
-            [&](int i) {
-                const int kx = i / B, l = i - kx * B;
-                return A[(size_t)(yb * B + l) + (size_t)N * (kx + (size_t)H * z)];
-            },
-            [&](int i, cx<T> v) {
-                const int kx = i / B, l = i - kx * B;
-                lds[EX::idx(l, kx)] = v;
-            });
-    }
-    // Z[k] = (X[k] + conj X[NC-k]) + i (X[k] - conj X[NC-k]) W_N^-k, for k and NC-k together;
-    // the imaginary parts of X[0] and X[NC] are dropped (c2r of a Hermitian line)
-    DJ_HD static void xi_pre(cx<T> *lds, const cx<T> *twN, int tid) {
-        constexpr int HP = NC / 2 + 1;
-        for (int i = tid; i < B * HP; i += NT) {
-            int l = i / HP, k = i - l * HP;
-            cx<T> a = lds[EX::idx(l, k)], b = lds[EX::idx(l, NC - k)];
+    // The load fused with the packed-real pre-processing of c2r: work item (k, l), k <= NC/2, reads
+    // X[k] and X[NC-k] of line l and writes Z[k] = (X[k] + conj X[NC-k]) + i (X[k] - conj X[NC-k]) W_N^-k
+    // and Z[NC-k] likewise; the imaginary parts of X[0] and X[NC] are dropped (c2r of a Hermitian line).
+    // All reads of a thread are issued before its first LDS write.
+    DJ_HD static void xi_load_pre(const cx<T> *A, cx<T> *lds, const cx<T> *twN, int yb, int z, int tid) {
+        constexpr int HP = NC / 2 + 1, TOTAL = B * HP, CNT = (TOTAL + NT - 1) / NT;
+        cx<T> va[CNT], vb[CNT];
+#pragma unroll
+        for (int c = 0; c < CNT; c++) {
+            const int i = tid + c * NT;
+            if (TOTAL % NT == 0 || i < TOTAL) {
+                const int k = i / B, l = i - k * B;
+                const size_t row = (size_t)(yb * B + l) + (size_t)N * (size_t)H * z;
+                va[c] = A[row + (size_t)N * k];
+                vb[c] = A[row + (size_t)N * (NC - k)];
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < CNT; c++) {
+            const int i = tid + c * NT;
+            if (!(TOTAL % NT == 0 || i < TOTAL)) continue;
+            const int k = i / B, l = i - k * B;
+            cx<T> a = va[c], b = vb[c];
             if (k == 0) { a.y = 0; b.y = 0; }
-            // Z[k]
-            cx<T> e = a + conj(b);
-            cx<T> o = cmulc(a - conj(b), twN[k]);
-            cx<T> zk = {e.x - o.y, e.y + o.x};
+            const cx<T> e = a + conj(b);
+            const cx<T> o = cmulc(a - conj(b), twN[k]);
+            lds[EX::idx(l, k)] = {e.x - o.y, e.y + o.x};
             if (k != 0 && 2 * k != NC) {
-                cx<T> e2 = b + conj(a);
-                cx<T> o2 = cmulc(b - conj(a), twN[NC - k]);
+                const cx<T> e2 = b + conj(a);
+                const cx<T> o2 = cmulc(b - conj(a), twN[NC - k]);
                 lds[EX::idx(l, NC - k)] = {e2.x - o2.y, e2.y + o2.x};
             }
-            lds[EX::idx(l, k)] = zk;
         }
     }
     DJ_HD static void xi_store(const cx<T> *lds, T *mesh, int yb, int z, int tid) {
@@ -288,28 +290,6 @@ struct Passes {
                 const int l = i / NC, m = i - l * NC;
                 lds[EX::idx(l, m)] = v;
             });
-    }
-    DJ_HD static void xi_pre_w(cx<T> *lds, const cx<T> *twN, int tid) {
-        constexpr int HP = NC / 2 + 1, E = EX::LW * HP, CNT = (E + 63) / 64;
-        const int wave = tid >> 6, lane = tid & 63;
-#pragma unroll
-        for (int c = 0; c < CNT; c++) {
-            const int il = lane + 64 * c;
-            if (!(E % 64 == 0 || il < E)) continue;
-            const int i = wave * E + il;
-            const int l = i / HP, k = i - l * HP;
-            cx<T> a = lds[EX::idx(l, k)], b = lds[EX::idx(l, NC - k)];
-            if (k == 0) { a.y = 0; b.y = 0; }
-            cx<T> e = a + conj(b);
-            cx<T> o = cmulc(a - conj(b), twN[k]);
-            cx<T> zk = {e.x - o.y, e.y + o.x};
-            if (k != 0 && 2 * k != NC) {
-                cx<T> e2 = b + conj(a);
-                cx<T> o2 = cmulc(b - conj(a), twN[NC - k]);
-                lds[EX::idx(l, NC - k)] = {e2.x - o2.y, e2.y + o2.x};
-            }
-            lds[EX::idx(l, k)] = zk;
-        }
     }
     DJ_HD static void xi_store_w(const cx<T> *lds, T *mesh, int yb, int z, int tid) {
         const size_t base = (size_t)N * N * z + (size_t)N * yb * B;
