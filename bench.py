@@ -133,6 +133,9 @@ def main():
             kernels[name] = {"calls": s["calls"], "ms_per_call": round(ms, 4), "ms_total": round(s["ms_total"], 3),
                              "algorithmic_GB": round(s["bytes_per_call"] / 1e9, 4), "achieved_GBs": round(gbs, 1),
                              "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    if "scatter" in kernels:
+        # unit-mass atomic adds of one deposit: W^3 per particle (TSC: 27), as the reference's scatter makes
+        kernels["scatter"]["atomic_adds_per_s"] = 27 * n**3 / (kernels["scatter"]["ms_per_call"] * 1e-3)
     dom = max(kernels, key=lambda k: kernels[k]["ms_total"])
     roof = {"kernel": dom, "bound": "hbm", "achieved": kernels[dom]["achieved_GBs"], "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": kernels[dom]["frac"], "traffic": None,
