@@ -105,15 +105,21 @@ __device__ __forceinline__ int hash_slot(unsigned *keys, unsigned key) {
     return -1;  // table full: the caller falls back to a global atomic
 }
 
-// tiles of one particle: own first, then the distinct ghosts; returns the count (<= 8).  The stencil
-// cells of every shift along an axis lie within c0 + [rlo, rhi], a span shorter than the tile, so the
-// axis touches at most the two tiles {tile(c0 + rlo), tile(c0 + rhi)}, one of which is the own tile;
-// the ghosts are the other combinations of those per-axis tiles.  The cells are computed with the
-// arithmetic of the deposit (same X, same shift, same base rule).
-template <typename T, int W>
-__device__ __forceinline__ int particle_tiles(const T X[3], const TileGeom &g, const double *shifts, int ns, T L,
-                                              unsigned *tiles) {
+// Tiles of one particle.  The stencil cells of every shift along an axis lie within c0 + [rlo, rhi],
+// a span shorter than the tile, so the axis touches at most the two tiles {tile(c0 + rlo),
+// tile(c0 + rhi)}.  The particle's ghost tiles are the combinations of those per-axis tiles other
+// than its own tile (that of c0, whose own list it always joins, even when a shift moves every
+// stencil cell out of it): combination c (0 <= c < 8) takes th along the axes whose bit of c is set
+// (x = 1, y = 2, z = 4) and tl along the others, and exists when th != tl on each set axis.  The
+// cells are computed with the arithmetic of the deposit (same X, same shift, same base rule).
+struct PartTiles {
     int tl[3], th[3], to[3];
+};
+
+template <typename T, int W>
+__device__ __forceinline__ PartTiles particle_tiles(const T X[3], const TileGeom &g, const double *shifts, int ns,
+                                                   T L) {
+    PartTiles P;
     for (int a = 0; a < 3; a++) {
         const T g0 = tjmod(X[a], L) / L * (T)g.N;
         const int c0 = wrapn((int)floor(g0), g.N);
@@ -127,21 +133,33 @@ __device__ __forceinline__ int particle_tiles(const T X[3], const TileGeom &g, c
             rlo = min(rlo, d + twlo<W>());
             rhi = max(rhi, d + twlo<W>() + W - 1);
         }
-        to[a] = c0 >> g.lT[a];
-        tl[a] = wrapn(c0 + rlo, g.N) >> g.lT[a];
-        th[a] = wrapn(c0 + rhi, g.N) >> g.lT[a];
+        P.to[a] = c0 >> g.lT[a];
+        P.tl[a] = wrapn(c0 + rlo, g.N) >> g.lT[a];
+        P.th[a] = wrapn(c0 + rhi, g.N) >> g.lT[a];
     }
-    int n = 0;
-    tiles[n++] = (unsigned)(to[0] + g.ntx * (to[1] + g.nty * to[2]));
-    const int nz = th[2] != tl[2] ? 2 : 1, ny = th[1] != tl[1] ? 2 : 1, nx = th[0] != tl[0] ? 2 : 1;
-    for (int cz = 0; cz < nz; cz++)
-        for (int cy = 0; cy < ny; cy++)
-            for (int cx = 0; cx < nx; cx++) {
-                const int tx = cx ? th[0] : tl[0], ty = cy ? th[1] : tl[1], tz = cz ? th[2] : tl[2];
-                if (tx == to[0] && ty == to[1] && tz == to[2]) continue;
-                tiles[n++] = (unsigned)(tx + g.ntx * (ty + g.nty * tz));
-            }
-    return n;
+    return P;
+}
+
+// list keys (tile * 2, + 1 for a ghost list): key[8] is the own tile's, key[c] that of ghost
+// combination c, or EMPTY when the combination does not exist or is the own tile
+template <int C>
+__device__ __forceinline__ unsigned combo_key(const PartTiles &P, const TileGeom &g) {
+    if ((C & 1) && P.th[0] == P.tl[0]) return EMPTY;
+    if ((C & 2) && P.th[1] == P.tl[1]) return EMPTY;
+    if ((C & 4) && P.th[2] == P.tl[2]) return EMPTY;
+    const int tx = (C & 1) ? P.th[0] : P.tl[0], ty = (C & 2) ? P.th[1] : P.tl[1], tz = (C & 4) ? P.th[2] : P.tl[2];
+    if (tx == P.to[0] && ty == P.to[1] && tz == P.to[2]) return EMPTY;
+    return (unsigned)(tx + g.ntx * (ty + g.nty * tz)) * 2u + 1u;
+}
+
+template <int C = 0>
+__device__ __forceinline__ void combo_keys(const PartTiles &P, const TileGeom &g, unsigned (&key)[9]) {
+    if constexpr (C < 8) {
+        key[C] = combo_key<C>(P, g);
+        combo_keys<C + 1>(P, g, key);
+    } else {
+        key[8] = (unsigned)(P.to[0] + g.ntx * (P.to[1] + g.nty * P.to[2])) * 2u;
+    }
 }
 
 template <typename T, int W, bool FILL>
@@ -157,33 +175,39 @@ __global__ __launch_bounds__(256) void k_tile_bin(const T *__restrict__ psi, con
     const int i = blockIdx.x * 16 + (threadIdx.x & 15), j = blockIdx.y * 4 + ((threadIdx.x >> 4) & 3),
               k = blockIdx.z * 4 + (threadIdx.x >> 6);
     const bool ok = i < n && j < n && k < n;
-    unsigned tiles[8];
-    int slot[8], rank[8], nt = 0;
+    // the particle's list keys (every index below is a compile-time constant once the loops are
+    // unrolled, so the arrays stay in registers without indexed selects)
+    unsigned key[9];
+    int slot[9], rank[9];
     T X[3];
     size_t p = 0;
+#pragma unroll
+    for (int c = 0; c < 9; c++) key[c] = EMPTY;
     if (ok) {
         const size_t np = (size_t)n * n * n;
         p = (size_t)i + (size_t)n * ((size_t)j + (size_t)n * k);
         X[0] = psi[p] + q1[i];
         X[1] = psi[p + np] + q1[j];
         X[2] = psi[p + 2 * np] + q1[k];
-        nt = particle_tiles<T, W>(X, g, shifts, ns, L, tiles);
+        combo_keys(particle_tiles<T, W>(X, g, shifts, ns, L), g, key);
     }
-    // Wave-aggregated ranks: per list position m, the lanes that share a key form one group; its
+    // Wave-aggregated ranks: per key position c, the lanes that share a key form one group; its
     // first lane makes the hash lookup and one LDS atomic for the whole group, and each lane's rank
     // is the group base plus its position in the group.  A wave's particles thus stay contiguous
     // (in Lagrangian order) within their tile's list.
     const int lane = __lane_id();
     const unsigned long long below = (1ull << lane) - 1ull;
 #pragma unroll
-    for (int m = 0; m < 8; m++) {
-        const bool act = m < nt;
-        const unsigned key = act ? tiles[m] * 2u + (m > 0 ? 1u : 0u) : EMPTY;
+    for (int c = 0; c < 9; c++) {
+        const bool act = key[c] != EMPTY;
+        slot[c] = -1;
+        rank[c] = -1;
         unsigned long long pend = __ballot(act);
         while (pend) {
+            // the leader is wave-uniform: its values reach every lane by a scalar read of its lane
             const int leader = __ffsll((long long)pend) - 1;
-            const unsigned lk = (unsigned)__shfl((int)key, leader);
-            const bool mine = act && key == lk;
+            const unsigned lk = (unsigned)__builtin_amdgcn_readlane((int)key[c], leader);
+            const bool mine = act && key[c] == lk;
             const unsigned long long grp = __ballot(mine);
             int s = -1;
             unsigned b = 0;
@@ -191,11 +215,11 @@ __global__ __launch_bounds__(256) void k_tile_bin(const T *__restrict__ psi, con
                 s = hash_slot(keys, lk);
                 if (s >= 0) b = atomicAdd(&vals[s], (unsigned)__popcll(grp));
             }
-            s = __shfl(s, leader);
-            b = (unsigned)__shfl((int)b, leader);
+            s = __builtin_amdgcn_readlane(s, leader);
+            b = (unsigned)__builtin_amdgcn_readlane((int)b, leader);
             if (mine) {
-                slot[m] = s;
-                rank[m] = s >= 0 ? (int)(b + (unsigned)__popcll(grp & below)) : -1;
+                slot[c] = s;
+                rank[c] = s >= 0 ? (int)(b + (unsigned)__popcll(grp & below)) : -1;
             }
             pend &= ~grp;
         }
@@ -205,9 +229,9 @@ __global__ __launch_bounds__(256) void k_tile_bin(const T *__restrict__ psi, con
     // group's first entry and its list's end (own list [start[t], gstart[t]), ghost list
     // [gstart[t], start[t + 1]))
     for (int s = threadIdx.x; s < HASH; s += 256) {
-        unsigned key = keys[s];
-        if (key == EMPTY) continue;
-        unsigned t = key >> 1, gh = key & 1u;
+        unsigned kk = keys[s];
+        if (kk == EMPTY) continue;
+        unsigned t = kk >> 1, gh = kk & 1u;
         if (!FILL) {
             atomicAdd(&cnt[gh * g.nt + t], vals[s]);
         } else {
@@ -219,23 +243,26 @@ __global__ __launch_bounds__(256) void k_tile_bin(const T *__restrict__ psi, con
     }
     if (!FILL) {
         // overflow pairs (table full) count directly
-        if (ok)
-            for (int m = 0; m < nt; m++)
-                if (slot[m] < 0) atomicAdd(&cnt[(m > 0 ? g.nt : 0) + tiles[m]], 1u);
+#pragma unroll
+        for (int c = 0; c < 9; c++)
+            if (key[c] != EMPTY && slot[c] < 0) atomicAdd(&cnt[(key[c] & 1u) * g.nt + (key[c] >> 1)], 1u);
         return;
     }
     __syncthreads();
     bool over = false;
-    for (int m = 0; m < nt; m++) {
-        const unsigned t = tiles[m];
+#pragma unroll
+    for (int c = 0; c < 9; c++) {
+        if (key[c] == EMPTY) continue;
+        const unsigned t = key[c] >> 1;
+        const bool gh = key[c] & 1u;
         unsigned at, end;
-        if (slot[m] >= 0) {
-            at = base[slot[m]] + (unsigned)rank[m];
-            end = lend[slot[m]];
+        if (slot[c] >= 0) {
+            at = base[slot[c]] + (unsigned)rank[c];
+            end = lend[slot[c]];
         } else {  // pair that did not fit the workgroup's hash table
-            const unsigned r = atomicAdd(&cur[(m > 0 ? g.nt : 0) + t], 1u);
-            at = (m > 0 ? gstart[t] : start[t]) + r;
-            end = m > 0 ? start[t + 1] : gstart[t];
+            const unsigned r = atomicAdd(&cur[(gh ? g.nt : 0) + t], 1u);
+            at = (gh ? gstart[t] : start[t]) + r;
+            end = gh ? start[t + 1] : gstart[t];
         }
         if (at < end && at < ent_cap) {
             Entry<T> e;
