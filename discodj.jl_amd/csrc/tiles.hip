@@ -740,14 +740,17 @@ __device__ __forceinline__ void gather_fetch(const Entry<T> *__restrict__ ent, u
 // neighbouring tiles share is fetched into that XCD's L2 once.
 __device__ __forceinline__ int xcd_tile(int b, int nt) { return (nt & 7) ? b : (b >> 3) + (b & 7) * (nt >> 3); }
 
-// Gather order: within each XCD's run of workgroups, tiles go brick by brick, 2 x 4 x 8 tiles
-// (64^3 cells for 32 x 16 x 8 tiles, about as many workgroups as an XCD runs at once), x fastest
-// inside a brick.  The staged regions of a tile reach its neighbours along every axis; in a brick
-// the neighbours along z are in flight together too, and their shared halo planes are fetched into
-// the XCD's L2 once instead of once per tile.  Tile counts that the bricks do not divide keep the
-// row order.
+// Gather order: tiles go brick by brick, 2 x 4 x 8 tiles (64^3 cells for 32 x 16 x 8 tiles, about
+// as many workgroups as an XCD runs at once), x fastest inside a brick, and each brick on one XCD.
+// The staged regions of a tile reach its neighbours along every axis; in a brick the neighbours
+// along z are in flight together too, and their shared halo planes are fetched into the XCD's L2
+// once instead of once per tile.  When the brick count is a multiple of 8, XCD x takes bricks x,
+// x + 8, x + 16, ... spread over the whole mesh, so that clustered regions (long tile lists) load
+// every XCD alike; otherwise each XCD takes a contiguous eighth of the tiles.  Tile counts that the
+// bricks do not divide keep the row order.
 constexpr int BRICK_X = 2, BRICK_Y = 4, BRICK_Z = 8;
 __device__ __forceinline__ void gather_tile(int b, const TileGeom &g, int &tx, int &ty, int &tz) {
+    constexpr int BV = BRICK_X * BRICK_Y * BRICK_Z;
     const int loc = xcd_tile(b, g.nt);
     if (g.ntx % BRICK_X || g.nty % BRICK_Y || g.ntz % BRICK_Z) {
         tx = loc % g.ntx;
@@ -755,8 +758,14 @@ __device__ __forceinline__ void gather_tile(int b, const TileGeom &g, int &tx, i
         tz = loc / (g.ntx * g.nty);
         return;
     }
-    constexpr int BV = BRICK_X * BRICK_Y * BRICK_Z;
-    const int w = loc % BV, br = loc / BV;
+    int w = loc % BV, br = loc / BV;
+    if ((g.nt / BV) % 8 == 0) {
+        // XCD b % 8 takes bricks b % 8, b % 8 + 8, ...: its k-th workgroup (k = b / 8) takes tile
+        // k % BV of its brick k / BV, so that every XCD's bricks spread over the whole mesh
+        const int k = b >> 3;
+        w = k % BV;
+        br = (k / BV) * 8 + (b & 7);
+    }
     const int nbx = g.ntx / BRICK_X, nby = g.nty / BRICK_Y;
     tx = (br % nbx) * BRICK_X + w % BRICK_X;
     ty = ((br / nbx) % nby) * BRICK_Y + (w / BRICK_X) % BRICK_Y;
