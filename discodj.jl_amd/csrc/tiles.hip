@@ -511,7 +511,7 @@ __global__ __launch_bounds__(512) void k_tile_deposit(const Entry<T> *__restrict
 // per tile in three padded LDS accumulators of T (the layout of the padded deposit above), stored as the
 // tile's cells: no device atomics, no zero fill.  Every axis of the mesh holds at least two tiles.
 template <typename T, int W>
-__global__ __launch_bounds__(256) void k_tile_deposit3(const Entry<T> *__restrict__ ent, const unsigned *__restrict__ start,
+__global__ __launch_bounds__(512) void k_tile_deposit3(const Entry<T> *__restrict__ ent, const unsigned *__restrict__ start,
                                                        const unsigned *__restrict__ gstart, const unsigned *__restrict__ cnt,
                                                        TileGeom g, T sx, T sy, T sz, T L, const T *__restrict__ vals,
                                                        size_t np, T vscale, T *__restrict__ out) {
@@ -1021,7 +1021,7 @@ struct TileLaunch {
         const size_t shm = 3 * (size_t)(g.TX + 2 * (W - 1)) * (g.TY * g.TZ) * sizeof(T);
         (void)hipFuncSetAttribute((const void *)k_tile_deposit3<T, W>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)shm);
-        k_tile_deposit3<T, W><<<g.nt, 256, shm, st>>>((const Entry<T> *)a.ent, a.start, a.gstart, a.cur, g, (T)sh[0],
+        k_tile_deposit3<T, W><<<g.nt, 512, shm, st>>>((const Entry<T> *)a.ent, a.start, a.gstart, a.cur, g, (T)sh[0],
                                                       (T)sh[1], (T)sh[2], (T)a.L, (const T *)vals,
                                                       (size_t)a.n * a.n * a.n, (T)scale, (T *)out);
     }
