@@ -9,14 +9,20 @@ workload's cosmology, 2LPT at z = 19, generated on the device (outside the timed
 particle state is resident in HBM when the timed region starts.
 
 Reported beside the throughput:
-  accuracy       max |P32(k)/P64(k) - 1| for kf <= k < k_Nyq/2 and rms(Psi32 - Psi64)/(L/n) of the
-                 FP32 run against the same run in FP64 (same kernels, same ICs)
+  timing         median of 3 runs of the 20 steps from the initial conditions (state reset between
+                 runs), after a warm-up run of --warmup steps; --steps K < 20 times the first K steps
+  accuracy       max |P32(k)/P64(k) - 1| for kf/2 <= k < k_Nyq/2 and rms(Psi32 - Psi64)/(L/n) of the
+                 FP32 run against the CPU oracle's FP64 run of the same workload from the same ICs
+                 (tests/golden/pm512_mesh1024_fastpm20_fp64.npz), and the force at the ICs against it
   roofline       the dominant kernel's algorithmic bytes per launch / its mean launch time (HIP
-                 events on the library's stream over the timed region) against 8 TB/s
+                 events on the library's stream, in one more run of the same steps) against 8 TB/s
   cpu_baseline   the CPU oracle (the C restatement of the reference path, FP32, OpenMP) on the host
-                 cores: one interlacing pass of the same force, the step counted as two passes
+                 cores: the force of one step (two interlacing passes) at the ICs; the GPU force at
+                 the ICs is compared with it
+  extra lines    256^3 particles on a 256^3 mesh, BullFrog 10 (particle-updates/s) and the density-loss
+                 gradient at 256^3 (gradients/s)
 
-usage: python bench.py [--gpus 1] [--steps 19] [--warmup 1] [--no-accuracy] [--no-cpu] [--small]
+usage: python bench.py [--gpus 1] [--steps 20] [--warmup 2] [--no-accuracy] [--no-cpu] [--no-extra] [--small]
        python bench.py --workload gradient|learned      (the differentiation and learned-correction
                                                         workloads at 256^3; one JSON line each)
 """
@@ -40,10 +46,11 @@ def log(msg):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=19)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the 256^3 throughput and gradient lines")
     ap.add_argument("--small", action="store_true", help="128^3 particles on a 256^3 mesh (quick check)")
     ap.add_argument("--workload", choices=["pm", "gradient", "learned"], default="pm")
     ap.add_argument("--copy-rate", action="store_true", help="only time a 1 GiB read + 1 GiB write copy kernel")
@@ -89,42 +96,58 @@ def main():
 
     import discodj_amd as D
 
-    metric = "PM particle-updates/s at 512\u00b3 on 1024\u00b3 mesh, \u22641e-3 P(k) err vs FP64; %HBM roofline"
+    metric = "PM particle-updates/s at 512³ on 1024³ mesh, ≤1e-3 P(k) err vs FP64; %HBM roofline"
     n, N, L = (128, 256, 250.0) if args.small else (512, 1024, 1000.0)
-    a_ini, a_end, seed = 0.05, 1.0, 42
+    a_ini, a_end, seed, n_steps = 0.05, 1.0, 42, 20
     W, K = args.warmup, args.steps
-    total = max(20, W + K)
+    if not 1 <= K <= n_steps or W < 0:
+        print(f"--steps must be in 1..{n_steps} (the workload is the {n_steps}-step FastPM schedule from z = 19 "
+              f"to 0; a timed run is its first K steps from the initial conditions) and --warmup >= 0",
+              file=sys.stderr)
+        return 2
     cfg = D.PMConfig(res_pm=N, n_part=n, boxsize=L, worder=3, antialias=1, grad_order=6, lap_order=6)
     c = D.compute_timetables(D.Cosmology(**COSMO))
     k_tab, pk_tab = D.linear_power_spectrum(c)
-    co = D.stepper_coefficients(c, "fastpm", a_ini, a_end, total, "D")
+    co = D.stepper_coefficients(c, "fastpm", a_ini, a_end, n_steps, "D")
     Dg = c.Dplus(a_ini)
     coef = [co["ddrift1"], co["alpha"], co["beta"], co["ddrift2"]]
 
-    log(f"workload {n}^3 particles, {N}^3 mesh, L={L}, FastPM {total} steps; ICs on the device")
+    log(f"workload {n}^3 particles, {N}^3 mesh, L={L}, FastPM {n_steps} steps; ICs on the device")
     S = D.PMSolverHIP(cfg, precision=32)
     t = time.perf_counter()
     S.ics_lpt(k_tab, pk_tab, Dg, order=2, seed=seed)
+    S.synchronize()
     log(f"2LPT ICs in {time.perf_counter() - t:.1f}s")
-    ic_psi, ic_pi = S.get_state() if not args.no_cpu else (None, None)
+    ic_psi, ic_pi = S.get_state()
 
+    # warm-up: the first W steps of the schedule, then back to the initial conditions; each timed run
+    # steps the schedule's first K steps (all of them by default) from the initial conditions
     if W:
         S.run(*(a[:W] for a in coef))
-    S.synchronize()
+        S.synchronize()
+    times = []
+    for rep in range(3):
+        S.set_state(ic_psi, ic_pi)
+        S.synchronize()
+        t0 = time.perf_counter()
+        S.run(*(a[:K] for a in coef))
+        S.synchronize()
+        times.append(time.perf_counter() - t0)
+        log(f"timed run {rep}: {K} steps in {times[-1]:.4f} s ({times[-1] / K * 1e3:.2f} ms/step)")
+    dt = float(np.median(times))
+    value = n**3 * K / dt
+    log(f"median {dt / K * 1e3:.2f} ms/step, {value:.3e} particle-updates/s")
+    psi32 = S.get_state()[0] if K == n_steps else None
+
+    # per-kernel times: one more run of the same steps with HIP events around every launch (on the
+    # library's stream), outside the timed runs
+    S.set_state(ic_psi, ic_pi)
     S.set_profiling(True)
     S.reset_stats()
-    log(f"timing {K} steps after {W} warm-up steps")
-    t0 = time.perf_counter()
-    S.run(*(a[W:W + K] for a in coef))
+    S.run(*(a[:K] for a in coef))
     S.synchronize()
-    dt = time.perf_counter() - t0
     stats = S.kernel_stats()
     S.set_profiling(False)
-    value = n**3 * K / dt
-    log(f"{dt / K * 1e3:.2f} ms/step, {value:.3e} particle-updates/s")
-    if W + K < total:
-        S.run(*(a[W + K:] for a in coef))
-
     kernels = {}
     for name, s in stats.items():
         if s["calls"]:
@@ -140,14 +163,15 @@ def main():
     roof = {"kernel": dom, "bound": "hbm", "achieved": kernels[dom]["achieved_GBs"], "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": kernels[dom]["frac"], "traffic": None,
             "algorithmic_bytes_per_launch": int(kernels[dom]["algorithmic_GB"] * 1e9)}
-    # HBM bytes per launch of the same kernel, measured by rocprofv3 FETCH_SIZE / WRITE_SIZE passes over
-    # this workload and scaled by a copy kernel of known size (profiles/pm_step_c3_counters.md)
+    # HBM bytes per launch of the same kernel: not measured in this run, but taken from rocprofv3
+    # FETCH_SIZE / WRITE_SIZE passes over this workload, scaled by a copy kernel of known size
     tpath = os.path.join(ROOT, "profiles", "pm_step_c3_traffic.json")
     if not args.small and os.path.exists(tpath):
         tk = json.load(open(tpath))["kernels"].get(dom)
         if tk:
             roof["traffic"] = int(tk["traffic_bytes"])
-            roof["traffic_source"] = "profiles/pm_step_c3_counters.md (" + tk["rocprof_kernel"] + ")"
+            roof["traffic_source"] = ("rocprofv3 counters of a separate profiling run of this workload, "
+                                      "profiles/pm_step_c3_counters.md (" + tk["rocprof_kernel"] + ")")
 
     # a plain streaming kernel of known size (1 GiB read + 1 GiB written, torch.neg) on the same device,
     # in a child process (torch brings its own HIP runtime): the bandwidth a kernel of this chip reaches
@@ -162,26 +186,11 @@ def main():
         log(f"streaming copy {roof['copy_GBs']} GB/s")
 
     accuracy = None
+    acc_ini32 = None
     if not args.no_accuracy:
-        psi32, _ = S.get_state()
-        S.close()
-        log("FP64 run of the same workload for the accuracy reference")
-        S64 = D.PMSolverHIP(cfg, precision=64)
-        S64.ics_lpt(k_tab, pk_tab, Dg, order=2, seed=seed)
-        S64.run(*coef)
-        psi64, _ = S64.get_state()
-        k, p64, nm = S64.power_spectrum(W=2)
-        _, p32, _ = S64.power_spectrum(psi=psi32.astype(np.float64), W=2)
-        S64.close()
-        ok = nm > 0
-        rel = np.abs(p32[ok] / p64[ok] - 1)
-        disp = float(np.sqrt(np.mean((psi32.astype(np.float64) - psi64) ** 2)) / (L / n))
-        accuracy = {"pk_max_rel_err": float(rel.max()), "pk_k_range": [float(k[ok][0]), float(k[ok][-1])],
-                    "pk_bins": int(ok.sum()), "disp_rms_over_spacing": disp, "reference": "same run in FP64"}
-        log(f"P(k) max rel err {rel.max():.2e}, displacement rms / spacing {disp:.2e}")
-        del psi32, psi64
-    else:
-        S.close()
+        acc_ini32 = S.pm_acceleration(ic_psi)
+        accuracy = accuracy_vs_oracle(np, S, psi32, acc_ini32, c, n, N, L) if not args.small else None
+    S.close()
 
     cpu = None
     if not args.no_cpu:
@@ -189,32 +198,103 @@ def main():
         import oracle as O
 
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        ocfg = O.pm_config(res_pm=N, n_part=n, boxsize=L, worder=3, antialias=0, grad_order=6, lap_order=6)
-        log(f"CPU oracle: one force pass on {threads} threads")
+        ocfg = O.pm_config(res_pm=N, n_part=n, boxsize=L, worder=3, antialias=1, grad_order=6, lap_order=6)
+        log(f"CPU oracle: the FP32 force of the workload (two interlacing passes) at the ICs on {threads} threads")
         t = time.perf_counter()
-        O.pm_acceleration(ic_psi, ocfg, "solver")
-        t_pass = time.perf_counter() - t
-        cpu = {"value": n**3 / (2 * t_pass), "unit": "particle-updates/s", "cores": threads, "kind": "port",
-               "sample": f"one interlacing pass (deposit, FFT Poisson solve, FD gradient, gather) of the "
-                         f"{n}^3 / {N}^3 FP32 force at the z=19 ICs, {t_pass:.1f} s; a step is two passes"}
-        log(f"CPU pass {t_pass:.1f}s -> {cpu['value']:.3e} particle-updates/s")
+        acc_cpu = O.pm_acceleration(ic_psi, ocfg, "solver")
+        t_force = time.perf_counter() - t
+        cpu = {"value": n**3 / t_force, "unit": "particle-updates/s", "cores": threads, "kind": "port",
+               "sample": f"the force of one step (two interlacing passes: deposit, FFT Poisson solve, FD "
+                         f"gradient, gather) of the {n}^3 / {N}^3 FP32 workload at the z=19 ICs, {t_force:.1f} s; "
+                         f"a step's kick/drift is not counted"}
+        log(f"CPU force {t_force:.1f}s -> {cpu['value']:.3e} particle-updates/s")
+        if acc_ini32 is not None:
+            e = float(np.max(np.abs(acc_ini32.astype(np.float64) - acc_cpu)) / np.max(np.abs(acc_cpu)))
+            cpu["gpu_force_vs_this_force"] = e
+            if accuracy is not None:
+                accuracy["force_ini_fp32_vs_oracle_fp32"] = e
+            log(f"GPU FP32 force at the ICs vs the oracle's FP32 force: max rel err {e:.2e}")
+        del acc_cpu
+
+    extra = {}
+    if not args.small and not args.no_extra:
+        extra["pm256_mesh256_bullfrog10"] = bench_pm256(np, D, c, k_tab, pk_tab)
+        extra["gradient256"] = bench_gradient(args, as_dict=True)
 
     out = {"metric": metric, "value": value, "unit": "particle-updates/s", "n_gpus": 1, "steps": K, "warmup": W,
            "ms_per_step": dt / K * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f32", "data": "synthetic",
            "config": {"workload": f"PM N-body, {n}^3 particles on a {N}^3 mesh, L={L} Mpc/h, TSC, interlacing "
-                                  f"(2 passes), grad/Laplace order 6, FastPM {total} steps z=19->0, 2LPT ICs "
+                                  f"(2 passes), grad/Laplace order 6, FastPM {n_steps} steps z=19->0, 2LPT ICs "
                                   f"(N-GenIC seed {seed}), FP32 state and mesh, FP64 coefficients",
-                      "particles": n**3, "mesh": N**3, "boxsize": L, "stepper": "fastpm", "n_steps": total},
+                      "particles": n**3, "mesh": N**3, "boxsize": L, "stepper": "fastpm", "n_steps": n_steps,
+                      "timed_steps": K, "timing": f"median of 3 runs of the first {K} steps from the ICs, after "
+                                                  f"a {W}-step warm-up run",
+                      "run_seconds": [round(x, 5) for x in times]},
            "roofline": roof, "kernels": kernels, "accuracy": accuracy, "cpu_baseline": cpu}
+    out.update(extra)
     print(json.dumps(out), flush=True)
     return 0
 
 
-def bench_gradient(args):
+REF512 = os.path.join(ROOT, "tests", "golden", "pm512_mesh1024_fastpm20_fp64.npz")
+
+
+def accuracy_vs_oracle(np, S, psi32, acc_ini32, c, n, N, L):
+    """The FP32 device run against the CPU oracle's FP64 run of the same workload (same ICs: seed 42, 2LPT),
+    stored by tests/golden/make_pm512_mesh1024_reference.py: P(k) of the final positions (CIC, deconvolved,
+    255 linear bins kf/2 .. k_Nyq/2), the displacement and the force at the ICs on the stored subsample."""
+    if psi32 is None or not os.path.exists(REF512):
+        log("no oracle reference for the accuracy (partial run or missing fixture)")
+        return None
+    ref = np.load(REF512)
+    s = int(ref["sub"])
+    sub = (slice(None),) + (slice(None, None, s),) * 3
+    _, p32, nm = S.power_spectrum(psi=psi32, W=2)
+    ok = ref["nmodes"] > 0
+    rel = np.abs(p32[ok] / ref["pk"][ok] - 1)
+    disp = float(np.sqrt(np.mean((psi32[sub].astype(np.float64) - ref["psi_fin"]) ** 2)) / (L / n))
+    e_acc = float(np.max(np.abs(acc_ini32[sub].astype(np.float64) - ref["acc_ini"])) / np.max(np.abs(ref["acc_ini"])))
+    log(f"vs the oracle's FP64 run: P(k) max rel err {rel.max():.2e}, displacement rms / spacing {disp:.2e}, "
+        f"force at the ICs {e_acc:.2e}")
+    return {"pk_max_rel_err": float(rel.max()), "pk_bins": int(ok.sum()),
+            "pk_k_range": [float(ref["k"][ok][0]), float(ref["k"][ok][-1])],
+            "disp_rms_over_spacing": disp, "disp_sample": f"{(n // s) ** 3} particles (every {s}th along each axis)",
+            "force_ini_max_rel_err": e_acc,
+            "reference": "CPU oracle (FP64) run of the same workload from the same ICs, "
+                         "tests/golden/pm512_mesh1024_fastpm20_fp64.npz"}
+
+
+def bench_pm256(np, D, c, k_tab, pk_tab):
+    """256^3 particles on a 256^3 mesh, L = 500, CIC, interlacing, gradient order 4, BullFrog 10 (FP32):
+    particle-updates/s over the 10 steps from the 2LPT ICs of seed 42, median of 3 after a warm-up run."""
+    n, L = 256, 500.0
+    cfg = D.PMConfig(res_pm=n, n_part=n, boxsize=L, worder=2, antialias=1, grad_order=4)
+    co = D.stepper_coefficients(c, "bullfrog", 0.05, 1.0, 10, "D")
+    coef = [co["ddrift1"], co["alpha"], co["beta"], co["ddrift2"]]
+    with D.PMSolverHIP(cfg, precision=32) as S:
+        S.ics_lpt(k_tab, pk_tab, c.Dplus(0.05), order=2, seed=42)
+        p0, m0 = S.get_state()
+        times = []
+        for rep in range(4):
+            S.set_state(p0, m0)
+            S.synchronize()
+            t0 = time.perf_counter()
+            S.run(*coef)
+            S.synchronize()
+            if rep:
+                times.append(time.perf_counter() - t0)
+    dt = float(np.median(times))
+    log(f"256^3 / 256^3 BullFrog 10: {dt / 10 * 1e3:.3f} ms/step")
+    return {"value": n**3 * 10 / dt, "unit": "particle-updates/s", "ms_per_step": dt / 10 * 1e3,
+            "workload": "256^3 particles, 256^3 mesh, L=500, CIC, interlacing, grad order 4, BullFrog 10, FP32",
+            "timing": "median of 3 runs of the 10 steps from the ICs, after a warm-up run"}
+
+
+def bench_gradient(args, as_dict=False):
     """Gradient of sum (delta - target)^2 on the 256^3 mesh with respect to the 256^3 white noise through
     2LPT and 10 BullFrog CIC steps (grad order 4), FP32: seconds per gradient, split into the forward run
-    with step checkpoints and the reverse pass."""
+    with step checkpoints and the reverse pass (as_dict: returned instead of printed)."""
     import numpy as np
 
     import discodj_amd as D
@@ -245,11 +325,14 @@ def bench_gradient(args):
             times.append((t1 - t0, t2 - t1))
             log(f"gradient {rep}: forward {t1 - t0:.3f}s, reverse {t2 - t1:.3f}s, |wbar| {np.linalg.norm(wb):.3e}")
     fw, bw = np.median([t[0] for t in times]), np.median([t[1] for t in times])
-    print(json.dumps({"metric": "gradients/s of the density loss w.r.t. the white noise (2LPT + 10 BullFrog steps)",
-                      "value": 1.0 / (fw + bw), "unit": "gradients/s", "forward_s": fw, "reverse_s": bw,
-                      "dtype": "f32", "data": "synthetic", "n_gpus": 1,
-                      "config": {"workload": f"{n}^3 particles, {n}^3 mesh, L={L}, CIC, grad order 4, BullFrog 10",
-                                 "median_of": 3}}), flush=True)
+    out = {"metric": "gradients/s of the density loss w.r.t. the white noise (2LPT + 10 BullFrog steps)",
+           "value": 1.0 / (fw + bw), "unit": "gradients/s", "forward_s": fw, "reverse_s": bw,
+           "dtype": "f32", "data": "synthetic", "n_gpus": 1,
+           "config": {"workload": f"{n}^3 particles, {n}^3 mesh, L={L}, CIC, grad order 4, BullFrog 10",
+                      "median_of": 3}}
+    if as_dict:
+        return out
+    print(json.dumps(out), flush=True)
     return 0
 
 

@@ -540,11 +540,21 @@ static int SFX(acc_single)(const REAL *Xs, const orc_pm_config *cfg, const SFX(k
         }
         free(Xr);
     }
-    CPLX *fd = (CPLX *)malloc(sizeof(CPLX) * H), *cb = (CPLX *)malloc(sizeof(CPLX) * H);
+    /* One force component at a time (F holds one mesh): the per-particle sums of each component are
+       the same additions in the same order as in a joint three-component loop, so the result does
+       not depend on this, and the 512^3 / 1024^3 FP64 force fits in ~50 GB of host memory. */
+    CPLX *fd = (CPLX *)malloc(sizeof(CPLX) * H);
+    CPLX *cb = cfg->grad_order == 0 ? (CPLX *)malloc(sizeof(CPLX) * H) : fd;
     SFX(orc_rfft3)(res, mesh, (REAL *)fd);
-    REAL *F = (REAL *)malloc(sizeof(REAL) * 3 * M), *phi = mesh;
-    if (cfg->grad_order == 0) {
-        for (int ax = 0; ax < 3; ax++) {
+    REAL *F = (REAL *)malloc(sizeof(REAL) * M), *phi = mesh;
+    REAL c[3];
+    if (cfg->grad_order != 0) {
+        for (size_t i = 0; i < H; i++) cb[i] = fd[i] * Kp[i];
+        SFX(orc_brfft3)(res, (const REAL *)cb, phi);
+        SFX(fd_coeffs)(cfg->grad_order, cfg->boxsize / res, c);
+    }
+    for (int ax = 0; ax < 3; ax++) {
+        if (cfg->grad_order == 0) {
             for (int z = 0; z < nh; z++)
                 for (int y = 0; y < res; y++)
                     for (int x = 0; x < res; x++) {
@@ -552,32 +562,28 @@ static int SFX(acc_single)(const REAL *Xs, const orc_pm_config *cfg, const SFX(k
                         CPLX g = ax == 0 ? K->g[0][x] : ax == 1 ? K->g[1][y] : K->g[2][z];
                         cb[i] = -((g * Kp[i]) * fd[i]);
                     }
-            SFX(orc_brfft3)(res, (const REAL *)cb, phi);
-            memcpy(F + ax * M, phi, sizeof(REAL) * M);
+            SFX(orc_brfft3)(res, (const REAL *)cb, F);
+        } else {
+            SFX(fdgrad1)(F, phi, res, ax, c);
         }
-    } else {
-        for (size_t i = 0; i < H; i++) cb[i] = fd[i] * Kp[i];
-        SFX(orc_brfft3)(res, (const REAL *)cb, phi);
-        REAL c[3];
-        SFX(fd_coeffs)(cfg->grad_order, cfg->boxsize / res, c);
-        for (int ax = 0; ax < 3; ax++) SFX(fdgrad1)(F + ax * M, phi, res, ax, c);
-    }
-    /* gather3 (pm.jl:344-364) */
+        /* gather3 (pm.jl:344-364), component ax */
 #pragma omp parallel for schedule(static)
-    for (int64_t p = 0; p < np; p++) {
-        SFX(stencil) st;
-        SFX(make_stencil)(&st, Xs[p], Xs[p + np], Xs[p + 2 * np], res, L, W);
-        REAL s1 = 0, s2 = 0, s3 = 0;
-        for (int oz = 0; oz < W; oz++)
-            for (int oy = 0; oy < W; oy++)
-                for (int ox = 0; ox < W; ox++) {
-                    size_t cell = st.cell[0][ox] + R * (st.cell[1][oy] + R * st.cell[2][oz]);
-                    REAL w = st.w[0][ox] * st.w[1][oy] * st.w[2][oz];
-                    s1 += F[cell] * w; s2 += F[cell + M] * w; s3 += F[cell + 2 * M] * w;
-                }
-        acc[p] += s1; acc[p + np] += s2; acc[p + 2 * np] += s3;
+        for (int64_t p = 0; p < np; p++) {
+            SFX(stencil) st;
+            SFX(make_stencil)(&st, Xs[p], Xs[p + np], Xs[p + 2 * np], res, L, W);
+            REAL s1 = 0;
+            for (int oz = 0; oz < W; oz++)
+                for (int oy = 0; oy < W; oy++)
+                    for (int ox = 0; ox < W; ox++) {
+                        size_t cell = st.cell[0][ox] + R * (st.cell[1][oy] + R * st.cell[2][oz]);
+                        REAL w = st.w[0][ox] * st.w[1][oy] * st.w[2][oz];
+                        s1 += F[cell] * w;
+                    }
+            acc[p + ax * np] += s1;
+        }
     }
-    free(mesh); free(fd); free(cb); free(F);
+    free(mesh); free(fd); free(F);
+    if (cb != fd) free(cb);
     return 0;
 }
 
@@ -593,6 +599,8 @@ int SFX(orc_pm_acceleration)(const orc_pm_config *cfg, const REAL *psi, REAL *ac
     int ns = SFX(aa_shifts)(cfg->antialias, 0.5 * cfg->boxsize / cfg->res_pm, sh);
     if (ns < 0) { SFX(kernels_free)(&K); return -1; }
     REAL *Kp = SFX(kphi)(&K, cfg, cfg->n_resample * cfg->n_resample * cfg->n_resample);
+    free(K.invlap); K.invlap = NULL;
+    free(K.mak); K.mak = NULL;
     REAL *q1 = (REAL *)malloc(sizeof(REAL) * n);
     SFX(lagr1d)(n, cfg->boxsize, q1);
     REAL *X = (REAL *)malloc(sizeof(REAL) * 3 * np), *Xs = (REAL *)malloc(sizeof(REAL) * 3 * np);

@@ -1,0 +1,238 @@
+"""The benchmark workloads at their own sizes on the device, against the CPU oracle.
+
+  64^3 particles on a 64^3 mesh, L = 250, CIC, gradient order 2, BullFrog 10 (FP64 and FP32)
+  256^3 particles on a 256^3 mesh, L = 500, CIC, interlacing, gradient order 4, BullFrog 10 (FP32 against
+      the FP64 oracle; the fixed-point deposit bit-reproducible across runs)
+  the density-loss gradient at 256^3 (2LPT + 10 BullFrog CIC steps) against central differences
+  the fitted transfer function at 256^3 particles on a 256^3 mesh against the 1024^3-mesh run
+  512^3 particles on a 1024^3 mesh, TSC, interlacing, order 6, FastPM 20 (FP32) against the oracle's FP64
+      run of the same workload, committed as a fixture (tests/golden/make_pm512_mesh1024_reference.py)
+
+All runs start from the 2LPT initial conditions of the N-GenIC white noise of seed 42 (Eisenstein-Hu
+P(k), Omega_c = 0.2621, Omega_b = 0.049, h = 0.6766, sigma8 = 0.8102, n_s = 0.9665) at a = 0.05 and end
+at a = 1 (time variable D).  FP32 results are judged as in the reference's studies: P(k) of the final
+positions (CIC, window-deconvolved, linear bins of width kf to k_Nyq/2, the oracle's estimator) and the
+displacement rms relative to the mean interparticle spacing L / n.
+"""
+import functools
+import os
+
+import numpy as np
+import pytest
+
+import golden as G
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+D = pytest.importorskip("discodj_amd")
+
+COSMO = (0.2621, 0.049, 0.6766, 0.8102, 0.9665)
+A_INI, A_END, SEED = 0.05, 1.0, 42
+
+
+@functools.lru_cache(maxsize=None)
+def _oc():
+    oc = O.Cosmology(*COSMO)
+    k, pk = O.linear_power_spectrum(oc)
+    return oc, k, pk
+
+
+def _dc():
+    return D.compute_timetables(D.Cosmology(Omega_c=COSMO[0], Omega_b=COSMO[1], h=COSMO[2], sigma8=COSMO[3],
+                                            n_s=COSMO[4]))
+
+
+@functools.lru_cache(maxsize=None)
+def _oracle_ics(n, L):
+    oc, k, pk = _oc()
+    fphi = O.white_noise_to_fphi(O.ic_operator(n, L, k, pk), O.ngenic_wnoise_real(SEED, n))
+    psi1, psi2 = O.compute_core2(fphi, L)
+    return O.nbody_ics_lpt2(oc, psi1, psi2, A_INI)
+
+
+@functools.lru_cache(maxsize=None)
+def _oracle_run(n, N, L, W, aa, grad, lap, stepper, steps):
+    oc, _, _ = _oc()
+    psi, pi = _oracle_ics(n, L)
+    cfg = O.pm_config(N, n, L, worder=W, antialias=aa, grad_order=grad, lap_order=lap)
+    return O.run_nbody(oc, psi, pi, cfg, A_INI, A_END, steps, stepper, "D")
+
+
+def _pk(psi, N, L):
+    """P(k) of q + psi (CIC on the N^3 mesh, deconvolved, linear bins to k_Nyq/2) with the oracle."""
+    n = psi.shape[1]
+    mesh = O.pm_scatter(O.lagrangian_positions(np.asarray(psi, np.float64), L), N, L, 2)
+    mesh *= N**3 / n**3
+    mesh -= 1.0
+    return O.power_spectrum_lin(mesh, L, W=2, nbins=N // 4 - 1)
+
+
+def _device_run(cfg, precision, stepper, steps, deposit="float", ics=None):
+    """Device ICs (seed 42, FP64 on the device, rounded to the precision) or the given ones, then the
+    steps: -> (Psi, P canonical, the ICs' Psi, Pi)."""
+    oc, k, pk = _oc()
+    c = _dc()
+    co = D.stepper_coefficients(c, stepper, A_INI, A_END, steps, "D")
+    with D.PMSolverHIP(cfg, precision=precision, deposit=deposit) as S:
+        if ics is None:
+            S.ics_lpt(k, pk, oc.Dplus(A_INI), order=2, seed=SEED)
+        else:
+            S.set_state(*ics)
+        p0, m0 = S.get_state()
+        S.run(co["ddrift1"], co["alpha"], co["beta"], co["ddrift2"])
+        psi, mom = S.get_state()
+    return psi, mom.astype(np.float64) * c.Fplus(A_END), p0, m0
+
+
+def _fp32_errors(psi32, psi64, N, L):
+    n = psi64.shape[1]
+    _, p64, nm = _pk(psi64, N, L)
+    _, p32, _ = _pk(psi32, N, L)
+    ok = nm > 0
+    pk_err = float(np.max(np.abs(p32[ok] / p64[ok] - 1)))
+    disp = float(np.sqrt(np.mean((np.asarray(psi32, np.float64) - psi64) ** 2)) / (L / n))
+    return pk_err, disp
+
+
+# ------------------------------------------------------------------ 64^3 on 64^3, BullFrog 10
+def test_pm64_mesh64_bullfrog_fp64_and_fp32_vs_oracle():
+    n, N, L = 64, 64, 250.0
+    cfg = D.PMConfig(res_pm=N, n_part=n, boxsize=L, worder=2, grad_order=2)
+    ref_psi, ref_p = _oracle_run(n, N, L, 2, 0, 2, 0, "bullfrog", 10)
+    psi_ics, pi_ics = _oracle_ics(n, L)
+    psi, P, p0, m0 = _device_run(cfg, 64, "bullfrog", 10)
+    print(f"FP64: ICs {G.rel(p0, psi_ics):.2e} / {G.rel(m0, pi_ics):.2e}, final Psi {G.rel(psi, ref_psi):.2e}, "
+          f"P {G.rel(P, ref_p):.2e}")
+    assert G.rel(p0, psi_ics) < 1e-12 and G.rel(m0, pi_ics) < 1e-12
+    assert G.rel(psi, ref_psi) < 1e-11 and G.rel(P, ref_p) < 1e-11
+    psi32, _, _, _ = _device_run(cfg, 32, "bullfrog", 10)
+    pk_err, disp = _fp32_errors(psi32, ref_psi, N, L)
+    print(f"FP32: P(k) max rel err {pk_err:.2e}, displacement rms / spacing {disp:.2e}")
+    assert pk_err < 1e-3 and disp < 1e-3
+
+
+# ------------------------------------------------------------------ 256^3 on 256^3, BullFrog 10
+def _cfg256(**kw):
+    return D.PMConfig(res_pm=256, n_part=256, boxsize=500.0, worder=2, antialias=1, grad_order=4, **kw)
+
+
+def test_pm256_mesh256_bullfrog_fp32_vs_fp64_oracle():
+    n, N, L = 256, 256, 500.0
+    ref_psi, ref_p = _oracle_run(n, N, L, 2, 1, 4, 0, "bullfrog", 10)
+    psi_ics, pi_ics = _oracle_ics(n, L)
+    psi, P, p0, m0 = _device_run(_cfg256(), 64, "bullfrog", 10)
+    e_psi, e_p = G.rel(psi, ref_psi), G.rel(P, ref_p)
+    print(f"FP64: ICs {G.rel(p0, psi_ics):.2e} / {G.rel(m0, pi_ics):.2e}, final Psi {e_psi:.2e}, P {e_p:.2e}")
+    assert G.rel(p0, psi_ics) < 1e-12 and G.rel(m0, pi_ics) < 1e-12
+    assert e_psi < 1e-10 and e_p < 1e-10
+    del psi, P, p0, m0
+    psi32, _, _, _ = _device_run(_cfg256(), 32, "bullfrog", 10)
+    pk_err, disp = _fp32_errors(psi32, ref_psi, N, L)
+    print(f"FP32: P(k) max rel err {pk_err:.2e}, displacement rms / spacing {disp:.2e}")
+    assert pk_err < 1e-3 and disp < 1e-3
+
+
+def test_pm256_mesh256_fixed64_runs_bit_identical():
+    outs = [_device_run(_cfg256(), 32, "bullfrog", 10, deposit="fixed64") for _ in range(2)]
+    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+    ref_psi, _ = _oracle_run(256, 256, 500.0, 2, 1, 4, 0, "bullfrog", 10)
+    pk_err, disp = _fp32_errors(outs[0][0], ref_psi, 256, 500.0)
+    print(f"fixed-point FP32: P(k) max rel err {pk_err:.2e}, displacement rms / spacing {disp:.2e}")
+    assert pk_err < 1e-3 and disp < 1e-3
+
+
+# ------------------------------------------------------------------ density-loss gradient at 256^3
+def test_density_loss_gradient_256_central_differences():
+    """Gradient of sum (delta - target)^2 on the 256^3 mesh with respect to the 256^3 white noise through
+    2LPT and 10 BullFrog CIC steps (grad order 4), FP64, against central differences of the whole pipeline
+    along 4 random directions; the central differences use the fixed-point deposit, whose runs are
+    deterministic, so that the difference of two losses is not blurred by the order of float additions.
+    The FP32 gradient against the FP64 one."""
+    n, L = 256, 500.0
+    c = _dc()
+    _, k, pk = _oc()
+    white = O.ngenic_wnoise_real(SEED, n)
+    target = np.zeros((n, n, n))
+    kw = dict(boxsize=L, res_pm=n, n_steps=10, stepper="bullfrog", order=2, worder=2, grad_kernel_order=4,
+              k_tab=k, pk_tab=pk)
+    cfg = D.PMConfig(res_pm=n, n_part=n, boxsize=L, worder=2, grad_order=4)
+    loss, wb, _ = D.density_loss_gradient(c, white, target, precision=64, **kw)
+    co = D.stepper_coefficients(c, "bullfrog", A_INI, A_END, 10, "D")
+    frac = 2.0**-32
+
+    with D.PMSolverHIP(cfg, precision=64, deposit="fixed64") as S:
+        def lossf(w):
+            S.ics_lpt(k, pk, c.Dplus(A_INI), order=2, white=w)
+            S.run(co["ddrift1"], co["alpha"], co["beta"], co["ddrift2"])
+            delta = S.deposit().astype(np.float64) * frac - 1.0
+            return float(np.sum(delta * delta))
+
+        l0 = lossf(white)
+        assert abs(l0 - loss) < 1e-8 * loss, (l0, loss)
+        rng = np.random.default_rng(11)
+        eps = 1e-5
+        for i in range(4):
+            v = rng.normal(size=(n, n, n))
+            fd = (lossf(white + eps * v) - lossf(white - eps * v)) / (2 * eps)
+            g = float(np.sum(wb * v))
+            print(f"direction {i}: gradient {g:.8e}, central difference {fd:.8e}, rel {abs(g - fd) / abs(fd):.2e}")
+            assert abs(g - fd) < 1e-4 * abs(fd), (i, g, fd)
+    _, wb32, _ = D.density_loss_gradient(c, white, target, precision=32, **kw)
+    e32 = G.rel(wb32, wb)
+    print(f"FP32 gradient against FP64: {e32:.2e}")
+    assert e32 < 1e-2
+
+
+# ------------------------------------------------------- fitted transfer function, 256^3 vs 1024^3 mesh
+def test_transfer_function_256_against_1024_mesh():
+    """The learned-correction workload at its own size: 256^3 particles on a 256^3 mesh (window
+    deconvolution) against the same ICs on a 1024^3 mesh, BullFrog 10; the transfer function of the
+    Poisson kernel fitted on seeds 1000 and 1001 brings the held-out seed 42 within 2 % of the fine
+    mesh's P(k) to k_Nyq/2 (plain PM is off by about 11 %)."""
+    c = _dc()
+    work = D.learned.Workload(c, n=256, N=256, L=500.0, ratio=4, n_steps=10, deconvolve=True)
+    plain = D.learned.Workload(c, n=256, N=256, L=500.0, ratio=4, n_steps=10)
+    fine = D.learned.fine_states(work, [1000, 1001, SEED])
+    g, kmax = D.learned.fit_transfer(work, [1000, 1001], psi_fine=fine[:2])
+    pk_plain, _, _ = D.learned.power_spectrum_errors(plain, SEED, psi_fine=fine[2])
+    pk_dec, pk_corr, _ = D.learned.power_spectrum_errors(work, SEED, kgain=(g, kmax), psi_fine=fine[2])
+    print(f"max |P/P_fine - 1| to k_Nyq/2: plain {pk_plain:.4f}, deconvolved {pk_dec:.4f}, "
+          f"fitted transfer function {pk_corr:.4f}")
+    assert pk_corr < 0.02 < pk_plain
+
+
+# ------------------------------------------------- 512^3 on 1024^3, FastPM 20, against the oracle's FP64 run
+REF512 = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "pm512_mesh1024_fastpm20_fp64.npz")
+
+
+@pytest.mark.skipif(not os.path.exists(REF512), reason="the oracle's FP64 512^3 / 1024^3 run is not committed")
+def test_pm512_mesh1024_fastpm_fp32_vs_oracle_fp64_run():
+    ref = np.load(REF512)
+    s = int(ref["sub"])
+    sub = (slice(None),) + (slice(None, None, s),) * 3
+    n, N, L = 512, 1024, 1000.0
+    oc, k, pk = _oc()
+    c = _dc()
+    co = D.stepper_coefficients(c, "fastpm", A_INI, A_END, 20, "D")
+    cfg = D.PMConfig(res_pm=N, n_part=n, boxsize=L, worder=3, antialias=1, grad_order=6, lap_order=6)
+    with D.PMSolverHIP(cfg, precision=32) as S:
+        S.ics_lpt(k, pk, oc.Dplus(A_INI), order=2, seed=SEED)
+        p0, m0 = S.get_state()
+        e_ics = G.rel(p0[sub], ref["psi_ini"]), G.rel(m0[sub], ref["pi_ini"])
+        acc = S.pm_acceleration(p0)
+        e_acc = G.rel(acc[sub], ref["acc_ini"])
+        del p0, m0, acc
+        S.run(co["ddrift1"], co["alpha"], co["beta"], co["ddrift2"])
+        kk, p32, nm = S.power_spectrum(W=2)
+        psi, mom = S.get_state()
+    ok = ref["nmodes"] > 0
+    assert np.array_equal(nm, ref["nmodes"])
+    pk_err = float(np.max(np.abs(p32[ok] / ref["pk"][ok] - 1)))
+    disp = float(np.sqrt(np.mean((psi[sub].astype(np.float64) - ref["psi_fin"]) ** 2)) / (L / n))
+    e_p = G.rel(mom[sub].astype(np.float64) * c.Fplus(A_END), ref["p_fin"])
+    print(f"ICs {e_ics[0]:.2e} / {e_ics[1]:.2e}, force at the ICs {e_acc:.2e}, final P(k) {pk_err:.2e}, "
+          f"displacement rms / spacing {disp:.2e}, momentum {e_p:.2e}")
+    assert e_ics[0] < 1e-6 and e_ics[1] < 1e-6
+    assert e_acc < 2e-4
+    assert pk_err < 1e-3 and disp < 1e-3
