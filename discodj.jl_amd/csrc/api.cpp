@@ -83,6 +83,7 @@ struct dj_ctx {
     AdjointOps adj{};
     void *vb_psi1 = nullptr, *vb_abar = nullptr, *vb_psibar = nullptr, *vb_pibar = nullptr;
     void *vb_fbar = nullptr, *vb_pbar = nullptr;
+    bool vb_resident = false;  // vb_psibar / vb_pibar hold cotangents left resident by dj_vjp_deposit / dj_vjp_run
     void *ckpt = nullptr;  // (steps, 2, 3 n^3): Psi and Pi before each step
     int ck_steps = 0;
     std::vector<double> ck_dd1, ck_alpha, ck_beta, ck_dd2;
@@ -98,6 +99,11 @@ struct dj_ctx {
     void *nn_dphi = nullptr, *nn_dens = nullptr, *nn_tgt = nullptr, *nn_sum = nullptr;
     void *acc_list = nullptr;  // (entries, 3): the acceleration between interlacing passes, in tile-list order
     size_t acc_list_cap = 0;
+    // binned spectra: bin sums and device copies of the outputs (allocated on first use)
+    double *pk_work = nullptr;
+    // transforms on the particle grid (n != res_pm) for the displacement spectrum (allocated on first use)
+    FftOps pg_fft{};
+    void *pg_tw = nullptr, *pg_A = nullptr, *pg_B = nullptr;
     std::string err;
     bool prof = false;
     std::vector<ProfEvent> pending;
@@ -320,7 +326,8 @@ void free_all(dj_ctx *c) {
                     c->nn_p, c->nn_g, c->nn_m, c->nn_v, c->nn_mask, c->nn_wf, c->nn_wb, c->nn_pf, c->nn_pb,
                     c->nn_t[0], c->nn_t[1],
                     c->nn_t[2], c->nn_t[3], c->nn_t[4], c->nn_t[5], c->nn_t[6], c->nn_t[7], c->nn_t[8], c->nn_t[9],
-                    c->nn_dphi, c->nn_dens, c->nn_tgt, c->nn_sum, c->acc_list, c->kgain};
+                    c->nn_dphi, c->nn_dens, c->nn_tgt, c->nn_sum, c->acc_list, c->kgain, c->pk_work,
+                    c->pg_tw, c->pg_A, c->pg_B};
     // teardown: failures here have no caller to report to
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -777,6 +784,9 @@ dj_status dj_create(const dj_pm_config *cfg, dj_ctx **out) {
     };
     hipError_t e = hipSetDevice(c->dev);
     if (e != hipSuccess) { hip_fail(c, e, "hipSetDevice"); return bail(DJ_ERR_DEVICE); }
+    // the device's copy of the allocation-free spectrum's scratch array (once per device)
+    e = pk_scratch_init(c->dev);
+    if (e != hipSuccess) { hip_fail(c, e, "hipGetSymbolAddress(scratch)"); return bail(DJ_ERR_DEVICE); }
     e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
     if (e != hipSuccess) { hip_fail(c, e, "hipStreamCreate"); return bail(DJ_ERR_DEVICE); }
     r = c->tsize == 8 ? setup_ops<double>(c) : setup_ops<float>(c);
@@ -1045,6 +1055,7 @@ dj_status dj_vjp_acceleration(dj_ctx *c, const void *psi, const void *acc_bar, v
         src = c->vb_psi1;
     }
     HIPCHK(c, hipMemcpyAsync(c->vb_abar, acc_bar, pb, hipMemcpyHostToDevice, c->st));
+    c->vb_resident = false;
     HIPCHK(c, hipMemsetAsync(c->vb_psibar, 0, pb, c->st));
     r = vjp_force(c, src, c->vb_abar, c->vb_psibar);
     if (r) return r;
@@ -1065,6 +1076,7 @@ dj_status dj_vjp_deposit(dj_ctx *c, const void *psi, const void *mesh_bar, void 
         src = c->vb_psi1;
     }
     HIPCHK(c, hipMemcpyAsync(c->vb_pbar, mesh_bar, c->M * c->tsize, hipMemcpyHostToDevice, c->st));
+    c->vb_resident = false;
     HIPCHK(c, hipMemsetAsync(c->vb_psibar, 0, pb, c->st));
     const double zero[3] = {0, 0, 0};
     c->adj.gather_grad(src, c->q1, c->n, c->res, c->cfg.boxsize, zero, c->vb_pbar, nullptr, nullptr, 0.0, c->fdc,
@@ -1077,6 +1089,7 @@ dj_status dj_vjp_deposit(dj_ctx *c, const void *psi, const void *mesh_bar, void 
         HIPCHK(c, hipMemsetAsync(c->vb_pibar, 0, pb, c->st));  // resident cotangents (Psibar, Pibar = 0)
     }
     HIPCHK(c, hipStreamSynchronize(c->st));
+    c->vb_resident = !psi_bar_out;
     return DJ_OK;
 }
 
@@ -1084,6 +1097,9 @@ dj_status dj_run_checkpointed(dj_ctx *c, int32_t n_steps, const double *dd1, con
                               const double *dd2) {
     if (!c || n_steps < 1 || !dd1 || !alpha || !beta || !dd2)
         return fail(c, DJ_ERR_INVALID_ARGUMENT, "invalid run arguments");
+    // options without a reverse pass are refused before the checkpoints are allocated and the run made
+    dj_status ra = adj_supported(c);
+    if (ra) return ra;
     HIPCHK(c, hipSetDevice(c->dev));
     const size_t pb = (size_t)c->n3 * c->tsize;
     if (c->ckpt && c->ck_steps < n_steps) {
@@ -1113,6 +1129,8 @@ dj_status dj_run_checkpointed(dj_ctx *c, int32_t n_steps, const double *dd1, con
 dj_status dj_vjp_run(dj_ctx *c, void *psi_bar, void *pi_bar) {
     if (!c || (!psi_bar) != (!pi_bar)) return fail(c, DJ_ERR_INVALID_ARGUMENT, "psi_bar and pi_bar: both or neither");
     if (!c->ckpt || c->ck_steps < 1) return fail(c, DJ_ERR_INVALID_ARGUMENT, "no checkpointed run to reverse");
+    if (!psi_bar && !c->vb_resident)
+        return fail(c, DJ_ERR_INVALID_ARGUMENT, "no resident cotangents (dj_vjp_deposit with psi_bar_out NULL first)");
     dj_status r = adj_supported(c);
     if (r) return r;
     HIPCHK(c, hipSetDevice(c->dev));
@@ -1140,6 +1158,7 @@ dj_status dj_vjp_run(dj_ctx *c, void *psi_bar, void *pi_bar) {
         HIPCHK(c, hipMemcpyAsync(pi_bar, c->vb_pibar, pb, hipMemcpyDeviceToHost, c->st));
     }
     HIPCHK(c, hipStreamSynchronize(c->st));
+    c->vb_resident = !psi_bar;
     return DJ_OK;
 }
 
@@ -1149,6 +1168,8 @@ dj_status dj_vjp_ics_lpt(dj_ctx *c, uint32_t seed, const double *white, const do
     if (!c || !k_tab || !pk_tab || ntab < 2 || (order != 1 && order != 2) || (!psi_bar) != (!pi_bar) ||
         !white_bar_out)
         return fail(c, DJ_ERR_INVALID_ARGUMENT, "invalid IC adjoint arguments");
+    if (!psi_bar && !c->vb_resident)
+        return fail(c, DJ_ERR_INVALID_ARGUMENT, "no resident cotangents (dj_vjp_deposit / dj_vjp_run with NULL first)");
     const int n = c->n;
     FftOps probe{};
     if (n % 2 || !fft_dispatch<double>(n, probe) || (order == 2 && !fft_dispatch<double>(3 * n / 2, probe)))
@@ -1160,6 +1181,7 @@ dj_status dj_vjp_ics_lpt(dj_ctx *c, uint32_t seed, const double *white, const do
     if (psi_bar) {
         HIPCHK(c, hipMemcpyAsync(c->vb_psibar, psi_bar, pb, hipMemcpyHostToDevice, c->st));
         HIPCHK(c, hipMemcpyAsync(c->vb_pibar, pi_bar, pb, hipMemcpyHostToDevice, c->st));
+        c->vb_resident = false;
     }
     HIPCHK(c, hipStreamSynchronize(c->st));
     std::vector<double> field;
@@ -1469,6 +1491,197 @@ dj_status dj_power_spectrum(dj_ctx *c, const void *psi, int32_t W, int32_t nbins
         pk_out[b] = h[nbins + b] > 0 ? h[b] / h[nbins + b] : 0.0;
         if (nmodes_out) nmodes_out[b] = (int64_t)hc[b];
     }
+    return DJ_OK;
+}
+
+// ------------------------------------------------------------------------ binned spectra
+static_assert(DJ_PK_MAX_BINS == PK_MAX_BINS, "the header's bin limit is the kernels'");
+extern "C++" {
+namespace {
+// evaluate_power_spectrum's bins (power_spectrum.jl:46-51): nbins + 1 edges between kf = 2 pi / L and
+// k_Nyq sqrt(3), k_Nyq = pi N / L, as LinRange ((1 - t) a + t b, t = j / nbins) of k or of log10 k
+PkEdges reference_edges(int nbins, int logarithmic, int N, double L) {
+    PkEdges ed{};
+    ed.nbins = nbins;
+    const double dk = 2 * M_PI / L, kny = M_PI * N / L, kmin = dk, kmax = kny * std::sqrt(3.0);
+    for (int b = 0; b <= nbins; b++) {
+        const double t = (double)b / nbins;
+        ed.e[b] = logarithmic ? exp10((1 - t) * std::log10(kmin) + t * std::log10(kmax)) : (1 - t) * kmin + t * kmax;
+    }
+    return ed;
+}
+
+dj_status pk_work(dj_ctx *c) {
+    // [0, 4B) bin sums, [4B, 8B) k, P, mode counts, a second P; [8B, 12B) a second set of bin sums
+    if (!c->pk_work) HIPCHK(c, hipMalloc(&c->pk_work, sizeof(double) * 12 * DJ_PK_MAX_BINS));
+    return DJ_OK;
+}
+
+// rfft of a real (res^3) device field of the context's precision into the spectrum S (x-halved B layout)
+void field_fft(dj_ctx *c, const void *field, void *S) {
+    c->fft.xf(field, false, c->A, c->tw, 1.0, c->st);
+    c->fft.yf(c->A, S, c->tw, c->st);
+    c->fft.zm(ZM_NONE, true, false, S, S, c->tw, c->kmult, c->st);
+}
+
+void bin(dj_ctx *c, const void *F1, const void *F2, int N, int W, int wmode, double scale, const PkEdges &ed,
+         double *sums) {
+    if (c->tsize == 8) pk_edges<double>(F1, F2, N, c->cfg.boxsize, W, wmode, scale, ed, sums, c->st);
+    else pk_edges<float>(F1, F2, N, c->cfg.boxsize, W, wmode, scale, ed, sums, c->st);
+}
+
+// outputs of the bins (device pk_work areas) to the host
+dj_status pk_download(dj_ctx *c, int nbins, double *k_out, double *pk_out, int64_t *nm_out, double *pk2_out) {
+    const size_t B = DJ_PK_MAX_BINS;
+    double *w = c->pk_work;
+    if (k_out) HIPCHK(c, hipMemcpyAsync(k_out, w + 4 * B, nbins * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    if (pk_out) HIPCHK(c, hipMemcpyAsync(pk_out, w + 5 * B, nbins * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    if (nm_out) HIPCHK(c, hipMemcpyAsync(nm_out, w + 6 * B, nbins * sizeof(int64_t), hipMemcpyDeviceToHost, c->st));
+    if (pk2_out) HIPCHK(c, hipMemcpyAsync(pk2_out, w + 7 * B, nbins * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return check_launch(c, "binned spectrum");
+}
+
+template <typename T>
+dj_status setup_pgrid(dj_ctx *c) {
+    if (c->pg_A) return DJ_OK;
+    const int n = c->n;
+    if (!fft_dispatch<T>(n, c->pg_fft))
+        return fail(c, DJ_ERR_UNSUPPORTED, "the displacement spectrum needs n_part in {16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512, 768, 1024}");
+    std::vector<cx<T>> tw(n);
+    for (int k = 0; k < n; k++) {
+        const double a = -2.0 * M_PI * (double)k / (double)n;
+        tw[k] = {(T)std::cos(a), (T)std::sin(a)};
+    }
+    const size_t hc = (size_t)n * n * (n / 2 + 1);
+    HIPCHK(c, hipMalloc(&c->pg_tw, sizeof(cx<T>) * n));
+    HIPCHK(c, hipMemcpy(c->pg_tw, tw.data(), sizeof(cx<T>) * n, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMalloc(&c->pg_B, hc * sizeof(cx<T>)));
+    HIPCHK(c, hipMalloc(&c->pg_A, hc * sizeof(cx<T>)));
+    return DJ_OK;
+}
+}  // namespace
+}  // extern "C++"
+
+dj_status dj_field_power_spectrum(dj_ctx *c, const void *field, int32_t nbins, int32_t logarithmic, double *k_out,
+                                  double *pk_out, int64_t *nmodes_out) {
+    if (!c || !field || !k_out || !pk_out || nbins < 1 || nbins > DJ_PK_MAX_BINS)
+        return fail(c, DJ_ERR_INVALID_ARGUMENT, "invalid power spectrum arguments (1 <= nbins <= 256)");
+    HIPCHK(c, hipSetDevice(c->dev));
+    dj_status r = pk_work(c);
+    if (r) return r;
+    const int N = c->res;
+    const PkEdges ed = reference_edges(nbins, logarithmic, N, c->cfg.boxsize);
+    const double V = c->cfg.boxsize * c->cfg.boxsize * c->cfg.boxsize, n3 = (double)N * N * N;
+    HIPCHK(c, hipMemcpyAsync(c->mesh, field, c->M * c->tsize, hipMemcpyHostToDevice, c->st));
+    field_fft(c, c->mesh, c->B);
+    const size_t B = DJ_PK_MAX_BINS;
+    zero_doubles(c->pk_work, 4 * B, c->st);
+    bin(c, c->B, nullptr, N, 0, 0, V / (n3 * n3), ed, c->pk_work);
+    pk_finalize(c->pk_work, ed, 1.0, 1, false, c->pk_work + 4 * B, c->pk_work + 5 * B, (int64_t *)(c->pk_work + 6 * B),
+                c->st);
+    return pk_download(c, nbins, k_out, pk_out, nmodes_out, nullptr);
+}
+
+dj_status dj_field_cross_power_spectrum(dj_ctx *c, const void *field1, const void *field2, int32_t nbins,
+                                        int32_t logarithmic, double *k_out, double *pk_out) {
+    if (!c || !field1 || !field2 || !k_out || !pk_out || nbins < 1 || nbins > DJ_PK_MAX_BINS)
+        return fail(c, DJ_ERR_INVALID_ARGUMENT, "invalid cross power spectrum arguments (1 <= nbins <= 256)");
+    HIPCHK(c, hipSetDevice(c->dev));
+    dj_status r = pk_work(c);
+    if (r) return r;
+    if (!c->C) HIPCHK(c, hipMalloc(&c->C, c->HC * 2 * c->tsize));
+    const int N = c->res;
+    const PkEdges ed = reference_edges(nbins, logarithmic, N, c->cfg.boxsize);
+    const double V = c->cfg.boxsize * c->cfg.boxsize * c->cfg.boxsize, n3 = (double)N * N * N;
+    HIPCHK(c, hipMemcpyAsync(c->mesh, field1, c->M * c->tsize, hipMemcpyHostToDevice, c->st));
+    field_fft(c, c->mesh, c->B);
+    HIPCHK(c, hipMemcpyAsync(c->mesh, field2, c->M * c->tsize, hipMemcpyHostToDevice, c->st));
+    field_fft(c, c->mesh, c->C);
+    const size_t B = DJ_PK_MAX_BINS;
+    zero_doubles(c->pk_work, 4 * B, c->st);
+    bin(c, c->B, c->C, N, 0, 1, V / (n3 * n3), ed, c->pk_work);
+    pk_finalize(c->pk_work, ed, 1.0, 1, false, c->pk_work + 4 * B, c->pk_work + 5 * B, nullptr, c->st);
+    return pk_download(c, nbins, k_out, pk_out, nullptr, nullptr);
+}
+
+dj_status dj_power_spectra_binned(dj_ctx *c, const void *psi, int32_t nbins, const double *edges, double *k_out,
+                                  double *pk_delta_out, double *pk_psi_out) {
+    if (!c || !edges || nbins < 1 || nbins > DJ_PK_MAX_BINS || (!pk_delta_out && !pk_psi_out))
+        return fail(c, DJ_ERR_INVALID_ARGUMENT, "invalid binned spectrum arguments (1 <= nbins <= 256)");
+    PkEdges ed{};
+    ed.nbins = nbins;
+    for (int b = 0; b <= nbins; b++) {
+        ed.e[b] = edges[b];
+        if (!std::isfinite(edges[b]) || (b && !(edges[b] > edges[b - 1])))
+            return fail(c, DJ_ERR_INVALID_ARGUMENT, "bin edges must be finite and ascending");
+    }
+    HIPCHK(c, hipSetDevice(c->dev));
+    dj_status r = pk_work(c);
+    if (r) return r;
+    const void *src = c->psi;
+    if (psi) {
+        if (!c->psi_in) HIPCHK(c, hipMalloc(&c->psi_in, (size_t)c->n3 * c->tsize));
+        HIPCHK(c, hipMemcpyAsync(c->psi_in, psi, (size_t)c->n3 * c->tsize, hipMemcpyHostToDevice, c->st));
+        src = c->psi_in;
+    }
+    const size_t B = DJ_PK_MAX_BINS;
+    const double L = c->cfg.boxsize, V = L * L * L;
+    if (pk_delta_out) {
+        // CIC density of q + psi on the mesh, delta = mesh N^3 / n^3 - 1: |delta_k|^2 V / N^6 = |mesh_k|^2 V / n^6
+        const double zero3[3] = {0, 0, 0};
+        HIPCHK(c, hipMemcpyAsync(c->d_shifts + 48, zero3, sizeof(double) * 3, hipMemcpyHostToDevice, c->st));
+        r = tile_sort(c, src, c->d_shifts + 48, 1, 2);
+        if (r) return r;
+        TileArgs ta = tile_args(c, src, c->d_shifts + 48, 1);
+        c->tiles.deposit(ta, 2, false, zero3, c->mesh, 1.0, c->st);
+        field_fft(c, c->mesh, c->B);
+        zero_doubles(c->pk_work, 4 * B, c->st);
+        bin(c, c->B, nullptr, c->res, 2, 0, V / ((double)c->np * (double)c->np), ed, c->pk_work);
+        pk_finalize(c->pk_work, ed, 1.0, 1, true, c->pk_work + 4 * B, c->pk_work + 5 * B, nullptr, c->st);
+    }
+    if (pk_psi_out) {
+        // the displacement's components on the particle grid, transformed one at a time
+        const int n = c->n;
+        const FftOps *f = &c->fft;
+        const void *tw = c->tw;
+        void *A = c->A, *S = c->B;
+        if (n != c->res) {
+            r = c->tsize == 8 ? setup_pgrid<double>(c) : setup_pgrid<float>(c);
+            if (r) return r;
+            f = &c->pg_fft; tw = c->pg_tw; A = c->pg_A; S = c->pg_B;
+        }
+        double *sums = c->pk_work + 8 * B;
+        zero_doubles(sums, 4 * B, c->st);
+        for (int comp = 0; comp < 3; comp++) {
+            f->xf((const char *)src + (size_t)comp * c->np * c->tsize, false, A, tw, 1.0, c->st);
+            f->yf(A, S, tw, c->st);
+            f->zm(ZM_NONE, true, false, S, S, tw, c->kmult, c->st);
+            bin(c, S, nullptr, n, 0, 0, V / ((double)c->np * (double)c->np), ed, sums);
+        }
+        pk_finalize(sums, ed, 3.0, 3, true, c->pk_work + 4 * B, c->pk_work + 7 * B, nullptr, c->st);
+    }
+    return pk_download(c, nbins, k_out, pk_delta_out, nullptr, pk_psi_out);
+}
+
+dj_status dj_field_power_spectrum_noalloc(dj_ctx *c, const void *d_field, int32_t nbins, int32_t logarithmic,
+                                          double *d_k_out, double *d_pk_out, int64_t *d_nmodes_out) {
+    if (!c || !d_field || !d_pk_out)
+        return fail(c, DJ_ERR_INVALID_ARGUMENT, "null argument");
+    if (nbins < 1 || nbins > DJ_PK_MAX_BINS)
+        return fail(c, DJ_ERR_INVALID_ARGUMENT,
+                    "nbins " + std::to_string(nbins) + " outside 1..256, the bins the scratch array holds");
+    double *sums = pk_scratch(c->dev);
+    if (!sums) return fail(c, DJ_ERR_DEVICE, "the scratch array of this device was not set up");
+    HIPCHK(c, hipSetDevice(c->dev));
+    const int N = c->res;
+    const PkEdges ed = reference_edges(nbins, logarithmic, N, c->cfg.boxsize);
+    const double V = c->cfg.boxsize * c->cfg.boxsize * c->cfg.boxsize, n3 = (double)N * N * N;
+    field_fft(c, d_field, c->B);
+    zero_doubles(sums, 4 * nbins, c->st);
+    bin(c, c->B, nullptr, N, 0, 0, V / (n3 * n3), ed, sums);
+    pk_finalize(sums, ed, 1.0, 1, false, d_k_out, d_pk_out, d_nmodes_out, c->st);
+    HIPCHK(c, hipGetLastError());
     return DJ_OK;
 }
 

@@ -163,6 +163,26 @@ void nn_ops(NnOps &ops);
 template <typename T>
 void pk_bin(const void *B, int N, double L, int W, int nbins, double scale, double *sum_p, double *sum_w,
             unsigned long long *cnt, hipStream_t st);
+// bin edges, passed by value to the kernels (no device copy): bin b is [e[b], e[b + 1])
+enum { PK_MAX_BINS = 256 };
+struct PkEdges {
+    double e[PK_MAX_BINS + 1];
+    int nbins;
+};
+// sums (4 nbins: sum |k|, sum w v, sum w, count) += the binned modes of the spectrum F1 (|F1|^2) or of
+// F1, F2 (Re conj(F1) F2), v scaled by scale and divided by the window of order W (0: none); wmode 0:
+// weights 1/2 on the reference's half-axis end planes, 1: unweighted
+template <typename T>
+void pk_edges(const void *F1, const void *F2, int N, double L, int W, int wmode, double scale, const PkEdges &ed,
+              double *sums, hipStream_t st);
+// k_out, pk_out = mult sum(w v) / sum(w), nm_out = count / ncomp (device pointers, each may be null)
+void pk_finalize(const double *sums, const PkEdges &ed, double mult, int ncomp, bool centre_only, double *k_out,
+                 double *pk_out, int64_t *nm_out, hipStream_t st);
+void zero_doubles(double *p, int n, hipStream_t st);
+// the allocation-free entry point's bin sums (4 PK_MAX_BINS doubles) on device dev: set up once per
+// device with that device current (pk_scratch_init), then its address (pk_scratch; null before set-up)
+hipError_t pk_scratch_init(int dev);
+double *pk_scratch(int dev);
 
 // initial conditions (ics.hip, ngenic.cpp)
 void ranlxd1_uniforms(uint32_t seed, int n, double *out);

@@ -369,11 +369,14 @@ __global__ __launch_bounds__(256) void k_scan_write(const unsigned *src, int nt,
 // ------------------------------------------------------------------------------------ deposit
 // Accumulators of a tile in LDS:
 //   ACC_T      T atomics (FP64 meshes)
-//   ACC_Q31    FP32 meshes: each weight w <= 1 enters as the integer trunc(w 2^31) (exact in 32 bits),
-//              summed in 64-bit integers and rounded once to T at the tile store.  Integer LDS atomics
-//              run at about five times the rate of float ones on gfx950, the sum is independent of
-//              the order of the additions (the deposit is reproducible bit for bit), and its error
-//              (below 2^-31 per term, then one rounding) is below that of an FP32 running sum.
+//   ACC_Q31    FP32 meshes: each weight w <= 1 enters as the integer nearest to w 2^31 (the float
+//              product wx (wy wz) with the 2^31 in wx, plus one half in the same fused multiply-add,
+//              then truncated), summed in 64-bit integers and rounded once to T at the tile store.
+//              Integer LDS atomics run at about five times the rate of float ones on gfx950, and the
+//              sum is independent of the order of the additions (the deposit is reproducible bit for
+//              bit).  Each term carries the rounding of the float product (relative 2^-23 at most, no
+//              bias) and, below 2^-8, one of at most 2^-32; the sum adds no error until the final
+//              rounding, so the result is closer to the exact mass than an FP32 running sum.
 //   ACC_FIXED  the fixed-point mode: llrint(w 2^frac) in 64-bit integers, stored as integers
 enum { ACC_T = 0, ACC_Q31 = 1, ACC_FIXED = 2 };
 
@@ -384,7 +387,7 @@ __device__ __forceinline__ void dep_add(Acc *a, T wx, T wy, T wz, T wyz, double 
         const T wv = wx * wy * wz;
         atomicAdd(a, (unsigned long long)__double2ll_rn((double)wv * fscale));
     } else if constexpr (ACC == ACC_Q31) {
-        atomicAdd(a, (unsigned long long)(unsigned)(wx * wyz));  // wx carries the 2^31
+        atomicAdd(a, (unsigned long long)(unsigned)fmaf(wx, wyz, 0.5f));  // wx carries the 2^31
     } else {
         atomicAdd(a, wx * wy * wz);
     }

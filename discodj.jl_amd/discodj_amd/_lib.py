@@ -64,6 +64,14 @@ _SIGS = {
                                C.c_double, C.c_int32]),
     "dj_power_spectrum": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
                                       C.c_void_p]),
+    "dj_field_power_spectrum": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p,
+                                            C.c_void_p, C.c_void_p]),
+    "dj_field_cross_power_spectrum": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32,
+                                                  C.c_void_p, C.c_void_p]),
+    "dj_power_spectra_binned": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                            C.c_void_p]),
+    "dj_field_power_spectrum_noalloc": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p,
+                                                    C.c_void_p, C.c_void_p]),
     "dj_ngenic_field": (C.c_int32, [C.c_uint32, C.c_int32, C.c_void_p]),
     "dj_ranlxd1_uniforms": (C.c_int32, [C.c_uint32, C.c_int32, C.c_void_p]),
     "dj_vjp_acceleration": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),

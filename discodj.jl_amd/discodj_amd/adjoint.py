@@ -11,7 +11,7 @@ reference has no adjoint of its PM path (NR/README.md:169); the pieces follow it
 import numpy as np
 
 from .cosmology import linear_power_spectrum, stepper_coefficients
-from .pm import PMConfig, PMSolverHIP
+from .pm import PMConfig, PMSolverHIP, check_solver
 
 
 def density_loss_gradient(cosmo, white, target, *, boxsize, res_pm, a_ini=0.05, a_end=1.0, n_steps=10,
@@ -30,6 +30,9 @@ def density_loss_gradient(cosmo, white, target, *, boxsize, res_pm, a_ini=0.05, 
     co = stepper_coefficients(cosmo, stepper, a_ini, a_end, n_steps, time_var)
     D = cosmo.Dplus(a_ini)
     own = solver is None
+    if not own:
+        # the loss is built on the float deposit (the fixed-point one returns integer sums)
+        check_solver(solver, cfg, precision, deposit="float")
     S = solver or PMSolverHIP(cfg, precision=precision)
     try:
         S.ics_lpt(k_tab, pk_tab, D, order=order, white=white)

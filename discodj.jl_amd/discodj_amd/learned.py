@@ -13,7 +13,7 @@ corrected N^3 force and the fine-mesh force at snapshots of the fine run.  Plain
 import numpy as np
 
 from .cosmology import linear_power_spectrum, stepper_coefficients
-from .pm import PMConfig, PMSolverHIP
+from .pm import PMConfig, PMSolverHIP, check_solver
 from ._lib import NN_PARAMS
 
 PER_LAYER = 27 * 16 * 16 + 16
@@ -88,6 +88,8 @@ def train(work, data, params=None, iters=600, lr=1e-3, seed=0, solver=None, log=
     spacing, so that the net's output is of order one.  cosine: the learning rate decays as
     lr (1 + cos(pi t / iters)) / 2."""
     own = solver is None
+    if not own:
+        check_solver(solver, work.cfg_lo, 32, deposit="float")
     S = solver or PMSolverHIP(work.cfg_lo, precision=32)
     try:
         e0 = [float(np.mean((S.pm_acceleration(p).astype(np.float64) - f) ** 2)) for p, f, _ in data]

@@ -6,7 +6,7 @@ whole loop (dj_run), with callbacks only at the steps callback_when selects.
 import numpy as np
 
 from .cosmology import stepper_coefficients
-from .pm import PMConfig, PMSolverHIP
+from .pm import PMConfig, PMSolverHIP, check_solver
 
 
 def lagrangian_q(n, boxsize, dtype):
@@ -42,6 +42,8 @@ def run_nbody(c, psi_ini, pi_ini, *, boxsize, a_ini, a_end, n_steps, res_pm, tim
                    antialias=antialias, grad_order=grad_kernel_order, lap_order=laplace_kernel_order,
                    n_resample=n_resample, resampling=resampling_method)
     own = solver is None
+    if not own:
+        check_solver(solver, cfg, precision, deposit=deposit, lean=lean)
     S = solver or PMSolverHIP(cfg, precision=precision, deposit=deposit, lean=lean)
     try:
         mom = np.asarray(pi_ini, np.float64)

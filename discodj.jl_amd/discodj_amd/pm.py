@@ -54,6 +54,7 @@ class PMSolverHIP:
         self.n = cfg.n_part
         self.shape = (3, self.n, self.n, self.n)
         self.deposit_mode = deposit
+        self.lean = bool(lean)
         L = _lib.lib()
         ptr = C.c_void_p()
         c = _c_config(cfg, precision, deposit, lean, device, fixed_frac_bits)
@@ -146,6 +147,41 @@ class PMSolverHIP:
         self._chk(self._L.dj_power_spectrum(self._ctx, None if src is None else src.ctypes.data, W, nbins,
                                             k.ctypes.data, pk.ctypes.data, nm.ctypes.data))
         return k, pk, nm
+
+    def field_power_spectrum(self, field, bins=20, logarithmic=True):
+        """evaluate_power_spectrum(field, boxsize; bins, logarithmic) (power_spectrum.jl:25-79) of a real
+        (N,N,N) field [iz,iy,ix] on the solver's mesh -> (k, Pk, N_modes)."""
+        N = self.cfg.res_pm
+        f = np.ascontiguousarray(field, self.dtype).reshape(N, N, N)
+        k, pk, nm = np.empty(bins), np.empty(bins), np.empty(bins, np.int64)
+        self._chk(self._L.dj_field_power_spectrum(self._ctx, f.ctypes.data, bins, int(bool(logarithmic)),
+                                                  k.ctypes.data, pk.ctypes.data, nm.ctypes.data))
+        return k, pk, nm
+
+    def field_cross_power_spectrum(self, field1, field2, bins=20, logarithmic=True):
+        """evaluate_cross_power_spectrum (power_spectrum.jl:86-123) -> (k, Pk12)."""
+        N = self.cfg.res_pm
+        f1 = np.ascontiguousarray(field1, self.dtype).reshape(N, N, N)
+        f2 = np.ascontiguousarray(field2, self.dtype).reshape(N, N, N)
+        k, pk = np.empty(bins), np.empty(bins)
+        self._chk(self._L.dj_field_cross_power_spectrum(self._ctx, f1.ctypes.data, f2.ctypes.data, bins,
+                                                        int(bool(logarithmic)), k.ctypes.data, pk.ctypes.data))
+        return k, pk
+
+    def power_spectra_binned(self, edges, psi=None, density=True, displacement=True):
+        """pk_full of the study estimator (measure.jl:403-447) of q + psi (None: the solver's state) over the
+        given bin edges -> (k centres, P_delta or None, P_Psi or None)."""
+        e = np.ascontiguousarray(edges, np.float64)
+        nb = len(e) - 1
+        k = np.empty(nb)
+        pd = np.empty(nb) if density else None
+        pp = np.empty(nb) if displacement else None
+        src = None if psi is None else self._arr(psi)
+        self._chk(self._L.dj_power_spectra_binned(self._ctx, None if src is None else src.ctypes.data, nb,
+                                                  e.ctypes.data, k.ctypes.data,
+                                                  None if pd is None else pd.ctypes.data,
+                                                  None if pp is None else pp.ctypes.data))
+        return k, pd, pp
 
     def check_finite(self):
         self._chk(self._L.dj_check_finite(self._ctx))
@@ -268,6 +304,19 @@ class PMSolverHIP:
         self._chk(self._L.dj_kernel_stats(self._ctx, arr, 32, C.byref(n)))
         return {arr[i].name.decode(): {"calls": arr[i].calls, "ms_total": arr[i].ms_total,
                                        "bytes_per_call": arr[i].bytes_per_call} for i in range(n.value)}
+
+
+def check_solver(S, cfg, precision, deposit=None, lean=None):
+    """A solver handed to a driver in place of the one its arguments describe must compute the same
+    model: the same PMConfig, precision and deposit mode (and lean form).  Raises ValueError otherwise."""
+    if S.cfg != cfg:
+        raise ValueError(f"the solver's configuration {S.cfg} differs from the requested {cfg}")
+    if S.precision != precision:
+        raise ValueError(f"the solver's precision {S.precision} differs from the requested {precision}")
+    if deposit is not None and S.deposit_mode != deposit:
+        raise ValueError(f"the solver's deposit mode {S.deposit_mode!r} differs from the requested {deposit!r}")
+    if lean is not None and bool(S.lean) != bool(lean):
+        raise ValueError(f"the solver's lean form ({S.lean}) differs from the requested one ({lean})")
 
 
 def pm_acceleration(psi, cfg, precision=None, **kw):

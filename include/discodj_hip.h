@@ -41,8 +41,9 @@ typedef int32_t dj_status;
 #define DJ_ERR_NONFINITE (-6)        /* NaN/Inf found in the particle state (dj_check_finite) */
 
 #define DJ_DEPOSIT_FLOAT 0   /* density mesh in the working precision.  FP32: each tile sums the
-                              * weights as integers round(w 2^31) in 64 bits and rounds once to
-                              * float (independent of the order of the adds); FP64: double atomics */
+                              * float weight products w, as the integers nearest to w 2^31, in 64
+                              * bits and rounds once to float (independent of the order of the
+                              * adds); FP64: double atomics */
 #define DJ_DEPOSIT_FIXED64 1 /* int64 fixed-point weights: bit-reproducible density */
 
 /* PMConfig (pm.jl:35-46) plus the device-side choices.  Field meanings and accepted values are the
@@ -133,6 +134,45 @@ dj_status dj_ics_lpt(dj_ctx *ctx, uint32_t seed, const double *white, const doub
 dj_status dj_power_spectrum(dj_ctx *ctx, const void *psi, int32_t W, int32_t nbins, double *k_out, double *pk_out,
                             int64_t *nmodes_out);
 
+/* Binned power spectra on the context's mesh (res_pm^3, boxsize L).  A mode k != 0 of the half spectrum
+ * (the reference halves z: kz >= 0) is in bin b when edges[b] <= |k| < edges[b + 1], |k| formed as the
+ * reference does in FP64; at most DJ_PK_MAX_BINS bins.  Host arrays, blocking calls. */
+#define DJ_PK_MAX_BINS 256
+
+/* evaluate_power_spectrum(field, boxsize; bins, logarithmic) (analysis/power_spectrum.jl:25-79) of a real
+ * (N,N,N) host field of the context's precision, N = res_pm: P(k) = V sum w |f_k|^2 / sum w / N^6 with
+ * w = 1/2 on the kz = 0 and kz = N/2 planes and 1 elsewhere, nbins bins from kf = 2 pi / L to
+ * k_Nyq sqrt(3) (k_Nyq = pi N / L), equal in k or, logarithmic != 0, in log10 k; k_out the mean |k| of a
+ * bin's modes (the geometric centre of an empty bin), nmodes_out (may be NULL) their number */
+dj_status dj_field_power_spectrum(dj_ctx *ctx, const void *field, int32_t nbins, int32_t logarithmic, double *k_out,
+                                  double *pk_out, int64_t *nmodes_out);
+/* evaluate_cross_power_spectrum(field1, field2, boxsize; bins, logarithmic) (power_spectrum.jl:86-123):
+ * P12(k) = V sum Re(conj(f1_k) f2_k) / n_k / N^6, the unweighted mean over a bin's n_k modes; bins and
+ * k_out as dj_field_power_spectrum */
+dj_status dj_field_cross_power_spectrum(dj_ctx *ctx, const void *field1, const void *field2, int32_t nbins,
+                                        int32_t logarithmic, double *k_out, double *pk_out);
+/* pk_full of the study estimator (studies/eft_shell_crossing/src/measure.jl:403-447) of X = q + psi (psi
+ * NULL: the context's Psi) over the given edges (nbins + 1, ascending, finite): pk_delta_out the CIC
+ * density's P(k) on the mesh, delta = mesh N^3 / n^3 - 1, divided by the CIC window |W(k)|^2; pk_psi_out
+ * P_Psi(k) = V sum_c sum w |Psi_c,k|^2 / (sum w / 3) / n^6 of the displacement on the particle grid
+ * (n_part in the transform sizes of dj_pm_config's Fourier resampling); mode weights 1 on the kz = 0 and
+ * kz = N/2 planes, 2 elsewhere; k_out the geometric centres sqrt(edges[b] edges[b + 1]).  k_out and
+ * either spectrum may be NULL. */
+dj_status dj_power_spectra_binned(dj_ctx *ctx, const void *psi, int32_t nbins, const double *edges, double *k_out,
+                                  double *pk_delta_out, double *pk_psi_out);
+
+/* Allocation-free dj_field_power_spectrum, for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc); no counterpart in the
+ * reference.  All pointers are device pointers: the field (res_pm^3 values of the context's precision)
+ * and the outputs (d_k_out and d_nmodes_out may be NULL).  The call only enqueues work on the
+ * context's stream (results are ready after dj_synchronize) and allocates nothing: its bin sums live in
+ * a static device array of DJ_PK_MAX_BINS x 4 doubles, one per GPU, held until the process ends, which
+ * every call on that GPU shares, so calls that overlap on one GPU (contexts used from several threads)
+ * race.  nbins above DJ_PK_MAX_BINS returns DJ_ERR_INVALID_ARGUMENT.  It uses the context's own
+ * spectrum buffers, allocated by dj_create. */
+dj_status dj_field_power_spectrum_noalloc(dj_ctx *ctx, const void *d_field, int32_t nbins, int32_t logarithmic,
+                                          double *d_k_out, double *d_pk_out, int64_t *d_nmodes_out);
+
 /* Host helpers of the IC pipeline: the N-GenIC Fourier white noise of `seed` (ngenic_field_gsl,
  * ngenic_gsl.jl:126-165), (res, res, res/2+1) interleaved complex, halved along z, element (i,j,k)
  * at i + res*(j + res*k); and the GSL ranlxd1 stream (ngenic_gsl.jl:34-98). */
@@ -148,7 +188,8 @@ dj_status dj_ranlxd1_uniforms(uint32_t seed, int32_t n, double *out);
 dj_status dj_vjp_acceleration(dj_ctx *ctx, const void *psi, const void *acc_bar, void *psi_bar_out);
 /* psi_bar_out = (d mesh / d psi)^T mesh_bar of dj_deposit (no shift) at psi (NULL: the context's Psi).
  * psi_bar_out NULL keeps the result on the device as the context's resident cotangents (Psibar, Pibar = 0),
- * which dj_vjp_run and dj_vjp_ics_lpt take when their cotangent pointers are NULL. */
+ * which dj_vjp_run and dj_vjp_ics_lpt take when their cotangent pointers are NULL (DJ_ERR_INVALID_ARGUMENT
+ * when none are resident: dj_vjp_acceleration, and calls given host cotangents, overwrite them). */
 dj_status dj_vjp_deposit(dj_ctx *ctx, const void *psi, const void *mesh_bar, void *psi_bar_out);
 /* dj_run's n_steps DKD steps from the current state, keeping (Psi, Pi) before each step on the device
  * (2 x 3 n^3 values per step) for dj_vjp_run; the final state is the context's state as after dj_run */

@@ -155,6 +155,14 @@ int orc_power_spectrum_d(int res, const double *field, double boxsize, int bins,
    window of order W (0 = none) — the accuracy measurement of the PM path */
 int orc_power_spectrum_lin_d(int res, const double *field, double boxsize, int W, int nbins,
                              double *k_out, double *pk_out, int64_t *nmodes_out);
+/* evaluate_cross_power_spectrum (power_spectrum.jl:86-123) */
+int orc_cross_power_spectrum_d(int res, const double *f1, const double *f2, double boxsize, int bins,
+                               int logarithmic, double *k_out, double *pk_out);
+/* pk_full of the study estimator (measure.jl:403-447) over given bin edges: P_delta of a density
+   contrast on the deposit mesh (deconv_W: window order divided out, 0 none) and P_psi of a
+   displacement (3, n^3) on the particle grid; NULL skips a part */
+void orc_pk_binned_d(int res, const double *delta, int n, const double *psi, double boxsize, int nbins,
+                     const double *edges, int deconv_W, double *pk_delta, double *pk_psi);
 
 #ifdef __cplusplus
 }

@@ -103,6 +103,9 @@ _sig("orc_ngenic_field_gsl", None, [C.c_uint32, C.c_int, _dp])
 _sig("orc_ngenic_wnoise_real", None, [C.c_uint32, C.c_int, _dp])
 _sig("orc_power_spectrum_d", C.c_int, [C.c_int, _dp, C.c_double, C.c_int, C.c_int, _dp, _dp, _i64p])
 _sig("orc_power_spectrum_lin_d", C.c_int, [C.c_int, _dp, C.c_double, C.c_int, C.c_int, _dp, _dp, _i64p])
+_sig("orc_cross_power_spectrum_d", C.c_int, [C.c_int, _dp, _dp, C.c_double, C.c_int, C.c_int, _dp, _dp])
+_sig("orc_pk_binned_d", None, [C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_double, C.c_int, _dp, C.c_int,
+                               C.c_void_p, C.c_void_p])
 
 STEP_CB = C.CFUNCTYPE(None, C.c_int, C.c_double, C.c_void_p)
 
@@ -351,6 +354,29 @@ def power_spectrum_lin(field, boxsize, W=2, nbins=None):
     k, pk, nm = np.empty(nbins), np.empty(nbins), np.empty(nbins, np.int64)
     lib.orc_power_spectrum_lin_d(res, np.ascontiguousarray(field, np.float64), boxsize, W, nbins, k, pk, nm)
     return k, pk, nm
+
+
+def cross_power_spectrum(f1, f2, boxsize, bins=20, logarithmic=True):
+    """evaluate_cross_power_spectrum (power_spectrum.jl:86-123) -> (k, Pk12)."""
+    res = f1.shape[0]
+    k, pk = np.empty(bins), np.empty(bins)
+    lib.orc_cross_power_spectrum_d(res, np.ascontiguousarray(f1, np.float64), np.ascontiguousarray(f2, np.float64),
+                                   boxsize, bins, int(logarithmic), k, pk)
+    return k, pk
+
+
+def pk_binned(boxsize, edges, delta=None, psi=None, deconv_W=2):
+    """pk_full of the study estimator (measure.jl:403-447) over the given edges -> (P_delta, P_psi); delta is a
+    density contrast (N,N,N) on the deposit mesh, psi the displacement (3,n,n,n); either may be None."""
+    edges = np.ascontiguousarray(edges, np.float64)
+    nb = len(edges) - 1
+    d = None if delta is None else np.ascontiguousarray(delta, np.float64)
+    p = None if psi is None else np.ascontiguousarray(psi, np.float64)
+    pd, pp = np.zeros(nb), np.zeros(nb)
+    lib.orc_pk_binned_d(0 if d is None else d.shape[0], None if d is None else d.ctypes.data,
+                        0 if p is None else p.shape[1], None if p is None else p.ctypes.data, boxsize, nb, edges,
+                        deconv_W, pd.ctypes.data, pp.ctypes.data)
+    return (None if d is None else pd), (None if p is None else pp)
 
 
 def lagrangian_positions(psi, boxsize):

@@ -191,3 +191,36 @@ def test_missing_library_fails_loudly():
             "    print('raised:', e)\n") % os.path.dirname(os.path.dirname(os.path.abspath(D.__file__)))
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
     assert "raised:" in r.stdout and "build it" in r.stdout, r.stdout + r.stderr
+
+
+class _FakeSolver:
+    """Stands in for a PMSolverHIP (no device here): the attributes check_solver reads."""
+
+    def __init__(self, cfg, precision=32, deposit_mode="float", lean=False):
+        self.cfg, self.precision, self.deposit_mode, self.lean = cfg, precision, deposit_mode, lean
+
+
+def test_a_passed_solver_must_match_the_requested_model():
+    from discodj_amd.pm import check_solver
+    cfg = D.PMConfig(res_pm=32, n_part=16, boxsize=100.0, worder=3, grad_order=4)
+    check_solver(_FakeSolver(cfg), D.PMConfig(res_pm=32, n_part=16, boxsize=100, worder=3, grad_order=4), 32)
+    for bad in (dict(worder=2), dict(antialias=1), dict(grad_order=6), dict(deconvolve=True)):
+        other = D.PMConfig(**{**cfg.__dict__, **bad})
+        with pytest.raises(ValueError):
+            check_solver(_FakeSolver(other), cfg, 32)
+    with pytest.raises(ValueError):
+        check_solver(_FakeSolver(cfg, precision=64), cfg, 32)
+    with pytest.raises(ValueError):
+        check_solver(_FakeSolver(cfg, deposit_mode="fixed64"), cfg, 32, deposit="float")
+    # the drivers check before any device work
+    c = D.Cosmology.named("Planck18EEBAOSN")
+    n = 16
+    with pytest.raises(ValueError):
+        D.density_loss_gradient(c, np.zeros((n, n, n)), np.zeros((32, 32, 32)), boxsize=100.0, res_pm=32,
+                                precision=32, solver=_FakeSolver(D.PMConfig(res_pm=32, n_part=n, boxsize=100.0,
+                                                                             worder=2, grad_order=4),
+                                                                 deposit_mode="fixed64"))
+    with pytest.raises(ValueError):
+        D.run_nbody(c, np.zeros((3, n, n, n), np.float32), np.zeros((3, n, n, n), np.float32), boxsize=100.0,
+                    a_ini=0.05, a_end=1.0, n_steps=2, res_pm=32, worder=3,
+                    solver=_FakeSolver(D.PMConfig(res_pm=32, n_part=n, boxsize=100.0, worder=2)))

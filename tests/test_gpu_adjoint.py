@@ -185,3 +185,62 @@ def test_density_loss_gradient_directions():
     with D.PMSolverHIP(cfg, precision=32) as S32:
         _, wb32, _ = D.density_loss_gradient(c, white, target, precision=32, solver=S32, **kw)
     assert G.rel(wb32, wb) < 2e-3
+
+
+@pytest.mark.parametrize("W", [2, 3, 4])
+@pytest.mark.parametrize("aa", [0, 1])
+def test_vjp_acceleration_fp32_tiled_vector_deposit(W, aa):
+    """The FP32 force VJP on a 64^3 mesh, where every axis holds at least two 32x16x8 FP32 tiles, so the
+    cotangent's vector deposit runs tiled in LDS (the 16^3-particle tests above have one tile per axis
+    in FP32 and fall back to device atomics): against the FP64 VJP of the same positions."""
+    n, res, L = 32, 64, 100.0
+    rng = np.random.default_rng(100 + 10 * W + aa)
+    psi, abar = _psi(n, L, 30 + W + aa), rng.normal(size=(3, n, n, n))
+    cfg = D.PMConfig(res_pm=res, n_part=n, boxsize=L, worder=W, grad_order=4, antialias=aa)
+    with D.PMSolverHIP(cfg, precision=64) as S:
+        ref = S.vjp_acceleration(psi, abar)
+    with D.PMSolverHIP(cfg, precision=32) as S:
+        got = S.vjp_acceleration(psi.astype(np.float32), abar.astype(np.float32))
+    assert G.rel(got, ref) < 5e-4, G.rel(got, ref)
+
+
+def test_density_loss_gradient_fp32_tiled_64():
+    """The density-loss gradient at n = res_pm = 64 (tiled FP32 vector deposit in every step) against
+    the FP64 gradient."""
+    n, L = 64, 125.0
+    c = D.Cosmology.named("Planck18EEBAOSN")
+    white = O.ngenic_wnoise_real(9, n)
+    target = np.random.default_rng(4).normal(scale=0.5, size=(n, n, n))
+    kw = dict(boxsize=L, res_pm=n, n_steps=10, stepper="bullfrog", order=2, worder=2, grad_kernel_order=4)
+    _, wb64, _ = D.density_loss_gradient(c, white, target, precision=64, **kw)
+    _, wb32, _ = D.density_loss_gradient(c, white, target, precision=32, **kw)
+    assert G.rel(wb32, wb64) < 5e-3, G.rel(wb32, wb64)
+
+
+def test_resident_cotangents_are_checked():
+    """dj_vjp_run and dj_vjp_ics_lpt with NULL cotangents need resident ones from dj_vjp_deposit (or a
+    resident dj_vjp_run); dj_vjp_acceleration overwrites them; options without an adjoint are refused by
+    dj_run_checkpointed before it runs."""
+    n, L = 16, 64.0
+    c = D.Cosmology.named("Planck18EEBAOSN")
+    co = D.stepper_coefficients(c, "bullfrog", 0.05, 0.5, 2, "D")
+    coef = [co["ddrift1"], co["alpha"], co["beta"], co["ddrift2"]]
+    oc = O.Cosmology(*COSMO)
+    k, pk = O.linear_power_spectrum(oc)
+    white = O.ngenic_wnoise_real(3, n)
+    with D.PMSolverHIP(D.PMConfig(res_pm=32, n_part=n, boxsize=L), precision=64) as S:
+        S.ics_lpt(k, pk, oc.Dplus(0.05), white=white)
+        S.run_checkpointed(*coef)
+        with pytest.raises(D.DJError):
+            S.vjp_run()
+        with pytest.raises(D.DJError):
+            S.vjp_ics_lpt(k, pk, oc.Dplus(0.05), white=white)
+        S.vjp_deposit(None, np.ones((32, 32, 32)), resident=True)
+        S.vjp_run()
+        S.vjp_acceleration(None, np.ones((3, n, n, n)))
+        with pytest.raises(D.DJError):
+            S.vjp_ics_lpt(k, pk, oc.Dplus(0.05), white=white)
+    with D.PMSolverHIP(D.PMConfig(res_pm=32, n_part=n, boxsize=L, grad_order=0), precision=64) as S:
+        S.set_state(np.zeros((3, n, n, n)))
+        with pytest.raises(D.DJError):
+            S.run_checkpointed(*coef)

@@ -68,3 +68,89 @@ def test_power_spectrum_vs_oracle(prec):
     assert np.allclose(k, ko, rtol=1e-14)
     tol = 1e-10 if prec == 64 else 2e-4
     assert np.max(np.abs(pk / pko - 1)) < tol
+
+
+# ---------------------------------------------------- field, cross and binned spectra (power_spectrum.jl)
+def _fields(N, seed):
+    rng = np.random.default_rng(seed)
+    f = rng.normal(size=(N, N, N))
+    # a correlated second field: f smoothed along x plus noise, so the cross spectrum has both signs
+    g = 0.6 * np.roll(f, 1, axis=2) - 0.3 * f + 0.5 * rng.normal(size=(N, N, N))
+    return f, g
+
+
+@pytest.mark.parametrize("N", [32, 64])
+@pytest.mark.parametrize("log", [True, False])
+@pytest.mark.parametrize("prec", [32, 64])
+def test_field_power_spectrum_vs_oracle(N, log, prec):
+    """evaluate_power_spectrum of a field: the same modes in every bin (bit-exact |k| and bins), P and the
+    mean |k| of each bin against the oracle (FP64 1e-10; FP32: the field and its transform in FP32)."""
+    L = 3.0 * N
+    f, _ = _fields(N, N + int(log))
+    dt = np.float32 if prec == 32 else np.float64
+    with D.PMSolverHIP(D.PMConfig(res_pm=N, n_part=16, boxsize=L), precision=prec) as S:
+        k, pk, nm = S.field_power_spectrum(f.astype(dt), bins=20, logarithmic=log)
+    ko, pko, nmo = O.power_spectrum(f.astype(dt).astype(np.float64), L, bins=20, logarithmic=log)
+    assert np.array_equal(nm, nmo)
+    assert np.allclose(k, ko, rtol=1e-12, atol=0)
+    tol = 1e-10 if prec == 64 else 2e-5
+    ok = nmo > 0
+    assert np.max(np.abs(pk[ok] / pko[ok] - 1)) < tol
+    assert np.all(pk[~ok] == 0)
+
+
+@pytest.mark.parametrize("log", [True, False])
+@pytest.mark.parametrize("prec", [32, 64])
+def test_field_cross_power_spectrum_vs_oracle(log, prec):
+    N, L = 48, 150.0
+    f, g = _fields(N, 7)
+    dt = np.float32 if prec == 32 else np.float64
+    with D.PMSolverHIP(D.PMConfig(res_pm=N, n_part=16, boxsize=L), precision=prec) as S:
+        k, pc = S.field_cross_power_spectrum(f.astype(dt), g.astype(dt), bins=16, logarithmic=log)
+        _, pa = S.field_cross_power_spectrum(f.astype(dt), f.astype(dt), bins=16, logarithmic=log)
+    ko, pco = O.cross_power_spectrum(f.astype(dt).astype(np.float64), g.astype(dt).astype(np.float64), L, bins=16,
+                                     logarithmic=log)
+    assert np.allclose(k, ko, rtol=1e-12, atol=0)
+    scale = np.max(np.abs(pco))
+    tol = 1e-10 if prec == 64 else 2e-5
+    assert np.max(np.abs(pc - pco)) < tol * scale
+    assert np.any(pco < 0) and np.any(pco > 0)
+    _, pao = O.cross_power_spectrum(f.astype(dt).astype(np.float64), f.astype(dt).astype(np.float64), L, bins=16,
+                                    logarithmic=log)
+    assert np.max(np.abs(pa - pao)) < tol * np.max(pao)
+
+
+@pytest.mark.parametrize("n, N", [(32, 64), (32, 32), (24, 48)])
+@pytest.mark.parametrize("prec", [32, 64])
+def test_power_spectra_binned_vs_oracle(n, N, prec):
+    """pk_full of the study estimator (measure.jl:403-447): P_delta of the CIC density of q + psi on the
+    mesh, window-deconvolved, and P_Psi of the displacement on the particle grid, over logarithmic edges
+    from 0.999 kf to k_Nyq of the mesh, against the oracle."""
+    L = 2.0 * n
+    rng = np.random.default_rng(n + N)
+    psi = rng.normal(scale=1.5, size=(3, n, n, n))
+    dt = np.float32 if prec == 32 else np.float64
+    psi_t = psi.astype(dt)
+    edges = np.exp(np.linspace(np.log(2 * np.pi / L * 0.999), np.log(np.pi * N / L), 21))
+    with D.PMSolverHIP(D.PMConfig(res_pm=N, n_part=n, boxsize=L), precision=prec) as S:
+        k, pd, pp = S.power_spectra_binned(edges, psi_t)
+        S.set_state(psi_t)
+        _, pd2, _ = S.power_spectra_binned(edges, None, displacement=False)
+    mesh = O.pm_scatter(O.lagrangian_positions(psi_t.astype(np.float64), L), N, L, 2)
+    delta = mesh * (N**3 / n**3) - 1.0
+    pdo, ppo = O.pk_binned(L, edges, delta=delta, psi=psi_t.astype(np.float64), deconv_W=2)
+    assert np.allclose(k, np.sqrt(edges[:-1] * edges[1:]), rtol=1e-15, atol=0)
+    tol = 1e-10 if prec == 64 else 5e-5
+    okd, okp = pdo > 0, ppo > 0
+    assert np.max(np.abs(pd[okd] / pdo[okd] - 1)) < tol and np.all(pd[~okd] == 0)
+    assert np.max(np.abs(pp[okp] / ppo[okp] - 1)) < tol and np.all(pp[~okp] == 0)
+    assert np.allclose(pd, pd2, rtol=1e-12, atol=0)  # the bin sums are double atomics (order-dependent)
+
+
+def test_spectra_reject_bad_arguments():
+    with D.PMSolverHIP(D.PMConfig(res_pm=32, n_part=16, boxsize=100.0), precision=64) as S:
+        f = np.zeros((32, 32, 32))
+        with pytest.raises(D.DJError):
+            S.field_power_spectrum(f, bins=257)
+        with pytest.raises(D.DJError):
+            S.power_spectra_binned(np.array([1.0, 0.5, 2.0]))

@@ -146,9 +146,13 @@ def test_pm256_mesh256_fixed64_runs_bit_identical():
 def test_density_loss_gradient_256_central_differences():
     """Gradient of sum (delta - target)^2 on the 256^3 mesh with respect to the 256^3 white noise through
     2LPT and 10 BullFrog CIC steps (grad order 4), FP64, against central differences of the whole pipeline
-    along 4 random directions; the central differences use the fixed-point deposit, whose runs are
-    deterministic, so that the difference of two losses is not blurred by the order of float additions.
-    The FP32 gradient against the FP64 one."""
+    along the gradient's own direction and 4 random directions.  The central differences use the
+    fixed-point deposit, whose runs are deterministic, so that the difference of two losses is not blurred
+    by the order of float additions.  CIC weights are piecewise linear: over a difference step some
+    thousand of the 16.8M particles cross a cell face, each adding an error of the size of its own
+    contribution with a random sign, so along a random direction v ~ N(0, I) (whose derivative has the
+    size |grad|) the error is judged against |grad|, and along the gradient (derivative |grad| sqrt(n^3))
+    relatively.  The FP32 gradient against the FP64 one."""
     n, L = 256, 500.0
     c = _dc()
     _, k, pk = _oc()
@@ -158,6 +162,7 @@ def test_density_loss_gradient_256_central_differences():
               k_tab=k, pk_tab=pk)
     cfg = D.PMConfig(res_pm=n, n_part=n, boxsize=L, worder=2, grad_order=4)
     loss, wb, _ = D.density_loss_gradient(c, white, target, precision=64, **kw)
+    gnorm = float(np.sqrt(np.sum(wb * wb)))
     co = D.stepper_coefficients(c, "bullfrog", A_INI, A_END, 10, "D")
     frac = 2.0**-32
 
@@ -169,15 +174,20 @@ def test_density_loss_gradient_256_central_differences():
             return float(np.sum(delta * delta))
 
         l0 = lossf(white)
+        print(f"loss {loss:.10e} (fixed-point forward {l0:.10e}), |grad| {gnorm:.4e}")
         assert abs(l0 - loss) < 1e-8 * loss, (l0, loss)
         rng = np.random.default_rng(11)
-        eps = 1e-5
-        for i in range(4):
-            v = rng.normal(size=(n, n, n))
+        eps = 2e-6
+        dirs = [wb * (np.sqrt(n**3) / gnorm)] + [rng.normal(size=(n, n, n)) for _ in range(4)]
+        for i, v in enumerate(dirs):
             fd = (lossf(white + eps * v) - lossf(white - eps * v)) / (2 * eps)
             g = float(np.sum(wb * v))
-            print(f"direction {i}: gradient {g:.8e}, central difference {fd:.8e}, rel {abs(g - fd) / abs(fd):.2e}")
-            assert abs(g - fd) < 1e-4 * abs(fd), (i, g, fd)
+            print(f"direction {i}: gradient {g:.8e}, central difference {fd:.8e}, |g - fd| / |grad| "
+                  f"{abs(g - fd) / gnorm:.2e}, relative {abs(g - fd) / abs(fd):.2e}")
+            if i == 0:
+                assert abs(g - fd) < 1e-4 * abs(fd), (i, g, fd)
+            else:
+                assert abs(g - fd) < 2e-3 * gnorm, (i, g, fd, gnorm)
     _, wb32, _ = D.density_loss_gradient(c, white, target, precision=32, **kw)
     e32 = G.rel(wb32, wb)
     print(f"FP32 gradient against FP64: {e32:.2e}")

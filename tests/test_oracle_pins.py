@@ -125,6 +125,72 @@ def test_power_spectrum_single_mode():
     assert np.sum(pk > 1e-20) == 1
 
 
+def _half_grid(N, L):
+    kf = 2 * np.pi / L
+    m = np.arange(N)
+    kx = np.where(m <= N // 2, m, m - N) * kf
+    kz = np.arange(N // 2 + 1) * kf
+    return np.sqrt(kz[:, None, None] ** 2 + kx[None, :, None] ** 2 + kx[None, None, :] ** 2)
+
+
+def test_cross_power_spectrum_of_two_phased_modes():
+    """f = A cos(k1.x + a), g = B cos(k1.x + b): Re(conj f_k g_k) V / N^6 = A B V cos(b - a) / 4 at +-k1;
+    with k1 off the kz = 0 plane only +k1 is in the half spectrum, and the bin's unweighted mean divides
+    by all its modes.  (The sign of the imaginary product is pinned: re re - im im would give cos(a + b).)"""
+    N, L, A, B, a, b = 16, 80.0, 0.7, 1.3, 0.4, 1.5
+    kf = 2 * np.pi / L
+    x = np.arange(N) * L / N
+    ph = kf * (2 * x[None, None, :] + 1 * x[None, :, None] + 3 * x[:, None, None])  # k1 = (2, 1, 3) kf
+    f, g = A * np.cos(ph + a), B * np.cos(ph + b)
+    bins = 12
+    k, pc = O.cross_power_spectrum(f, g, L, bins=bins, logarithmic=False)
+    edges = [(1 - t) * kf + t * (np.pi * N / L * np.sqrt(3)) for t in np.arange(bins + 1) / bins]
+    kk = _half_grid(N, L)
+    k1 = kf * np.sqrt(14.0)
+    bb = max(i for i in range(bins) if edges[i] <= k1)
+    n_k = np.sum((kk >= edges[bb]) & (kk < edges[bb + 1]))
+    assert pc[bb] == pytest.approx(A * B * L**3 * np.cos(b - a) / 4 / n_k, rel=1e-10)
+    assert np.sum(np.abs(pc) > 1e-12 * abs(pc[bb])) == 1
+    # symmetric, and Re(conj f f) of one field is the unweighted mean of |f_k|^2
+    assert np.allclose(O.cross_power_spectrum(g, f, L, bins=bins, logarithmic=False)[1], pc, rtol=1e-12, atol=0)
+    ks, ps = O.cross_power_spectrum(f, f, L, bins=bins, logarithmic=False)
+    assert ps[bb] == pytest.approx(A * A * L**3 / 4 / n_k, rel=1e-10)
+    # bin centres: the mean |k| of the modes in each non-empty bin (logarithmic bins)
+    kl, _ = O.cross_power_spectrum(f, g, L, bins=bins, logarithmic=True)
+    le = [10 ** ((1 - t) * np.log10(kf) + t * np.log10(np.pi * N / L * np.sqrt(3))) for t in np.arange(bins + 1) / bins]
+    for i in range(bins):
+        m = (kk >= le[i]) & (kk < le[i + 1])
+        if m.any():
+            assert kl[i] == pytest.approx(kk[m].mean(), rel=1e-12)
+
+
+def test_binned_estimator_closed_forms():
+    """pk_full over given edges.  Density part with linear edges kf (b + 1/2) equals the pinned linear-bin
+    estimator (same weights and window); displacement part of psi_c = A_c cos(k1.q) is
+    L^3 sum_c w1 A_c^2 / 4 / (sum_w / 3) with w1 = 2 off the kz = 0, N/2 planes."""
+    N, L = 32, 100.0
+    kf = 2 * np.pi / L
+    delta = np.random.default_rng(3).normal(size=(N, N, N))
+    nb = N // 4 - 1
+    edges = kf * (np.arange(nb + 1) + 0.5)
+    pd, _ = O.pk_binned(L, edges, delta=delta, deconv_W=2)
+    _, plin, _ = O.power_spectrum_lin(delta, L, W=2, nbins=nb)
+    assert np.allclose(pd, plin, rtol=1e-12, atol=0)
+    n = 16
+    q = np.arange(n) * L / n
+    ph = kf * (1 * q[None, None, :] + 2 * q[None, :, None] + 2 * q[:, None, None])  # |k1| = 3 kf
+    Ac = np.array([0.5, 1.0, 2.0])
+    psi = Ac[:, None, None, None] * np.cos(ph)[None]
+    e2 = kf * np.array([0.9, 2.5, 3.5, 6.0])
+    _, pp = O.pk_binned(L, e2, psi=psi)
+    kk = _half_grid(n, L)
+    w = np.full_like(kk, 2.0)
+    w[0] = w[-1] = 1.0
+    sw = w[(kk >= e2[1]) & (kk < e2[2])].sum()
+    assert pp[1] == pytest.approx(L**3 * np.sum(2.0 * Ac**2 / 4) / sw, rel=1e-10)
+    assert abs(pp[0]) < 1e-20 * pp[1] and abs(pp[2]) < 1e-20 * pp[1]
+
+
 def test_power_spectrum_white_noise_amplitude():
     # runtests.jl:174-186: unit white noise on N^3 cells of volume (L/N)^3 has P = (L/N)^3
     N, L = 32, 200.0
