@@ -280,6 +280,27 @@ bad:
 }
 
 /* transfer.jl:24-77: Eisenstein & Hu (1998) with baryons */
+/* The scales of EH98 that T(k) below is built from, for the tests that pin it (sound horizon against
+   the paper's approximation, k_eq against its definition): out = (k_eq [h/Mpc], z_eq, z_d, s [Mpc/h]) */
+void orc_eh98_scales(const orc_cosmo *c, double Tcmb, double *out) {
+    double th = Tcmb / 2.7, th2 = th * th, th4 = th2 * th2;
+    double Om = orc_Omega_m(c);
+    double omh2 = Om * c->h * c->h;
+    double f_b = c->Omega_b / Om;
+    double obh2 = omh2 * f_b;
+    double z_eq = 2.50e4 * omh2 / th4;
+    double k_eq = 0.0746 * omh2 / th2 / c->h;
+    double z_d1 = 0.313 * pow(omh2, -0.419) * (1 + 0.607 * pow(omh2, 0.674));
+    double z_d2 = 0.238 * pow(omh2, 0.223);
+    double z_d = 1291.0 * pow(omh2, 0.251) / (1 + 0.659 * pow(omh2, 0.828)) * (1 + z_d1 * pow(obh2, z_d2));
+    double R_d = 31.5 * obh2 / th4 * (1000 / (1 + z_d));
+    double R_eq = 31.5 * obh2 / th4 * (1000 / z_eq);
+    out[0] = k_eq;
+    out[1] = z_eq;
+    out[2] = z_d;
+    out[3] = 2.0 / 3.0 / k_eq * sqrt(6 / R_eq) * log((sqrt(1 + R_d) + sqrt(R_d + R_eq)) / (1 + sqrt(R_eq)));
+}
+
 void orc_eisenstein_hu(const orc_cosmo *c, const double *k, int n, double Tcmb, double *T) {
     double th = Tcmb / 2.7, th2 = th * th, th4 = th2 * th2;
     double Om = orc_Omega_m(c);

@@ -63,6 +63,7 @@ int orc_stepper_coefficients(const orc_cosmo *c, int stepper, double a_ini, doub
                              double *dd1, double *dd2, double *all_a);
 
 void orc_eisenstein_hu(const orc_cosmo *c, const double *k, int n, double Tcmb, double *T);
+void orc_eh98_scales(const orc_cosmo *c, double Tcmb, double *out);
 void orc_linear_power_spectrum(const orc_cosmo *c, double kmin, double kmax, int n_modes,
                                double *k_out, double *pk_out);
 

@@ -73,6 +73,7 @@ _sig("orc_stepper_coefficients", C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_
                                            C.c_void_p, _dp, _dp, _dp, _dp, _dp])
 _sig("orc_eisenstein_hu", None, [C.c_void_p, _dp, C.c_int, C.c_double, _dp])
 _sig("orc_linear_power_spectrum", None, [C.c_void_p, C.c_double, C.c_double, C.c_int, _dp, _dp])
+_sig("orc_eh98_scales", None, [C.c_void_p, C.c_double, _dp])
 _sig("orc_fft1_d", None, [C.c_int, _dp, _dp, C.c_int])
 _sig("orc_fft1_f", None, [C.c_int, _fp, _fp, C.c_int])
 _sig("orc_rfft3_d", None, [C.c_int, _dp, _dp])
@@ -306,6 +307,13 @@ def eisenstein_hu(cosmo, k, Tcmb=2.72548):
     T = np.empty_like(k)
     lib.orc_eisenstein_hu(cosmo.ptr, k, len(k), Tcmb, T)
     return T
+
+
+def eh98_scales(cosmo, Tcmb=2.72548):
+    """(k_eq [h/Mpc], z_eq, z_d, sound horizon s [Mpc/h]) of the EH98 fit."""
+    out = np.empty(4)
+    lib.orc_eh98_scales(cosmo.ptr, Tcmb, out)
+    return tuple(out)
 
 
 def ic_operator(res, boxsize, k_tab, pk_tab):

@@ -8,10 +8,10 @@ Omega_b = 0.049, h = 0.6766, sigma8 = 0.8102, n_s = 0.9665.
 
 The oracle (oracle/, FP64, OpenMP) computes everything: white noise, 2LPT (compute_core on the 3n/2
 grid), the force at the initial conditions, the 20 steps and the P(k) of the final positions (CIC on the
-1024^3 mesh, window-deconvolved, linear bins of width kf from kf/2 to k_Nyq/2).  Stored (about 1.5 MB):
+1024^3 mesh, window-deconvolved, linear bins of width kf from kf/2 to k_Nyq/2).  Stored (about 0.6 MB):
   k, pk, nmodes                   the final P(k) (255 bins)
-  sub                             the subsample of particles kept: i, j, k all multiples of 16 (32^3)
-  psi_ini, pi_ini                 the initial state on the subsample, (3, 32, 32, 32) [c, k, j, i]
+  sub                             the subsample of particles kept: i, j, k all multiples of 32 (16^3)
+  psi_ini, pi_ini                 the initial state on the subsample, (3, 16, 16, 16) [c, k, j, i]
   acc_ini                         the force at the initial conditions on the subsample
   psi_fin, p_fin                  the final displacement and canonical momentum on the subsample
   psi_rms_fin                     rms of the final displacement over all particles
@@ -32,7 +32,7 @@ import oracle as O  # noqa: E402
 
 COSMO = (0.2621, 0.049, 0.6766, 0.8102, 0.9665)
 n, N, L, SEED, A_INI, A_END, STEPS = 512, 1024, 1000.0, 42, 0.05, 1.0, 20
-SUB = 16
+SUB = 32
 OUT = os.path.join(HERE, "pm512_mesh1024_fastpm20_fp64.npz")
 
 

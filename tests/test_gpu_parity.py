@@ -296,3 +296,32 @@ def test_snapshots_through_the_step_callback(tmp_path):
         S.run(co2["ddrift1"], co2["alpha"], co2["beta"], co2["ddrift2"])
         psi2, _ = S.get_state()
     assert G.rel(psi2, psi) < 1e-12
+
+
+def test_plane_wave_linear_growth_on_the_device():
+    """nbody_parity_tests.jl:68-90 on the device (FP64): a tiny fundamental plane wave follows D(a) under
+    every stepper (BullFrog 1 and 16 steps, FastPM 16, symplectic 64), stays one-dimensional, and its
+    growth error converges with the lattice (16^3 -> 32^3 at least 3x smaller); the device run also
+    equals the oracle's."""
+    c = D.Cosmology.named("Planck18EEBAOSN")
+    oc = O.Cosmology.named("Planck18EEBAOSN")
+    L, a0 = 100.0, 0.05
+    D0, D1 = c.Dplus(a0), c.Dplus(1.0)
+    err = {}
+    runs = (("bullfrog", 1), ("bullfrog", 16), ("fastpm", 16), ("symplectic", 64))
+    for n in (16, 32):
+        q = np.arange(n) * L / n
+        psi1 = np.zeros((3, n, n, n))
+        psi1[0] = 1e-3 * np.sin(2 * np.pi * q / L)[None, None, :]
+        for st, ns in runs:
+            psi, _, _ = D.run_nbody(c, D0 * psi1, psi1.copy(), boxsize=L, a_ini=a0, a_end=1.0, n_steps=ns,
+                                    res_pm=2 * n, stepper=st, return_displacement=True, precision=64)
+            assert np.max(np.abs(psi[1:])) < 1e-12
+            err[(n, st, ns)] = abs(np.sum(psi[0] * psi1[0]) / np.sum(psi1[0] ** 2) / D1 - 1)
+            if n == 16:
+                ref, _ = O.run_nbody(oc, D0 * psi1, psi1.copy(), O.pm_config(res_pm=2 * n, n_part=n, boxsize=L),
+                                     a0, 1.0, ns, stepper=st)
+                assert G.rel(psi, ref) < 1e-11
+    for st, ns in runs:
+        assert err[(32, st, ns)] < 1e-2
+        assert err[(16, st, ns)] / err[(32, st, ns)] > 3

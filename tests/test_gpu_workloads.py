@@ -177,7 +177,7 @@ def test_density_loss_gradient_256_central_differences():
         print(f"loss {loss:.10e} (fixed-point forward {l0:.10e}), |grad| {gnorm:.4e}")
         assert abs(l0 - loss) < 1e-8 * loss, (l0, loss)
         rng = np.random.default_rng(11)
-        eps = 2e-6
+        eps = 5e-7
         dirs = [wb * (np.sqrt(n**3) / gnorm)] + [rng.normal(size=(n, n, n)) for _ in range(4)]
         for i, v in enumerate(dirs):
             fd = (lossf(white + eps * v) - lossf(white - eps * v)) / (2 * eps)
@@ -185,7 +185,7 @@ def test_density_loss_gradient_256_central_differences():
             print(f"direction {i}: gradient {g:.8e}, central difference {fd:.8e}, |g - fd| / |grad| "
                   f"{abs(g - fd) / gnorm:.2e}, relative {abs(g - fd) / abs(fd):.2e}")
             if i == 0:
-                assert abs(g - fd) < 1e-4 * abs(fd), (i, g, fd)
+                assert abs(g - fd) < 3e-4 * abs(fd), (i, g, fd)
             else:
                 assert abs(g - fd) < 2e-3 * gnorm, (i, g, fd, gnorm)
     _, wb32, _ = D.density_loss_gradient(c, white, target, precision=32, **kw)

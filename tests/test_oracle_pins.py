@@ -90,6 +90,43 @@ def test_eisenstein_hu_limits():
     assert np.all(np.diff(T) < 0) and T[-1] > 0            # monotone decreasing, positive
 
 
+def test_eisenstein_hu_zero_baryon_limit():
+    """As Omega_b -> 0 the EH98 fit (arXiv:astro-ph/9709112) reduces to its zero-baryon form T0(q) of
+    eqs. 28-29: T0 = L0 / (L0 + C0 q^2), L0 = ln(e + 1.8 q), C0 = 14.2 + 386 / (1 + 69.9 q^1.08),
+    q = k / (13.41 k_eq) with k_eq = 0.0746 Omega_m h^2 / Theta^2 Mpc^-1 (eq. 3).  Pins the CDM branch,
+    the shape parameter q and k_eq's units (h/Mpc)."""
+    h, Om, Tcmb = 0.6766, 0.3111, 2.72548
+    c = O.Cosmology(Om - 1e-9, 1e-9, h, 0.8, 0.96)
+    k = np.logspace(-4, 1.5, 200)
+    T = O.eisenstein_hu(c, k, Tcmb)
+    theta = Tcmb / 2.7
+    k_eq = 0.0746 * Om * h * h / theta**2 / h          # h/Mpc
+    q = k / (13.41 * k_eq)
+    L0 = np.log(np.e + 1.8 * q)
+    C0 = 14.2 + 386.0 / (1.0 + 69.9 * q**1.08)
+    assert np.max(np.abs(T - L0 / (L0 + C0 * q * q))) < 1e-6
+    # with baryons the transfer function is suppressed below the zero-baryon form at k > k_eq
+    Tb = O.eisenstein_hu(O.Cosmology(0.2621, 0.049, h, 0.8, 0.96), k, Tcmb)
+    hi = k > 0.1
+    assert np.all(Tb[hi] < L0[hi] / (L0[hi] + C0[hi] * q[hi] ** 2))
+
+
+def test_eisenstein_hu_scales_against_their_definitions():
+    """k_eq = (2 Omega_m H0^2 z_eq)^(1/2) / c (EH98 eq. 3 is its numerical value, 0.0746 Omega_m h^2 Theta^-2
+    Mpc^-1 with z_eq of eq. 2), and the sound horizon of eq. 6 against the paper's approximation eq. 26,
+    s = 44.5 ln(9.83 / Omega_m h^2) / sqrt(1 + 10 (Omega_b h^2)^(3/4)) Mpc, good to 2 %: a wrong baryon
+    term in R_d, R_eq or z_d moves s by more."""
+    for (Oc, Ob, h) in ((0.2621, 0.049, 0.6766), (0.25, 0.03, 0.7), (0.3, 0.06, 0.65)):
+        c = O.Cosmology(Oc, Ob, h, 0.8, 0.96)
+        k_eq, z_eq, z_d, s = O.eh98_scales(c)
+        om, ob = (Oc + Ob) * h * h, Ob * h * h
+        k_def = np.sqrt(2 * (Oc + Ob) * (100.0 / 299792.458) ** 2 * z_eq)  # h/Mpc, H0 = 100 h km/s/Mpc
+        assert k_eq == pytest.approx(k_def, rel=2e-3)
+        s_fit = 44.5 * np.log(9.83 / om) / np.sqrt(1 + 10 * ob**0.75)        # Mpc
+        assert s / h == pytest.approx(s_fit, rel=0.02)
+        assert 1000 < z_d < 1100 < z_eq
+
+
 def test_sigma8_normalisation_recomputed():
     # runtests.jl:70-86: sigma8 recomputed from the normalised table by an independent integrator
     c = O.Cosmology(0.2621, 0.049, 0.6766, 0.8102, 0.9665)
@@ -288,7 +325,7 @@ def test_fixed_point_deposit_matches_float_and_is_order_free():
 
 
 def test_plane_wave_linear_growth():
-    # nbody_parity_tests.jl:68-90 for BullFrog with one and 16 steps and FastPM with 16 steps
+    # nbody_parity_tests.jl:68-90: BullFrog with one and 16 steps, FastPM with 16, symplectic with 64
     c = O.Cosmology.named("Planck18EEBAOSN")
     L, a0 = 100.0, 0.05
     D0, D1 = c.Dplus(a0), c.Dplus(1.0)
@@ -297,12 +334,12 @@ def test_plane_wave_linear_growth():
         q = np.arange(n) * L / n
         psi1 = np.zeros((3, n, n, n))
         psi1[0] = 1e-3 * np.sin(2 * np.pi * q / L)[None, None, :]
-        for st, ns in (("bullfrog", 1), ("bullfrog", 16), ("fastpm", 16)):
+        for st, ns in (("bullfrog", 1), ("bullfrog", 16), ("fastpm", 16), ("symplectic", 64)):
             cfg = O.pm_config(res_pm=2 * n, n_part=n, boxsize=L)
             psi, _ = O.run_nbody(c, D0 * psi1, psi1.copy(), cfg, a0, 1.0, ns, stepper=st)
             assert np.max(np.abs(psi[1:])) < 1e-12
             err[(n, st, ns)] = abs(np.sum(psi[0] * psi1[0]) / np.sum(psi1[0] ** 2) / D1 - 1)
-    for st, ns in (("bullfrog", 1), ("bullfrog", 16), ("fastpm", 16)):
+    for st, ns in (("bullfrog", 1), ("bullfrog", 16), ("fastpm", 16), ("symplectic", 64)):
         assert err[(32, st, ns)] < 1e-2
         assert err[(16, st, ns)] / err[(32, st, ns)] > 3
 
