@@ -969,13 +969,8 @@ dj_status dj_ics_lpt(dj_ctx *c, uint32_t seed, const double *white, const double
         return fail(c, DJ_ERR_UNSUPPORTED, "n_part " + std::to_string(n) + " has no built IC transform size");
     HIPCHK(c, hipSetDevice(c->dev));
     HIPCHK(c, hipStreamSynchronize(c->st));
-    std::vector<double> field;
-    if (!white) {
-        field.resize(2 * (size_t)n * n * (n / 2 + 1));
-        ngenic_field(seed, n, field.data());
-    }
-    int e = ics_lpt_device(n, c->cfg.boxsize, order, white, white ? nullptr : field.data(), k_tab, pk_tab, ntab, D,
-                           c->cfg.precision, c->psi, c->mom, c->st);
+    int e = ics_lpt_device(n, c->cfg.boxsize, order, white, seed, k_tab, pk_tab, ntab, D, c->cfg.precision, c->psi,
+                           c->mom, c->st);
     if (e) return hip_fail(c, (hipError_t)e, "dj_ics_lpt");
     return DJ_OK;
 }
@@ -1184,12 +1179,7 @@ dj_status dj_vjp_ics_lpt(dj_ctx *c, uint32_t seed, const double *white, const do
         c->vb_resident = false;
     }
     HIPCHK(c, hipStreamSynchronize(c->st));
-    std::vector<double> field;
-    if (!white) {
-        field.resize(2 * (size_t)n * n * (n / 2 + 1));
-        ngenic_field(seed, n, field.data());
-    }
-    int e = ics_lpt_vjp_device(n, c->cfg.boxsize, order, white, white ? nullptr : field.data(), k_tab, pk_tab, ntab,
+    int e = ics_lpt_vjp_device(n, c->cfg.boxsize, order, white, seed, k_tab, pk_tab, ntab,
                                D, c->cfg.precision, c->vb_psibar, c->vb_pibar, white_bar_out, c->st);
     if (e) return hip_fail(c, (hipError_t)e, "dj_vjp_ics_lpt");
     return DJ_OK;
@@ -1683,6 +1673,23 @@ dj_status dj_field_power_spectrum_noalloc(dj_ctx *c, const void *d_field, int32_
     pk_finalize(sums, ed, 1.0, 1, false, d_k_out, d_pk_out, d_nmodes_out, c->st);
     HIPCHK(c, hipGetLastError());
     return DJ_OK;
+}
+
+dj_status dj_ngenic_field_on_device(uint32_t seed, int32_t res, int32_t device, double *out) {
+    if (!out || res < 2 || res % 2) return fail(nullptr, DJ_ERR_INVALID_ARGUMENT, "invalid N-GenIC arguments");
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess) return hip_fail(nullptr, e, "hipSetDevice");
+    hipStream_t st = nullptr;
+    double *d = nullptr;
+    const size_t bytes = 2 * sizeof(double) * (size_t)res * res * (res / 2 + 1);
+    e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc(&d, bytes);
+    if (e == hipSuccess) e = (hipError_t)ngenic_field_device(seed, res, d, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d, bytes, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (d) (void)hipFree(d);
+    if (st) (void)hipStreamDestroy(st);
+    return e == hipSuccess ? DJ_OK : hip_fail(nullptr, e, "dj_ngenic_field_on_device");
 }
 
 dj_status dj_ngenic_field(uint32_t seed, int32_t res, double *out) {

@@ -187,10 +187,15 @@ double *pk_scratch(int dev);
 // initial conditions (ics.hip, ngenic.cpp)
 void ranlxd1_uniforms(uint32_t seed, int n, double *out);
 void ngenic_field(uint32_t seed, int res, double *out);
-int ics_lpt_device(int n, double L, int order, const double *white_host, const double *field_host,
+void ngenic_seed_table(uint32_t seed, int res, uint32_t *tab);
+// the N-GenIC field of seed on the device (the columns' generators run one per thread), (res, res, res/2+1)
+// complex, element (i, j, k) at i + res (j + res k), into device memory d_field; returns a HIP error code
+int ngenic_field_device(uint32_t seed, int res, double *d_field, hipStream_t st);
+// white noise: white_host (n^3 real values), or else the N-GenIC field of seed generated on the device
+int ics_lpt_device(int n, double L, int order, const double *white_host, uint32_t seed,
                    const double *kt, const double *pt, int ntab, double D, int prec, void *psi_out,
                    void *pi_out, hipStream_t st);
-int ics_lpt_vjp_device(int n, double L, int order, const double *white_host, const double *field_host,
+int ics_lpt_vjp_device(int n, double L, int order, const double *white_host, uint32_t seed,
                        const double *kt, const double *pt, int ntab, double D, int prec, const void *psibar,
                        const void *pibar, double *wbar_host, hipStream_t st);
 

@@ -1,5 +1,6 @@
 // Initial conditions of the PM path on the device, in FP64 (the IC pipeline of the reference):
-//   white noise w ............ N-GenIC field (host, ngenic.cpp) -> real space by the library's c2r
+//   white noise w ............ N-GenIC field (seed table on the host, ngenic.cpp; the column
+//                              generators on the device) -> real space by the library's c2r
 //                              (ngenic_wnoise_real, NR/src/ics/ngenic_gsl.jl:174-177)
 //   phi(k) ................... rfft(w) * (-sqrt(P(k) (n/L)^3) / k^2), 0 at k = 0 (grf.jl:118-144)
 //   psi_1 .................... irfft(-(i k_c) phi)  (nlpt_core.jl:285-297)
@@ -178,28 +179,139 @@ __global__ void k_ic_combine_adj(const T *psibar, const T *pibar, double *p1b, d
 
 static unsigned g1(size_t n) { return (unsigned)((n + 255) / 256); }
 
-// N-GenIC field (z halved, (n, n, n/2+1)) -> B layout (x halved) by Hermitian symmetry, times n^(3/2)
-static void field_to_blayout(int n, const double *field_host, cx<double> *hb) {
-    const int Hn = n / 2 + 1;
-    const double sc = std::pow((double)n, 1.5);
-    const cx<double> *f = (const cx<double> *)field_host;
-#pragma omp parallel for collapse(2)
-    for (int ky = 0; ky < n; ky++)
-        for (int kx = 0; kx < Hn; kx++)
-            for (int kz = 0; kz < n; kz++) {
-                cx<double> v;
-                if (kz <= n / 2) {
-                    v = f[(size_t)kx + n * ((size_t)ky + (size_t)n * kz)];
-                } else {
-                    int mx = (n - kx) % n, my = (n - ky) % n, mz = n - kz;
-                    v = conj(f[(size_t)mx + n * ((size_t)my + (size_t)n * mz)]);
-                }
-                hb[(size_t)kz + (size_t)n * (kx + (size_t)Hn * ky)] = scale(v, sc);
+// ---------------------------------------------------------------- N-GenIC white noise on the device
+// The column generators of ngenic_field_gsl (NR/src/ics/ngenic_gsl.jl:126-165): GSL ranlxd1 (RANLUX,
+// 48-bit subtract-with-borrow in exact double arithmetic, lags 12 and 5, 202 steps per 12 numbers
+// delivered; ngenic_gsl.jl:35-98), seeded from the host's seed table, one thread per (i, j) column.
+// The twelve state words of each thread live in LDS (stride 256, so a wave's accesses are consecutive)
+// because the recurrence indexes them by a running position.  The uniforms are the same bits as the
+// host's; sqrt(-log u), cos and sin are the device's FP64 functions (within an ulp or two of the host
+// libm's).
+namespace {
+constexpr double kNgOneBit = 1.0 / 281474976710656.0;  // 2^-48
+
+struct NgGen {
+    double *x;  // 12 words at stride 256
+    double carry;
+    int ir, jr, ir_old;
+    __device__ __forceinline__ double &w(int k) { return x[k * 256]; }
+    __device__ void seed(uint32_t s) {
+        if (s == 0) s = 1;
+        uint32_t bits = s & 0x7fffffffu;  // the 31 seed bits xbit[0..30]
+        int ib = 0, jb = 18;
+        for (int k = 0; k < 12; k++) {
+            double v = 0;
+            for (int m = 0; m < 48; m++) {
+                const uint32_t b = (bits >> ib) & 1u;
+                v += v + (double)(1u - b);
+                const uint32_t nb = b ^ ((bits >> jb) & 1u);
+                bits = (bits & ~(1u << ib)) | (nb << ib);
+                ib = ib == 30 ? 0 : ib + 1;
+                jb = jb == 30 ? 0 : jb + 1;
             }
+            w(k) = kNgOneBit * v;
+        }
+        carry = 0;
+        ir = 11;
+        jr = 7;
+        ir_old = 0;
+    }
+    __device__ __forceinline__ void swb() {
+        double y = w(jr) - w(ir) - carry;
+        if (y < 0) { carry = kNgOneBit; y += 1; } else { carry = 0; }
+        w(ir) = y;
+        ir = ir == 11 ? 0 : ir + 1;
+        jr = jr == 11 ? 0 : jr + 1;
+    }
+    __device__ double next() {
+        ir = ir == 11 ? 0 : ir + 1;
+        if (ir == ir_old) {
+            for (int k = 0; k < 202; k++) swb();
+            ir_old = ir;
+        }
+        return w(ir);
+    }
+};
+}  // namespace
+
+__global__ __launch_bounds__(256) void k_ngenic_columns(const uint32_t *__restrict__ tab, int res, double *out) {
+    __shared__ double xs[12 * 256];
+    const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= (size_t)res * res) return;  // no barrier below: each thread's words are its own
+    const int i = (int)(t % res), j = (int)(t / res), half = res / 2;
+    const int ii = i > 0 ? res - i : 0, jj = j > 0 ? res - j : 0;
+    const size_t R = (size_t)res;
+    NgGen g;
+    g.x = xs + threadIdx.x;
+    g.seed(tab[(size_t)i * R + j]);
+    const double twopi = 2 * M_PI;
+    for (int k = 0; k <= half; k++) {
+        const double phase = g.next() * twopi;
+        double ampl;
+        do { ampl = g.next(); } while (ampl == 0.0 || ampl == 1.0);
+        if (i == half || j == half || k == half) continue;
+        if (i == 0 && j == 0 && k == 0) continue;
+        ampl = sqrt(-log(ampl));
+        const double re = ampl * cos(phase), im = ampl * sin(phase);
+        auto put = [&](int a, int b, int c, double xr, double xi) {
+            const size_t e = (size_t)a + R * ((size_t)b + R * c);
+            out[2 * e] = xr;
+            out[2 * e + 1] = xi;
+        };
+        if (k > 0) {
+            put(i, j, k, re, im);
+        } else if (i == 0) {
+            if (j < half) {
+                put(0, j, 0, re, im);
+                put(0, jj, 0, re, -im);
+            }
+        } else if (i < half) {
+            put(i, j, 0, re, im);
+            put(ii, jj, 0, re, -im);
+        }
+    }
+}
+
+__global__ void k_fill_zero_f64(double *p, size_t n) {
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+        p[e] = 0.0;
+}
+
+int ngenic_field_device(uint32_t seed, int res, double *d_field, hipStream_t st) {
+    const size_t R = (size_t)res, nc = R * R * (res / 2 + 1);
+    std::vector<uint32_t> tab(R * R);
+    ngenic_seed_table(seed, res, tab.data());
+    uint32_t *d_tab = nullptr;
+    hipError_t e = hipMalloc(&d_tab, sizeof(uint32_t) * R * R);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_tab, tab.data(), sizeof(uint32_t) * R * R, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) {
+        k_fill_zero_f64<<<(unsigned)std::min<size_t>(8192, (2 * nc + 255) / 256), 256, 0, st>>>(d_field, 2 * nc);
+        k_ngenic_columns<<<g1(R * R), 256, 0, st>>>(d_tab, res, d_field);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (d_tab) (void)hipFree(d_tab);
+    return (int)e;
+}
+
+// N-GenIC field (z halved, (n, n, n/2+1)) -> B layout (x halved) by Hermitian symmetry, times n^(3/2)
+__global__ void k_field_to_blayout(const cx<double> *__restrict__ f, int n, double sc, cx<double> *__restrict__ B) {
+    const int H = n / 2 + 1;
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x, tot = (size_t)n * H * n;
+    if (e >= tot) return;
+    const BIdx k = bidx(e, n);
+    cx<double> v;
+    if (k.kz <= n / 2) {
+        v = f[(size_t)k.kx + n * ((size_t)k.ky + (size_t)n * k.kz)];
+    } else {
+        const int mx = (n - k.kx) % n, my = (n - k.ky) % n, mz = n - k.kz;
+        v = conj(f[(size_t)mx + n * ((size_t)my + (size_t)n * mz)]);
+    }
+    B[e] = scale(v, sc);
 }
 
 // host orchestration; returns a HIP error code (0 = success)
-int ics_lpt_device(int n, double L, int order, const double *white_host, const double *field_host,
+int ics_lpt_device(int n, double L, int order, const double *white_host, uint32_t seed,
                    const double *kt_h, const double *pt_h, int ntab, double D, int prec, void *psi_out,
                    void *pi_out, hipStream_t st) {
     FftOps fn{}, fe{};
@@ -249,10 +361,11 @@ int ics_lpt_device(int n, double L, int order, const double *white_host, const d
     if (white_host) {
         err = hipMemcpy(w, white_host, M * 8, hipMemcpyHostToDevice);
     } else {
-        std::vector<cx<double>> hb(HC);
-        field_to_blayout(n, field_host, hb.data());
-        err = hipMemcpy(Bm, hb.data(), HC * 16, hipMemcpyHostToDevice);
+        // the z-halved field (n n (n/2+1) complex values, the size of F, which is free until phi(k)),
+        // then to the B layout
+        err = (hipError_t)ngenic_field_device(seed, n, (double *)F, st);
         if (err == hipSuccess) {
+            k_field_to_blayout<<<g1(HC), 256, 0, st>>>(F, n, std::pow((double)n, 1.5), Bm);
             inv(fn, Bm, A, w, twn);
             k_ic_scale_real<<<g1(M), 256, 0, st>>>(w, M, 1.0 / (double)M);
         }
@@ -320,7 +433,7 @@ done:
 // cotangent wbar (host, n^3 FP64).  The forward fields are recomputed (phi(k), the nine A_ik on the ext
 // grid, three at a time); each linear step is transposed with the adjoint rules above, and the 2LPT
 // source mu is a quadratic form whose cotangent per field is mubar times its partner in mu.
-int ics_lpt_vjp_device(int n, double L, int order, const double *white_host, const double *field_host,
+int ics_lpt_vjp_device(int n, double L, int order, const double *white_host, uint32_t seed,
                        const double *kt_h, const double *pt_h, int ntab, double D, int prec, const void *psibar,
                        const void *pibar, double *wbar_host, hipStream_t st) {
     FftOps fn{}, fe{};
@@ -370,10 +483,11 @@ int ics_lpt_vjp_device(int n, double L, int order, const double *white_host, con
     if (white_host) {
         err = hipMemcpy(w, white_host, M * 8, hipMemcpyHostToDevice);
     } else {
-        std::vector<cx<double>> hb(HC);
-        field_to_blayout(n, field_host, hb.data());
-        err = hipMemcpy(Bm, hb.data(), HC * 16, hipMemcpyHostToDevice);
+        // the z-halved field (n n (n/2+1) complex values, the size of F, which is free until phi(k)),
+        // then to the B layout
+        err = (hipError_t)ngenic_field_device(seed, n, (double *)F, st);
         if (err == hipSuccess) {
+            k_field_to_blayout<<<g1(HC), 256, 0, st>>>(F, n, std::pow((double)n, 1.5), Bm);
             inv(fn, Bm, A, w, twn);
             k_ic_scale_real<<<g1(M), 256, 0, st>>>(w, M, 1.0 / (double)M);
         }

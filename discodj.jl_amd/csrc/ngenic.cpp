@@ -85,12 +85,13 @@ void ranlxd1_uniforms(uint32_t seed, int n, double *out) {
     for (int i = 0; i < n; i++) out[i] = ranlxd_next(st);
 }
 
-// field (res, res, res/2+1) complex interleaved, element (i, j, k) at i + res (j + res k)
-void ngenic_field(uint32_t seed, int res, double *out) {
-    const int half = res / 2, nh = res / 2 + 1;
+// The seed of every (i, j) column's generator, tab[i res + j] (ngenic_gsl.jl:103-119): res^2 draws of
+// the master generator along the symmetric spiral; sequential, a few milliseconds at res = 1024
+void ngenic_seed_table(uint32_t seed, int res, uint32_t *tab_out) {
+    const int half = res / 2;
     const size_t R = (size_t)res;
-    std::memset(out, 0, sizeof(double) * 2 * R * R * nh);
-    std::vector<uint32_t> tab(R * R, 0);
+    uint32_t *tab = tab_out;
+    std::memset(tab, 0, sizeof(uint32_t) * R * R);
     Ranlxd master;
     ranlxd_seed(master, seed);
     auto draw = [&]() { return (uint32_t)std::floor(2147483647.0 * ranlxd_next(master)); };
@@ -104,6 +105,15 @@ void ngenic_field(uint32_t seed, int res, double *out) {
         for (int j = 0; j < i; j++) tab[(R - 1 - i) * R + (R - 1 - j)] = draw();
         for (int j = 0; j <= i; j++) tab[(R - 1 - j) * R + (R - 1 - i)] = draw();
     }
+}
+
+// field (res, res, res/2+1) complex interleaved, element (i, j, k) at i + res (j + res k)
+void ngenic_field(uint32_t seed, int res, double *out) {
+    const int half = res / 2, nh = res / 2 + 1;
+    const size_t R = (size_t)res;
+    std::memset(out, 0, sizeof(double) * 2 * R * R * nh);
+    std::vector<uint32_t> tab(R * R, 0);
+    ngenic_seed_table(seed, res, tab.data());
     const double twopi = 2 * M_PI;
 #pragma omp parallel for collapse(2) schedule(dynamic, 16)
     for (int i = 0; i < res; i++)

@@ -73,6 +73,7 @@ _SIGS = {
     "dj_field_power_spectrum_noalloc": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p,
                                                     C.c_void_p, C.c_void_p]),
     "dj_ngenic_field": (C.c_int32, [C.c_uint32, C.c_int32, C.c_void_p]),
+    "dj_ngenic_field_on_device": (C.c_int32, [C.c_uint32, C.c_int32, C.c_int32, C.c_void_p]),
     "dj_ranlxd1_uniforms": (C.c_int32, [C.c_uint32, C.c_int32, C.c_void_p]),
     "dj_vjp_acceleration": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "dj_vjp_deposit": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),

@@ -345,6 +345,13 @@ def fft3d(x, direction=-1, device=0):
     return out
 
 
+def ngenic_field_on_device(seed, res, device=0):
+    """The N-GenIC Fourier white noise generated on the GPU: (res//2+1, res, res) complex [kz, j, i]."""
+    out = np.empty((res // 2 + 1, res, res), np.complex128)
+    check(_lib.lib().dj_ngenic_field_on_device(seed, res, device, out.ctypes.data))
+    return out
+
+
 def ngenic_field(seed, res):
     """N-GenIC Fourier white noise (host): (res//2+1, res, res) complex [kz, j, i] (halved along z)."""
     out = np.empty((res // 2 + 1, res, res), np.complex128)

@@ -119,7 +119,7 @@ dj_status dj_deposit(dj_ctx *ctx, const void *psi, const double *shift, void *me
  * context's precision: the IC pipeline of the reference (ngenic_ic, ngenic_gsl.jl:187-191; ic_operator /
  * white_noise_to_fphi, grf.jl:118-144; compute_core, nlpt_core.jl:311-362 and nlpt_fast.jl:208-249;
  * nbody_ics_lpt, run_nbody.jl:180-208).  White noise: `white` (host, n^3 real values, column-major) or,
- * when NULL, the N-GenIC field of `seed` (GSL ranlxd1).  phi(k) = rfft(w) sqrt(P(k) (n/L)^3) (-1/k^2)
+ * when NULL, the N-GenIC field of `seed` (GSL ranlxd1, generated on the device).  phi(k) = rfft(w) sqrt(P(k) (n/L)^3) (-1/k^2)
  * with P interpolated linearly in the table (k_tab, pk_tab; ntab entries, k ascending).  order 1 or 2
  * (de-aliased on the 3n/2 grid).  Psi = sum_m D^m psi_m, Pi = sum_m m D^(m-1) psi_m (EdS growth). */
 dj_status dj_ics_lpt(dj_ctx *ctx, uint32_t seed, const double *white, const double *k_tab, const double *pk_tab,
@@ -177,6 +177,11 @@ dj_status dj_field_power_spectrum_noalloc(dj_ctx *ctx, const void *d_field, int3
  * ngenic_gsl.jl:126-165), (res, res, res/2+1) interleaved complex, halved along z, element (i,j,k)
  * at i + res*(j + res*k); and the GSL ranlxd1 stream (ngenic_gsl.jl:34-98). */
 dj_status dj_ngenic_field(uint32_t seed, int32_t res, double *out);
+/* the same field generated on GPU `device` (ngenic_field_gsl, ngenic_gsl.jl:126-165): the seed table on the
+ * host, each (i, j) column's ranlxd1 generator in one device thread; the uniforms are the host's bits,
+ * sqrt(-log u), cos and sin the device's FP64 functions (within 2 ulp of the host's).  dj_ics_lpt without
+ * white noise generates its field this way. */
+dj_status dj_ngenic_field_on_device(uint32_t seed, int32_t res, int32_t device, double *out);
 dj_status dj_ranlxd1_uniforms(uint32_t seed, int32_t n, double *out);
 
 /* Reverse mode (no counterpart in the reference, whose PM path has no adjoint: NR/README.md:169; its

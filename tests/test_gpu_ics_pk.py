@@ -154,3 +154,17 @@ def test_spectra_reject_bad_arguments():
             S.field_power_spectrum(f, bins=257)
         with pytest.raises(D.DJError):
             S.power_spectra_binned(np.array([1.0, 0.5, 2.0]))
+
+
+@pytest.mark.parametrize("seed, res", [(42, 8), (7, 16), (123, 24), (42, 64), (1000, 128)])
+def test_ngenic_field_on_the_device(seed, res):
+    """The N-GenIC field generated on the device against the host's (ngenic_field_gsl): the same zero
+    pattern (Nyquist planes, DC, the Hermitian kz = 0 plane), values within 2 ulp (the device's sqrt, log,
+    cos and sin against the host libm's; the ranlxd1 uniforms underneath are integer-exact), and the
+    reference's known answer |field(42, 8)| = 13.448527 (runtests.jl:417)."""
+    a = D.ngenic_field_on_device(seed, res)
+    b = D.ngenic_field(seed, res)
+    assert np.array_equal(a == 0, b == 0)
+    assert np.max(np.abs(a - b)) <= 2 * np.finfo(np.float64).eps * np.max(np.abs(b))
+    if (seed, res) == (42, 8):
+        assert np.linalg.norm(a) == pytest.approx(G.KNOWN["ngenic_field_gsl_42_8_norm"], abs=G.KNOWN["ngenic_field_gsl_42_8_norm_atol"])
