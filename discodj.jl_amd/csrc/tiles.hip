@@ -969,6 +969,155 @@ __global__ __launch_bounds__((gather_threads<T, W>()), (gather_min_waves<T, W>()
     }
 }
 
+// Gather that marches along z.  A workgroup takes a segment of G consecutive tiles of one column (fixed
+// tx, ty) and keeps the staged potential in an LDS ring of S2 planes: tile t + 1's region is tile t's
+// shifted by TZ planes, so after tile t only its TZ new planes are staged, into the ring slots of the TZ
+// planes tile t + 1 no longer needs (TZ of S2 = TZ + RL + RS + 2H planes instead of S2: 8 of 17 for TSC
+// at order 6).  A particle's Q planes sit at ring slots (sb + lb_z + z) mod S2, so each of its z planes
+// gets one base register and every LDS read is that base plus an immediate offset, as in the staged
+// gather above.  The segments are dealt so that the workgroups an XCD (workgroup b % 8) runs together
+// are the same z segment of a 4 x 4 patch of neighbouring columns, whose x and y halos then meet in that
+// XCD's L2; the XCDs' patches interleave over the mesh, which spreads clustered regions over all of them.
+// Used for the FP32 32 x 16 x 8 tiles with the compile-time staging geometry; the arithmetic per particle
+// is the staged gather's (same sums, same order).
+struct MarchMap {
+    int G;          // tiles per segment (divides ntz)
+    bool patches;   // 4 x 4 column patches dealt round-robin to the XCDs
+};
+__host__ __device__ __forceinline__ MarchMap march_map(const TileGeom &g) {
+    MarchMap m;
+    m.G = (g.ntz % 16 == 0) ? 16 : (g.ntz % 8 == 0) ? 8 : (g.ntz % 4 == 0) ? 4 : (g.ntz % 2 == 0) ? 2 : 1;
+    m.patches = g.ntx % 4 == 0 && g.nty % 4 == 0 && ((g.ntx / 4) * (g.nty / 4)) % 8 == 0;
+    return m;
+}
+__device__ __forceinline__ void march_segment(int b, const TileGeom &g, const MarchMap &m, int &tx, int &ty, int &tz0) {
+    const int ncol = g.ntx * g.nty;
+    int col, seg;
+    if (m.patches) {
+        const int xcd = b & 7, k = b >> 3;
+        const int per = ncol / 8;  // columns per XCD: (ncol / 16) / 8 patches of 16
+        seg = k / per;
+        const int r = k % per, pi = r >> 4, ci = r & 15;
+        const int gp = pi * 8 + xcd, npx = g.ntx / 4;
+        tx = (gp % npx) * 4 + (ci & 3);
+        ty = (gp / npx) * 4 + (ci >> 2);
+        tz0 = seg * m.G;
+        return;
+    }
+    col = b % ncol;
+    seg = b / ncol;
+    tx = col % g.ntx;
+    ty = col / g.ntx;
+    tz0 = seg * m.G;
+}
+
+template <typename T, int W, int ORDER, int MODE>
+__global__ __launch_bounds__((gather_threads<T, W>()), (gather_min_waves<T, W>())) void k_tile_gather_march(
+    const Entry<T> *__restrict__ ent, const unsigned *__restrict__ start, const unsigned *__restrict__ own, TileGeom g,
+    T sx, T sy, T sz, T L, const T *__restrict__ phi, T c1, T c2, T c3, const T *psi_in, T *psi_out,
+    T *__restrict__ mom, T *__restrict__ acc, size_t np, T alpha, T beta, T dd2, T dd1n, int has_next, int ns,
+    int lacc) {
+    constexpr int H = ORDER / 2, Q = W + 2 * H, NT = gather_threads<T, W>();
+    extern __shared__ unsigned char smem[];
+    T *ph = (T *)smem;
+    const StageGeom q = gather_geom<T, W, true>(g, H);
+    constexpr int TXL = 5, TYL = 4, TZ = FIXED_TZ;
+    const int S[3] = {q.s[0], q.s[1], q.s[2]};
+    const int N = g.N, SX = S[0], SXY = S[0] * S[1], R = S[2];  // R: ring planes
+    const MarchMap m = march_map(g);
+    int tx, ty, tz0;
+    march_segment(blockIdx.x, g, m, tx, ty, tz0);
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int rx = (tx << TXL) - q.lo[0], ry = (ty << TYL) - q.lo[1];
+    const int zs0 = tz0 * TZ - q.lo[2];  // first staged plane of the segment (ring slot 0)
+    const T xsh[3] = {sx, sy, sz};
+
+    for (int it = 0; it < m.G; it++) {
+        const int tz = tz0 + it;
+        const int t = tx + g.ntx * (ty + g.nty * tz);
+        const int r0[3] = {rx, ry, tz * TZ - q.lo[2]};
+        const unsigned e0 = start[t], e1 = e0 + own[t];
+        unsigned e = e0 + threadIdx.x;
+        Entry<T> en{};
+        T m3[3] = {0, 0, 0}, p3[3] = {0, 0, 0}, a3[3] = {0, 0, 0};
+        if (e < e1) gather_fetch<T, MODE>(ent, e, psi_in, mom, acc, np, ns, lacc, en, m3, p3, a3);
+        // stage: the whole region for the first tile, else the TZ planes after the previous region,
+        // into the ring slots (zg - zs0) mod R, split where the slots wrap
+        {
+            const int zfirst = it == 0 ? r0[2] : r0[2] + S[2] - TZ;
+            const int nplanes = it == 0 ? S[2] : TZ;
+            const int slot = (zfirst - zs0) % R;
+            const int n1 = min(nplanes, R - slot);
+            const int ra[3] = {rx, ry, zfirst}, sa[3] = {S[0], S[1], n1};
+            stage_groups<T, NT>(phi, ph + (size_t)slot * SXY, ra, sa, N, lane, wave);
+            if (n1 < nplanes) {
+                const int rb[3] = {rx, ry, zfirst + n1}, sb2[3] = {S[0], S[1], nplanes - n1};
+                stage_groups<T, NT>(phi, ph, rb, sb2, N, lane, wave);
+            }
+            __builtin_amdgcn_s_waitcnt(0);
+        }
+        __syncthreads();
+        const int sbase = (r0[2] - zs0) % R;  // ring slot of this tile's first staged plane
+        while (e < e1) {
+            const T xe[3] = {en.x, en.y, en.z};
+            T w[3][W];
+            int c0[3], lb[3];
+            bool staged = true;
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                const T gg = tjmod(xe[a] + xsh[a], L) / L * (T)N;
+                const int b = twbase<W>(gg);
+#pragma unroll
+                for (int o = 0; o < W; o++) w[a][o] = twk_at<W>(o, fabs(gg - (T)(b + twlo<W>() + o)));
+                c0[a] = b + twlo<W>() - H;
+                lb[a] = wrapn(c0[a] - r0[a], N);
+                staged &= lb[a] + Q <= S[a];
+            }
+            T s1 = 0, s2 = 0, s3 = 0;
+            if (staged) {
+                // plane z of the stencil at ring slot sl0 + z, wrapping once, at plane zw: base A for the
+                // planes before it, B = A - R SXY after it, the rest an immediate offset
+                int sl0 = sbase + lb[2];
+                sl0 = sl0 >= R ? sl0 - R : sl0;
+                const int A = lb[0] + SX * lb[1] + sl0 * SXY, B = A - R * SXY, zw = R - sl0;
+                auto ld = [&](int x, int y, int z) { return ph[(z >= zw ? B : A) + x + SX * y + SXY * z]; };
+                gather_sums_union<T, W, ORDER>(ld, w, c1, c2, c3, s1, s2, s3);
+            } else {
+                auto ld = [&](int x, int y, int z) {
+                    return phi[(size_t)wrapn(c0[0] + x, N) +
+                               (size_t)N * ((size_t)wrapn(c0[1] + y, N) + (size_t)N * wrapn(c0[2] + z, N))];
+                };
+                gather_sums<T, W, ORDER>(ld, w, c1, c2, c3, s1, s2, s3);
+            }
+            const size_t p = (size_t)en.idx;
+            const T f3[3] = {s1, s2, s3};
+            const size_t ai = lacc ? 3 * (size_t)e : p, as = lacc ? 1 : np;
+            if constexpr (MODE == GATHER_ACC) {
+#pragma unroll
+                for (int c = 0; c < 3; c++) acc[ai + c * as] = a3[c] + f3[c];
+            } else if constexpr (MODE == GATHER_ASSIGN) {
+#pragma unroll
+                for (int c = 0; c < 3; c++) acc[ai + c * as] = f3[c];
+            } else {
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    T ac = f3[c];
+                    if (ns > 1) ac = (acc[ai + c * as] + ac) / (T)ns;
+                    T mm = alpha * m3[c] + beta * ac;
+                    T sp = p3[c] + dd2 * mm;
+                    if (has_next) sp = sp + dd1n * mm;
+                    mom[p + c * np] = mm;
+                    psi_out[p + c * np] = sp;
+                }
+            }
+            e += NT;
+            if (e < e1) gather_fetch<T, MODE>(ent, e, psi_in, mom, acc, np, ns, lacc, en, m3, p3, a3);
+        }
+        // every wave is done with this tile's first TZ planes before the next tile's staging overwrites them
+        __syncthreads();
+    }
+}
+
 // ------------------------------------------------------------------------------------ launchers
 template <typename T>
 struct TileLaunch {
@@ -1059,9 +1208,26 @@ struct TileLaunch {
                                         (T)k.beta, (T)k.dd2, (T)k.dd1_next, k.has_next, k.nshifts, a.acc_by_entry);
     }
     template <int W, int O, int M>
+    static void gat_march(const TileArgs &a, const double *sh, const void *phi, const double *c, const KickArgs &k,
+                          void *psi_out, hipStream_t st) {
+        constexpr int H = O / 2;
+        const StageGeom q = gather_geom<T, W, true>(a.g, H);
+        const size_t shm = sizeof(T) * (size_t)q.s[0] * q.s[1] * q.s[2];
+        const MarchMap mm = march_map(a.g);
+        auto kern = k_tile_gather_march<T, W, O, M>;
+        (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+        kern<<<a.g.nt / mm.G, gather_threads<T, W>(), shm, st>>>(
+            (const Entry<T> *)a.ent, a.start, a.cur, a.g, (T)sh[0], (T)sh[1], (T)sh[2], (T)a.L, (const T *)phi,
+            (T)c[0], (T)c[1], (T)c[2], (const T *)a.psi, (T *)psi_out, (T *)a.mom, (T *)a.acc,
+            (size_t)a.n * a.n * a.n, (T)k.alpha, (T)k.beta, (T)k.dd2, (T)k.dd1_next, k.has_next, k.nshifts,
+            a.acc_by_entry);
+    }
+    template <int W, int O, int M>
     static void gat_wom(const TileArgs &a, const double *sh, const void *phi, const double *c, const KickArgs &k,
                         void *psi_out, hipStream_t st) {
         if constexpr (sizeof(T) == 4) {
+            if (fixed_geom_fits<W>(a.g) && gather_geom<T, W, true>(a.g, O / 2).ok)
+                return gat_march<W, O, M>(a, sh, phi, c, k, psi_out, st);
             if (fixed_geom_fits<W>(a.g)) return gat_womf<W, O, M, true>(a, sh, phi, c, k, psi_out, st);
         }
         gat_womf<W, O, M, false>(a, sh, phi, c, k, psi_out, st);
