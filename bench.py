@@ -23,8 +23,8 @@ Reported beside the throughput:
                  gradient at 256^3 (gradients/s)
 
 usage: python bench.py [--gpus 1] [--steps 20] [--warmup 2] [--no-accuracy] [--no-cpu] [--no-extra] [--small]
-       python bench.py --workload gradient|learned      (the differentiation and learned-correction
-                                                        workloads at 256^3; one JSON line each)
+       python bench.py --workload gradient|learned|studies  (the differentiation, learned-correction and
+                                                        discreteness-suppressed workloads; one JSON line each)
 """
 import argparse
 import json
@@ -52,7 +52,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the 256^3 throughput and gradient lines")
     ap.add_argument("--small", action="store_true", help="128^3 particles on a 256^3 mesh (quick check)")
-    ap.add_argument("--workload", choices=["pm", "gradient", "learned"], default="pm")
+    ap.add_argument("--workload", choices=["pm", "gradient", "learned", "studies"], default="pm")
     ap.add_argument("--copy-rate", action="store_true", help="only time a 1 GiB read + 1 GiB write copy kernel")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy kernel three times (the reference point that scales "
@@ -87,6 +87,8 @@ def main():
         return bench_gradient(args)
     if args.workload == "learned":
         return bench_learned(args)
+    if args.workload == "studies":
+        return bench_studies(args)
     if args.gpus != 1:
         print(f"bench.py measures one GPU (one MI355X); --gpus {args.gpus} is not run: the multi-GPU slab/pencil "
               f"decomposition of the mesh is designed, not implemented", file=sys.stderr)
@@ -333,6 +335,60 @@ def bench_gradient(args, as_dict=False):
     if as_dict:
         return out
     print(json.dumps(out), flush=True)
+    return 0
+
+
+def bench_studies(args):
+    """The discreteness-suppressed PM of the reference's shell-crossing study (studies/eft_shell_crossing/
+    config/fiducial.toml:17-30): 256^3 particles on a 512^3 mesh, L = 500 Mpc/h, PCS with window
+    deconvolution, spectral ik gradient, interlacing, 2^3 Fourier sheet resampling, BullFrog 200 steps uniform
+    in D from D = 0.02 to 1.  ICs: 2LPT of the N-GenIC seed 42 field (the study starts from 3LPT), FP32.
+    Seconds per step over the 200 steps (median of 3 runs from the ICs), beside the reference's 6.1 min per
+    job of 200 steps on an RTX A6000 (about 1.8 s per step, diagnostics included)."""
+    import numpy as np
+
+    import discodj_amd as D
+
+    n, N, L, steps = (64, 128, 125.0, 20) if args.small else (256, 512, 500.0, 200)
+    c = D.compute_timetables(D.Cosmology(**COSMO))
+    k_tab, pk_tab = D.linear_power_spectrum(c)
+    a_ini = c.a_of_Dplus(0.02)
+    co = D.stepper_coefficients(c, "bullfrog", a_ini, 1.0, steps, "D")
+    coef = [co["ddrift1"], co["alpha"], co["beta"], co["ddrift2"]]
+    cfg = D.PMConfig(res_pm=N, n_part=n, boxsize=L, worder=4, deconvolve=True, antialias=1, grad_order=0,
+                     n_resample=2, resampling="fourier")
+    with D.PMSolverHIP(cfg, precision=32) as S:
+        S.ics_lpt(k_tab, pk_tab, c.Dplus(a_ini), order=2, seed=42)
+        p0, m0 = S.get_state()
+        S.run(*(a[:2] for a in coef))  # warm-up
+        times = []
+        for rep in range(3):
+            S.set_state(p0, m0)
+            S.synchronize()
+            t0 = time.perf_counter()
+            S.run(*coef)
+            S.synchronize()
+            times.append(time.perf_counter() - t0)
+            log(f"studies run {rep}: {steps} steps in {times[-1]:.3f} s")
+        S.set_state(p0, m0)
+        S.set_profiling(True)
+        S.reset_stats()
+        S.run(*(a[:20] for a in coef))
+        S.synchronize()
+        stats = S.kernel_stats()
+        S.set_profiling(False)
+    dt = float(np.median(times)) / steps
+    kernels = {k: {"calls": v["calls"], "ms_per_call": round(v["ms_total"] / v["calls"], 4)}
+               for k, v in stats.items() if v["calls"]}
+    print(json.dumps({"metric": "seconds per step of the discreteness-suppressed PM (shell-crossing study settings)",
+                      "value": dt, "unit": "s/step", "higher_is_better": False,
+                      "particle_updates_per_s": n**3 / dt, "reference": {"value": 1.8, "unit": "s/step",
+                      "hardware": "1x RTX A6000, diagnostics included (studies/eft_shell_crossing/README.md:1023)"},
+                      "kernels_first_20_steps": kernels, "dtype": "f32", "data": "synthetic", "n_gpus": 1,
+                      "config": {"workload": f"{n}^3 particles, {N}^3 mesh, L={L}, PCS, deconvolution, ik gradient, "
+                                             f"interlacing, 2^3 Fourier sheet resampling, BullFrog {steps} steps "
+                                             f"D = 0.02 -> 1, 2LPT ICs", "timing": "median of 3 full runs"}}),
+          flush=True)
     return 0
 
 

@@ -42,6 +42,15 @@ struct ProfEvent {
 
 }  // namespace
 
+// tile lists of one point set: counts, list starts, fill cursors, overflow flag, scan block sums, entries
+struct TileLists {
+    unsigned *cnt = nullptr, *start = nullptr, *gstart = nullptr, *cur = nullptr, *flag = nullptr;
+    unsigned long long *part = nullptr;  // block sums of the tile-list scan
+    bool lists = false;   // cur holds the counts of a completed sort (capacities for the next one)
+    void *ent = nullptr;
+    size_t ent_cap = 0;   // entries
+};
+
 struct dj_ctx {
     dj_pm_config cfg{};
     int tsize = 4;  // bytes of the working precision
@@ -62,11 +71,9 @@ struct dj_ctx {
     double fdc[3] = {0, 0, 0};
     TileOps tiles{};
     TileGeom tg{};
-    unsigned *tcnt = nullptr, *tstart = nullptr, *tgstart = nullptr, *tcur = nullptr, *tflag = nullptr;
-    unsigned long long *tpart = nullptr;  // block sums of the tile-list scan
-    bool tlists = false;   // tcur holds the counts of a completed sort (capacities for the next one)
-    void *tent = nullptr;
-    size_t tent_cap = 0;   // entries
+    TileLists tl{};   // the particles' lists
+    TileLists tlr{};  // the lists of the sheet-resampling point sets (n_resample > 1, allocated on first use)
+    void *rl_pts = nullptr, *rl_q0 = nullptr;  // the point sets' positions (3, (2n)^3), a zero lattice (2n)
     double *d_shifts = nullptr;  // (16 + 1, 3): the interlacing shifts, then one free slot
     // sheet resampling (n_resample > 1): particle-grid FFT ops and buffers
     ResampleOps rs{};
@@ -79,6 +86,10 @@ struct dj_ctx {
     void *rs_phase = nullptr;                   // px (n), py (n), pz (n/2+1) of the current offset
     int n_offsets = 0;
     double offsets[64][3];
+    int ioffsets[64][3];  // the offsets in units of 1 / n_resample cells
+    // the pass's point sets (the particles and every offset's) deposited together through tile lists
+    // (tlr, rl_pts) when their (n_resample n)^3 points index in 31 bits; device atomics otherwise
+    bool rs_tiled = false;
     // reverse mode: work buffers (allocated on first use) and the step checkpoints of dj_run_checkpointed
     AdjointOps adj{};
     void *vb_psi1 = nullptr, *vb_abar = nullptr, *vb_psibar = nullptr, *vb_pibar = nullptr;
@@ -318,9 +329,11 @@ int aa_shifts(int aa, double dh, double (*s)[3]) {
     return n;
 }
 
+void free_lists(TileLists &tl);
+
 void free_all(dj_ctx *c) {
     void *bufs[] = {c->psi, c->mom, c->acc, c->psi_in, c->q1, c->mesh, c->meshfix, c->A, c->B, c->C, c->tw, c->tabs,
-                    c->tcnt, c->tstart, c->tgstart, c->tcur, c->tflag, c->tpart, c->tent, c->d_shifts,
+                    c->d_shifts, c->rl_pts, c->rl_q0,
                     c->rs_tw, c->rs_psi, c->rs_hat, c->rs_A, c->rs_B, c->rs_xr, c->rs_phase,
                     c->vb_psi1, c->vb_abar, c->vb_psibar, c->vb_pibar, c->vb_fbar, c->vb_pbar, c->ckpt,
                     c->nn_p, c->nn_g, c->nn_m, c->nn_v, c->nn_mask, c->nn_wf, c->nn_wb, c->nn_pf, c->nn_pb,
@@ -331,6 +344,8 @@ void free_all(dj_ctx *c) {
     // teardown: failures here have no caller to report to
     for (void *b : bufs)
         if (b) (void)hipFree(b);
+    free_lists(c->tl);
+    free_lists(c->tlr);
     for (auto &p : c->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
     for (auto e : c->pool) (void)hipEventDestroy(e);
     if (c->st) (void)hipStreamDestroy(c->st);
@@ -367,90 +382,123 @@ void set_bytes(dj_ctx *c) {
 double fscale_dep(const dj_pm_config &cfg) { return std::ldexp(1.0, cfg.fixed_frac_bits ? cfg.fixed_frac_bits : 32); }
 
 // ------------------------------------------------------------------------ tile binning
-TileArgs tile_args(dj_ctx *c, const void *psi, const double *d_shifts, int ns) {
+// the tile lists tl of a point set: positions psi + q1 on the lattice n^3 (the particles: Psi and q; the
+// sheet-resampling point sets: their positions and a zero lattice)
+TileArgs tile_args_of(dj_ctx *c, TileLists &tl, const void *psi, const void *q1, int n, const double *d_shifts,
+                      int ns) {
     TileArgs a{};
     a.psi = psi;
     a.mom = c->mom;
     a.acc = c->acc;
-    a.q1 = c->q1;
-    a.n = c->n;
+    a.q1 = q1;
+    a.n = n;
     a.L = c->cfg.boxsize;
     a.shifts = d_shifts;
     a.ns = ns;
     a.g = c->tg;
-    a.cnt = c->tcnt;
-    a.start = c->tstart;
-    a.gstart = c->tgstart;
-    a.cur = c->tcur;
-    a.flag = c->tflag;
-    a.scan_part = c->tpart;
-    a.ent = c->tent;
-    a.ent_cap = c->tent_cap;
+    a.cnt = tl.cnt;
+    a.start = tl.start;
+    a.gstart = tl.gstart;
+    a.cur = tl.cur;
+    a.flag = tl.flag;
+    a.scan_part = tl.part;
+    a.ent = tl.ent;
+    a.ent_cap = tl.ent_cap;
     return a;
+}
+TileArgs tile_args(dj_ctx *c, const void *psi, const double *d_shifts, int ns) {
+    return tile_args_of(c, c->tl, psi, c->q1, c->n, d_shifts, ns);
+}
+
+dj_status alloc_lists(dj_ctx *c, TileLists &tl, size_t ent_cap) {
+    const TileGeom &g = c->tg;
+    HIPCHK(c, hipMalloc(&tl.cnt, sizeof(unsigned) * 2 * g.nt));
+    HIPCHK(c, hipMalloc(&tl.cur, sizeof(unsigned) * 2 * g.nt));
+    HIPCHK(c, hipMalloc(&tl.start, sizeof(unsigned) * (g.nt + 1)));
+    HIPCHK(c, hipMalloc(&tl.gstart, sizeof(unsigned) * g.nt));
+    HIPCHK(c, hipMalloc(&tl.flag, sizeof(unsigned)));
+    HIPCHK(c, hipMalloc(&tl.part, sizeof(unsigned long long) * (g.nt / 1024 + 2)));
+    tl.lists = false;
+    tl.ent_cap = ent_cap;
+    HIPCHK(c, hipMalloc(&tl.ent, tl.ent_cap * c->tiles.entry_bytes));
+    return DJ_OK;
+}
+
+void free_lists(TileLists &tl) {
+    void *b[] = {tl.cnt, tl.start, tl.gstart, tl.cur, tl.flag, tl.part, tl.ent};
+    for (void *p : b)
+        if (p) (void)hipFree(p);
+    tl = TileLists{};
 }
 
 // Sort of the particles by mesh tile for the given shifts (device array of ns x 3).  Tile lists are
 // reserved from the previous sort's counts with room to double (particles move less than a tile per step), so
 // one fill pass writes them; a list that outgrows its reserve (or a full entry buffer) raises a flag,
 // and then the sort is repeated exactly: counting pass, lists of the exact sizes, fill.
-dj_status tile_sort(dj_ctx *c, const void *psi, const double *d_shifts, int ns, int W) {
-    TileArgs a = tile_args(c, psi, d_shifts, ns);
+dj_status tile_sort_of(dj_ctx *c, TileLists &tl, const void *psi, const void *q1, int n, const double *d_shifts,
+                       int ns, int W) {
+    TileArgs a = tile_args_of(c, tl, psi, q1, n, d_shifts, ns);
     for (int attempt = 0; attempt < 3; attempt++) {
-        const bool exact = !c->tlists || attempt > 0;
+        const bool exact = !tl.lists || attempt > 0;
         {
             Prof p(c, exact ? K_SORT_EXACT : K_SORT);
             if (exact) {
-                c->part.zero(c->tcnt, sizeof(unsigned) * 2 * c->tg.nt, c->st);
+                c->part.zero(tl.cnt, sizeof(unsigned) * 2 * c->tg.nt, c->st);
                 c->tiles.count(a, W, c->st);
-                c->tiles.scan(a, c->tcnt, 0, c->st);
+                c->tiles.scan(a, tl.cnt, 0, c->st);
             } else {
-                c->tiles.scan(a, c->tcur, 1, c->st);
+                c->tiles.scan(a, tl.cur, 1, c->st);
             }
             c->tiles.fill(a, W, c->st);
         }
         unsigned hv[2] = {0, 0};  // overflow flag, entries reserved
-        HIPCHK(c, hipMemcpyAsync(&hv[0], c->tflag, sizeof(unsigned), hipMemcpyDeviceToHost, c->st));
-        HIPCHK(c, hipMemcpyAsync(&hv[1], c->tstart + c->tg.nt, sizeof(unsigned), hipMemcpyDeviceToHost, c->st));
+        HIPCHK(c, hipMemcpyAsync(&hv[0], tl.flag, sizeof(unsigned), hipMemcpyDeviceToHost, c->st));
+        HIPCHK(c, hipMemcpyAsync(&hv[1], tl.start + c->tg.nt, sizeof(unsigned), hipMemcpyDeviceToHost, c->st));
         HIPCHK(c, hipStreamSynchronize(c->st));
-        if (hv[0] == 0 && hv[1] <= c->tent_cap) {
-            c->tlists = true;
+        if (hv[0] == 0 && hv[1] <= tl.ent_cap) {
+            tl.lists = true;
             // After an exact sort hv[1] is the number of entries E; the next sort reserves 2 E + 128 nt.
             // Grow the buffer now (keeping this sort's lists), so that the next fill does not overflow it
             // and repeat the sort.
             const size_t next = 2 * (size_t)hv[1] + 128 * (size_t)c->tg.nt;
-            if (exact && next > c->tent_cap) {
+            if (exact && next > tl.ent_cap) {
                 const size_t cap = next + next / 8 + 1024;
                 void *grown = nullptr;
                 HIPCHK(c, hipMalloc(&grown, cap * c->tiles.entry_bytes));
-                HIPCHK(c, hipMemcpyAsync(grown, c->tent, (size_t)hv[1] * c->tiles.entry_bytes,
+                HIPCHK(c, hipMemcpyAsync(grown, tl.ent, (size_t)hv[1] * c->tiles.entry_bytes,
                                          hipMemcpyDeviceToDevice, c->st));
                 HIPCHK(c, hipStreamSynchronize(c->st));
-                HIPCHK(c, hipFree(c->tent));
-                c->tent = grown;
-                c->tent_cap = cap;
+                HIPCHK(c, hipFree(tl.ent));
+                tl.ent = grown;
+                tl.ent_cap = cap;
             }
             return check_launch(c, "tile binning");
         }
-        c->tlists = false;
-        if (hv[1] > c->tent_cap) {
+        tl.lists = false;
+        if (hv[1] > tl.ent_cap) {
             // grow the entry buffer (exact sizes follow in the next attempt)
-            if (c->tent) HIPCHK(c, hipFree(c->tent));
-            c->tent = nullptr;
+            if (tl.ent) HIPCHK(c, hipFree(tl.ent));
+            tl.ent = nullptr;
             const size_t cap = (size_t)hv[1] + hv[1] / 8 + 1024;
-            HIPCHK(c, hipMalloc(&c->tent, cap * c->tiles.entry_bytes));
-            c->tent_cap = cap;
-            a.ent = c->tent;
-            a.ent_cap = c->tent_cap;
+            HIPCHK(c, hipMalloc(&tl.ent, cap * c->tiles.entry_bytes));
+            tl.ent_cap = cap;
+            a.ent = tl.ent;
+            a.ent_cap = tl.ent_cap;
         }
     }
     return fail(c, DJ_ERR_DEVICE, "tile binning: lists did not fit after an exact recount");
+}
+
+dj_status tile_sort(dj_ctx *c, const void *psi, const double *d_shifts, int ns, int W) {
+    return tile_sort_of(c, c->tl, psi, c->q1, c->n, d_shifts, ns, W);
 }
 
 // ------------------------------------------------------------------------ one force evaluation
 enum { FORCE_ACC = 0, FORCE_KICK = 1 };
 
 template <typename T>
-dj_status resample_deposit(dj_ctx *c, const void *psi, const double *sh, void *dens, bool fixed, double fscale);
+dj_status resample_deposit(dj_ctx *c, const void *psi, const double *sh, void *dens, bool fixed, double fscale,
+                           void *pts = nullptr);
 void nn_forward_dev(dj_ctx *c, const void *dens, double a);
 
 dj_status force(dj_ctx *c, const void *psi, int mode, const KickArgs &kick) {
@@ -474,17 +522,22 @@ dj_status force(dj_ctx *c, const void *psi, int mode, const KickArgs &kick) {
         Prof p(c, K_ZERO);
         c->part.zero(c->acc, (size_t)c->n3 * c->tsize, c->st);
     }
-    dj_status rs = tile_sort(c, psi, c->d_shifts, ns, cfg.worder);
-    if (rs) return rs;
+    // the point sets of sheet resampling deposited together through their own tile lists (the particles'
+    // lists then serve only a finite-difference gather)
+    const bool rs_tiled = cfg.n_resample > 1 && c->rs_tiled;
+    if (!(rs_tiled && ik)) {
+        dj_status rs = tile_sort(c, psi, c->d_shifts, ns, cfg.worder);
+        if (rs) return rs;
+    }
     (void)mesh_bytes;
     // in a step the passes hand the acceleration over in tile-list order: both walk the same lists, so
     // the hand-over is read and written contiguously instead of at the particles' positions
     const bool lacc = mode == FORCE_KICK && ns > 1 && !ik;
-    if (lacc && c->acc_list_cap < c->tent_cap) {
+    if (lacc && c->acc_list_cap < c->tl.ent_cap) {
         if (c->acc_list) HIPCHK(c, hipFree(c->acc_list));
         c->acc_list = nullptr;
-        HIPCHK(c, hipMalloc(&c->acc_list, 3 * c->tent_cap * c->tsize));
-        c->acc_list_cap = c->tent_cap;
+        HIPCHK(c, hipMalloc(&c->acc_list, 3 * c->tl.ent_cap * c->tsize));
+        c->acc_list_cap = c->tl.ent_cap;
     }
     for (int s = 0; s < ns; s++) {
         pa.shift[0] = c->shifts[s][0];
@@ -495,12 +548,30 @@ dj_status force(dj_ctx *c, const void *psi, int mode, const KickArgs &kick) {
             ta.acc = c->acc_list;
             ta.acc_by_entry = 1;
         }
-        { Prof p(c, K_SCATTER); c->tiles.deposit(ta, cfg.worder, fixed, c->shifts[s], dens, fscale_dep(cfg), c->st); }
-        if (cfg.n_resample > 1) {
-            Prof p(c, K_RESAMPLE);
-            dj_status rr = c->tsize == 8 ? resample_deposit<double>(c, psi, c->shifts[s], dens, fixed, fscale_dep(cfg))
-                                         : resample_deposit<float>(c, psi, c->shifts[s], dens, fixed, fscale_dep(cfg));
-            if (rr) return rr;
+        if (rs_tiled) {
+            const int nrn = cfg.n_resample * c->n;
+            {
+                Prof p(c, K_RESAMPLE);
+                c->rs.shifted(psi, c->q1, c->n, c->shifts[s], cfg.n_resample, c->rl_pts, c->st);
+                dj_status rr = c->tsize == 8
+                                   ? resample_deposit<double>(c, psi, c->shifts[s], dens, fixed, fscale_dep(cfg), c->rl_pts)
+                                   : resample_deposit<float>(c, psi, c->shifts[s], dens, fixed, fscale_dep(cfg), c->rl_pts);
+                if (rr) return rr;
+                rr = tile_sort_of(c, c->tlr, c->rl_pts, c->rl_q0, nrn, c->d_shifts + 48, 1, cfg.worder);
+                if (rr) return rr;
+            }
+            const double zero3[3] = {0, 0, 0};
+            TileArgs tr = tile_args_of(c, c->tlr, c->rl_pts, c->rl_q0, nrn, c->d_shifts + 48, 1);
+            Prof p(c, K_SCATTER);
+            c->tiles.deposit(tr, cfg.worder, fixed, zero3, dens, fscale_dep(cfg), c->st);
+        } else {
+            { Prof p(c, K_SCATTER); c->tiles.deposit(ta, cfg.worder, fixed, c->shifts[s], dens, fscale_dep(cfg), c->st); }
+            if (cfg.n_resample > 1) {
+                Prof p(c, K_RESAMPLE);
+                dj_status rr = c->tsize == 8 ? resample_deposit<double>(c, psi, c->shifts[s], dens, fixed, fscale_dep(cfg))
+                                             : resample_deposit<float>(c, psi, c->shifts[s], dens, fixed, fscale_dep(cfg));
+                if (rr) return rr;
+            }
         }
         if (c->nn_on) {
             // the learned potential correction of this pass, from the density before the transform
@@ -569,9 +640,22 @@ dj_status setup_resampling(dj_ctx *c) {
                 c->offsets[c->n_offsets][0] = (double)a / nr;
                 c->offsets[c->n_offsets][1] = (double)b / nr;
                 c->offsets[c->n_offsets][2] = (double)d / nr;
+                c->ioffsets[c->n_offsets][0] = a;
+                c->ioffsets[c->n_offsets][1] = b;
+                c->ioffsets[c->n_offsets][2] = d;
                 c->n_offsets++;
             }
     const size_t n3 = (size_t)c->np, hc = (size_t)n * n * (n / 2 + 1);
+    const double npts = std::pow((double)nr * n, 3);
+    c->rs_tiled = npts < 2147483648.0;
+    if (c->rs_tiled) {
+        const size_t NR = (size_t)nr * n;
+        HIPCHK(c, hipMalloc(&c->rl_pts, 3 * NR * NR * NR * sizeof(T)));
+        HIPCHK(c, hipMalloc(&c->rl_q0, NR * sizeof(T)));
+        HIPCHK(c, hipMemset(c->rl_q0, 0, NR * sizeof(T)));
+        dj_status ra = alloc_lists(c, c->tlr, (size_t)(2.5 * npts) + 1024);
+        if (ra) return ra;
+    }
     HIPCHK(c, hipMalloc(&c->rs_xr, 3 * n3 * sizeof(T)));
     if (c->cfg.resampling == 1) {
         HIPCHK(c, hipMalloc(&c->rs_psi, 4 * n3 * sizeof(T)));  // 3 wrapped components + 1 shifted
@@ -614,8 +698,11 @@ dj_status setup_resampling(dj_ctx *c) {
 
 // The extra point sets of one interlacing pass (shift sh) deposited into dens (pm.jl:471-483 with
 // _resample_prepare! / _resample_offset! for Fourier, _spawn for linear).
+// pts: instead of depositing, write each offset's positions into the point-set lattice pts (3, (nr n)^3)
 template <typename T>
-dj_status resample_deposit(dj_ctx *c, const void *psi, const double *sh, void *dens, bool fixed, double fscale) {
+dj_status resample_deposit(dj_ctx *c, const void *psi, const double *sh, void *dens, bool fixed, double fscale,
+                           void *pts) {
+    const int nr = pts ? c->cfg.n_resample : 1, o0[3] = {0, 0, 0};
     const int n = c->n, H = n / 2 + 1;
     const double L = c->cfg.boxsize;
     const size_t n3 = (size_t)c->np, hc = (size_t)n * n * H;
@@ -628,9 +715,10 @@ dj_status resample_deposit(dj_ctx *c, const void *psi, const double *sh, void *d
             const double *d = c->offsets[o];
             for (int comp = 0; comp < 3; comp++) {
                 c->rs.trilinear(w + comp * n3, n, d, w + 3 * n3, c->st);
-                c->rs.place(w + 3 * n3, c->q1, n, comp, (double)(T)(d[comp] * L / n), L, false, xr, c->st);
+                c->rs.place(w + 3 * n3, c->q1, n, comp, (double)(T)(d[comp] * L / n), L, false, pts ? pts : xr, nr,
+                            pts ? c->ioffsets[o] : o0, c->st);
             }
-            c->rs.scatter(xr, n, c->cfg.worder, dens, c->res, L, fixed, fscale, c->st);
+            if (!pts) c->rs.scatter(xr, n, c->cfg.worder, dens, c->res, L, fixed, fscale, c->st);
         }
         return check_launch(c, "linear sheet resampling");
     }
@@ -650,9 +738,10 @@ dj_status resample_deposit(dj_ctx *c, const void *psi, const double *sh, void *d
             c->fftn.zm(ZM_NONE, false, true, c->rs_B, c->rs_B, c->rs_tw, c->kmult, c->st);
             c->fftn.yi(c->rs_B, c->rs_A, c->rs_tw, c->st);
             c->fftn.xi(c->rs_A, c->rs_psi, c->rs_tw, c->st);
-            c->rs.place(c->rs_psi, c->q1, n, comp, (double)(T)(d[comp] * L / n), L, true, xr, c->st);
+            c->rs.place(c->rs_psi, c->q1, n, comp, (double)(T)(d[comp] * L / n), L, true, pts ? pts : xr, nr,
+                        pts ? c->ioffsets[o] : o0, c->st);
         }
-        c->rs.scatter(xr, n, c->cfg.worder, dens, c->res, L, fixed, fscale, c->st);
+        if (!pts) c->rs.scatter(xr, n, c->cfg.worder, dens, c->res, L, fixed, fscale, c->st);
     }
     return check_launch(c, "Fourier sheet resampling");
 }
@@ -698,21 +787,14 @@ dj_status setup_tiles(dj_ctx *c) {
             rs = std::max(rs, (int)std::ceil((W % 2 ? 1.5 : 1.0) + sig) - 1);
         }
     g.RS = std::min(g.RR, rs + hi);
-    HIPCHK(c, hipMalloc(&c->tcnt, sizeof(unsigned) * 2 * g.nt));
-    HIPCHK(c, hipMalloc(&c->tcur, sizeof(unsigned) * 2 * g.nt));
-    HIPCHK(c, hipMalloc(&c->tstart, sizeof(unsigned) * (g.nt + 1)));
-    HIPCHK(c, hipMalloc(&c->tgstart, sizeof(unsigned) * g.nt));
-    HIPCHK(c, hipMalloc(&c->tflag, sizeof(unsigned)));
     if (g.nt > (1 << 20)) return fail(c, DJ_ERR_UNSUPPORTED, "more than 2^20 mesh tiles");
-    HIPCHK(c, hipMalloc(&c->tpart, sizeof(unsigned long long) * (g.nt / 1024 + 2)));
-    c->tlists = false;
     HIPCHK(c, hipMalloc(&c->d_shifts, sizeof(double) * 17 * 3));
-    HIPCHK(c, hipMemcpy(c->d_shifts, c->shifts, sizeof(double) * 16 * 3, hipMemcpyHostToDevice));
+    double sh17[17][3] = {};
+    std::memcpy(sh17, c->shifts, sizeof(double) * 16 * 3);
+    HIPCHK(c, hipMemcpy(c->d_shifts, sh17, sizeof(double) * 17 * 3, hipMemcpyHostToDevice));
     // own + ghost entries: about 2.1 per particle for 32 x 16 x 8 tiles with TSC and interlacing, twice
     // that plus 64 per list once the lists are reserved from a previous sort (grown on demand)
-    c->tent_cap = (size_t)(2.5 * (double)c->np) + 1024;
-    HIPCHK(c, hipMalloc(&c->tent, c->tent_cap * c->tiles.entry_bytes));
-    return DJ_OK;
+    return alloc_lists(c, c->tl, (size_t)(2.5 * (double)c->np) + 1024);
 }
 
 dj_status validate(const dj_pm_config &cfg, std::string &msg) {

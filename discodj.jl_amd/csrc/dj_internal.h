@@ -112,9 +112,12 @@ struct ResampleOps {
                   hipStream_t st);
     // periodic trilinear shift by d (grid cells)
     void (*trilinear)(const void *f, int n, const double *d, void *out, hipStream_t st);
-    // Xr_c = mod(q_c + npsi_c + shift + L, L)
+    // Xr_c = mod(q_c + npsi_c + shift + L, L), at point (nr i + o[0], nr j + o[1], nr k + o[2]) of the
+    // (nr n)^3 point-set lattice xr (3 components); nr = 1, o = 0: the particle layout
     void (*place)(const void *npsi, const void *q1, int n, int c, double shift, double L, bool transposed, void *xr,
-                  hipStream_t st);
+                  int nr, const int *o, hipStream_t st);
+    // the particles' shifted positions (psi + q) + sh at points (nr i, nr j, nr k) of the point-set lattice
+    void (*shifted)(const void *psi, const void *q1, int n, const double *sh, int nr, void *out, hipStream_t st);
     // unit-mass deposit of positions (3, n^3) with device atomics (float, or int64 fixed point)
     void (*scatter)(const void *X, int n, int W, void *mesh, int res, double L, bool fixed, double fscale,
                     hipStream_t st);

@@ -73,16 +73,33 @@ __global__ __launch_bounds__(256) void k_rs_trilinear(const T *__restrict__ f, i
         F(i1, j1, k1) * dx * dy * dz;
 }
 
-// Xr_c = mod(q_c + npsi_c + shift + L, L); npsi in the natural or the transposed layout
+// Xr_c = mod(q_c + npsi_c + shift + L, L); npsi in the natural or the transposed layout.  Written at
+// point (nr i + o0, nr j + o1, nr k + o2) of the (nr n)^3 lattice of all point sets of a pass (offset
+// (o0, o1, o2) / nr cells: the sets interleave into a finer lattice; nr = 1, o = 0 is the plain layout)
 template <typename T, bool TRANSPOSED>
 __global__ __launch_bounds__(256) void k_rs_place(const T *__restrict__ npsi, const T *__restrict__ q1, int n, int c,
-                                                  T shift, T L, T *__restrict__ xr) {
+                                                  T shift, T L, T *__restrict__ xr, int nr, int o0, int o1, int o2) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
     if (i >= n) return;
-    const size_t N = (size_t)n, p = (size_t)i + N * ((size_t)j + N * k);
+    const size_t N = (size_t)n, p = (size_t)i + N * ((size_t)j + N * k), NR = (size_t)nr * n;
     const T qc = q1[c == 0 ? i : c == 1 ? j : k];
     const T np_ = npsi[TRANSPOSED ? (size_t)k + N * ((size_t)j + N * i) : p];
-    xr[p + (size_t)c * N * N * N] = rs_jmod(((qc + np_) + shift) + L, L);
+    const size_t pr = ((size_t)nr * i + o0) + NR * (((size_t)nr * j + o1) + NR * ((size_t)nr * k + o2));
+    xr[pr + (size_t)c * NR * NR * NR] = rs_jmod(((qc + np_) + shift) + L, L);
+}
+
+// the particles' own positions of the pass, X_c + s_c = (psi_c + q_c) + s_c (the deposit's arithmetic), at
+// points (nr i, nr j, nr k) of the point-set lattice
+template <typename T>
+__global__ __launch_bounds__(256) void k_rs_shifted(const T *__restrict__ psi, const T *__restrict__ q1, int n, T s0,
+                                                    T s1, T s2, int nr, T *__restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+    if (i >= n) return;
+    const size_t N = (size_t)n, p = (size_t)i + N * ((size_t)j + N * k), NR = (size_t)nr * n, NR3 = NR * NR * NR;
+    const size_t pr = (size_t)nr * i + NR * ((size_t)nr * j + NR * ((size_t)nr * k));
+    out[pr] = (psi[p] + q1[i]) + s0;
+    out[pr + NR3] = (psi[p + N * N * N] + q1[j]) + s1;
+    out[pr + 2 * NR3] = (psi[p + 2 * N * N * N] + q1[k]) + s2;
 }
 
 // 1-D assignment weights and base cell (pm.jl:49-55), as in the particle kernels
@@ -162,11 +179,17 @@ struct RsLaunch {
         k_rs_trilinear<T><<<pgrid(n), 256, 0, st>>>((const T *)f, n, (T)d[0], (T)d[1], (T)d[2], (T *)out);
     }
     static void place(const void *npsi, const void *q1, int n, int c, double shift, double L, bool transposed, void *xr,
-                      hipStream_t st) {
+                      int nr, const int *o, hipStream_t st) {
         if (transposed)
-            k_rs_place<T, true><<<pgrid(n), 256, 0, st>>>((const T *)npsi, (const T *)q1, n, c, (T)shift, (T)L, (T *)xr);
+            k_rs_place<T, true><<<pgrid(n), 256, 0, st>>>((const T *)npsi, (const T *)q1, n, c, (T)shift, (T)L, (T *)xr,
+                                                          nr, o[0], o[1], o[2]);
         else
-            k_rs_place<T, false><<<pgrid(n), 256, 0, st>>>((const T *)npsi, (const T *)q1, n, c, (T)shift, (T)L, (T *)xr);
+            k_rs_place<T, false><<<pgrid(n), 256, 0, st>>>((const T *)npsi, (const T *)q1, n, c, (T)shift, (T)L, (T *)xr,
+                                                           nr, o[0], o[1], o[2]);
+    }
+    static void shifted(const void *psi, const void *q1, int n, const double *sh, int nr, void *out, hipStream_t st) {
+        k_rs_shifted<T><<<pgrid(n), 256, 0, st>>>((const T *)psi, (const T *)q1, n, (T)sh[0], (T)sh[1], (T)sh[2], nr,
+                                                  (T *)out);
     }
     template <int W>
     static void scatter_w(const void *X, int n, void *mesh, int res, double L, bool fixed, double fscale, hipStream_t st) {
@@ -186,6 +209,7 @@ template <typename T> void resample_ops(ResampleOps &ops) {
     ops.phase = RsLaunch<T>::phase;
     ops.trilinear = RsLaunch<T>::trilinear;
     ops.place = RsLaunch<T>::place;
+    ops.shifted = RsLaunch<T>::shifted;
     ops.scatter = RsLaunch<T>::scatter;
 }
 
