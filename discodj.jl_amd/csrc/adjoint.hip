@@ -11,6 +11,9 @@
 //   phibar   = sum_a D_a Fbar_a                (D_a^T = -D_a: the stencil is antisymmetric)   (divergence)
 //   rhobar   = A phibar                        (K_phi is real and even in k: A^T = A)
 //   psibar  += sum_c grad W(X - c) (rhobar_c + sum_a abar_a F_a(c))                           (gather_grad)
+// With the spectral gradient (grad_order 0) F_a = B_a rho, B_a = c2r (-g_a K_phi) r2c with g_a = i k_a (its
+// Nyquist zeroed): -g_a K_phi is imaginary and odd in k, so B_a^T = -B_a and rhobar = -sum_a B_a Fbar_a, and
+// the weight-gradient gather reads the three recomputed force meshes (gather_grad_f).
 // The weights' slopes dW/dg follow the same branches as the weights; g = mod(X, L) / L N.
 #include <hip/hip_runtime.h>
 
@@ -170,6 +173,47 @@ __global__ __launch_bounds__(256) void k_adj_gather(const T *__restrict__ psi, c
     psibar[p + 2 * n3] += k * g2;
 }
 
+// The same for the spectral (ik) gradient, whose forward gathers the three force meshes F (3 N^3):
+// psibar_p += N/L sum_c grad W(X_p - c) (rhobar_c + sum_a abar_p,a s F_a(c))
+template <typename T, int W>
+__global__ __launch_bounds__(256) void k_adj_gather_f(const T *__restrict__ psi, const T *__restrict__ q1, int n, int N,
+                                                      T L, T sx, T sy, T sz, const T *__restrict__ rhobar,
+                                                      const T *__restrict__ F, const T *__restrict__ abar, T s,
+                                                      T *__restrict__ psibar) {
+    const size_t n3 = (size_t)n * n * n, M = (size_t)N * N * N;
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n3) return;
+    const T sh[3] = {sx, sy, sz};
+    int cell[3][W];
+    T w[3][W], dw[3][W];
+    adj_stencil<T, W>(psi, q1, n, n3, p, N, L, sh, cell, w, dw);
+    const T ab[3] = {abar[p] * s, abar[p + n3] * s, abar[p + 2 * n3] * s};
+    T g0 = 0, g1 = 0, g2 = 0;
+#pragma unroll
+    for (int oz = 0; oz < W; oz++)
+#pragma unroll
+        for (int oy = 0; oy < W; oy++)
+#pragma unroll
+            for (int ox = 0; ox < W; ox++) {
+                const size_t c = (size_t)cell[0][ox] + (size_t)N * ((size_t)cell[1][oy] + (size_t)N * cell[2][oz]);
+                const T v = rhobar[c] + (ab[0] * F[c] + ab[1] * F[c + M] + ab[2] * F[c + 2 * M]);
+                g0 += dw[0][ox] * w[1][oy] * w[2][oz] * v;
+                g1 += w[0][ox] * dw[1][oy] * w[2][oz] * v;
+                g2 += w[0][ox] * w[1][oy] * dw[2][oz] * v;
+            }
+    const T k = (T)N / L;
+    psibar[p] += k * g0;
+    psibar[p + n3] += k * g1;
+    psibar[p + 2 * n3] += k * g2;
+}
+
+// y += s x
+template <typename T>
+__global__ __launch_bounds__(256) void k_adj_axpy(T *__restrict__ y, const T *__restrict__ x, size_t n, T s) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = y[i] + s * x[i];
+}
+
 // backward of the kick and second drift of a DKD step (run_nbody.jl:156-158):
 //   Pibar_t = Pibar + dd2 Psibar;  abar = beta Pibar_t;  Pibar = alpha Pibar_t
 template <typename T>
@@ -220,6 +264,22 @@ struct AdjLaunch {
         else ADJ_G(4);
 #undef ADJ_G
     }
+    static void gather_grad_f(const void *psi, const void *q1, int n, int N, double L, const double *sh,
+                              const void *rhobar, const void *F, const void *abar, double s, int W, void *psibar,
+                              hipStream_t st) {
+        const size_t n3 = (size_t)n * n * n;
+#define ADJ_F(WW)                                                                                                    \
+    k_adj_gather_f<T, WW><<<agrid(n3), 256, 0, st>>>((const T *)psi, (const T *)q1, n, N, (T)L, (T)sh[0], (T)sh[1], \
+                                                     (T)sh[2], (const T *)rhobar, (const T *)F, (const T *)abar, (T)s, \
+                                                     (T *)psibar)
+        if (W == 2) ADJ_F(2);
+        else if (W == 3) ADJ_F(3);
+        else ADJ_F(4);
+#undef ADJ_F
+    }
+    static void axpy(void *y, const void *x, size_t n, double s, hipStream_t st) {
+        k_adj_axpy<T><<<agrid(n), 256, 0, st>>>((T *)y, (const T *)x, n, (T)s);
+    }
     static void kick(const void *psibar, void *pibar, void *abar, size_t n, double alpha, double beta, double dd2,
                      hipStream_t st) {
         k_adj_kick<T><<<agrid(n), 256, 0, st>>>((const T *)psibar, (T *)pibar, (T *)abar, n, (T)alpha, (T)beta, (T)dd2);
@@ -233,6 +293,8 @@ template <typename T> void adjoint_ops(AdjointOps &ops) {
     ops.scatter3 = AdjLaunch<T>::scatter3;
     ops.divergence = AdjLaunch<T>::divergence;
     ops.gather_grad = AdjLaunch<T>::gather_grad;
+    ops.gather_grad_f = AdjLaunch<T>::gather_grad_f;
+    ops.axpy = AdjLaunch<T>::axpy;
     ops.kick = AdjLaunch<T>::kick;
     ops.drift = AdjLaunch<T>::drift;
 }

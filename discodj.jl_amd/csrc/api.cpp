@@ -94,6 +94,7 @@ struct dj_ctx {
     AdjointOps adj{};
     void *vb_psi1 = nullptr, *vb_abar = nullptr, *vb_psibar = nullptr, *vb_pibar = nullptr;
     void *vb_fbar = nullptr, *vb_pbar = nullptr;
+    void *vb_fik = nullptr;  // the three force meshes of the spectral gradient, recomputed in the reverse pass
     bool vb_resident = false;  // vb_psibar / vb_pibar hold cotangents left resident by dj_vjp_deposit / dj_vjp_run
     void *ckpt = nullptr;  // (steps, 2, 3 n^3): Psi and Pi before each step
     int ck_steps = 0;
@@ -335,7 +336,7 @@ void free_all(dj_ctx *c) {
     void *bufs[] = {c->psi, c->mom, c->acc, c->psi_in, c->q1, c->mesh, c->meshfix, c->A, c->B, c->C, c->tw, c->tabs,
                     c->d_shifts, c->rl_pts, c->rl_q0,
                     c->rs_tw, c->rs_psi, c->rs_hat, c->rs_A, c->rs_B, c->rs_xr, c->rs_phase,
-                    c->vb_psi1, c->vb_abar, c->vb_psibar, c->vb_pibar, c->vb_fbar, c->vb_pbar, c->ckpt,
+                    c->vb_psi1, c->vb_abar, c->vb_psibar, c->vb_pibar, c->vb_fbar, c->vb_pbar, c->vb_fik, c->ckpt,
                     c->nn_p, c->nn_g, c->nn_m, c->nn_v, c->nn_mask, c->nn_wf, c->nn_wb, c->nn_pf, c->nn_pb,
                     c->nn_t[0], c->nn_t[1],
                     c->nn_t[2], c->nn_t[3], c->nn_t[4], c->nn_t[5], c->nn_t[6], c->nn_t[7], c->nn_t[8], c->nn_t[9],
@@ -1061,8 +1062,6 @@ dj_status dj_ics_lpt(dj_ctx *c, uint32_t seed, const double *white, const double
 namespace {
 dj_status adj_supported(dj_ctx *c) {
     if (c->nn_on) return fail(c, DJ_ERR_UNSUPPORTED, "the adjoint is built without the learned correction");
-    if (c->cfg.grad_order == 0)
-        return fail(c, DJ_ERR_UNSUPPORTED, "the adjoint is built for finite-difference gradients (grad_order 2/4/6)");
     if (c->cfg.n_resample > 1) return fail(c, DJ_ERR_UNSUPPORTED, "the adjoint is built without sheet resampling");
     return DJ_OK;
 }
@@ -1074,6 +1073,7 @@ dj_status adj_buffers(dj_ctx *c) {
         if (!*b) HIPCHK(c, hipMalloc(b, pb));
     if (!c->vb_fbar) HIPCHK(c, hipMalloc(&c->vb_fbar, 3 * mb));
     if (!c->vb_pbar) HIPCHK(c, hipMalloc(&c->vb_pbar, mb));
+    if (c->cfg.grad_order == 0 && !c->vb_fik) HIPCHK(c, hipMalloc(&c->vb_fik, 3 * mb));
     return DJ_OK;
 }
 
@@ -1093,12 +1093,44 @@ dj_status vjp_force(dj_ctx *c, const void *psi, const void *abar, void *psibar) 
         c->tiles.deposit(ta, cfg.worder, fixed, c->shifts[s], dens, fscale_dep(cfg), c->st);
         c->fft.xf(dens, fixed, c->A, c->tw, fscale, c->st);
         c->fft.yf(c->A, c->B, c->tw, c->st);
+        if (cfg.grad_order == 0) {
+            // F_a = B_a rho (the forward's force meshes), Fbar = G^T abar / ns, rhobar = -sum_a B_a Fbar_a
+            const size_t M = c->M;
+            char *F = (char *)c->vb_fik, *Fb = (char *)c->vb_fbar;
+            const size_t mb = M * c->tsize;
+            c->fft.zm(ZM_NONE, true, false, c->B, c->B, c->tw, c->kmult, c->st);
+            for (int ax = 0; ax < 3; ax++) {
+                c->fft.zm(ZM_GRADX + ax, false, true, c->B, c->C, c->tw, c->kmult, c->st);
+                c->fft.yi(c->C, c->A, c->tw, c->st);
+                c->fft.xi(c->A, F + ax * mb, c->tw, c->st);
+            }
+            if (!c->tiles.deposit3(ta, cfg.worder, c->shifts[s], abar, 1.0 / ns, c->vb_fbar, c->st)) {
+                c->part.zero(c->vb_fbar, 3 * c->M * c->tsize, c->st);
+                c->adj.scatter3(psi, c->q1, c->n, N, cfg.boxsize, c->shifts[s], abar, 1.0 / ns, cfg.worder, c->vb_fbar,
+                                c->st);
+            }
+            c->part.zero(c->vb_pbar, mb, c->st);
+            for (int ax = 0; ax < 3; ax++) {
+                c->fft.xf(Fb + ax * mb, false, c->A, c->tw, 1.0, c->st);
+                c->fft.yf(c->A, c->B, c->tw, c->st);
+                c->fft.zm(ZM_NONE, true, false, c->B, c->B, c->tw, c->kmult, c->st);
+                c->fft.zm(ZM_GRADX + ax, false, true, c->B, c->C, c->tw, c->kmult, c->st);
+                c->fft.yi(c->C, c->A, c->tw, c->st);
+                c->fft.xi(c->A, c->mesh, c->tw, c->st);
+                c->adj.axpy(c->vb_pbar, c->mesh, M, -1.0, c->st);
+            }
+            c->adj.gather_grad_f(psi, c->q1, c->n, N, cfg.boxsize, c->shifts[s], c->vb_pbar, c->vb_fik, abar, 1.0 / ns,
+                                 cfg.worder, psibar, c->st);
+            r = check_launch(c, "force adjoint (spectral gradient)");
+            if (r) return r;
+            continue;
+        }
         c->fft.zm(ZM_POT, true, true, c->B, c->B, c->tw, c->kmult, c->st);
         c->fft.yi(c->B, c->A, c->tw, c->st);
         c->fft.xi(c->A, c->mesh, c->tw, c->st);
         // Fbar = G^T abar / ns: the tiled vector deposit, or device atomics on a single-tile axis
         if (!c->tiles.deposit3(ta, cfg.worder, c->shifts[s], abar, 1.0 / ns, c->vb_fbar, c->st)) {
-            HIPCHK(c, hipMemsetAsync(c->vb_fbar, 0, 3 * c->M * c->tsize, c->st));
+            c->part.zero(c->vb_fbar, 3 * c->M * c->tsize, c->st);
             c->adj.scatter3(psi, c->q1, c->n, N, cfg.boxsize, c->shifts[s], abar, 1.0 / ns, cfg.worder, c->vb_fbar,
                             c->st);
         }

@@ -135,6 +135,11 @@ struct AdjointOps {
     void (*gather_grad)(const void *psi, const void *q1, int n, int N, double L, const double *sh, const void *rhobar,
                         const void *phi, const void *abar, double s, const double *c, int W, void *psibar,
                         hipStream_t st);
+    // psibar += N/L sum_c grad W(X - c) (rhobar_c + sum_a abar_a s F_a(c)) with F the three force meshes (3 N^3)
+    void (*gather_grad_f)(const void *psi, const void *q1, int n, int N, double L, const double *sh, const void *rhobar,
+                          const void *F, const void *abar, double s, int W, void *psibar, hipStream_t st);
+    // y += s x
+    void (*axpy)(void *y, const void *x, size_t n, double s, hipStream_t st);
     // kick and second drift backward: Pibar_t = Pibar + dd2 Psibar; abar = beta Pibar_t; Pibar = alpha Pibar_t
     void (*kick)(const void *psibar, void *pibar, void *abar, size_t n, double alpha, double beta, double dd2,
                  hipStream_t st);

@@ -186,8 +186,9 @@ dj_status dj_ranlxd1_uniforms(uint32_t seed, int32_t n, double *out);
 
 /* Reverse mode (no counterpart in the reference, whose PM path has no adjoint: NR/README.md:169; its
  * conventions follow the CIC rrule of field/sheet_deposit.jl:118-132 and the FFT rrules of
- * lpt/nlpt_core.jl:682-713).  Built for finite-difference gradients and n_resample = 1; other options
- * return DJ_ERR_UNSUPPORTED.  All arrays are host arrays in the context's layout and precision. */
+ * lpt/nlpt_core.jl:682-713).  Built for finite-difference and spectral (ik) gradients with n_resample = 1
+ * and without the learned correction; other options return DJ_ERR_UNSUPPORTED.  All arrays are host arrays
+ * in the context's layout and precision. */
 
 /* psi_bar_out = (d acc / d psi)^T acc_bar of dj_acceleration at psi (NULL: the context's Psi) */
 dj_status dj_vjp_acceleration(dj_ctx *ctx, const void *psi, const void *acc_bar, void *psi_bar_out);

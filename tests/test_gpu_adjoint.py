@@ -240,7 +240,68 @@ def test_resident_cotangents_are_checked():
         S.vjp_acceleration(None, np.ones((3, n, n, n)))
         with pytest.raises(D.DJError):
             S.vjp_ics_lpt(k, pk, oc.Dplus(0.05), white=white)
-    with D.PMSolverHIP(D.PMConfig(res_pm=32, n_part=n, boxsize=L, grad_order=0), precision=64) as S:
+    with D.PMSolverHIP(D.PMConfig(res_pm=32, n_part=n, boxsize=L, n_resample=2), precision=64) as S:
         S.set_state(np.zeros((3, n, n, n)))
         with pytest.raises(D.DJError):
             S.run_checkpointed(*coef)
+
+
+@pytest.mark.parametrize("W", [2, 3, 4])
+@pytest.mark.parametrize("aa", [0, 1])
+@pytest.mark.parametrize("deconvolve", [False, True])
+def test_vjp_acceleration_ik_fd(W, aa, deconvolve):
+    """The force VJP with the spectral (ik) gradient (grad_order 0, the discreteness-suppressed settings use
+    it with PCS and deconvolution): against central differences of the device forward and of the oracle's."""
+    n, res, L = 16, 32, 100.0
+    rng = np.random.default_rng(200 + 10 * W + 2 * aa + int(deconvolve))
+    psi, abar, v = _psi(n, L, 40 + W + aa), rng.normal(size=(3, n, n, n)), rng.normal(size=(3, n, n, n))
+    cfg = D.PMConfig(res_pm=res, n_part=n, boxsize=L, worder=W, grad_order=0, antialias=aa, deconvolve=deconvolve)
+    eps = 1e-5
+    with D.PMSolverHIP(cfg, precision=64) as S:
+        g = _dot(S.vjp_acceleration(psi, abar), v)
+        fd = (_dot(abar, S.pm_acceleration(psi + eps * v)) - _dot(abar, S.pm_acceleration(psi - eps * v))) / (2 * eps)
+    ocfg = O.pm_config(res, n, L, worder=W, grad_order=0, antialias=aa, deconvolve=deconvolve)
+    fdo = (_dot(abar, O.pm_acceleration(psi + eps * v, ocfg, "solver")) -
+           _dot(abar, O.pm_acceleration(psi - eps * v, ocfg, "solver"))) / (2 * eps)
+    tol = 1e-4 if W == 2 else 1e-6
+    assert _rel(g, fd) < tol, (g, fd)
+    assert _rel(g, fdo) < tol, (g, fdo)
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_vjp_run_ik_fd(prec):
+    """Three BullFrog steps with PCS, ik gradient, deconvolution and interlacing (64^3 mesh, so the FP32
+    vector deposit runs tiled): the reverse pass against central differences (FP64), FP32 against FP64."""
+    n, res, L, steps = 32, 64, 100.0, 3
+    rng = np.random.default_rng(9)
+    psi0 = _psi(n, L, 12, 1.0)
+    pi0 = rng.normal(scale=2.0, size=(3, n, n, n))
+    pb, mb = rng.normal(size=(3, n, n, n)), rng.normal(size=(3, n, n, n))
+    v, u = rng.normal(size=(3, n, n, n)), rng.normal(size=(3, n, n, n))
+    c = D.Cosmology.named("Planck18EEBAOSN")
+    co = D.stepper_coefficients(c, "bullfrog", 0.05, 0.5, steps, "D")
+    coef = [co["ddrift1"], co["alpha"], co["beta"], co["ddrift2"]]
+    cfg = D.PMConfig(res_pm=res, n_part=n, boxsize=L, worder=4, grad_order=0, antialias=1, deconvolve=True)
+    dt = np.float64 if prec == 64 else np.float32
+    with D.PMSolverHIP(cfg, precision=prec) as S:
+        S.set_state(psi0.astype(dt), pi0.astype(dt))
+        S.run_checkpointed(*coef)
+        g0, g1 = S.vjp_run(pb.astype(dt), mb.astype(dt))
+        if prec == 32:
+            with D.PMSolverHIP(cfg, precision=64) as S64:
+                S64.set_state(psi0, pi0)
+                S64.run_checkpointed(*coef)
+                r0, r1 = S64.vjp_run(pb, mb)
+            assert G.rel(g0, r0) < 1e-3 and G.rel(g1, r1) < 1e-3
+            return
+        eps = 1e-6
+
+        def f(e):
+            S.set_state(psi0 + e * v, pi0 + e * u)
+            S.run(*coef)
+            p, m = S.get_state()
+            return _dot(pb, p) + _dot(mb, m)
+
+        fd = (f(eps) - f(-eps)) / (2 * eps)
+    g = _dot(g0, v) + _dot(g1, u)
+    assert _rel(g, fd) < 1e-6, (g, fd)
