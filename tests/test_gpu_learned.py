@@ -47,10 +47,11 @@ def test_nn_forward_and_gradient_match_oracle(seed):
 def test_training_reduces_force_error():
     """The learned-correction workload at reduced size: 64^3 particles on a 64^3 mesh (window
     deconvolution) against a 256^3 mesh, BullFrog 10 steps; the net trained on six seeds (every step),
-    evaluated on a seventh.  Asserted: the held-out force error falls; the net alone does not lower the
-    P(k) error (a force-MSE optimum is smoother than the fine force), and a transfer function fitted on
-    top of the net (two seeds) takes back most of what the net loses (to 3 % on this seed, 1.4 % on
-    others; the transfer function alone meets 2 %, test_fitted_transfer_function_matches_fine_mesh_power)."""
+    evaluated on a seventh.  Asserted: the held-out force error falls; the net alone raises the P(k)
+    error above the deconvolved PM's and misses 2 % (a force-MSE optimum is smoother than the fine
+    force), within a bounded factor; a transfer function fitted on top of the net (two seeds) takes back
+    most of what the net loses (to 3.4 % on this seed; the transfer function alone meets 2 %,
+    test_fitted_transfer_function_matches_fine_mesh_power)."""
     c = D.Cosmology.named("Planck18EEBAOSN")
     work = D.learned.Workload(c, n=64, N=64, L=128.0, ratio=4, n_steps=10, deconvolve=True)
     data = work.snapshots([1, 2, 4, 5, 6, 7], every=1)
@@ -68,6 +69,11 @@ def test_training_reduces_force_error():
     _, pk_both, _ = D.learned.power_spectrum_errors(work, 3, params, scale, kgain=(g, kmax), psi_fine=fine[2])
     print(f"max |P/P_fine - 1| to k_Nyq/2: deconvolved {pk_plain:.4f}, net {pk_nn:.4f}, net + transfer "
           f"function {pk_both:.4f}")
+    # what the net itself does to P(k), stated as it is: trained on forces, it raises the P(k) error of the
+    # deconvolved PM (0.034 -> 0.095 on this seed: its force-MSE optimum removes small-scale power) and
+    # misses the 2 % bar; the rise is bounded, and the transfer function fitted on top of the net takes
+    # the error back to the deconvolved PM's level
+    assert pk_plain < pk_nn < 4 * pk_plain and pk_nn > 0.02, (pk_plain, pk_nn)
     assert pk_both < 0.5 * pk_nn and pk_both < 0.05
 
 
