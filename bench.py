@@ -296,7 +296,9 @@ def bench_pm256(np, D, c, k_tab, pk_tab):
 def bench_gradient(args, as_dict=False):
     """Gradient of sum (delta - target)^2 on the 256^3 mesh with respect to the 256^3 white noise through
     2LPT and 10 BullFrog CIC steps (grad order 4), FP32: seconds per gradient, split into the forward run
-    with step checkpoints and the reverse pass (as_dict: returned instead of printed)."""
+    with step checkpoints and the reverse pass (as_dict: returned instead of printed).  The white noise, the
+    target and the white-noise cotangent stay in device memory and the loss is formed on the device
+    (dj_density_loss_device); the same gradient with host arrays is timed beside it."""
     import numpy as np
 
     import discodj_amd as D
@@ -308,30 +310,47 @@ def bench_gradient(args, as_dict=False):
     target = np.zeros((n, n, n))
     co = D.stepper_coefficients(c, "bullfrog", 0.05, 1.0, 10, "D")
     cfg = D.PMConfig(res_pm=n, n_part=n, boxsize=L, worder=2, grad_order=4)
+    rho = 1.0
     with D.PMSolverHIP(cfg, precision=32) as S:
+        w_d = D.DeviceArray.from_numpy(white)
+        t_d = D.DeviceArray.from_numpy(target)
+        wb_d = D.DeviceArray(white.shape, np.float64)
         times = []
-        for rep in range(3):
+        for rep in range(4):  # the first is a warm-up (it allocates the IC and reverse-pass buffers)
             S.synchronize()
             t0 = time.perf_counter()
-            S.ics_lpt(k_tab, pk_tab, c.Dplus(0.05), order=2, white=white)
+            S.ics_lpt_device(k_tab, pk_tab, c.Dplus(0.05), w_d, order=2)
             S.run_checkpointed(co["ddrift1"], co["alpha"], co["beta"], co["ddrift2"])
-            rho = n**3 / n**3
-            delta = S.deposit().astype(np.float64) / rho - 1.0
-            S.synchronize()
+            loss = S.density_loss_device(t_d)
             t1 = time.perf_counter()
-            S.vjp_deposit(None, 2.0 * (delta - target) / rho, resident=True)
             S.vjp_run()
-            wb = S.vjp_ics_lpt(k_tab, pk_tab, c.Dplus(0.05), order=2, white=white)
+            S.vjp_ics_lpt_device(k_tab, pk_tab, c.Dplus(0.05), w_d, wb_d, order=2)
             S.synchronize()
             t2 = time.perf_counter()
-            times.append((t1 - t0, t2 - t1))
-            log(f"gradient {rep}: forward {t1 - t0:.3f}s, reverse {t2 - t1:.3f}s, |wbar| {np.linalg.norm(wb):.3e}")
+            if rep:
+                times.append((t1 - t0, t2 - t1))
+            log(f"gradient {rep}: forward {t1 - t0:.3f}s, reverse {t2 - t1:.3f}s, loss {loss:.6e}")
+        log(f"|wbar| {np.linalg.norm(wb_d.numpy()):.4e}")
+        # the same gradient with host arrays (white noise up, mesh down, loss on the host, cotangent up,
+        # white-noise cotangent down)
+        th = time.perf_counter()
+        S.ics_lpt(k_tab, pk_tab, c.Dplus(0.05), order=2, white=white)
+        S.run_checkpointed(co["ddrift1"], co["alpha"], co["beta"], co["ddrift2"])
+        delta = S.deposit().astype(np.float64) / rho - 1.0
+        S.vjp_deposit(None, 2.0 * (delta - target) / rho, resident=True)
+        S.vjp_run()
+        wb_h = S.vjp_ics_lpt(k_tab, pk_tab, c.Dplus(0.05), order=2, white=white)
+        S.synchronize()
+        t_host = time.perf_counter() - th
+        log(f"host arrays: {t_host:.3f}s per gradient, |wbar| {np.linalg.norm(wb_h):.4e}")
+        for a in (w_d, t_d, wb_d):
+            a.free()
     fw, bw = np.median([t[0] for t in times]), np.median([t[1] for t in times])
     out = {"metric": "gradients/s of the density loss w.r.t. the white noise (2LPT + 10 BullFrog steps)",
            "value": 1.0 / (fw + bw), "unit": "gradients/s", "forward_s": fw, "reverse_s": bw,
-           "dtype": "f32", "data": "synthetic", "n_gpus": 1,
+           "host_arrays_s_per_gradient": t_host, "dtype": "f32", "data": "synthetic", "n_gpus": 1,
            "config": {"workload": f"{n}^3 particles, {n}^3 mesh, L={L}, CIC, grad order 4, BullFrog 10",
-                      "median_of": 3}}
+                      "median_of": 3, "arrays": "white noise, target, loss and cotangents in device memory"}}
     if as_dict:
         return out
     print(json.dumps(out), flush=True)

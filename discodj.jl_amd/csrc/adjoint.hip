@@ -233,6 +233,45 @@ __global__ __launch_bounds__(256) void k_adj_drift(const T *__restrict__ psibar,
     if (i < n) pibar[i] = pibar[i] + dd1 * psibar[i];
 }
 
+// The density loss L = sum (delta - target)^2, delta = mesh / rho - 1, and its cotangent mesh_bar =
+// (2 / rho)(delta - target) rounded to T (the host path's operations, in its order).  Each of the
+// LOSS_BLOCKS blocks sums its grid-stride share of r^2 in a fixed order into part[block]; k_loss_total
+// adds the block sums in a fixed order, so the loss is the same in every call.
+constexpr int LOSS_BLOCKS = 1024;
+template <typename T>
+__global__ __launch_bounds__(256) void k_adj_mse(const T *__restrict__ mesh, const double *__restrict__ target,
+                                                 size_t n, double rho, T *__restrict__ mesh_bar,
+                                                 double *__restrict__ part) {
+    __shared__ double red[256];
+    double acc = 0.0;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)LOSS_BLOCKS * 256) {
+        const double delta = (double)mesh[i] / rho - 1.0;
+        const double r = target ? delta - target[i] : delta;
+        mesh_bar[i] = (T)((2.0 / rho) * r);
+        acc += r * r;
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_loss_total(double *part) {
+    __shared__ double red[256];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < LOSS_BLOCKS; i += 256) acc += part[i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[LOSS_BLOCKS] = red[0];
+}
+
 static unsigned agrid(size_t n) { return (unsigned)((n + 255) / 256); }
 
 template <typename T>
@@ -287,6 +326,11 @@ struct AdjLaunch {
     static void drift(const void *psibar, void *pibar, size_t n, double dd1, hipStream_t st) {
         k_adj_drift<T><<<agrid(n), 256, 0, st>>>((const T *)psibar, (T *)pibar, n, (T)dd1);
     }
+    static void mse(const void *mesh, const double *target, size_t n, double rho, void *mesh_bar, double *part,
+                    hipStream_t st) {
+        k_adj_mse<T><<<LOSS_BLOCKS, 256, 0, st>>>((const T *)mesh, target, n, rho, (T *)mesh_bar, part);
+        k_loss_total<<<1, 256, 0, st>>>(part);
+    }
 };
 
 template <typename T> void adjoint_ops(AdjointOps &ops) {
@@ -297,6 +341,7 @@ template <typename T> void adjoint_ops(AdjointOps &ops) {
     ops.axpy = AdjLaunch<T>::axpy;
     ops.kick = AdjLaunch<T>::kick;
     ops.drift = AdjLaunch<T>::drift;
+    ops.mse = AdjLaunch<T>::mse;
 }
 
 template void adjoint_ops<float>(AdjointOps &);

@@ -96,6 +96,7 @@ struct dj_ctx {
     void *vb_fbar = nullptr, *vb_pbar = nullptr;
     void *vb_fik = nullptr;  // the three force meshes of the spectral gradient, recomputed in the reverse pass
     bool vb_resident = false;  // vb_psibar / vb_pibar hold cotangents left resident by dj_vjp_deposit / dj_vjp_run
+    double *loss_work = nullptr;  // block sums of the density loss (LOSS_PARTS doubles)
     void *ckpt = nullptr;  // (steps, 2, 3 n^3): Psi and Pi before each step
     int ck_steps = 0;
     std::vector<double> ck_dd1, ck_alpha, ck_beta, ck_dd2;
@@ -116,6 +117,8 @@ struct dj_ctx {
     // transforms on the particle grid (n != res_pm) for the displacement spectrum (allocated on first use)
     FftOps pg_fft{};
     void *pg_tw = nullptr, *pg_A = nullptr, *pg_B = nullptr;
+    // the IC pipeline's device buffers (dj_ics_lpt and its reverse), kept between calls
+    IcWork icw{};
     std::string err;
     bool prof = false;
     std::vector<ProfEvent> pending;
@@ -341,10 +344,11 @@ void free_all(dj_ctx *c) {
                     c->nn_t[0], c->nn_t[1],
                     c->nn_t[2], c->nn_t[3], c->nn_t[4], c->nn_t[5], c->nn_t[6], c->nn_t[7], c->nn_t[8], c->nn_t[9],
                     c->nn_dphi, c->nn_dens, c->nn_tgt, c->nn_sum, c->acc_list, c->kgain, c->pk_work,
-                    c->pg_tw, c->pg_A, c->pg_B};
+                    c->pg_tw, c->pg_A, c->pg_B, c->loss_work};
     // teardown: failures here have no caller to report to
     for (void *b : bufs)
         if (b) (void)hipFree(b);
+    ic_work_free(c->icw);
     free_lists(c->tl);
     free_lists(c->tlr);
     for (auto &p : c->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
@@ -1010,9 +1014,9 @@ dj_status dj_run(dj_ctx *c, int32_t n_steps, const double *dd1, const double *al
     return check_launch(c, "run");
 }
 
-dj_status dj_deposit(dj_ctx *c, const void *psi, const double *shift, void *mesh_out) {
-    if (!c || !mesh_out) return fail(c, DJ_ERR_INVALID_ARGUMENT, "null argument");
-    HIPCHK(c, hipSetDevice(c->dev));
+// the unshifted-or-shifted deposit of psi (NULL: the context's Psi) into the context's mesh (or its
+// fixed-point mesh); *dens_out names it
+static dj_status deposit_mesh(dj_ctx *c, const void *psi, const double *shift, void **dens_out) {
     const bool fixed = c->cfg.deposit == DJ_DEPOSIT_FIXED64;
     const void *src = c->psi;
     if (psi) {
@@ -1028,22 +1032,38 @@ dj_status dj_deposit(dj_ctx *c, const void *psi, const double *shift, void *mesh
     pa.L = c->cfg.boxsize;
     for (int a = 0; a < 3; a++) pa.shift[a] = shift ? shift[a] : 0.0;
     void *dens = fixed ? c->meshfix : c->mesh;
-    size_t bytes = c->M * (fixed ? 8 : c->tsize);
     HIPCHK(c, hipMemcpyAsync(c->d_shifts + 48, pa.shift, sizeof(double) * 3, hipMemcpyHostToDevice, c->st));
     dj_status r = tile_sort(c, src, c->d_shifts + 48, 1, c->cfg.worder);
     if (r) return r;
     TileArgs ta = tile_args(c, src, c->d_shifts + 48, 1);
     { Prof p(c, K_SCATTER); c->tiles.deposit(ta, c->cfg.worder, fixed, pa.shift, dens, fscale_dep(c->cfg), c->st); }
-    r = check_launch(c, "deposit");
+    *dens_out = dens;
+    return check_launch(c, "deposit");
+}
+
+static dj_status deposit_to(dj_ctx *c, const void *psi, const double *shift, void *mesh_out, hipMemcpyKind out_kind) {
+    if (!c || !mesh_out) return fail(c, DJ_ERR_INVALID_ARGUMENT, "null argument");
+    HIPCHK(c, hipSetDevice(c->dev));
+    void *dens = nullptr;
+    dj_status r = deposit_mesh(c, psi, shift, &dens);
     if (r) return r;
-    HIPCHK(c, hipMemcpyAsync(mesh_out, dens, bytes, hipMemcpyDeviceToHost, c->st));
+    const size_t bytes = c->M * (c->cfg.deposit == DJ_DEPOSIT_FIXED64 ? 8 : c->tsize);
+    HIPCHK(c, hipMemcpyAsync(mesh_out, dens, bytes, out_kind, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     drain_prof(c);
     return DJ_OK;
 }
 
-dj_status dj_ics_lpt(dj_ctx *c, uint32_t seed, const double *white, const double *k_tab, const double *pk_tab,
-                     int32_t ntab, double D, int32_t order) {
+dj_status dj_deposit(dj_ctx *c, const void *psi, const double *shift, void *mesh_out) {
+    return deposit_to(c, psi, shift, mesh_out, hipMemcpyDeviceToHost);
+}
+
+dj_status dj_deposit_device(dj_ctx *c, void *d_mesh_out) {
+    return deposit_to(c, nullptr, nullptr, d_mesh_out, hipMemcpyDeviceToDevice);
+}
+
+static dj_status ics_lpt(dj_ctx *c, uint32_t seed, const double *white, bool white_on_device, const double *k_tab,
+                         const double *pk_tab, int32_t ntab, double D, int32_t order) {
     if (!c || !k_tab || !pk_tab || ntab < 2 || (order != 1 && order != 2))
         return fail(c, DJ_ERR_INVALID_ARGUMENT, "invalid IC arguments");
     const int n = c->n;
@@ -1052,10 +1072,20 @@ dj_status dj_ics_lpt(dj_ctx *c, uint32_t seed, const double *white, const double
         return fail(c, DJ_ERR_UNSUPPORTED, "n_part " + std::to_string(n) + " has no built IC transform size");
     HIPCHK(c, hipSetDevice(c->dev));
     HIPCHK(c, hipStreamSynchronize(c->st));
-    int e = ics_lpt_device(n, c->cfg.boxsize, order, white, seed, k_tab, pk_tab, ntab, D, c->cfg.precision, c->psi,
-                           c->mom, c->st);
+    int e = ics_lpt_device(c->icw, n, c->cfg.boxsize, order, white, white_on_device, seed, k_tab, pk_tab, ntab, D,
+                           c->cfg.precision, c->psi, c->mom, c->st);
     if (e) return hip_fail(c, (hipError_t)e, "dj_ics_lpt");
     return DJ_OK;
+}
+
+dj_status dj_ics_lpt(dj_ctx *c, uint32_t seed, const double *white, const double *k_tab, const double *pk_tab,
+                     int32_t ntab, double D, int32_t order) {
+    return ics_lpt(c, seed, white, false, k_tab, pk_tab, ntab, D, order);
+}
+
+dj_status dj_ics_lpt_device(dj_ctx *c, uint32_t seed, const double *d_white, const double *k_tab,
+                            const double *pk_tab, int32_t ntab, double D, int32_t order) {
+    return ics_lpt(c, seed, d_white, true, k_tab, pk_tab, ntab, D, order);
 }
 
 // ------------------------------------------------------------------------ reverse mode
@@ -1173,7 +1203,8 @@ dj_status dj_vjp_acceleration(dj_ctx *c, const void *psi, const void *acc_bar, v
     return DJ_OK;
 }
 
-dj_status dj_vjp_deposit(dj_ctx *c, const void *psi, const void *mesh_bar, void *psi_bar_out) {
+static dj_status vjp_deposit(dj_ctx *c, const void *psi, const void *mesh_bar, hipMemcpyKind bar_kind,
+                             void *psi_bar_out) {
     if (!c || !mesh_bar) return fail(c, DJ_ERR_INVALID_ARGUMENT, "null argument");
     HIPCHK(c, hipSetDevice(c->dev));
     dj_status r = adj_buffers(c);
@@ -1184,7 +1215,7 @@ dj_status dj_vjp_deposit(dj_ctx *c, const void *psi, const void *mesh_bar, void 
         HIPCHK(c, hipMemcpyAsync(c->vb_psi1, psi, pb, hipMemcpyHostToDevice, c->st));
         src = c->vb_psi1;
     }
-    HIPCHK(c, hipMemcpyAsync(c->vb_pbar, mesh_bar, c->M * c->tsize, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->vb_pbar, mesh_bar, c->M * c->tsize, bar_kind, c->st));
     c->vb_resident = false;
     HIPCHK(c, hipMemsetAsync(c->vb_psibar, 0, pb, c->st));
     const double zero[3] = {0, 0, 0};
@@ -1199,6 +1230,44 @@ dj_status dj_vjp_deposit(dj_ctx *c, const void *psi, const void *mesh_bar, void 
     }
     HIPCHK(c, hipStreamSynchronize(c->st));
     c->vb_resident = !psi_bar_out;
+    return DJ_OK;
+}
+
+dj_status dj_vjp_deposit(dj_ctx *c, const void *psi, const void *mesh_bar, void *psi_bar_out) {
+    return vjp_deposit(c, psi, mesh_bar, hipMemcpyHostToDevice, psi_bar_out);
+}
+
+dj_status dj_vjp_deposit_device(dj_ctx *c, const void *d_mesh_bar) {
+    return vjp_deposit(c, nullptr, d_mesh_bar, hipMemcpyDeviceToDevice, nullptr);
+}
+
+dj_status dj_density_loss_device(dj_ctx *c, const double *d_target, double *loss_out) {
+    if (!c || !loss_out) return fail(c, DJ_ERR_INVALID_ARGUMENT, "null argument");
+    if (c->cfg.deposit == DJ_DEPOSIT_FIXED64)
+        return fail(c, DJ_ERR_UNSUPPORTED, "the density loss is built on the float deposit");
+    HIPCHK(c, hipSetDevice(c->dev));
+    dj_status r = adj_buffers(c);
+    if (r) return r;
+    if (!c->loss_work) HIPCHK(c, hipMalloc(&c->loss_work, sizeof(double) * LOSS_PARTS));
+    void *dens = nullptr;
+    r = deposit_mesh(c, nullptr, nullptr, &dens);
+    if (r) return r;
+    const double rho = (double)c->np / (double)c->M;  // particles per cell
+    c->adj.mse(dens, d_target, c->M, rho, c->vb_pbar, c->loss_work, c->st);
+    // the deposit's reverse at the context's Psi, left resident (Psibar, Pibar = 0)
+    const size_t pb = (size_t)c->n3 * c->tsize;
+    c->vb_resident = false;
+    HIPCHK(c, hipMemsetAsync(c->vb_psibar, 0, pb, c->st));
+    HIPCHK(c, hipMemsetAsync(c->vb_pibar, 0, pb, c->st));
+    const double zero[3] = {0, 0, 0};
+    c->adj.gather_grad(c->psi, c->q1, c->n, c->res, c->cfg.boxsize, zero, c->vb_pbar, nullptr, nullptr, 0.0, c->fdc,
+                       c->cfg.worder, c->vb_psibar, c->st);
+    r = check_launch(c, "density loss");
+    if (r) return r;
+    HIPCHK(c, hipMemcpyAsync(loss_out, c->loss_work + LOSS_PARTS - 1, sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    drain_prof(c);
+    c->vb_resident = true;
     return DJ_OK;
 }
 
@@ -1271,9 +1340,9 @@ dj_status dj_vjp_run(dj_ctx *c, void *psi_bar, void *pi_bar) {
     return DJ_OK;
 }
 
-dj_status dj_vjp_ics_lpt(dj_ctx *c, uint32_t seed, const double *white, const double *k_tab, const double *pk_tab,
-                         int32_t ntab, double D, int32_t order, const void *psi_bar, const void *pi_bar,
-                         double *white_bar_out) {
+static dj_status vjp_ics_lpt(dj_ctx *c, uint32_t seed, const double *white, bool on_device, const double *k_tab,
+                             const double *pk_tab, int32_t ntab, double D, int32_t order, const void *psi_bar,
+                             const void *pi_bar, double *white_bar_out) {
     if (!c || !k_tab || !pk_tab || ntab < 2 || (order != 1 && order != 2) || (!psi_bar) != (!pi_bar) ||
         !white_bar_out)
         return fail(c, DJ_ERR_INVALID_ARGUMENT, "invalid IC adjoint arguments");
@@ -1293,10 +1362,21 @@ dj_status dj_vjp_ics_lpt(dj_ctx *c, uint32_t seed, const double *white, const do
         c->vb_resident = false;
     }
     HIPCHK(c, hipStreamSynchronize(c->st));
-    int e = ics_lpt_vjp_device(n, c->cfg.boxsize, order, white, seed, k_tab, pk_tab, ntab,
-                               D, c->cfg.precision, c->vb_psibar, c->vb_pibar, white_bar_out, c->st);
+    int e = ics_lpt_vjp_device(c->icw, n, c->cfg.boxsize, order, white, on_device, seed, k_tab, pk_tab, ntab, D,
+                               c->cfg.precision, c->vb_psibar, c->vb_pibar, white_bar_out, on_device, c->st);
     if (e) return hip_fail(c, (hipError_t)e, "dj_vjp_ics_lpt");
     return DJ_OK;
+}
+
+dj_status dj_vjp_ics_lpt(dj_ctx *c, uint32_t seed, const double *white, const double *k_tab, const double *pk_tab,
+                         int32_t ntab, double D, int32_t order, const void *psi_bar, const void *pi_bar,
+                         double *white_bar_out) {
+    return vjp_ics_lpt(c, seed, white, false, k_tab, pk_tab, ntab, D, order, psi_bar, pi_bar, white_bar_out);
+}
+
+dj_status dj_vjp_ics_lpt_device(dj_ctx *c, uint32_t seed, const double *d_white, const double *k_tab,
+                                const double *pk_tab, int32_t ntab, double D, int32_t order, double *d_white_bar_out) {
+    return vjp_ics_lpt(c, seed, d_white, true, k_tab, pk_tab, ntab, D, order, nullptr, nullptr, d_white_bar_out);
 }
 
 // ------------------------------------------------------------------------ learned correction

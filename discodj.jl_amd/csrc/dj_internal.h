@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include <cstdint>
 
 namespace dj {
@@ -145,7 +147,12 @@ struct AdjointOps {
                  hipStream_t st);
     // first drift backward: Pibar += dd1 Psibar
     void (*drift)(const void *psibar, void *pibar, size_t n, double dd1, hipStream_t st);
+    // L = sum (mesh / rho - 1 - target)^2 into part[LOSS_PARTS - 1] (part: LOSS_PARTS doubles), mesh_bar = dL/dmesh;
+    // target nullable (zero)
+    void (*mse)(const void *mesh, const double *target, size_t n, double rho, void *mesh_bar, double *part,
+                hipStream_t st);
 };
+constexpr int LOSS_PARTS = 1025;
 template <typename T> void adjoint_ops(AdjointOps &ops);
 
 // learned correction (nn.hip): FP32 fields, BF16 channel-last tensors t[v * 16 + c]
@@ -199,12 +206,26 @@ void ngenic_seed_table(uint32_t seed, int res, uint32_t *tab);
 // the N-GenIC field of seed on the device (the columns' generators run one per thread), (res, res, res/2+1)
 // complex, element (i, j, k) at i + res (j + res k), into device memory d_field; returns a HIP error code
 int ngenic_field_device(uint32_t seed, int res, double *d_field, hipStream_t st);
-// white noise: white_host (n^3 real values), or else the N-GenIC field of seed generated on the device
-int ics_lpt_device(int n, double L, int order, const double *white_host, uint32_t seed,
+// Device buffers of the IC pipeline and its reverse for one (n, order), kept between calls (the 2LPT
+// transforms at 3n/2 need about 20 n^3 doubles: 5.3 GB at n = 256, 42 GB at n = 512); freed by
+// ic_work_free or when a call needs another size
+struct IcWork {
+    int n = 0, order = 0, ntab = 0;
+    std::vector<void *> bufs;
+    void *twn = nullptr, *twe = nullptr, *kt = nullptr, *pt = nullptr;
+    void *A = nullptr, *Bm = nullptr, *F = nullptr, *Fb = nullptr, *T1 = nullptr, *fLb = nullptr;
+    void *w = nullptr, *p1 = nullptr, *p2 = nullptr;
+    void *Ae = nullptr, *Be = nullptr, *Pe = nullptr, *mu = nullptr, *ab = nullptr, *fld[3] = {nullptr, nullptr, nullptr};
+};
+void ic_work_free(IcWork &ws);
+// White noise: `white` (n^3 real values, on the host, or on the device when white_on_device), or else
+// (NULL) the N-GenIC field of seed generated on the device.  The reverse writes the white-noise cotangent
+// to wbar (host, or device when wbar_on_device).  Both return a HIP error code.
+int ics_lpt_device(IcWork &ws, int n, double L, int order, const double *white, bool white_on_device, uint32_t seed,
                    const double *kt, const double *pt, int ntab, double D, int prec, void *psi_out,
                    void *pi_out, hipStream_t st);
-int ics_lpt_vjp_device(int n, double L, int order, const double *white_host, uint32_t seed,
-                       const double *kt, const double *pt, int ntab, double D, int prec, const void *psibar,
-                       const void *pibar, double *wbar_host, hipStream_t st);
+int ics_lpt_vjp_device(IcWork &ws, int n, double L, int order, const double *white, bool white_on_device,
+                       uint32_t seed, const double *kt, const double *pt, int ntab, double D, int prec,
+                       const void *psibar, const void *pibar, double *wbar, bool wbar_on_device, hipStream_t st);
 
 }  // namespace dj

@@ -310,23 +310,23 @@ __global__ void k_field_to_blayout(const cx<double> *__restrict__ f, int n, doub
     B[e] = scale(v, sc);
 }
 
-// host orchestration; returns a HIP error code (0 = success)
-int ics_lpt_device(int n, double L, int order, const double *white_host, uint32_t seed,
-                   const double *kt_h, const double *pt_h, int ntab, double D, int prec, void *psi_out,
-                   void *pi_out, hipStream_t st) {
-    FftOps fn{}, fe{};
-    const int ext = 3 * n / 2;
-    if (!fft_dispatch<double>(n, fn)) return (int)hipErrorInvalidValue;
-    if (order >= 2 && !fft_dispatch<double>(ext, fe)) return (int)hipErrorInvalidValue;
-    const int Hn = n / 2 + 1, He = ext / 2 + 1;
+void ic_work_free(IcWork &ws) {
+    for (void *p : ws.bufs) (void)hipFree(p);
+    ws = IcWork{};
+}
+
+// the buffers of both directions for (n, order) and room for ntab table entries, kept while they fit
+static hipError_t ic_work_ensure(IcWork &ws, int n, int order, int ntab) {
+    if (ws.n == n && ws.order >= order && ws.ntab >= ntab) return hipSuccess;
+    ic_work_free(ws);
+    const int ext = 3 * n / 2, Hn = n / 2 + 1, He = ext / 2 + 1;
     const size_t M = (size_t)n * n * n, HC = (size_t)n * Hn * n, ME = (size_t)ext * ext * ext,
                  HE = (size_t)ext * He * ext;
-    std::vector<void *> allocs;
     hipError_t err = hipSuccess;
     auto alloc = [&](size_t bytes) -> void * {
         void *p = nullptr;
         if (err == hipSuccess) err = hipMalloc(&p, bytes);
-        if (p) allocs.push_back(p);
+        if (p) ws.bufs.push_back(p);
         return p;
     };
     auto twiddles = [&](int N) -> void * {
@@ -336,15 +336,79 @@ int ics_lpt_device(int n, double L, int order, const double *white_host, uint32_
             tw[k] = {std::cos(a), std::sin(a)};
         }
         void *d = alloc(sizeof(cx<double>) * N);
-        if (d) err = hipMemcpy(d, tw.data(), sizeof(cx<double>) * N, hipMemcpyHostToDevice);
+        if (d && err == hipSuccess) err = hipMemcpy(d, tw.data(), sizeof(cx<double>) * N, hipMemcpyHostToDevice);
         return d;
     };
-    void *twn = twiddles(n);
-    cx<double> *A = (cx<double> *)alloc(HC * 16), *Bm = (cx<double> *)alloc(HC * 16), *F = (cx<double> *)alloc(HC * 16);
-    double *w = (double *)alloc(M * 8), *p1 = (double *)alloc(3 * M * 8), *p2 = (double *)alloc(3 * M * 8);
-    double *kt = (double *)alloc(ntab * 8), *pt = (double *)alloc(ntab * 8);
-    if (err == hipSuccess) err = hipMemcpy(kt, kt_h, ntab * 8, hipMemcpyHostToDevice);
-    if (err == hipSuccess) err = hipMemcpy(pt, pt_h, ntab * 8, hipMemcpyHostToDevice);
+    const int cap = std::max(ntab, 4096);
+    ws.twn = twiddles(n);
+    ws.kt = alloc((size_t)cap * 8);
+    ws.pt = alloc((size_t)cap * 8);
+    ws.A = alloc(HC * 16);
+    ws.Bm = alloc(HC * 16);
+    ws.F = alloc(HC * 16);
+    ws.Fb = alloc(HC * 16);
+    ws.T1 = alloc(HC * 16);
+    ws.w = alloc(M * 8);
+    ws.p1 = alloc(3 * M * 8);
+    ws.p2 = alloc(3 * M * 8);
+    if (order >= 2) {
+        ws.twe = twiddles(ext);
+        ws.fLb = alloc(HC * 16);
+        ws.Ae = alloc(HE * 16);
+        ws.Be = alloc(HE * 16);
+        ws.Pe = alloc(HE * 16);
+        ws.mu = alloc(ME * 8);
+        ws.ab = alloc(ME * 8);
+        for (auto &f : ws.fld) f = alloc(ME * 8);
+    }
+    if (err != hipSuccess) {
+        ic_work_free(ws);
+        return err;
+    }
+    ws.n = n;
+    ws.order = order;
+    ws.ntab = cap;
+    return hipSuccess;
+}
+
+// white noise into ws.w: copied from the host or the device, or the N-GenIC field of seed
+static hipError_t ic_white(IcWork &ws, FftOps &fn, int n, const double *white, bool on_device, uint32_t seed,
+                           hipStream_t st) {
+    const size_t M = (size_t)n * n * n, HC = (size_t)n * (n / 2 + 1) * n;
+    if (white)
+        return hipMemcpyAsync(ws.w, white, M * 8, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st);
+    // the z-halved field (n n (n/2+1) complex values, the size of F, which is free until phi(k)), then to
+    // the B layout and back to real space
+    hipError_t err = (hipError_t)ngenic_field_device(seed, n, (double *)ws.F, st);
+    if (err != hipSuccess) return err;
+    const KMult<double> none{};
+    cx<double> *Bm = (cx<double> *)ws.Bm;
+    k_field_to_blayout<<<g1(HC), 256, 0, st>>>((const cx<double> *)ws.F, n, std::pow((double)n, 1.5), Bm);
+    fn.zm(ZM_NONE, false, true, Bm, Bm, ws.twn, &none, st);
+    fn.yi(Bm, ws.A, ws.twn, st);
+    fn.xi(ws.A, ws.w, ws.twn, st);
+    k_ic_scale_real<<<g1(M), 256, 0, st>>>((double *)ws.w, M, 1.0 / (double)M);
+    return hipGetLastError();
+}
+
+// host orchestration; returns a HIP error code (0 = success)
+int ics_lpt_device(IcWork &ws, int n, double L, int order, const double *white, bool white_on_device, uint32_t seed,
+                   const double *kt_h, const double *pt_h, int ntab, double D, int prec, void *psi_out,
+                   void *pi_out, hipStream_t st) {
+    FftOps fn{}, fe{};
+    const int ext = 3 * n / 2;
+    if (!fft_dispatch<double>(n, fn)) return (int)hipErrorInvalidValue;
+    if (order >= 2 && !fft_dispatch<double>(ext, fe)) return (int)hipErrorInvalidValue;
+    const size_t M = (size_t)n * n * n, HC = (size_t)n * (n / 2 + 1) * n, ME = (size_t)ext * ext * ext,
+                 HE = (size_t)ext * (ext / 2 + 1) * ext;
+    hipError_t err = ic_work_ensure(ws, n, order, ntab);
+    if (err != hipSuccess) return (int)err;
+    void *twn = ws.twn;
+    cx<double> *A = (cx<double> *)ws.A, *Bm = (cx<double> *)ws.Bm, *F = (cx<double> *)ws.F;
+    double *w = (double *)ws.w, *p1 = (double *)ws.p1, *p2 = (double *)ws.p2;
+    double *kt = (double *)ws.kt, *pt = (double *)ws.pt;
+    err = hipMemcpyAsync(kt, kt_h, ntab * 8, hipMemcpyHostToDevice, st);
+    if (err == hipSuccess) err = hipMemcpyAsync(pt, pt_h, ntab * 8, hipMemcpyHostToDevice, st);
     const KMult<double> none{};  // the plain transforms apply no multiplier
     auto fwd = [&](FftOps &f, const double *real, cx<double> *a, cx<double> *b, void *tw) {
         f.xf(real, false, a, tw, 1.0, st);
@@ -356,20 +420,7 @@ int ics_lpt_device(int n, double L, int order, const double *white_host, uint32_
         f.yi(b, a, tw, st);
         f.xi(a, real, tw, st);
     };
-    if (err != hipSuccess) goto done;
-    // white noise
-    if (white_host) {
-        err = hipMemcpy(w, white_host, M * 8, hipMemcpyHostToDevice);
-    } else {
-        // the z-halved field (n n (n/2+1) complex values, the size of F, which is free until phi(k)),
-        // then to the B layout
-        err = (hipError_t)ngenic_field_device(seed, n, (double *)F, st);
-        if (err == hipSuccess) {
-            k_field_to_blayout<<<g1(HC), 256, 0, st>>>(F, n, std::pow((double)n, 1.5), Bm);
-            inv(fn, Bm, A, w, twn);
-            k_ic_scale_real<<<g1(M), 256, 0, st>>>(w, M, 1.0 / (double)M);
-        }
-    }
+    if (err == hipSuccess) err = ic_white(ws, fn, n, white, white_on_device, seed, st);
     if (err != hipSuccess) goto done;
     // phi(k)
     fwd(fn, w, A, F, twn);
@@ -380,11 +431,10 @@ int ics_lpt_device(int n, double L, int order, const double *white_host, uint32_
         k_ic_scale_real<<<g1(M), 256, 0, st>>>(p1 + c * M, M, 1.0 / (double)M);
     }
     if (order >= 2) {
-        void *twe = twiddles(ext);
-        cx<double> *Ae = (cx<double> *)alloc(HE * 16), *Be = (cx<double> *)alloc(HE * 16), *Pe = (cx<double> *)alloc(HE * 16);
-        double *mu = (double *)alloc(ME * 8);
-        double *fld[3] = {(double *)alloc(ME * 8), (double *)alloc(ME * 8), (double *)alloc(ME * 8)};
-        if (err != hipSuccess) goto done;
+        void *twe = ws.twe;
+        cx<double> *Ae = (cx<double> *)ws.Ae, *Be = (cx<double> *)ws.Be, *Pe = (cx<double> *)ws.Pe;
+        double *mu = (double *)ws.mu;
+        double *fld[3] = {(double *)ws.fld[0], (double *)ws.fld[1], (double *)ws.fld[2]};
         const double pad_s = std::pow((double)ext / n, 3), crop_s = std::pow((double)n / ext, 3);
         // A_ik = irfft_ext(i k_k pad(psi1_i)); psi1_i spectrum = -(i k_i) phi
         auto deriv = [&](int i, int k, double *out) {
@@ -425,49 +475,31 @@ int ics_lpt_device(int n, double L, int order, const double *white_host, uint32_
     if (err == hipSuccess) err = hipStreamSynchronize(st);
 done:
     (void)hipStreamSynchronize(st);
-    for (void *p : allocs) (void)hipFree(p);
     return (int)err;
 }
 
 // Reverse mode of ics_lpt_device: (Psibar, Pibar) of the ICs (device, precision prec) -> the white-noise
-// cotangent wbar (host, n^3 FP64).  The forward fields are recomputed (phi(k), the nine A_ik on the ext
-// grid, three at a time); each linear step is transposed with the adjoint rules above, and the 2LPT
+// cotangent wbar (n^3 FP64, host or device).  The forward fields are recomputed (phi(k), the nine A_ik on
+// the ext grid, three at a time); each linear step is transposed with the adjoint rules above, and the 2LPT
 // source mu is a quadratic form whose cotangent per field is mubar times its partner in mu.
-int ics_lpt_vjp_device(int n, double L, int order, const double *white_host, uint32_t seed,
-                       const double *kt_h, const double *pt_h, int ntab, double D, int prec, const void *psibar,
-                       const void *pibar, double *wbar_host, hipStream_t st) {
+int ics_lpt_vjp_device(IcWork &ws, int n, double L, int order, const double *white, bool white_on_device,
+                       uint32_t seed, const double *kt_h, const double *pt_h, int ntab, double D, int prec,
+                       const void *psibar, const void *pibar, double *wbar, bool wbar_on_device, hipStream_t st) {
     FftOps fn{}, fe{};
     const int ext = 3 * n / 2;
     if (!fft_dispatch<double>(n, fn)) return (int)hipErrorInvalidValue;
     if (order >= 2 && !fft_dispatch<double>(ext, fe)) return (int)hipErrorInvalidValue;
-    const int Hn = n / 2 + 1, He = ext / 2 + 1;
-    const size_t M = (size_t)n * n * n, HC = (size_t)n * Hn * n, ME = (size_t)ext * ext * ext,
-                 HE = (size_t)ext * He * ext;
-    std::vector<void *> allocs;
-    hipError_t err = hipSuccess;
-    auto alloc = [&](size_t bytes) -> void * {
-        void *p = nullptr;
-        if (err == hipSuccess) err = hipMalloc(&p, bytes);
-        if (p) allocs.push_back(p);
-        return p;
-    };
-    auto twiddles = [&](int N) -> void * {
-        std::vector<cx<double>> tw(N);
-        for (int k = 0; k < N; k++) {
-            double a = -2.0 * M_PI * (double)k / (double)N;
-            tw[k] = {std::cos(a), std::sin(a)};
-        }
-        void *d = alloc(sizeof(cx<double>) * N);
-        if (d) err = hipMemcpy(d, tw.data(), sizeof(cx<double>) * N, hipMemcpyHostToDevice);
-        return d;
-    };
-    void *twn = twiddles(n);
-    cx<double> *A = (cx<double> *)alloc(HC * 16), *Bm = (cx<double> *)alloc(HC * 16), *F = (cx<double> *)alloc(HC * 16),
-               *Fb = (cx<double> *)alloc(HC * 16), *T1 = (cx<double> *)alloc(HC * 16);
-    double *w = (double *)alloc(M * 8), *p1b = (double *)alloc(3 * M * 8), *p2b = (double *)alloc(3 * M * 8);
-    double *kt = (double *)alloc(ntab * 8), *pt = (double *)alloc(ntab * 8);
-    if (err == hipSuccess) err = hipMemcpy(kt, kt_h, ntab * 8, hipMemcpyHostToDevice);
-    if (err == hipSuccess) err = hipMemcpy(pt, pt_h, ntab * 8, hipMemcpyHostToDevice);
+    const size_t M = (size_t)n * n * n, HC = (size_t)n * (n / 2 + 1) * n, ME = (size_t)ext * ext * ext,
+                 HE = (size_t)ext * (ext / 2 + 1) * ext;
+    hipError_t err = ic_work_ensure(ws, n, order, ntab);
+    if (err != hipSuccess) return (int)err;
+    void *twn = ws.twn;
+    cx<double> *A = (cx<double> *)ws.A, *Bm = (cx<double> *)ws.Bm, *F = (cx<double> *)ws.F,
+               *Fb = (cx<double> *)ws.Fb, *T1 = (cx<double> *)ws.T1;
+    double *w = (double *)ws.w, *p1b = (double *)ws.p1, *p2b = (double *)ws.p2;
+    double *kt = (double *)ws.kt, *pt = (double *)ws.pt;
+    err = hipMemcpyAsync(kt, kt_h, ntab * 8, hipMemcpyHostToDevice, st);
+    if (err == hipSuccess) err = hipMemcpyAsync(pt, pt_h, ntab * 8, hipMemcpyHostToDevice, st);
     const KMult<double> none{};
     auto fwd = [&](FftOps &f, const double *real, cx<double> *a, cx<double> *b, void *tw) {
         f.xf(real, false, a, tw, 1.0, st);
@@ -479,19 +511,7 @@ int ics_lpt_vjp_device(int n, double L, int order, const double *white_host, uin
         f.yi(b, a, tw, st);
         f.xi(a, real, tw, st);
     };
-    if (err != hipSuccess) goto done;
-    if (white_host) {
-        err = hipMemcpy(w, white_host, M * 8, hipMemcpyHostToDevice);
-    } else {
-        // the z-halved field (n n (n/2+1) complex values, the size of F, which is free until phi(k)),
-        // then to the B layout
-        err = (hipError_t)ngenic_field_device(seed, n, (double *)F, st);
-        if (err == hipSuccess) {
-            k_field_to_blayout<<<g1(HC), 256, 0, st>>>(F, n, std::pow((double)n, 1.5), Bm);
-            inv(fn, Bm, A, w, twn);
-            k_ic_scale_real<<<g1(M), 256, 0, st>>>(w, M, 1.0 / (double)M);
-        }
-    }
+    if (err == hipSuccess) err = ic_white(ws, fn, n, white, white_on_device, seed, st);
     if (err != hipSuccess) goto done;
     // forward phi(k), then the state cotangents split into psi1bar, psi2bar
     fwd(fn, w, A, F, twn);
@@ -511,12 +531,11 @@ int ics_lpt_vjp_device(int n, double L, int order, const double *white_host, uin
         k_ic_axpy<<<g1(HC), 256, 0, st>>>(Fb, T1, HC);
     }
     if (order >= 2) {
-        void *twe = twiddles(ext);
-        cx<double> *Ae = (cx<double> *)alloc(HE * 16), *Be = (cx<double> *)alloc(HE * 16), *Pe = (cx<double> *)alloc(HE * 16);
-        double *mub = (double *)alloc(ME * 8), *ab = (double *)alloc(ME * 8);
-        double *fld[3] = {(double *)alloc(ME * 8), (double *)alloc(ME * 8), (double *)alloc(ME * 8)};
-        cx<double> *fLb = (cx<double> *)alloc(HC * 16);
-        if (err != hipSuccess) goto done;
+        void *twe = ws.twe;
+        cx<double> *Ae = (cx<double> *)ws.Ae, *Be = (cx<double> *)ws.Be, *Pe = (cx<double> *)ws.Pe;
+        double *mub = (double *)ws.mu, *ab = (double *)ws.ab;
+        double *fld[3] = {(double *)ws.fld[0], (double *)ws.fld[1], (double *)ws.fld[2]};
+        cx<double> *fLb = (cx<double> *)ws.fLb;
         const double pad_s = std::pow((double)ext / n, 3), crop_s = std::pow((double)n / ext, 3);
         // psi2_c = c2r((-1/k^2)(i k_c) fL) / M:  fLbar = sum_c (1/k^2)(i k_c) w r2c(psi2bar_c) / M
         (void)hipMemsetAsync(fLb, 0, HC * 16, st);
@@ -573,11 +592,11 @@ int ics_lpt_vjp_device(int n, double L, int order, const double *white_host, uin
     k_ic_halfw<<<g1(HC), 256, 0, st>>>(Fb, n, 1, 1.0);
     inv(fn, Fb, A, w, twn);
     err = hipGetLastError();
-    if (err == hipSuccess) err = hipMemcpyAsync(wbar_host, w, M * 8, hipMemcpyDeviceToHost, st);
+    if (err == hipSuccess)
+        err = hipMemcpyAsync(wbar, w, M * 8, wbar_on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st);
     if (err == hipSuccess) err = hipStreamSynchronize(st);
 done:
     (void)hipStreamSynchronize(st);
-    for (void *p : allocs) (void)hipFree(p);
     return (int)err;
 }
 

@@ -81,6 +81,13 @@ _SIGS = {
     "dj_vjp_run": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "dj_vjp_ics_lpt": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32,
                                    C.c_double, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "dj_ics_lpt_device": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32,
+                                      C.c_double, C.c_int32]),
+    "dj_vjp_ics_lpt_device": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32,
+                                          C.c_double, C.c_int32, C.c_void_p]),
+    "dj_deposit_device": (C.c_int32, [C.c_void_p, C.c_void_p]),
+    "dj_vjp_deposit_device": (C.c_int32, [C.c_void_p, C.c_void_p]),
+    "dj_density_loss_device": (C.c_int32, [C.c_void_p, C.c_void_p, C.POINTER(C.c_double)]),
     "dj_nn_set_params": (C.c_int32, [C.c_void_p, C.c_void_p]),
     "dj_nn_get_params": (C.c_int32, [C.c_void_p, C.c_void_p]),
     "dj_nn_enable": (C.c_int32, [C.c_void_p, C.c_int32, C.c_double]),
@@ -114,6 +121,29 @@ def lib():
             f.argtypes = args
         _lib = L
     return _lib
+
+
+_hip = None
+
+
+def hip():
+    """The HIP runtime the library was loaded with (the copy of libamdhip64 mapped into this process for it):
+    device memory for the _device entry points is allocated with it, so that library and caller share one
+    runtime and one view of the device."""
+    global _hip
+    if _hip is None:
+        lib()
+        paths = [ln.split()[-1] for ln in open("/proc/self/maps") if "libamdhip64.so" in ln]
+        if not paths:
+            raise RuntimeError("libamdhip64 is not mapped into this process")
+        H = C.CDLL(paths[0])
+        H.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        H.hipFree.argtypes = [C.c_void_p]
+        H.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        H.hipSetDevice.argtypes = [C.c_int]
+        H.hipDeviceSynchronize.argtypes = []
+        _hip = H
+    return _hip
 
 
 def check(rc, ctx=None):

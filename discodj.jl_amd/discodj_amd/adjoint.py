@@ -11,16 +11,27 @@ reference has no adjoint of its PM path (NR/README.md:169); the pieces follow it
 import numpy as np
 
 from .cosmology import linear_power_spectrum, stepper_coefficients
+from .device import DeviceArray
 from .pm import PMConfig, PMSolverHIP, check_solver
 
 
 def density_loss_gradient(cosmo, white, target, *, boxsize, res_pm, a_ini=0.05, a_end=1.0, n_steps=10,
                           stepper="bullfrog", time_var="D", order=2, worder=2, grad_kernel_order=4,
                           laplace_kernel_order=0, antialias=0, deconvolve=False, precision=32, k_tab=None,
-                          pk_tab=None, solver=None):
+                          pk_tab=None, solver=None, on_device=False):
     """-> (loss, white_bar, delta): L = sum (delta_final - target)^2 on the res_pm mesh and its gradient with
     respect to the (n,n,n) real white noise [k,j,i] of the ICs (FP64 array).  The momentum variable of the
-    steps is Pi (bullfrog, fastpm); the symplectic stepper's velocity scaling is applied and reversed."""
+    steps is Pi (bullfrog, fastpm); the symplectic stepper's velocity scaling is applied and reversed.
+    on_device: white and target are (or are uploaded to) DeviceArrays, the loss and its cotangent are
+    formed on the device (dj_density_loss_device) and white_bar is returned as a DeviceArray (delta as
+    None): nothing of size n^3 crosses PCIe once the inputs are resident (bullfrog and fastpm steppers)."""
+    if on_device:
+        return _density_loss_gradient_device(cosmo, white, target, boxsize=boxsize, res_pm=res_pm, a_ini=a_ini,
+                                             a_end=a_end, n_steps=n_steps, stepper=stepper, time_var=time_var,
+                                             order=order, worder=worder, grad_kernel_order=grad_kernel_order,
+                                             laplace_kernel_order=laplace_kernel_order, antialias=antialias,
+                                             deconvolve=deconvolve, precision=precision, k_tab=k_tab, pk_tab=pk_tab,
+                                             solver=solver)
     white = np.ascontiguousarray(white, np.float64)
     n = white.shape[0]
     if k_tab is None:
@@ -59,3 +70,36 @@ def density_loss_gradient(cosmo, white, target, *, boxsize, res_pm, a_ini=0.05, 
         if own:
             S.close()
     return loss, white_bar, delta
+
+
+def _density_loss_gradient_device(cosmo, white, target, *, boxsize, res_pm, a_ini, a_end, n_steps, stepper,
+                                  time_var, order, worder, grad_kernel_order, laplace_kernel_order, antialias,
+                                  deconvolve, precision, k_tab, pk_tab, solver):
+    if stepper == "symplectic":
+        raise ValueError("on_device: the symplectic stepper's velocity scaling runs on the host; use on_device=False")
+    dev = solver.device if solver is not None else 0
+    w = white if isinstance(white, DeviceArray) else DeviceArray.from_numpy(np.asarray(white, np.float64), dev)
+    tgt = None
+    if target is not None:
+        tgt = target if isinstance(target, DeviceArray) else DeviceArray.from_numpy(np.asarray(target, np.float64), dev)
+    n = w.shape[0]
+    if k_tab is None:
+        k_tab, pk_tab = linear_power_spectrum(cosmo)
+    cfg = PMConfig(res_pm=res_pm, n_part=n, boxsize=float(boxsize), worder=worder, deconvolve=deconvolve,
+                   antialias=antialias, grad_order=grad_kernel_order, lap_order=laplace_kernel_order)
+    co = stepper_coefficients(cosmo, stepper, a_ini, a_end, n_steps, time_var)
+    D = cosmo.Dplus(a_ini)
+    own = solver is None
+    if not own:
+        check_solver(solver, cfg, precision, deposit="float")
+    S = solver or PMSolverHIP(cfg, precision=precision)
+    try:
+        S.ics_lpt_device(k_tab, pk_tab, D, w, order=order)
+        S.run_checkpointed(co["ddrift1"], co["alpha"], co["beta"], co["ddrift2"])
+        loss = S.density_loss_device(tgt)
+        S.vjp_run()
+        white_bar = S.vjp_ics_lpt_device(k_tab, pk_tab, D, w, DeviceArray(w.shape, np.float64, dev), order=order)
+    finally:
+        if own:
+            S.close()
+    return loss, white_bar, None

@@ -16,6 +16,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib
+from .device import DeviceArray
 from ._lib import DJConfig, DJError, KernelStat, check
 
 
@@ -55,6 +56,7 @@ class PMSolverHIP:
         self.shape = (3, self.n, self.n, self.n)
         self.deposit_mode = deposit
         self.lean = bool(lean)
+        self.device = int(device)
         L = _lib.lib()
         ptr = C.c_void_p()
         c = _c_config(cfg, precision, deposit, lean, device, fixed_frac_bits)
@@ -137,6 +139,61 @@ class PMSolverHIP:
         w = None if white is None else np.ascontiguousarray(white, np.float64)
         self._chk(self._L.dj_ics_lpt(self._ctx, seed, None if w is None else w.ctypes.data, kt.ctypes.data,
                                      pt.ctypes.data, len(kt), float(D), order))
+
+    # device-resident arrays (DeviceArray: memory of the HIP runtime the library links) ----------------
+    def _dptr(self, t, nbytes, what):
+        """the device address of t, a DeviceArray of nbytes on the solver's device (None passes through)"""
+        if t is None:
+            return None
+        if not isinstance(t, DeviceArray):
+            raise TypeError(f"{what}: a DeviceArray is expected")
+        if t.nbytes != nbytes:
+            raise ValueError(f"{what}: {nbytes} bytes expected, the array has {t.nbytes}")
+        if t.device != self.device:
+            raise ValueError(f"{what}: on device {t.device}, the solver runs on {self.device}")
+        return t.ptr
+
+    def density_loss_device(self, target=None):
+        """L = sum (delta - target)^2 of the solver's Psi on the mesh (delta = mesh / rho - 1; target: an
+        (N,N,N) FP64 DeviceArray or None for zero); its cotangent stays resident for vjp_run (dj_density_loss_device)"""
+        N = self.cfg.res_pm
+        loss = C.c_double()
+        self._chk(self._L.dj_density_loss_device(self._ctx, self._dptr(target, 8 * N**3, "target"), C.byref(loss)))
+        return loss.value
+
+    def ics_lpt_device(self, k_tab, pk_tab, D, white, order=2, seed=42):
+        """ics_lpt with the (n,n,n) FP64 white noise a DeviceArray (None: the N-GenIC field of seed)"""
+        kt = np.ascontiguousarray(k_tab, np.float64)
+        pt = np.ascontiguousarray(pk_tab, np.float64)
+        n = self.cfg.n_part
+        w = self._dptr(white, 8 * n**3, "white")
+        self._chk(self._L.dj_ics_lpt_device(self._ctx, seed, w, kt.ctypes.data, pt.ctypes.data, len(kt), float(D),
+                                            order))
+
+    def deposit_device(self, out):
+        """deposit() of the solver's Psi into the DeviceArray out (N^3 values of the solver's precision, int64
+        in the fixed64 mode)"""
+        N = self.cfg.res_pm
+        es = 8 if self.deposit_mode == "fixed64" else np.dtype(self.dtype).itemsize
+        self._chk(self._L.dj_deposit_device(self._ctx, self._dptr(out, es * N**3, "mesh")))
+        return out
+
+    def vjp_deposit_device(self, mesh_bar):
+        """vjp_deposit(None, mesh_bar, resident=True) with mesh_bar (N^3, the solver's precision) a DeviceArray"""
+        N = self.cfg.res_pm
+        mb = self._dptr(mesh_bar, np.dtype(self.dtype).itemsize * N**3, "mesh_bar")
+        self._chk(self._L.dj_vjp_deposit_device(self._ctx, mb))
+
+    def vjp_ics_lpt_device(self, k_tab, pk_tab, D, white, white_bar, order=2, seed=42):
+        """vjp_ics_lpt from the resident cotangents into the (n,n,n) FP64 DeviceArray white_bar"""
+        kt = np.ascontiguousarray(k_tab, np.float64)
+        pt = np.ascontiguousarray(pk_tab, np.float64)
+        n = self.cfg.n_part
+        w = self._dptr(white, 8 * n**3, "white")
+        wb = self._dptr(white_bar, 8 * n**3, "white_bar")
+        self._chk(self._L.dj_vjp_ics_lpt_device(self._ctx, seed, w, kt.ctypes.data, pt.ctypes.data, len(kt),
+                                                float(D), order, wb))
+        return white_bar
 
     def power_spectrum(self, psi=None, W=2, nbins=None):
         """(k, P, nmodes) of the CIC density of q + psi on the solver's mesh, window-deconvolved, linear

@@ -114,6 +114,10 @@ dj_status dj_run(dj_ctx *ctx, int32_t n_steps, const double *dd1, const double *
  * mesh, unnormalised.  mesh_out is a host array of N^3 elements of the context's precision, or of
  * int64 fixed-point values (fixed_frac_bits fractional bits) in DJ_DEPOSIT_FIXED64 mode. */
 dj_status dj_deposit(dj_ctx *ctx, const void *psi, const double *shift, void *mesh_out);
+/* dj_deposit of the context's Psi (no shift) into device memory d_mesh_out (N^3 values of the context's
+ * precision, or int64 in DJ_DEPOSIT_FIXED64 mode), for callers that keep the mesh on the device (a
+ * ROCArray behind the rrule of the density, a PyTorch tensor); returns after the copy */
+dj_status dj_deposit_device(dj_ctx *ctx, void *d_mesh_out);
 
 /* Initial conditions into the context's state, computed on the device in FP64 and rounded to the
  * context's precision: the IC pipeline of the reference (ngenic_ic, ngenic_gsl.jl:187-191; ic_operator /
@@ -124,6 +128,11 @@ dj_status dj_deposit(dj_ctx *ctx, const void *psi, const double *shift, void *me
  * (de-aliased on the 3n/2 grid).  Psi = sum_m D^m psi_m, Pi = sum_m m D^(m-1) psi_m (EdS growth). */
 dj_status dj_ics_lpt(dj_ctx *ctx, uint32_t seed, const double *white, const double *k_tab, const double *pk_tab,
                      int32_t ntab, double D, int32_t order);
+/* dj_ics_lpt with the white noise in device memory (d_white: n^3 FP64 values, or NULL for the N-GenIC field
+ * of seed).  The IC pipeline keeps its device buffers in the context between calls (about 20 n^3 doubles
+ * for order 2: 5.3 GB at n = 256, 42 GB at n = 512), until dj_destroy. */
+dj_status dj_ics_lpt_device(dj_ctx *ctx, uint32_t seed, const double *d_white, const double *k_tab,
+                            const double *pk_tab, int32_t ntab, double D, int32_t order);
 
 /* Power spectrum of the CIC (W = 2; TSC 3, PCS 4) density of X = q + psi on the context's mesh
  * (psi NULL: the context's Psi): delta = mesh N^3/n^3 - 1, P(k) = V <|delta_k|^2 / |W(k)|^2> / N^6
@@ -188,7 +197,7 @@ dj_status dj_ranlxd1_uniforms(uint32_t seed, int32_t n, double *out);
  * conventions follow the CIC rrule of field/sheet_deposit.jl:118-132 and the FFT rrules of
  * lpt/nlpt_core.jl:682-713).  Built for finite-difference and spectral (ik) gradients with n_resample = 1
  * and without the learned correction; other options return DJ_ERR_UNSUPPORTED.  All arrays are host arrays
- * in the context's layout and precision. */
+ * in the context's layout and precision, except where an entry point's name ends in _device. */
 
 /* psi_bar_out = (d acc / d psi)^T acc_bar of dj_acceleration at psi (NULL: the context's Psi) */
 dj_status dj_vjp_acceleration(dj_ctx *ctx, const void *psi, const void *acc_bar, void *psi_bar_out);
@@ -197,6 +206,15 @@ dj_status dj_vjp_acceleration(dj_ctx *ctx, const void *psi, const void *acc_bar,
  * which dj_vjp_run and dj_vjp_ics_lpt take when their cotangent pointers are NULL (DJ_ERR_INVALID_ARGUMENT
  * when none are resident: dj_vjp_acceleration, and calls given host cotangents, overwrite them). */
 dj_status dj_vjp_deposit(dj_ctx *ctx, const void *psi, const void *mesh_bar, void *psi_bar_out);
+/* dj_vjp_deposit at the context's Psi with mesh_bar in device memory (N^3 values of the context's precision);
+ * the result stays resident */
+dj_status dj_vjp_deposit_device(dj_ctx *ctx, const void *d_mesh_bar);
+/* The differentiation workload's loss on the device: L = sum (delta - target)^2 over the mesh, delta =
+ * mesh / rho - 1 the unshifted deposit of the context's Psi, rho = n^3 / N^3 (float deposit mode only).
+ * d_target: N^3 FP64 values in device memory, or NULL for zero.  loss_out (host) gets L, summed in FP64 in
+ * a fixed order; the cotangent of Psi, (d mesh / d psi)^T (2 / rho)(delta - target), is left resident for
+ * dj_vjp_run / dj_vjp_ics_lpt(_device).  Replaces dj_deposit, a loss on the host and dj_vjp_deposit. */
+dj_status dj_density_loss_device(dj_ctx *ctx, const double *d_target, double *loss_out);
 /* dj_run's n_steps DKD steps from the current state, keeping (Psi, Pi) before each step on the device
  * (2 x 3 n^3 values per step) for dj_vjp_run; the final state is the context's state as after dj_run */
 dj_status dj_run_checkpointed(dj_ctx *ctx, int32_t n_steps, const double *dd1, const double *alpha,
@@ -210,6 +228,11 @@ dj_status dj_vjp_run(dj_ctx *ctx, void *psi_bar, void *pi_bar);
 dj_status dj_vjp_ics_lpt(dj_ctx *ctx, uint32_t seed, const double *white, const double *k_tab, const double *pk_tab,
                          int32_t ntab, double D, int32_t order, const void *psi_bar, const void *pi_bar,
                          double *white_bar_out);
+/* dj_vjp_ics_lpt from the resident cotangents with the white noise (or NULL: seed) and its cotangent
+ * d_white_bar_out (n^3 FP64) in device memory */
+dj_status dj_vjp_ics_lpt_device(dj_ctx *ctx, uint32_t seed, const double *d_white, const double *k_tab,
+                                const double *pk_tab, int32_t ntab, double D, int32_t order,
+                                double *d_white_bar_out);
 
 /* Learned force correction (no counterpart in the reference; for callers that run the corrected PM of
  * the learned-correction workload).  A residual 3-D convolution net on the mesh, BF16 weights and

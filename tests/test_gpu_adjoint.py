@@ -217,6 +217,42 @@ def test_density_loss_gradient_fp32_tiled_64():
     assert G.rel(wb32, wb64) < 5e-3, G.rel(wb32, wb64)
 
 
+@pytest.mark.parametrize("prec", [32, 64])
+def test_density_loss_gradient_on_device_matches_host_arrays(prec):
+    """The gradient with device arrays (white noise, target and white-noise cotangent in device memory, the
+    loss formed on the device by dj_density_loss_device) equals the host-array gradient: the same deposit,
+    the same cotangent bits into the reverse pass.  The solver is reused across calls, first with 1LPT ICs,
+    so the IC pipeline's kept buffers are regrown for 2LPT."""
+    n, res, L = 32, 64, 100.0
+    c = D.Cosmology.named("Planck18EEBAOSN")
+    white = O.ngenic_wnoise_real(5, n)
+    target = np.random.default_rng(6).normal(scale=0.5, size=(res, res, res))
+    kw = dict(boxsize=L, res_pm=res, n_steps=6, stepper="fastpm", order=2, worder=3, grad_kernel_order=4)
+    loss_h, wb_h, _ = D.density_loss_gradient(c, white, target, precision=prec, **kw)
+    cfg = D.PMConfig(res_pm=res, n_part=n, boxsize=L, worder=3, grad_order=4)
+    k_tab, pk_tab = D.linear_power_spectrum(c)
+    w_d, t_d = D.DeviceArray.from_numpy(white), D.DeviceArray.from_numpy(target)
+    with D.PMSolverHIP(cfg, precision=prec) as S:
+        S.ics_lpt(k_tab, pk_tab, c.Dplus(0.05), order=1, seed=3)
+        for _ in range(2):
+            loss_d, wb_d, _ = D.density_loss_gradient(c, w_d, t_d, precision=prec, solver=S, on_device=True, **kw)
+            wd = wb_d.numpy()
+            assert _rel(loss_d, loss_h) < 1e-12, (loss_d, loss_h)
+            # FP32: the vector deposit's float atomics sum in a varying order
+            assert G.rel(wd, wb_h) < (1e-4 if prec == 32 else 1e-11), G.rel(wd, wb_h)
+        # the loss alone, against the host's from the same state (zero target: target None)
+        mesh = S.deposit().astype(np.float64) / (n**3 / res**3) - 1.0
+        assert _rel(S.density_loss_device(None), float(np.sum(mesh * mesh))) < 1e-12
+        with pytest.raises(TypeError):
+            S.density_loss_device(target)
+        with pytest.raises(ValueError):
+            S.density_loss_device(D.DeviceArray((res, res, res - 1), np.float64))
+    with D.PMSolverHIP(cfg, precision=prec, deposit="fixed64") as S:
+        S.ics_lpt(k_tab, pk_tab, c.Dplus(0.05), order=1, seed=3)
+        with pytest.raises(D.DJError):
+            S.density_loss_device(t_d)
+
+
 def test_resident_cotangents_are_checked():
     """dj_vjp_run and dj_vjp_ics_lpt with NULL cotangents need resident ones from dj_vjp_deposit (or a
     resident dj_vjp_run); dj_vjp_acceleration overwrites them; options without an adjoint are refused by
