@@ -96,7 +96,7 @@ struct dj_ctx {
     void *vb_fbar = nullptr, *vb_pbar = nullptr;
     void *vb_fik = nullptr;  // the three force meshes of the spectral gradient, recomputed in the reverse pass
     bool vb_resident = false;  // vb_psibar / vb_pibar hold cotangents left resident by dj_vjp_deposit / dj_vjp_run
-    double *loss_work = nullptr;  // block sums of the density loss (LOSS_PARTS doubles)
+    double *loss_work = nullptr;  // block sums of the density loss (LOSS_PARTS doubles); the vector deposit's scales
     void *ckpt = nullptr;  // (steps, 2, 3 n^3): Psi and Pi before each step
     int ck_steps = 0;
     std::vector<double> ck_dd1, ck_alpha, ck_beta, ck_dd2;
@@ -1104,6 +1104,7 @@ dj_status adj_buffers(dj_ctx *c) {
     if (!c->vb_fbar) HIPCHK(c, hipMalloc(&c->vb_fbar, 3 * mb));
     if (!c->vb_pbar) HIPCHK(c, hipMalloc(&c->vb_pbar, mb));
     if (c->cfg.grad_order == 0 && !c->vb_fik) HIPCHK(c, hipMalloc(&c->vb_fik, 3 * mb));
+    if (!c->loss_work) HIPCHK(c, hipMalloc(&c->loss_work, sizeof(double) * LOSS_PARTS));
     return DJ_OK;
 }
 
@@ -1134,7 +1135,7 @@ dj_status vjp_force(dj_ctx *c, const void *psi, const void *abar, void *psibar) 
                 c->fft.yi(c->C, c->A, c->tw, c->st);
                 c->fft.xi(c->A, F + ax * mb, c->tw, c->st);
             }
-            if (!c->tiles.deposit3(ta, cfg.worder, c->shifts[s], abar, 1.0 / ns, c->vb_fbar, c->st)) {
+            if (!c->tiles.deposit3(ta, cfg.worder, c->shifts[s], abar, 1.0 / ns, c->vb_fbar, c->loss_work, c->st)) {
                 c->part.zero(c->vb_fbar, 3 * c->M * c->tsize, c->st);
                 c->adj.scatter3(psi, c->q1, c->n, N, cfg.boxsize, c->shifts[s], abar, 1.0 / ns, cfg.worder, c->vb_fbar,
                                 c->st);
@@ -1159,7 +1160,7 @@ dj_status vjp_force(dj_ctx *c, const void *psi, const void *abar, void *psibar) 
         c->fft.yi(c->B, c->A, c->tw, c->st);
         c->fft.xi(c->A, c->mesh, c->tw, c->st);
         // Fbar = G^T abar / ns: the tiled vector deposit, or device atomics on a single-tile axis
-        if (!c->tiles.deposit3(ta, cfg.worder, c->shifts[s], abar, 1.0 / ns, c->vb_fbar, c->st)) {
+        if (!c->tiles.deposit3(ta, cfg.worder, c->shifts[s], abar, 1.0 / ns, c->vb_fbar, c->loss_work, c->st)) {
             c->part.zero(c->vb_fbar, 3 * c->M * c->tsize, c->st);
             c->adj.scatter3(psi, c->q1, c->n, N, cfg.boxsize, c->shifts[s], abar, 1.0 / ns, cfg.worder, c->vb_fbar,
                             c->st);
@@ -1248,7 +1249,6 @@ dj_status dj_density_loss_device(dj_ctx *c, const double *d_target, double *loss
     HIPCHK(c, hipSetDevice(c->dev));
     dj_status r = adj_buffers(c);
     if (r) return r;
-    if (!c->loss_work) HIPCHK(c, hipMalloc(&c->loss_work, sizeof(double) * LOSS_PARTS));
     void *dens = nullptr;
     r = deposit_mesh(c, nullptr, nullptr, &dens);
     if (r) return r;

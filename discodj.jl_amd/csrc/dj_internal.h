@@ -93,9 +93,10 @@ struct TileOps {
     void (*scan)(const TileArgs &a, const unsigned *src, int slack, hipStream_t st);
     void (*fill)(const TileArgs &a, int W, hipStream_t st);
     // reverse mode: out_c = sum_p W(X_p + sh - cell) vals[p + c np] scale (3 meshes, every cell written);
-    // false when the mesh has a single tile along an axis or the accumulators exceed the LDS
+    // false when the mesh has a single tile along an axis or the accumulators exceed the LDS.  work: 16 bytes
+    // of device memory (the FP32 fixed-point scales)
     bool (*deposit3)(const TileArgs &a, int W, const double *sh, const void *vals, double scale, void *out,
-                     hipStream_t st);
+                     void *work, hipStream_t st);
     void (*deposit)(const TileArgs &a, int W, bool fixed, const double *shift, void *mesh, double fscale,
                     hipStream_t st);
     void (*gather)(const TileArgs &a, int W, int order, const double *shift, const void *phi, const double *c,

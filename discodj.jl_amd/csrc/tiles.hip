@@ -585,6 +585,116 @@ __global__ __launch_bounds__(512) void k_tile_deposit3(const Entry<T> *__restric
     }
 }
 
+// FP32 vector deposit in fixed point: the three components' values are scaled so that the largest
+// |value| is 2^30 (s_c = 2^30 / max_c |vals_c vscale|, from k_absmax3), every term w v s_c is rounded to an
+// integer (|term| <= 2^30, so the sum of up to 2^33 terms fits in 64 bits) and summed by 64-bit LDS integer
+// atomics, which run several times faster than float ones; the store divides by s_c.  The sum is
+// independent of the order of the additions; each term carries its float product's rounding plus at most
+// 2^-31 max_c.  Block (tile, component): one 64-bit accumulator per padded cell (the layout above).
+__global__ __launch_bounds__(256) void k_absmax3(const float *__restrict__ vals, size_t np, float vscale,
+                                                 unsigned *__restrict__ mx) {
+    __shared__ float red[3][256];
+    float m[3] = {0.f, 0.f, 0.f};
+    for (size_t p = (size_t)blockIdx.x * 256 + threadIdx.x; p < np; p += (size_t)gridDim.x * 256)
+#pragma unroll
+        for (int c = 0; c < 3; c++) m[c] = fmaxf(m[c], fabsf(vals[p + c * np] * vscale));
+#pragma unroll
+    for (int c = 0; c < 3; c++) red[c][threadIdx.x] = m[c];
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h)
+#pragma unroll
+            for (int c = 0; c < 3; c++) red[c][threadIdx.x] = fmaxf(red[c][threadIdx.x], red[c][threadIdx.x + h]);
+        __syncthreads();
+    }
+    // non-negative floats order as their bit patterns
+    if (threadIdx.x < 3) atomicMax(&mx[threadIdx.x], __float_as_uint(red[threadIdx.x][0]));
+}
+
+__global__ void k_zero3(unsigned *mx) {
+    if (threadIdx.x < 3) mx[threadIdx.x] = 0u;
+}
+
+__device__ __forceinline__ float fx_scale(unsigned mbits) {
+    const float m = __uint_as_float(mbits);
+    return m > 0.f ? 1073741824.0f / m : 0.f;
+}
+
+template <int W>
+__global__ __launch_bounds__(512) void k_tile_deposit3q(const Entry<float> *__restrict__ ent,
+                                                        const unsigned *__restrict__ start,
+                                                        const unsigned *__restrict__ gstart,
+                                                        const unsigned *__restrict__ cnt, TileGeom g, float sx, float sy,
+                                                        float sz, float L, const float *__restrict__ vals, size_t np,
+                                                        float vscale, const unsigned *__restrict__ mx,
+                                                        float *__restrict__ out) {
+    extern __shared__ unsigned char smem[];
+    unsigned long long *acc = (unsigned long long *)smem;
+    constexpr int P = W - 1;
+    const int t = blockIdx.x, comp = blockIdx.y;
+    const int SXD = g.TX + 2 * P, ROWS = g.TY * g.TZ, CELLS = SXD * ROWS;
+    const int SZ = SXD * g.TY, END = SXD * ROWS;
+    for (int c = threadIdx.x; c < CELLS; c += blockDim.x) acc[c] = 0ull;
+    const int tx = t % g.ntx, ty = (t / g.ntx) % g.nty, tz = t / (g.ntx * g.nty);
+    const int o3[3] = {tx << g.lT[0], ty << g.lT[1], tz << g.lT[2]};
+    const float s = fx_scale(mx[comp]);
+    const float *vc = vals + (size_t)comp * np;
+    __syncthreads();
+    const unsigned n_own = cnt[t], n_all = n_own + cnt[g.nt + t], s_own = start[t], s_gh = gstart[t] - n_own;
+    for (unsigned i = threadIdx.x; i < n_all; i += blockDim.x) {
+        const Entry<float> en = ent[i < n_own ? s_own + i : s_gh + i];
+        const float vs = vc[en.idx] * vscale * s;
+        const float xs[3] = {en.x + sx, en.y + sy, en.z + sz};
+        float w[3][W];
+        int l0[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            const float gg = tjmod(xs[a], L) / L * (float)g.N;
+            const int b = twbase<W>(gg);
+#pragma unroll
+            for (int o = 0; o < W; o++) w[a][o] = twk_at<W>(o, fabsf(gg - (float)(b + twlo<W>() + o)));
+            int l = b + twlo<W>() - o3[a];
+            l0[a] = l >= (g.N >> 1) ? l - g.N : (l < -(g.N >> 1) ? l + g.N : l);
+        }
+#pragma unroll
+        for (int o = 0; o < W; o++) w[0][o] *= vs;
+        const bool okx = l0[0] >= -P && l0[0] < g.TX;
+        const int xo = okx ? l0[0] + P : 0;
+        constexpr int BIG = 1 << 28;
+        int ly[W], lz[W];
+#pragma unroll
+        for (int o = 0; o < W; o++) {
+            const int y = l0[1] + o, z = l0[2] + o;
+            ly[o] = (unsigned)y < (unsigned)g.TY ? __mul24(y, SXD) : BIG;
+            lz[o] = (okx && (unsigned)z < (unsigned)g.TZ) ? __mul24(z, SZ) : BIG;
+        }
+#pragma unroll
+        for (int iz = 0; iz < W; iz++)
+#pragma unroll
+            for (int iy = 0; iy < W; iy++) {
+                const int ro = ly[iy] + lz[iz];
+                if (ro >= END) continue;  // a row outside the tile
+                unsigned long long *r = acc + (xo + ro);
+                const float wyz = w[1][iy] * w[2][iz];
+#pragma unroll
+                for (int ix = 0; ix < W; ix++) {
+                    const long long q = (long long)(int)rintf(w[0][ix] * wyz);
+                    atomicAdd(r + ix, (unsigned long long)q);
+                }
+            }
+    }
+    __syncthreads();
+    const unsigned N = (unsigned)g.N, lx = threadIdx.x & (g.TX - 1);
+    const size_t M = (size_t)N * N * N;
+    const double inv = s > 0.f ? 1.0 / (double)s : 0.0;
+    float *oc = out + (size_t)comp * M;
+    for (int r = threadIdx.x >> g.lT[0]; r < ROWS; r += blockDim.x >> g.lT[0]) {
+        const unsigned ly = r & (g.TY - 1), lz = r >> g.lT[1];
+        const unsigned gi = (o3[0] + lx) + N * ((o3[1] + ly) + N * (o3[2] + lz));
+        oc[gi] = (float)((double)(long long)acc[(int)lx + P + SXD * r] * inv);
+    }
+}
+
 // ------------------------------------------------------------------------------------ gather
 // -(c1 d1 + c2 d2 + c3 d3) of a line of potential values v[0 .. W+2H) at stencil point o (index o + H):
 // the reference's finite-difference gradient at one cell (global-memory path)
@@ -1177,10 +1287,32 @@ struct TileLaunch {
                                                       (T)sh[1], (T)sh[2], (T)a.L, (const T *)vals,
                                                       (size_t)a.n * a.n * a.n, (T)scale, (T *)out);
     }
+    template <int W>
+    static void dep3q_w(const TileArgs &a, const double *sh, const void *vals, double scale, void *out, void *work,
+                        hipStream_t st) {
+        const TileGeom &g = a.g;
+        const size_t np = (size_t)a.n * a.n * a.n;
+        unsigned *mx = (unsigned *)work;
+        k_zero3<<<1, 64, 0, st>>>(mx);
+        const unsigned nb = (unsigned)std::min<size_t>(2048, (np + 255) / 256);
+        k_absmax3<<<nb, 256, 0, st>>>((const float *)vals, np, (float)scale, mx);
+        const size_t shm = (size_t)(g.TX + 2 * (W - 1)) * (g.TY * g.TZ) * 8;
+        (void)hipFuncSetAttribute((const void *)k_tile_deposit3q<W>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)shm);
+        k_tile_deposit3q<W><<<dim3(g.nt, 3), 512, shm, st>>>((const Entry<float> *)a.ent, a.start, a.gstart, a.cur, g,
+                                                             (float)sh[0], (float)sh[1], (float)sh[2], (float)a.L,
+                                                             (const float *)vals, np, (float)scale, mx, (float *)out);
+    }
     static bool deposit3(const TileArgs &a, int W, const double *sh, const void *vals, double scale, void *out,
-                         hipStream_t st) {
+                         void *work, hipStream_t st) {
         const TileGeom &g = a.g;
         if (g.ntx < 2 || g.nty < 2 || g.ntz < 2) return false;
+        if constexpr (sizeof(T) == 4) {
+            if (W == 2) dep3q_w<2>(a, sh, vals, scale, out, work, st);
+            else if (W == 3) dep3q_w<3>(a, sh, vals, scale, out, work, st);
+            else dep3q_w<4>(a, sh, vals, scale, out, work, st);
+            return true;
+        }
         if ((size_t)3 * (g.TX + 2 * (W - 1)) * (g.TY * g.TZ) * sizeof(T) > 160 * 1024) return false;
         if (W == 2) dep3_w<2>(a, sh, vals, scale, out, st);
         else if (W == 3) dep3_w<3>(a, sh, vals, scale, out, st);
