@@ -94,7 +94,7 @@ struct dj_ctx {
     AdjointOps adj{};
     void *vb_psi1 = nullptr, *vb_abar = nullptr, *vb_psibar = nullptr, *vb_pibar = nullptr;
     void *vb_fbar = nullptr, *vb_pbar = nullptr;
-    void *vb_fik = nullptr;  // the three force meshes of the spectral gradient, recomputed in the reverse pass
+    void *vb_fik = nullptr;  // the three force meshes (spectral or FD gradient), recomputed in the reverse pass
     bool vb_resident = false;  // vb_psibar / vb_pibar hold cotangents left resident by dj_vjp_deposit / dj_vjp_run
     double *loss_work = nullptr;  // block sums of the density loss (LOSS_PARTS doubles); the vector deposit's scales
     void *ckpt = nullptr;  // (steps, 2, 3 n^3): Psi and Pi before each step
@@ -1103,7 +1103,7 @@ dj_status adj_buffers(dj_ctx *c) {
         if (!*b) HIPCHK(c, hipMalloc(b, pb));
     if (!c->vb_fbar) HIPCHK(c, hipMalloc(&c->vb_fbar, 3 * mb));
     if (!c->vb_pbar) HIPCHK(c, hipMalloc(&c->vb_pbar, mb));
-    if (c->cfg.grad_order == 0 && !c->vb_fik) HIPCHK(c, hipMalloc(&c->vb_fik, 3 * mb));
+    if (!c->vb_fik) HIPCHK(c, hipMalloc(&c->vb_fik, 3 * mb));
     if (!c->loss_work) HIPCHK(c, hipMalloc(&c->loss_work, sizeof(double) * LOSS_PARTS));
     return DJ_OK;
 }
@@ -1171,8 +1171,10 @@ dj_status vjp_force(dj_ctx *c, const void *psi, const void *abar, void *psibar) 
         c->fft.zm(ZM_POT, true, true, c->B, c->B, c->tw, c->kmult, c->st);
         c->fft.yi(c->B, c->A, c->tw, c->st);
         c->fft.xi(c->A, c->vb_pbar, c->tw, c->st);
-        c->adj.gather_grad(psi, c->q1, c->n, N, cfg.boxsize, c->shifts[s], c->vb_pbar, c->mesh, abar, 1.0 / ns,
-                           c->fdc, cfg.worder, psibar, c->st);
+        // the forward's force meshes from phi, then the weight-gradient gather over them
+        c->adj.fd_force3(c->mesh, N, c->fdc, c->vb_fik, c->st);
+        c->adj.gather_grad_f(psi, c->q1, c->n, N, cfg.boxsize, c->shifts[s], c->vb_pbar, c->vb_fik, abar, 1.0 / ns,
+                             cfg.worder, psibar, c->st);
         r = check_launch(c, "force adjoint");
         if (r) return r;
     }

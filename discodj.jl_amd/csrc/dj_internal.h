@@ -148,6 +148,8 @@ struct AdjointOps {
                  hipStream_t st);
     // first drift backward: Pibar += dd1 Psibar
     void (*drift)(const void *psibar, void *pibar, size_t n, double dd1, hipStream_t st);
+    // F_a = -D_a phi (FD coefficients c[3]) into F (3 N^3), summed as gather_grad sums them
+    void (*fd_force3)(const void *phi, int N, const double *c, void *F, hipStream_t st);
     // L = sum (mesh / rho - 1 - target)^2 into part[LOSS_PARTS - 1] (part: LOSS_PARTS doubles), mesh_bar = dL/dmesh;
     // target nullable (zero)
     void (*mse)(const void *mesh, const double *target, size_t n, double rho, void *mesh_bar, double *part,
