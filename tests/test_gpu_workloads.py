@@ -194,6 +194,49 @@ def test_density_loss_gradient_256_central_differences():
     assert e32 < 1e-2
 
 
+def test_density_loss_gradient_512_fits_in_hbm():
+    """The gradient at 512^3 particles on a 512^3 mesh (L = 1000, 2LPT + 10 BullFrog CIC steps, grad order 4)
+    with step checkpoints and recompute: it fits in HBM in FP32 and in FP64 (checkpoints, IC buffers and
+    reverse-pass buffers about 150 GB), with the white noise, the loss and the cotangents in device memory.
+    The FP64 gradient against the central difference of the fixed-point forward along its own direction,
+    and the FP32 gradient against the FP64 one."""
+    n, L = 512, 1000.0
+    c = _dc()
+    _, k, pk = _oc()
+    white = O.ngenic_wnoise_real(SEED, n)
+    kw = dict(boxsize=L, res_pm=n, n_steps=10, stepper="bullfrog", order=2, worder=2, grad_kernel_order=4,
+              k_tab=k, pk_tab=pk, on_device=True)
+    w_d = D.DeviceArray.from_numpy(white)
+    loss, wb_d, _ = D.density_loss_gradient(c, w_d, None, precision=64, **kw)
+    wb = wb_d.numpy()
+    wb_d.free()
+    gnorm = float(np.sqrt(np.sum(wb * wb)))
+    assert np.isfinite(loss) and np.isfinite(gnorm) and gnorm > 0
+    _, wb32_d, _ = D.density_loss_gradient(c, w_d, None, precision=32, **kw)
+    e32 = G.rel(wb32_d.numpy(), wb)
+    wb32_d.free()
+    w_d.free()
+    cfg = D.PMConfig(res_pm=n, n_part=n, boxsize=L, worder=2, grad_order=4)
+    co = D.stepper_coefficients(c, "bullfrog", A_INI, A_END, 10, "D")
+    with D.PMSolverHIP(cfg, precision=64, deposit="fixed64") as S:
+        def lossf(w):
+            S.ics_lpt(k, pk, c.Dplus(A_INI), order=2, white=w)
+            S.run(co["ddrift1"], co["alpha"], co["beta"], co["ddrift2"])
+            delta = S.deposit().astype(np.float64) * 2.0**-32 - 1.0
+            return float(np.sum(delta * delta))
+
+        l0 = lossf(white)
+        v = wb * (np.sqrt(float(n) ** 3) / gnorm)
+        eps = 5e-7
+        fd = (lossf(white + eps * v) - lossf(white - eps * v)) / (2 * eps)
+    g = float(np.sum(wb * v))
+    print(f"loss {loss:.10e} (fixed-point forward {l0:.10e}), gradient along itself {g:.8e}, central difference "
+          f"{fd:.8e} (relative {abs(g - fd) / abs(fd):.2e}); FP32 gradient against FP64 {e32:.2e}")
+    assert abs(l0 - loss) < 1e-8 * loss
+    assert abs(g - fd) < 3e-4 * abs(fd)
+    assert e32 < 1e-2
+
+
 # ------------------------------------------------------- fitted transfer function, 256^3 vs 1024^3 mesh
 def test_transfer_function_256_against_1024_mesh():
     """The learned-correction workload at its own size: 256^3 particles on a 256^3 mesh (window
