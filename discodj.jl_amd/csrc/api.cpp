@@ -74,7 +74,7 @@ struct dj_ctx {
     TileLists tl{};   // the particles' lists
     TileLists tlr{};  // the lists of the sheet-resampling point sets (n_resample > 1, allocated on first use)
     void *rl_pts = nullptr, *rl_q0 = nullptr;  // the point sets' positions (3, (2n)^3), a zero lattice (2n)
-    double *d_shifts = nullptr;  // (16 + 1, 3): the interlacing shifts, then one free slot
+    double *d_shifts = nullptr;  // (16 + 2, 3): the interlacing shifts, a free slot (+48), a zero shift (+51)
     // sheet resampling (n_resample > 1): particle-grid FFT ops and buffers
     ResampleOps rs{};
     FftOps fftn{};
@@ -97,6 +97,7 @@ struct dj_ctx {
     void *vb_fik = nullptr;  // the three force meshes (spectral or FD gradient), recomputed in the reverse pass
     bool vb_resident = false;  // vb_psibar / vb_pibar hold cotangents left resident by dj_vjp_deposit / dj_vjp_run
     double *loss_work = nullptr;  // block sums of the density loss (LOSS_PARTS doubles); the vector deposit's scales
+    void *vb_pl = nullptr;  // (3, (n_resample n)^3): the cotangent of the resampled points' positions
     void *ckpt = nullptr;  // (steps, 2, 3 n^3): Psi and Pi before each step
     int ck_steps = 0;
     std::vector<double> ck_dd1, ck_alpha, ck_beta, ck_dd2;
@@ -344,7 +345,7 @@ void free_all(dj_ctx *c) {
                     c->nn_t[0], c->nn_t[1],
                     c->nn_t[2], c->nn_t[3], c->nn_t[4], c->nn_t[5], c->nn_t[6], c->nn_t[7], c->nn_t[8], c->nn_t[9],
                     c->nn_dphi, c->nn_dens, c->nn_tgt, c->nn_sum, c->acc_list, c->kgain, c->pk_work,
-                    c->pg_tw, c->pg_A, c->pg_B, c->loss_work};
+                    c->pg_tw, c->pg_A, c->pg_B, c->loss_work, c->vb_pl};
     // teardown: failures here have no caller to report to
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -562,11 +563,11 @@ dj_status force(dj_ctx *c, const void *psi, int mode, const KickArgs &kick) {
                                    ? resample_deposit<double>(c, psi, c->shifts[s], dens, fixed, fscale_dep(cfg), c->rl_pts)
                                    : resample_deposit<float>(c, psi, c->shifts[s], dens, fixed, fscale_dep(cfg), c->rl_pts);
                 if (rr) return rr;
-                rr = tile_sort_of(c, c->tlr, c->rl_pts, c->rl_q0, nrn, c->d_shifts + 48, 1, cfg.worder);
+                rr = tile_sort_of(c, c->tlr, c->rl_pts, c->rl_q0, nrn, c->d_shifts + 51, 1, cfg.worder);
                 if (rr) return rr;
             }
             const double zero3[3] = {0, 0, 0};
-            TileArgs tr = tile_args_of(c, c->tlr, c->rl_pts, c->rl_q0, nrn, c->d_shifts + 48, 1);
+            TileArgs tr = tile_args_of(c, c->tlr, c->rl_pts, c->rl_q0, nrn, c->d_shifts + 51, 1);
             Prof p(c, K_SCATTER);
             c->tiles.deposit(tr, cfg.worder, fixed, zero3, dens, fscale_dep(cfg), c->st);
         } else {
@@ -751,6 +752,47 @@ dj_status resample_deposit(dj_ctx *c, const void *psi, const double *sh, void *d
     return check_launch(c, "Fourier sheet resampling");
 }
 
+// Reverse of the resampled points' deposit of one interlacing pass: psibar += (d density / d psi)^T rhobar
+// through the points of every offset (the particles' own term is the force adjoint's).  The point lattice
+// rl_pts holds the pass's points (the density of the same pass placed them); the gather of grad W rhobar
+// at them gives their cotangents, and the transposed interpolation (trilinear, or the conjugate Fourier
+// phase) takes those back to Psi.
+template <typename T>
+dj_status resample_vjp(dj_ctx *c, const void *rhobar, void *psibar) {
+    const int nr = c->cfg.n_resample, n = c->n, H = n / 2 + 1, N = c->res;
+    const size_t n3 = (size_t)c->np, NR = (size_t)nr * n, NR3 = NR * NR * NR;
+    const double L = c->cfg.boxsize;
+    if (!c->vb_pl) HIPCHK(c, hipMalloc(&c->vb_pl, 3 * NR3 * sizeof(T)));
+    c->part.zero(c->vb_pl, 3 * NR3 * sizeof(T), c->st);
+    const double zero3[3] = {0, 0, 0};
+    c->adj.gather_grad(c->rl_pts, c->rl_q0, (int)NR, N, L, zero3, rhobar, nullptr, nullptr, 0.0, c->fdc, c->cfg.worder,
+                       c->vb_pl, c->st);
+    T *pb = (T *)psibar;
+    if (c->cfg.resampling == 1) {
+        for (int o = 0; o < c->n_offsets; o++)
+            for (int comp = 0; comp < 3; comp++)
+                c->rs.trilinear_adj(c->vb_pl, n, nr, c->ioffsets[o], comp, c->offsets[o], pb + comp * n3, c->st);
+        return check_launch(c, "linear sheet resampling adjoint");
+    }
+    cx<T> *hb = (cx<T> *)c->rs_hat;  // the accumulated cotangent spectrum of one component
+    const double scale = (double)((T)1 / ((T)n * (T)n * (T)n));
+    for (int comp = 0; comp < 3; comp++) {
+        for (int o = 0; o < c->n_offsets; o++) {
+            const cx<T> *px = (const cx<T> *)c->rs_phase + (size_t)o * (2 * n + H), *py = px + n, *pz = px + 2 * n;
+            c->rs.extract(c->vb_pl, n, nr, c->ioffsets[o], comp, true, c->rs_psi, c->st);
+            c->fftn.xf(c->rs_psi, false, c->rs_A, c->rs_tw, 1.0, c->st);
+            c->fftn.yf(c->rs_A, c->rs_B, c->rs_tw, c->st);
+            c->fftn.zm(ZM_NONE, true, false, c->rs_B, c->rs_B, c->rs_tw, c->kmult, c->st);
+            c->rs.phase_adj(c->rs_B, hb, n, px, py, pz, scale, o > 0, c->st);
+        }
+        c->fftn.zm(ZM_NONE, false, true, hb, hb, c->rs_tw, c->kmult, c->st);
+        c->fftn.yi(hb, c->rs_A, c->rs_tw, c->st);
+        c->fftn.xi(c->rs_A, c->rs_psi, c->rs_tw, c->st);
+        c->rs.add_transposed(c->rs_psi, n, pb + comp * n3, c->st);
+    }
+    return check_launch(c, "Fourier sheet resampling adjoint");
+}
+
 // tile shape (FP32: 32 x 16 x 16 cells, FP64: 16^3) and the stencil reach over the shifts
 dj_status setup_tiles(dj_ctx *c) {
     const int N = c->res;
@@ -793,10 +835,10 @@ dj_status setup_tiles(dj_ctx *c) {
         }
     g.RS = std::min(g.RR, rs + hi);
     if (g.nt > (1 << 20)) return fail(c, DJ_ERR_UNSUPPORTED, "more than 2^20 mesh tiles");
-    HIPCHK(c, hipMalloc(&c->d_shifts, sizeof(double) * 17 * 3));
-    double sh17[17][3] = {};
-    std::memcpy(sh17, c->shifts, sizeof(double) * 16 * 3);
-    HIPCHK(c, hipMemcpy(c->d_shifts, sh17, sizeof(double) * 17 * 3, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMalloc(&c->d_shifts, sizeof(double) * 18 * 3));
+    double sh18[18][3] = {};
+    std::memcpy(sh18, c->shifts, sizeof(double) * 16 * 3);
+    HIPCHK(c, hipMemcpy(c->d_shifts, sh18, sizeof(double) * 18 * 3, hipMemcpyHostToDevice));
     // own + ghost entries: about 2.1 per particle for 32 x 16 x 8 tiles with TSC and interlacing, twice
     // that plus 64 per list once the lists are reserved from a previous sort (grown on demand)
     return alloc_lists(c, c->tl, (size_t)(2.5 * (double)c->np) + 1024);
@@ -1092,7 +1134,8 @@ dj_status dj_ics_lpt_device(dj_ctx *c, uint32_t seed, const double *d_white, con
 namespace {
 dj_status adj_supported(dj_ctx *c) {
     if (c->nn_on) return fail(c, DJ_ERR_UNSUPPORTED, "the adjoint is built without the learned correction");
-    if (c->cfg.n_resample > 1) return fail(c, DJ_ERR_UNSUPPORTED, "the adjoint is built without sheet resampling");
+    if (c->cfg.n_resample > 1 && !c->rs_tiled)
+        return fail(c, DJ_ERR_UNSUPPORTED, "the adjoint of sheet resampling needs (n_resample n_part)^3 < 2^31 points");
     return DJ_OK;
 }
 
@@ -1121,7 +1164,22 @@ dj_status vjp_force(dj_ctx *c, const void *psi, const void *abar, void *psibar) 
     if (r) return r;
     for (int s = 0; s < ns; s++) {
         TileArgs ta = tile_args(c, psi, c->d_shifts, ns);
-        c->tiles.deposit(ta, cfg.worder, fixed, c->shifts[s], dens, fscale_dep(cfg), c->st);
+        if (cfg.n_resample > 1) {
+            // the forward's density: the particles and every offset's points through the point lattice
+            const int nrn = cfg.n_resample * c->n;
+            c->rs.shifted(psi, c->q1, c->n, c->shifts[s], cfg.n_resample, c->rl_pts, c->st);
+            r = c->tsize == 8 ? resample_deposit<double>(c, psi, c->shifts[s], dens, fixed, fscale_dep(cfg), c->rl_pts)
+                              : resample_deposit<float>(c, psi, c->shifts[s], dens, fixed, fscale_dep(cfg), c->rl_pts);
+            if (r) return r;
+            r = tile_sort_of(c, c->tlr, c->rl_pts, c->rl_q0, nrn, c->d_shifts + 51, 1, cfg.worder);
+            if (r) return r;
+            const double zero3[3] = {0, 0, 0};
+            TileArgs tr = tile_args_of(c, c->tlr, c->rl_pts, c->rl_q0, nrn, c->d_shifts + 51, 1);
+            c->tiles.deposit(tr, cfg.worder, fixed, zero3, dens, fscale_dep(cfg), c->st);
+            ta = tile_args(c, psi, c->d_shifts, ns);  // the particles' lists, for the cotangent's vector deposit
+        } else {
+            c->tiles.deposit(ta, cfg.worder, fixed, c->shifts[s], dens, fscale_dep(cfg), c->st);
+        }
         c->fft.xf(dens, fixed, c->A, c->tw, fscale, c->st);
         c->fft.yf(c->A, c->B, c->tw, c->st);
         if (cfg.grad_order == 0) {
@@ -1152,6 +1210,11 @@ dj_status vjp_force(dj_ctx *c, const void *psi, const void *abar, void *psibar) 
             }
             c->adj.gather_grad_f(psi, c->q1, c->n, N, cfg.boxsize, c->shifts[s], c->vb_pbar, c->vb_fik, abar, 1.0 / ns,
                                  cfg.worder, psibar, c->st);
+            if (cfg.n_resample > 1) {
+                r = c->tsize == 8 ? resample_vjp<double>(c, c->vb_pbar, psibar)
+                                  : resample_vjp<float>(c, c->vb_pbar, psibar);
+                if (r) return r;
+            }
             r = check_launch(c, "force adjoint (spectral gradient)");
             if (r) return r;
             continue;
@@ -1175,6 +1238,11 @@ dj_status vjp_force(dj_ctx *c, const void *psi, const void *abar, void *psibar) 
         c->adj.fd_force3(c->mesh, N, c->fdc, c->vb_fik, c->st);
         c->adj.gather_grad_f(psi, c->q1, c->n, N, cfg.boxsize, c->shifts[s], c->vb_pbar, c->vb_fik, abar, 1.0 / ns,
                              cfg.worder, psibar, c->st);
+        if (cfg.n_resample > 1) {
+            r = c->tsize == 8 ? resample_vjp<double>(c, c->vb_pbar, psibar)
+                              : resample_vjp<float>(c, c->vb_pbar, psibar);
+            if (r) return r;
+        }
         r = check_launch(c, "force adjoint");
         if (r) return r;
     }

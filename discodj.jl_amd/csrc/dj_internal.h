@@ -124,6 +124,15 @@ struct ResampleOps {
     // unit-mass deposit of positions (3, n^3) with device atomics (float, or int64 fixed point)
     void (*scatter)(const void *X, int n, int W, void *mesh, int res, double L, bool fixed, double fscale,
                     hipStream_t st);
+    // reverse mode: point set o's component c of a point-lattice cotangent into an n^3 field (natural or
+    // transposed); hb (+)= in conj(px py pz scale); psibar_c += transposed v; psibar_c += the transposed
+    // trilinear shift by d of point set o's cotangent
+    void (*extract)(const void *pl, int n, int nr, const int *o, int c, bool transposed, void *out, hipStream_t st);
+    void (*phase_adj)(const void *in, void *hb, int n, const void *px, const void *py, const void *pz, double scale,
+                      bool acc, hipStream_t st);
+    void (*add_transposed)(const void *v, int n, void *psibar_c, hipStream_t st);
+    void (*trilinear_adj)(const void *pl, int n, int nr, const int *o, int c, const double *d, void *psibar_c,
+                          hipStream_t st);
 };
 template <typename T> void resample_ops(ResampleOps &ops);
 

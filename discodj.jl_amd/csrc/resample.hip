@@ -160,6 +160,74 @@ __global__ __launch_bounds__(256) void k_rs_scatter(const T *__restrict__ X, int
         }
 }
 
+// ------------------------------------------------------------------------ reverse mode
+// The resampled points enter the density only: the cotangent of their positions (the gather of grad W
+// times the density cotangent at each point, on the point lattice) goes back to Psi through the
+// transposed interpolation.  d Xr / d npsi and d psi_c / d Psi_c are the identity (the mods are locally
+// translations).  Linear: the transpose of the trilinear shift scatters a point's cotangent to its 8
+// corners, written here as a gather over the 8 points that use a lattice site.  Fourier: npsi = c2r(P s
+// r2c(psi)) has the transpose c2r(conj(P) s r2c(npsibar)) (the half-axis weights of c2r and r2c cancel).
+
+// the cotangent of point set (o0, o1, o2), component c, from the point lattice into the natural or the
+// transposed n^3 layout
+template <typename T, bool TRANSPOSED>
+__global__ __launch_bounds__(256) void k_rs_extract(const T *__restrict__ pl, int n, int nr, int o0, int o1, int o2,
+                                                    int c, T *__restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+    if (i >= n) return;
+    const size_t N = (size_t)n, NR = (size_t)nr * n;
+    const size_t pr = ((size_t)nr * i + o0) + NR * (((size_t)nr * j + o1) + NR * ((size_t)nr * k + o2));
+    out[TRANSPOSED ? (size_t)k + N * ((size_t)j + N * i) : (size_t)i + N * ((size_t)j + N * k)] =
+        pl[pr + (size_t)c * NR * NR * NR];
+}
+
+// hb (+)= in * conj((px py) pz scale): the forward's phase factor, conjugated
+template <typename T, bool ACC>
+__global__ __launch_bounds__(256) void k_rs_phase_adj(const cx<T> *__restrict__ in, cx<T> *__restrict__ hb, int n,
+                                                      const cx<T> *__restrict__ px, const cx<T> *__restrict__ py,
+                                                      const cx<T> *__restrict__ pz, T scale) {
+    const int H = n / 2 + 1;
+    const size_t tot = (size_t)n * H * n;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (size_t)gridDim.x * blockDim.x) {
+        const int w = (int)(e % n);
+        const size_t r = e / n;
+        const int h = (int)(r % H), v = (int)(r / H);
+        cx<T> f = cmul(cmul(px[w], py[v]), pz[h]);
+        f = {f.x * scale, -(f.y * scale)};
+        const cx<T> x = cmul(in[e], f);
+        hb[e] = ACC ? hb[e] + x : x;
+    }
+}
+
+// psibar_c (natural layout) += v (transposed layout)
+template <typename T>
+__global__ __launch_bounds__(256) void k_rs_add_transposed(const T *__restrict__ v, int n, T *__restrict__ psibar_c) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+    if (i >= n) return;
+    const size_t N = (size_t)n;
+    psibar_c[(size_t)i + N * ((size_t)j + N * k)] += v[(size_t)k + N * ((size_t)j + N * i)];
+}
+
+// psibar_c(i, j, k) += sum_{a, b, e in {0, 1}} wx(a) wy(b) wz(e) pbar(i - a, j - b, k - e), the transpose of
+// k_rs_trilinear (w(0) = 1 - d, w(1) = d per axis), with pbar the point set's cotangent on the lattice
+template <typename T>
+__global__ __launch_bounds__(256) void k_rs_trilinear_adj(const T *__restrict__ pl, int n, int nr, int o0, int o1, int o2,
+                                                          int c, T dx, T dy, T dz, T *__restrict__ psibar_c) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+    if (i >= n) return;
+    const size_t N = (size_t)n, NR = (size_t)nr * n;
+    const T *pc = pl + (size_t)c * NR * NR * NR;
+    const int im = i == 0 ? n - 1 : i - 1, jm = j == 0 ? n - 1 : j - 1, km = k == 0 ? n - 1 : k - 1;
+    auto P = [&](int a, int b, int e) {
+        return pc[((size_t)nr * a + o0) + NR * (((size_t)nr * b + o1) + NR * ((size_t)nr * e + o2))];
+    };
+    const T ex = (T)1 - dx, ey = (T)1 - dy, ez = (T)1 - dz;
+    const T v = P(i, j, k) * ex * ey * ez + P(im, j, k) * dx * ey * ez + P(i, jm, k) * ex * dy * ez +
+                P(i, j, km) * ex * ey * dz + P(im, jm, k) * dx * dy * ez + P(im, j, km) * dx * ey * dz +
+                P(i, jm, km) * ex * dy * dz + P(im, jm, km) * dx * dy * dz;
+    psibar_c[(size_t)i + N * ((size_t)j + N * k)] += v;
+}
+
 template <typename T>
 struct RsLaunch {
     static dim3 pgrid(int n) { return dim3((n + 255) / 256, n, n); }
@@ -187,6 +255,30 @@ struct RsLaunch {
             k_rs_place<T, false><<<pgrid(n), 256, 0, st>>>((const T *)npsi, (const T *)q1, n, c, (T)shift, (T)L, (T *)xr,
                                                            nr, o[0], o[1], o[2]);
     }
+    static void extract(const void *pl, int n, int nr, const int *o, int c, bool transposed, void *out,
+                        hipStream_t st) {
+        if (transposed)
+            k_rs_extract<T, true><<<pgrid(n), 256, 0, st>>>((const T *)pl, n, nr, o[0], o[1], o[2], c, (T *)out);
+        else
+            k_rs_extract<T, false><<<pgrid(n), 256, 0, st>>>((const T *)pl, n, nr, o[0], o[1], o[2], c, (T *)out);
+    }
+    static void phase_adj(const void *in, void *hb, int n, const void *px, const void *py, const void *pz, double scale,
+                          bool acc, hipStream_t st) {
+        if (acc)
+            k_rs_phase_adj<T, true><<<2048, 256, 0, st>>>((const cx<T> *)in, (cx<T> *)hb, n, (const cx<T> *)px,
+                                                          (const cx<T> *)py, (const cx<T> *)pz, (T)scale);
+        else
+            k_rs_phase_adj<T, false><<<2048, 256, 0, st>>>((const cx<T> *)in, (cx<T> *)hb, n, (const cx<T> *)px,
+                                                           (const cx<T> *)py, (const cx<T> *)pz, (T)scale);
+    }
+    static void add_transposed(const void *v, int n, void *psibar_c, hipStream_t st) {
+        k_rs_add_transposed<T><<<pgrid(n), 256, 0, st>>>((const T *)v, n, (T *)psibar_c);
+    }
+    static void trilinear_adj(const void *pl, int n, int nr, const int *o, int c, const double *d, void *psibar_c,
+                              hipStream_t st) {
+        k_rs_trilinear_adj<T><<<pgrid(n), 256, 0, st>>>((const T *)pl, n, nr, o[0], o[1], o[2], c, (T)d[0], (T)d[1],
+                                                        (T)d[2], (T *)psibar_c);
+    }
     static void shifted(const void *psi, const void *q1, int n, const double *sh, int nr, void *out, hipStream_t st) {
         k_rs_shifted<T><<<pgrid(n), 256, 0, st>>>((const T *)psi, (const T *)q1, n, (T)sh[0], (T)sh[1], (T)sh[2], nr,
                                                   (T *)out);
@@ -205,6 +297,10 @@ struct RsLaunch {
 };
 
 template <typename T> void resample_ops(ResampleOps &ops) {
+    ops.extract = RsLaunch<T>::extract;
+    ops.phase_adj = RsLaunch<T>::phase_adj;
+    ops.add_transposed = RsLaunch<T>::add_transposed;
+    ops.trilinear_adj = RsLaunch<T>::trilinear_adj;
     ops.wrap = RsLaunch<T>::wrap;
     ops.phase = RsLaunch<T>::phase;
     ops.trilinear = RsLaunch<T>::trilinear;

@@ -195,8 +195,10 @@ dj_status dj_ranlxd1_uniforms(uint32_t seed, int32_t n, double *out);
 
 /* Reverse mode (no counterpart in the reference, whose PM path has no adjoint: NR/README.md:169; its
  * conventions follow the CIC rrule of field/sheet_deposit.jl:118-132 and the FFT rrules of
- * lpt/nlpt_core.jl:682-713).  Built for finite-difference and spectral (ik) gradients with n_resample = 1
- * and without the learned correction; other options return DJ_ERR_UNSUPPORTED.  All arrays are host arrays
+ * lpt/nlpt_core.jl:682-713).  Built for finite-difference and spectral (ik) gradients, with sheet resampling
+ * (Fourier or linear; the resampled points' cotangents go back to Psi through the transposed interpolation)
+ * when (n_resample n_part)^3 < 2^31, and without the learned conv net; other options return
+ * DJ_ERR_UNSUPPORTED.  All arrays are host arrays
  * in the context's layout and precision, except where an entry point's name ends in _device. */
 
 /* psi_bar_out = (d acc / d psi)^T acc_bar of dj_acceleration at psi (NULL: the context's Psi) */

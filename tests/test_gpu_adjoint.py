@@ -276,8 +276,11 @@ def test_resident_cotangents_are_checked():
         S.vjp_acceleration(None, np.ones((3, n, n, n)))
         with pytest.raises(D.DJError):
             S.vjp_ics_lpt(k, pk, oc.Dplus(0.05), white=white)
-    with D.PMSolverHIP(D.PMConfig(res_pm=32, n_part=n, boxsize=L, n_resample=2), precision=64) as S:
-        S.set_state(np.zeros((3, n, n, n)))
+    # the conv net of the learned correction has no reverse pass through the steps
+    with D.PMSolverHIP(D.PMConfig(res_pm=64, n_part=n, boxsize=L), precision=32) as S:
+        S.set_state(np.zeros((3, n, n, n), np.float32))
+        S.nn_set_params(D.learned.init_params(1))
+        S.nn_enable(True, 1.0)
         with pytest.raises(D.DJError):
             S.run_checkpointed(*coef)
 
@@ -341,3 +344,71 @@ def test_vjp_run_ik_fd(prec):
         fd = (f(eps) - f(-eps)) / (2 * eps)
     g = _dot(g0, v) + _dot(g1, u)
     assert _rel(g, fd) < 1e-6, (g, fd)
+
+
+@pytest.mark.parametrize("resampling", ["fourier", "linear"])
+@pytest.mark.parametrize("W", [2, 3, 4])
+@pytest.mark.parametrize("grad", [4, 0])
+def test_vjp_acceleration_resampling_fd(resampling, W, grad):
+    """The force VJP with sheet resampling (n_resample 2, interlacing): the resampled points enter the
+    density, their cotangents go back to Psi through the transposed interpolation (trilinear, or the
+    conjugate Fourier phase); against central differences of the device forward and of the oracle's."""
+    n, res, L = 16, 32, 100.0
+    rng = np.random.default_rng(300 + 10 * W + grad + (resampling == "linear"))
+    psi, abar, v = _psi(n, L, 60 + W + grad), rng.normal(size=(3, n, n, n)), rng.normal(size=(3, n, n, n))
+    kw = dict(worder=W, grad_order=grad, antialias=1, n_resample=2, resampling=resampling)
+    cfg = D.PMConfig(res_pm=res, n_part=n, boxsize=L, **kw)
+    ocfg = O.pm_config(res, n, L, **kw)
+    # CIC weights are piecewise linear: a point (8 per particle here) crossing a cell face inside the
+    # difference step spoils that step's difference (by 1e-3 to 1e-1 at 1e-5); the best of two steps is
+    # taken, which a single crossing cannot spoil both of
+    steps = (1e-6, 3e-7)
+    with D.PMSolverHIP(cfg, precision=64) as S:
+        g = _dot(S.vjp_acceleration(psi, abar), v)
+        fd = [(_dot(abar, S.pm_acceleration(psi + e * v)) - _dot(abar, S.pm_acceleration(psi - e * v))) / (2 * e)
+              for e in steps]
+    fdo = [(_dot(abar, O.pm_acceleration(psi + e * v, ocfg, "solver")) -
+            _dot(abar, O.pm_acceleration(psi - e * v, ocfg, "solver"))) / (2 * e) for e in steps]
+    tol = 1e-5 if W == 2 else 1e-6
+    assert min(_rel(g, f) for f in fd) < tol, (g, fd)
+    assert min(_rel(g, f) for f in fdo) < tol, (g, fdo)
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_vjp_run_resampling_fd(prec):
+    """Three BullFrog steps with the discreteness-suppressed settings (PCS, ik gradient, deconvolution,
+    interlacing, 2^3 Fourier sheet resampling): the run's VJP against central differences (FP64); FP32
+    against the FP64 VJP."""
+    n, res, L, steps = 16, 32, 100.0, 3
+    rng = np.random.default_rng(7)
+    psi0 = _psi(n, L, 13, 1.0)
+    pi0 = rng.normal(scale=2.0, size=(3, n, n, n))
+    pb, mb = rng.normal(size=(3, n, n, n)), rng.normal(size=(3, n, n, n))
+    v, u = rng.normal(size=(3, n, n, n)), rng.normal(size=(3, n, n, n))
+    c = D.Cosmology.named("Planck18EEBAOSN")
+    co = D.stepper_coefficients(c, "bullfrog", 0.05, 0.5, steps, "D")
+    coef = [co["ddrift1"], co["alpha"], co["beta"], co["ddrift2"]]
+    cfg = D.PMConfig(res_pm=res, n_part=n, boxsize=L, worder=4, grad_order=0, antialias=1, deconvolve=True,
+                     n_resample=2, resampling="fourier")
+    eps = 1e-6
+    with D.PMSolverHIP(cfg, precision=64) as S:
+        S.set_state(psi0, pi0)
+        S.run_checkpointed(*coef)
+        g0, g1 = S.vjp_run(pb, mb)
+
+        def f(e):
+            S.set_state(psi0 + e * v, pi0 + e * u)
+            S.run(*coef)
+            p, m = S.get_state()
+            return _dot(pb, p) + _dot(mb, m)
+
+        fd = (f(eps) - f(-eps)) / (2 * eps)
+    g = _dot(g0, v) + _dot(g1, u)
+    if prec == 64:
+        assert _rel(g, fd) < 1e-6, (g, fd)
+        return
+    with D.PMSolverHIP(cfg, precision=32) as S:
+        S.set_state(psi0.astype(np.float32), pi0.astype(np.float32))
+        S.run_checkpointed(*coef)
+        h0, h1 = S.vjp_run(pb.astype(np.float32), mb.astype(np.float32))
+    assert G.rel(h0, g0) < 1e-3 and G.rel(h1, g1) < 1e-3, (G.rel(h0, g0), G.rel(h1, g1))
