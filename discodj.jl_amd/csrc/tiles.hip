@@ -88,22 +88,7 @@ template <int W, typename T> __device__ __forceinline__ T twk_at(int o, T d) {
 }
 
 // ------------------------------------------------------------------------------------ binning
-constexpr int HASH = 512;          // LDS hash slots per workgroup (keys: tile * 2 + ghost)
 constexpr unsigned EMPTY = 0xffffffffu;
-
-__device__ __forceinline__ int hash_slot(unsigned *keys, unsigned key) {
-    unsigned h = (key * 2654435761u) & (HASH - 1);
-    for (int probe = 0; probe < HASH; probe++) {
-        unsigned s = (h + probe) & (HASH - 1);
-        unsigned k = keys[s];
-        if (k == key) return (int)s;
-        if (k == EMPTY) {
-            unsigned old = atomicCAS(&keys[s], EMPTY, key);
-            if (old == EMPTY || old == key) return (int)s;
-        }
-    }
-    return -1;  // table full: the caller falls back to a global atomic
-}
 
 // Tiles of one particle.  The stencil cells of every shift along an axis lie within c0 + [rlo, rhi],
 // a span shorter than the tile, so the axis touches at most the two tiles {tile(c0 + rlo),
@@ -140,25 +125,52 @@ __device__ __forceinline__ PartTiles particle_tiles(const T X[3], const TileGeom
     return P;
 }
 
-// list keys (tile * 2, + 1 for a ghost list): key[8] is the own tile's, key[c] that of ghost
-// combination c, or EMPTY when the combination does not exist or is the own tile
+// Per workgroup, the lists its particles join are counted in a window of tiles around the tile of its
+// first particle: BWX x BWY x BWZ tiles (own and ghost list each), directly indexed, so a lane finds its
+// counter without a search.  The window covers the spread of a 16 x 4 x 4 particle block (32 x 8 x 8
+// cells at the start) plus displacements of some tens of cells; a list outside it counts with one global
+// atomic per particle.  Window cells that alias one tile on a small mesh never both occur (a tile's offset
+// from the window's centre is taken in (-n/2, n/2]).
+constexpr int BWX = 5, BWY = 7, BWZ = 13, BSLOTS = 2 * BWX * BWY * BWZ;
+
+__device__ __forceinline__ int tdiff(int t, int b, int nt) {
+    int d = t - b;
+    d = d > nt / 2 ? d - nt : d;
+    return d <= -((nt + 1) / 2) ? d + nt : d;
+}
+
+// list keys (tile * 2, + 1 for a ghost list) and window slots (-1: outside): index 8 is the own tile's,
+// c that of ghost combination c, or EMPTY when the combination does not exist or is the own tile
 template <int C>
-__device__ __forceinline__ unsigned combo_key(const PartTiles &P, const TileGeom &g) {
-    if ((C & 1) && P.th[0] == P.tl[0]) return EMPTY;
-    if ((C & 2) && P.th[1] == P.tl[1]) return EMPTY;
-    if ((C & 4) && P.th[2] == P.tl[2]) return EMPTY;
+__device__ __forceinline__ void combo_key(const PartTiles &P, const TileGeom &g, const int (&tb)[3], unsigned &key,
+                                          int &slot) {
+    key = EMPTY;
+    slot = -1;
+    if ((C & 1) && P.th[0] == P.tl[0]) return;
+    if ((C & 2) && P.th[1] == P.tl[1]) return;
+    if ((C & 4) && P.th[2] == P.tl[2]) return;
     const int tx = (C & 1) ? P.th[0] : P.tl[0], ty = (C & 2) ? P.th[1] : P.tl[1], tz = (C & 4) ? P.th[2] : P.tl[2];
-    if (tx == P.to[0] && ty == P.to[1] && tz == P.to[2]) return EMPTY;
-    return (unsigned)(tx + g.ntx * (ty + g.nty * tz)) * 2u + 1u;
+    if (tx == P.to[0] && ty == P.to[1] && tz == P.to[2]) return;
+    key = (unsigned)(tx + g.ntx * (ty + g.nty * tz)) * 2u + 1u;
+    const int dx = tdiff(tx, tb[0], g.ntx) + BWX / 2, dy = tdiff(ty, tb[1], g.nty) + BWY / 2,
+              dz = tdiff(tz, tb[2], g.ntz) + BWZ / 2;
+    if ((unsigned)dx < (unsigned)BWX && (unsigned)dy < (unsigned)BWY && (unsigned)dz < (unsigned)BWZ)
+        slot = 2 * (dx + BWX * (dy + BWY * dz)) + 1;
 }
 
 template <int C = 0>
-__device__ __forceinline__ void combo_keys(const PartTiles &P, const TileGeom &g, unsigned (&key)[9]) {
+__device__ __forceinline__ void combo_keys(const PartTiles &P, const TileGeom &g, const int (&tb)[3],
+                                           unsigned (&key)[9], int (&slot)[9]) {
     if constexpr (C < 8) {
-        key[C] = combo_key<C>(P, g);
-        combo_keys<C + 1>(P, g, key);
+        combo_key<C>(P, g, tb, key[C], slot[C]);
+        combo_keys<C + 1>(P, g, tb, key, slot);
     } else {
         key[8] = (unsigned)(P.to[0] + g.ntx * (P.to[1] + g.nty * P.to[2])) * 2u;
+        const int dx = tdiff(P.to[0], tb[0], g.ntx) + BWX / 2, dy = tdiff(P.to[1], tb[1], g.nty) + BWY / 2,
+                  dz = tdiff(P.to[2], tb[2], g.ntz) + BWZ / 2;
+        slot[8] = ((unsigned)dx < (unsigned)BWX && (unsigned)dy < (unsigned)BWY && (unsigned)dz < (unsigned)BWZ)
+                      ? 2 * (dx + BWX * (dy + BWY * dz))
+                      : -1;
     }
 }
 
@@ -168,81 +180,90 @@ __global__ __launch_bounds__(256) void k_tile_bin(const T *__restrict__ psi, con
                                                   unsigned *cnt, const unsigned *__restrict__ start,
                                                   const unsigned *__restrict__ gstart, unsigned *cur, unsigned *flag,
                                                   Entry<T> *__restrict__ ent, size_t ent_cap) {
-    __shared__ unsigned keys[HASH], vals[HASH], base[HASH], lend[HASH];
-    for (int s = threadIdx.x; s < HASH; s += 256) { keys[s] = EMPTY; vals[s] = 0; }
-    __syncthreads();
-    // 16 x 4 x 4 Lagrangian block of particles per workgroup
+    __shared__ unsigned vals[BSLOTS], base[BSLOTS], lend[BSLOTS];
+    __shared__ int tbs[3];
+    for (int s = threadIdx.x; s < BSLOTS; s += 256) vals[s] = 0;
+    // 16 x 4 x 4 Lagrangian block of particles per workgroup (thread 0's particle always exists)
     const int i = blockIdx.x * 16 + (threadIdx.x & 15), j = blockIdx.y * 4 + ((threadIdx.x >> 4) & 3),
               k = blockIdx.z * 4 + (threadIdx.x >> 6);
     const bool ok = i < n && j < n && k < n;
-    // the particle's list keys (every index below is a compile-time constant once the loops are
-    // unrolled, so the arrays stay in registers without indexed selects)
-    unsigned key[9];
-    int slot[9], rank[9];
-    T X[3];
+    T X[3] = {0, 0, 0};
     size_t p = 0;
-#pragma unroll
-    for (int c = 0; c < 9; c++) key[c] = EMPTY;
+    PartTiles P{};
     if (ok) {
         const size_t np = (size_t)n * n * n;
         p = (size_t)i + (size_t)n * ((size_t)j + (size_t)n * k);
         X[0] = psi[p] + q1[i];
         X[1] = psi[p + np] + q1[j];
         X[2] = psi[p + 2 * np] + q1[k];
-        combo_keys(particle_tiles<T, W>(X, g, shifts, ns, L), g, key);
+        P = particle_tiles<T, W>(X, g, shifts, ns, L);
     }
-    // Wave-aggregated ranks: per key position c, the lanes that share a key form one group; its
-    // first lane makes the hash lookup and one LDS atomic for the whole group, and each lane's rank
-    // is the group base plus its position in the group.  A wave's particles thus stay contiguous
-    // (in Lagrangian order) within their tile's list.
+    if (threadIdx.x == 0) {
+        tbs[0] = P.to[0];
+        tbs[1] = P.to[1];
+        tbs[2] = P.to[2];
+    }
+    __syncthreads();
+    const int tb[3] = {tbs[0], tbs[1], tbs[2]};
+    // the particle's list keys and window slots (every index below is a compile-time constant once the
+    // loops are unrolled, so the arrays stay in registers without indexed selects)
+    unsigned key[9];
+    int slot[9], rank[9];
+    if (ok) {
+        combo_keys(P, g, tb, key, slot);
+    } else {
+#pragma unroll
+        for (int c = 0; c < 9; c++) {
+            key[c] = EMPTY;
+            slot[c] = -1;
+        }
+    }
+    // Ranks within the workgroup's counters.  Where every active lane of the wave joins the same list
+    // (the common case), one lane adds the group's size and each lane takes the group base plus its
+    // position, so a wave's particles stay contiguous (in Lagrangian order) in their list; otherwise each
+    // lane makes its own LDS atomic.
     const int lane = __lane_id();
     const unsigned long long below = (1ull << lane) - 1ull;
 #pragma unroll
     for (int c = 0; c < 9; c++) {
-        const bool act = key[c] != EMPTY;
-        slot[c] = -1;
         rank[c] = -1;
-        unsigned long long pend = __ballot(act);
-        while (pend) {
-            // the leader is wave-uniform: its values reach every lane by a scalar read of its lane
-            const int leader = __ffsll((long long)pend) - 1;
-            const unsigned lk = (unsigned)__builtin_amdgcn_readlane((int)key[c], leader);
-            const bool mine = act && key[c] == lk;
-            const unsigned long long grp = __ballot(mine);
-            int s = -1;
-            unsigned b = 0;
-            if (lane == leader) {
-                s = hash_slot(keys, lk);
-                if (s >= 0) b = atomicAdd(&vals[s], (unsigned)__popcll(grp));
+        const bool act = key[c] != EMPTY && slot[c] >= 0;
+        const unsigned long long am = __ballot(act);
+        if (am) {
+            const int leader = __ffsll((long long)am) - 1;
+            const int ls = __builtin_amdgcn_readlane(slot[c], leader);
+            const unsigned long long sm = __ballot(act && slot[c] == ls);
+            if (sm == am) {
+                unsigned b = 0;
+                if (lane == leader) b = atomicAdd(&vals[ls], (unsigned)__popcll(am));
+                b = (unsigned)__builtin_amdgcn_readlane((int)b, leader);
+                if (act) rank[c] = (int)(b + (unsigned)__popcll(am & below));
+            } else if (act) {
+                rank[c] = (int)atomicAdd(&vals[slot[c]], 1u);
             }
-            s = __builtin_amdgcn_readlane(s, leader);
-            b = (unsigned)__builtin_amdgcn_readlane((int)b, leader);
-            if (mine) {
-                slot[c] = s;
-                rank[c] = s >= 0 ? (int)(b + (unsigned)__popcll(grp & below)) : -1;
-            }
-            pend &= ~grp;
         }
     }
     __syncthreads();
-    // one global atomic per distinct (tile, own/ghost) of the workgroup; the fill also records the
-    // group's first entry and its list's end (own list [start[t], gstart[t]), ghost list
-    // [gstart[t], start[t + 1]))
-    for (int s = threadIdx.x; s < HASH; s += 256) {
-        unsigned kk = keys[s];
-        if (kk == EMPTY) continue;
-        unsigned t = kk >> 1, gh = kk & 1u;
+    // one global atomic per window list in use; the fill also records the group's first entry and its
+    // list's end (own list [start[t], gstart[t]), ghost list [gstart[t], start[t + 1]))
+    for (int s = threadIdx.x; s < BSLOTS; s += 256) {
+        const unsigned v = vals[s];
+        if (!v) continue;
+        const unsigned gh = (unsigned)s & 1u;
+        const int r = s >> 1;
+        const int tx = timod(tb[0] + r % BWX - BWX / 2, g.ntx), ty = timod(tb[1] + (r / BWX) % BWY - BWY / 2, g.nty),
+                  tz = timod(tb[2] + r / (BWX * BWY) - BWZ / 2, g.ntz);
+        const unsigned t = (unsigned)(tx + g.ntx * (ty + g.nty * tz));
         if (!FILL) {
-            atomicAdd(&cnt[gh * g.nt + t], vals[s]);
+            atomicAdd(&cnt[gh * g.nt + t], v);
         } else {
-            const unsigned b = atomicAdd(&cur[gh * g.nt + t], vals[s]);
-            const unsigned lo = gh ? gstart[t] : start[t];
-            base[s] = lo + b;
+            const unsigned b = atomicAdd(&cur[gh * g.nt + t], v);
+            base[s] = (gh ? gstart[t] : start[t]) + b;
             lend[s] = gh ? start[t + 1] : gstart[t];
         }
     }
     if (!FILL) {
-        // overflow pairs (table full) count directly
+        // lists outside the window count directly
 #pragma unroll
         for (int c = 0; c < 9; c++)
             if (key[c] != EMPTY && slot[c] < 0) atomicAdd(&cnt[(key[c] & 1u) * g.nt + (key[c] >> 1)], 1u);
@@ -259,7 +280,7 @@ __global__ __launch_bounds__(256) void k_tile_bin(const T *__restrict__ psi, con
         if (slot[c] >= 0) {
             at = base[slot[c]] + (unsigned)rank[c];
             end = lend[slot[c]];
-        } else {  // pair that did not fit the workgroup's hash table
+        } else {  // a list outside the workgroup's window
             const unsigned r = atomicAdd(&cur[(gh ? g.nt : 0) + t], 1u);
             at = (gh ? gstart[t] : start[t]) + r;
             end = gh ? start[t + 1] : gstart[t];
