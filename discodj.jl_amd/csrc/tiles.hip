@@ -398,8 +398,13 @@ __global__ __launch_bounds__(256) void k_scan_write(const unsigned *src, int nt,
 //              bit).  Each term carries the rounding of the float product (relative 2^-23 at most, no
 //              bias) and, below 2^-8, one of at most 2^-32; the sum adds no error until the final
 //              rounding, so the result is closer to the exact mass than an FP32 running sum.
+//              A tile whose lists hold E < 2^12 entries sums in 32-bit integers instead (ACC_Q32):
+//              the weights enter as w 2^b with b = 32 - bits(E), so that no cell's sum (at most E
+//              terms of at most 2^b) can exceed 2^32; a 32-bit LDS atomic moves half the bytes of a
+//              64-bit one.  Each term then carries an absolute rounding of at most 2^-(b+1) <= 2^-21.
 //   ACC_FIXED  the fixed-point mode: llrint(w 2^frac) in 64-bit integers, stored as integers
-enum { ACC_T = 0, ACC_Q31 = 1, ACC_FIXED = 2 };
+enum { ACC_T = 0, ACC_Q31 = 1, ACC_FIXED = 2, ACC_Q32 = 3 };
+constexpr int Q32_MIN_BITS = 20;
 
 template <typename T, int ACC, typename Acc>
 __device__ __forceinline__ void dep_add(Acc *a, T wx, T wy, T wz, T wyz, double fscale) {
@@ -409,6 +414,8 @@ __device__ __forceinline__ void dep_add(Acc *a, T wx, T wy, T wz, T wyz, double 
         atomicAdd(a, (unsigned long long)__double2ll_rn((double)wv * fscale));
     } else if constexpr (ACC == ACC_Q31) {
         atomicAdd(a, (unsigned long long)(unsigned)fmaf(wx, wyz, 0.5f));  // wx carries the 2^31
+    } else if constexpr (ACC == ACC_Q32) {
+        atomicAdd(a, (unsigned)fmaf(wx, wyz, 0.5f));  // wx carries the tile's 2^b
     } else {
         atomicAdd(a, wx * wy * wz);
     }
@@ -423,10 +430,12 @@ __device__ __forceinline__ void dep_add(Acc *a, T wx, T wy, T wz, T wyz, double 
 // receive the same particles through their ghost lists.  With a single tile along an axis the stencil
 // wraps onto the tile itself, and the masked form (!PAD) wraps each cell.
 template <typename T, int W, int ACC, bool PAD>
-__global__ __launch_bounds__(512) void k_tile_deposit(const Entry<T> *__restrict__ ent, const unsigned *__restrict__ start,
-                                                      const unsigned *__restrict__ gstart, const unsigned *__restrict__ cnt,
-                                                      TileGeom g, T sx, T sy, T sz, T L, void *mesh, double fscale) {
-    using Acc = typename std::conditional<ACC == ACC_T, T, unsigned long long>::type;
+__device__ __forceinline__ void tile_deposit(const Entry<T> *__restrict__ ent, const unsigned *__restrict__ start,
+                                             const unsigned *__restrict__ gstart, const unsigned *__restrict__ cnt,
+                                             const TileGeom &g, T sx, T sy, T sz, T L, void *mesh, double fscale,
+                                             int qbits) {
+    using Acc = typename std::conditional<ACC == ACC_T, T,
+                                          typename std::conditional<ACC == ACC_Q32, unsigned, unsigned long long>::type>::type;
     extern __shared__ unsigned char smem[];
     Acc *acc = (Acc *)smem;
     constexpr int P = W - 1;
@@ -452,10 +461,11 @@ __global__ __launch_bounds__(512) void k_tile_deposit(const Entry<T> *__restrict
             for (int o = 0; o < W; o++) w[a][o] = twk_at<W>(o, fabs(gg - (T)(b + twlo<W>() + o)));
             b0[a] = b + twlo<W>();
         }
-        if constexpr (ACC == ACC_Q31) {
-            // x weights scaled by 2^31 (exact); one w_y w_z product per row
+        if constexpr (ACC == ACC_Q31 || ACC == ACC_Q32) {
+            // x weights scaled by 2^31 or the tile's 2^b (exact); one w_y w_z product per row
+            const T q = ACC == ACC_Q31 ? (T)2147483648.0f : (T)ldexpf(1.0f, qbits);
 #pragma unroll
-            for (int ix = 0; ix < W; ix++) w[0][ix] *= (T)2147483648.0f;
+            for (int ix = 0; ix < W; ix++) w[0][ix] *= q;
         }
         if constexpr (PAD) {
             // first stencil cell relative to the tile, nearest periodic image (N >= 2 T per axis)
@@ -527,8 +537,26 @@ __global__ __launch_bounds__(512) void k_tile_deposit(const Entry<T> *__restrict
         const unsigned gi = (ox + lx) + N * ((oy + ly) + N * (oz + lz));
         const Acc v = acc[(PAD ? lx + P : lx) + SXD * r];
         if constexpr (ACC == ACC_Q31) ((T *)mesh)[gi] = (T)((double)v * 0x1p-31);
+        else if constexpr (ACC == ACC_Q32) ((T *)mesh)[gi] = (T)ldexp((double)v, -qbits);
         else ((Acc *)mesh)[gi] = v;
     }
+}
+
+// FP32 tiles (ACC_Q31) with fewer than 2^12 entries sum in 32 bits (ACC_Q32); the LDS holds 64-bit
+// accumulators either way
+template <typename T, int W, int ACC, bool PAD>
+__global__ __launch_bounds__(512) void k_tile_deposit(const Entry<T> *__restrict__ ent, const unsigned *__restrict__ start,
+                                                      const unsigned *__restrict__ gstart, const unsigned *__restrict__ cnt,
+                                                      TileGeom g, T sx, T sy, T sz, T L, void *mesh, double fscale) {
+    if constexpr (ACC == ACC_Q31) {
+        const unsigned t = blockIdx.x, e = cnt[t] + cnt[g.nt + t];
+        const int qbits = 32 - (32 - __clz(e));  // 32 - bits(e): e 2^qbits < 2^32
+        if (qbits >= Q32_MIN_BITS) {
+            tile_deposit<T, W, ACC_Q32, PAD>(ent, start, gstart, cnt, g, sx, sy, sz, L, mesh, fscale, min(qbits, 31));
+            return;
+        }
+    }
+    tile_deposit<T, W, ACC, PAD>(ent, start, gstart, cnt, g, sx, sy, sz, L, mesh, fscale, 31);
 }
 
 // Vector deposit (reverse mode): out_c(cell) = sum_p W(X_p + s - cell) vals[p + c np] scale for c = 0, 1, 2,
