@@ -140,37 +140,48 @@ __device__ __forceinline__ int tdiff(int t, int b, int nt) {
 }
 
 // list keys (tile * 2, + 1 for a ghost list) and window slots (-1: outside): index 8 is the own tile's,
-// c that of ghost combination c, or EMPTY when the combination does not exist or is the own tile
+// c that of ghost combination c, or EMPTY when the combination does not exist or is the own tile.  The
+// per-axis parts (list index, window offset, in-window and own-tile tests of tl, th and to) are formed
+// once; each combination then selects and adds them (its selections are compile-time constants).
+struct AxisParts {
+    int k[3], sl[3];       // [tl, th, to]: tile index term, window slot term
+    bool in[3], eqo[2];    // [tl, th, to] inside the window; tl, th equal to the own tile's coordinate
+    bool two;              // th != tl
+};
+
+__device__ __forceinline__ void axis_parts(int tl, int th, int to, int tb, int nt, int wd, int mulk, int muls,
+                                           AxisParts &A) {
+    const int t3[3] = {tl, th, to};
+#pragma unroll
+    for (int h = 0; h < 3; h++) {
+        const int d = tdiff(t3[h], tb, nt) + wd / 2;
+        A.in[h] = (unsigned)d < (unsigned)wd;
+        A.k[h] = t3[h] * mulk;
+        A.sl[h] = d * muls;
+    }
+    A.eqo[0] = tl == to;
+    A.eqo[1] = th == to;
+    A.two = th != tl;
+}
+
 template <int C>
-__device__ __forceinline__ void combo_key(const PartTiles &P, const TileGeom &g, const int (&tb)[3], unsigned &key,
-                                          int &slot) {
-    key = EMPTY;
-    slot = -1;
-    if ((C & 1) && P.th[0] == P.tl[0]) return;
-    if ((C & 2) && P.th[1] == P.tl[1]) return;
-    if ((C & 4) && P.th[2] == P.tl[2]) return;
-    const int tx = (C & 1) ? P.th[0] : P.tl[0], ty = (C & 2) ? P.th[1] : P.tl[1], tz = (C & 4) ? P.th[2] : P.tl[2];
-    if (tx == P.to[0] && ty == P.to[1] && tz == P.to[2]) return;
-    key = (unsigned)(tx + g.ntx * (ty + g.nty * tz)) * 2u + 1u;
-    const int dx = tdiff(tx, tb[0], g.ntx) + BWX / 2, dy = tdiff(ty, tb[1], g.nty) + BWY / 2,
-              dz = tdiff(tz, tb[2], g.ntz) + BWZ / 2;
-    if ((unsigned)dx < (unsigned)BWX && (unsigned)dy < (unsigned)BWY && (unsigned)dz < (unsigned)BWZ)
-        slot = 2 * (dx + BWX * (dy + BWY * dz)) + 1;
+__device__ __forceinline__ void combo_key(const AxisParts (&A)[3], unsigned &key, int &slot) {
+    constexpr int sx = C & 1, sy = (C >> 1) & 1, sz = (C >> 2) & 1;
+    const bool exists = (!sx || A[0].two) && (!sy || A[1].two) && (!sz || A[2].two) &&
+                        !(A[0].eqo[sx] && A[1].eqo[sy] && A[2].eqo[sz]);
+    key = exists ? (unsigned)(A[0].k[sx] + A[1].k[sy] + A[2].k[sz]) * 2u + 1u : EMPTY;
+    slot = exists && A[0].in[sx] && A[1].in[sy] && A[2].in[sz] ? 2 * (A[0].sl[sx] + A[1].sl[sy] + A[2].sl[sz]) + 1
+                                                                : -1;
 }
 
 template <int C = 0>
-__device__ __forceinline__ void combo_keys(const PartTiles &P, const TileGeom &g, const int (&tb)[3],
-                                           unsigned (&key)[9], int (&slot)[9]) {
+__device__ __forceinline__ void combo_keys(const AxisParts (&A)[3], unsigned (&key)[9], int (&slot)[9]) {
     if constexpr (C < 8) {
-        combo_key<C>(P, g, tb, key[C], slot[C]);
-        combo_keys<C + 1>(P, g, tb, key, slot);
+        combo_key<C>(A, key[C], slot[C]);
+        combo_keys<C + 1>(A, key, slot);
     } else {
-        key[8] = (unsigned)(P.to[0] + g.ntx * (P.to[1] + g.nty * P.to[2])) * 2u;
-        const int dx = tdiff(P.to[0], tb[0], g.ntx) + BWX / 2, dy = tdiff(P.to[1], tb[1], g.nty) + BWY / 2,
-                  dz = tdiff(P.to[2], tb[2], g.ntz) + BWZ / 2;
-        slot[8] = ((unsigned)dx < (unsigned)BWX && (unsigned)dy < (unsigned)BWY && (unsigned)dz < (unsigned)BWZ)
-                      ? 2 * (dx + BWX * (dy + BWY * dz))
-                      : -1;
+        key[8] = (unsigned)(A[0].k[2] + A[1].k[2] + A[2].k[2]) * 2u;
+        slot[8] = A[0].in[2] && A[1].in[2] && A[2].in[2] ? 2 * (A[0].sl[2] + A[1].sl[2] + A[2].sl[2]) : -1;
     }
 }
 
@@ -210,7 +221,11 @@ __global__ __launch_bounds__(256) void k_tile_bin(const T *__restrict__ psi, con
     unsigned key[9];
     int slot[9], rank[9];
     if (ok) {
-        combo_keys(P, g, tb, key, slot);
+        AxisParts A[3];
+        axis_parts(P.tl[0], P.th[0], P.to[0], tb[0], g.ntx, BWX, 1, 1, A[0]);
+        axis_parts(P.tl[1], P.th[1], P.to[1], tb[1], g.nty, BWY, g.ntx, BWX, A[1]);
+        axis_parts(P.tl[2], P.th[2], P.to[2], tb[2], g.ntz, BWZ, g.ntx * g.nty, BWX * BWY, A[2]);
+        combo_keys(A, key, slot);
     } else {
 #pragma unroll
         for (int c = 0; c < 9; c++) {
