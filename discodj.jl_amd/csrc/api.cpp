@@ -30,10 +30,10 @@ namespace {
 thread_local std::string g_create_error;
 
 enum KernelId { K_ZERO = 0, K_SCATTER, K_XF, K_YF, K_ZM, K_YI, K_XI, K_GATHER, K_GATHER_KICK, K_KICK, K_DRIFT, K_SORT,
-                K_SORT_EXACT, K_RESAMPLE, K_NN, K_COUNT };
+                K_SORT_EXACT, K_RESAMPLE, K_NN, K_GATHER_LAST, K_KICK_LISTED, K_COUNT };
 const char *kernel_names[K_COUNT] = {"zero", "scatter", "fft_x_r2c", "fft_y_fwd", "fft_z_mult", "fft_y_inv",
                                      "fft_x_c2r", "gather", "gather_kick", "kick_drift", "drift", "bin",
-                                     "bin_recount", "resample", "nn_correction"};
+                                     "bin_recount", "resample", "nn_correction", "gather_last", "kick_listed"};
 
 struct ProfEvent {
     int id;
@@ -112,6 +112,7 @@ struct dj_ctx {
     void *nn_t[10] = {};                      // z1 z2 z3 h1 h2 h3 and four gradient tensors (N^3 x 16 BF16)
     void *nn_dphi = nullptr, *nn_dens = nullptr, *nn_tgt = nullptr, *nn_sum = nullptr;
     void *acc_list = nullptr;  // (entries, 3): the acceleration between interlacing passes, in tile-list order
+    unsigned *own_at = nullptr;  // (Np): each particle's entry in its own tile list (written by the bin's fill)
     size_t acc_list_cap = 0;
     // binned spectra: bin sums and device copies of the outputs (allocated on first use)
     double *pk_work = nullptr;
@@ -345,7 +346,7 @@ void free_all(dj_ctx *c) {
                     c->nn_t[0], c->nn_t[1],
                     c->nn_t[2], c->nn_t[3], c->nn_t[4], c->nn_t[5], c->nn_t[6], c->nn_t[7], c->nn_t[8], c->nn_t[9],
                     c->nn_dphi, c->nn_dens, c->nn_tgt, c->nn_sum, c->acc_list, c->kgain, c->pk_work,
-                    c->pg_tw, c->pg_A, c->pg_B, c->loss_work, c->vb_pl};
+                    c->pg_tw, c->pg_A, c->pg_B, c->loss_work, c->vb_pl, c->own_at};
     // teardown: failures here have no caller to report to
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -374,6 +375,10 @@ void set_bytes(dj_ctx *c) {
     // shifts), read phi once, write psi and mom
     c->bytes[K_GATHER_KICK] = 3 * T * np * (c->nshifts > 1 ? 5 : 4) + T * M;
     c->bytes[K_KICK] = 3 * T * np * 5;
+    // the last pass's gather when interlacing: read the positions, the hand-over and phi, write the hand-over
+    c->bytes[K_GATHER_LAST] = 3 * T * np * 3 + T * M;
+    // kick and drifts in particle order: read the own-entry index, the summed force, psi and mom; write psi, mom
+    c->bytes[K_KICK_LISTED] = (4.0 + 3 * T * 5) * np;
     c->bytes[K_DRIFT] = 3 * T * np * 3;
     // binning: read psi twice (count, fill), write one entry per (particle, tile)
     c->bytes[K_SORT] = 3 * T * np + (double)c->tiles.entry_bytes * np;
@@ -410,6 +415,7 @@ TileArgs tile_args_of(dj_ctx *c, TileLists &tl, const void *psi, const void *q1,
     a.scan_part = tl.part;
     a.ent = tl.ent;
     a.ent_cap = tl.ent_cap;
+    a.own_at = &tl == &c->tl ? c->own_at : nullptr;
     return a;
 }
 TileArgs tile_args(dj_ctx *c, const void *psi, const double *d_shifts, int ns) {
@@ -531,6 +537,10 @@ dj_status force(dj_ctx *c, const void *psi, int mode, const KickArgs &kick) {
     // the point sets of sheet resampling deposited together through their own tile lists (the particles'
     // lists then serve only a finite-difference gather)
     const bool rs_tiled = cfg.n_resample > 1 && c->rs_tiled;
+    // with interlacing the last pass adds its force into the list-ordered hand-over and the kick runs in
+    // particle order (k_kick_listed), reading the sum through the own-list entry the bin records
+    const bool listed_kick = mode == FORCE_KICK && ns > 1 && !ik;
+    if (listed_kick && !c->own_at) HIPCHK(c, hipMalloc(&c->own_at, sizeof(unsigned) * (size_t)c->np));
     if (!(rs_tiled && ik)) {
         dj_status rs = tile_sort(c, psi, c->d_shifts, ns, cfg.worder);
         if (rs) return rs;
@@ -593,10 +603,19 @@ dj_status force(dj_ctx *c, const void *psi, int mode, const KickArgs &kick) {
             { Prof p(c, K_XI); c->fft.xi(c->A, c->mesh, c->tw, c->st); }
             if (c->nn_on) c->nn.axpy(c->mesh, c->nn_dphi, c->M, 1.0, c->st);
             const bool last = s == ns - 1;
-            // first pass stores acc, later passes add to it; the last one of a step kicks and drifts
-            const int gmode = (mode == FORCE_KICK && last) ? GATHER_KICK : (s == 0 ? GATHER_ASSIGN : GATHER_ACC);
-            Prof p(c, gmode == GATHER_KICK ? K_GATHER_KICK : K_GATHER);
-            c->tiles.gather(ta, cfg.worder, cfg.grad_order, c->shifts[s], c->mesh, c->fdc, gmode, kick, c->psi, c->st);
+            // first pass stores acc, later passes add to it; the last one of a step kicks and drifts (fused
+            // into the gather without interlacing, in particle order after it with)
+            const int gmode = (mode == FORCE_KICK && last && !listed_kick) ? GATHER_KICK
+                                                                           : (s == 0 ? GATHER_ASSIGN : GATHER_ACC);
+            {
+                Prof p(c, gmode == GATHER_KICK ? K_GATHER_KICK : (listed_kick && last ? K_GATHER_LAST : K_GATHER));
+                c->tiles.gather(ta, cfg.worder, cfg.grad_order, c->shifts[s], c->mesh, c->fdc, gmode, kick, c->psi,
+                                c->st);
+            }
+            if (listed_kick && last) {
+                Prof p(c, K_KICK_LISTED);
+                c->tiles.kick_listed(c->own_at, c->acc_list, c->psi, c->mom, (size_t)c->np, kick, c->st);
+            }
         } else {
             { Prof p(c, K_ZM); c->fft.zm(ZM_NONE, true, false, c->B, c->B, c->tw, c->kmult, c->st); }
             for (int ax = 0; ax < 3; ax++) {

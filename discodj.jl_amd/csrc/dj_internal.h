@@ -86,6 +86,7 @@ struct TileArgs {
     void *ent;             // entries (X, particle index)
     size_t ent_cap;        // entries the buffer holds
     int acc_by_entry;      // the gather's acc is (ent_cap, 3) in list order (between interlacing passes)
+    unsigned *own_at;      // (Np, nullable): the fill records each particle's own-list entry
 };
 struct TileOps {
     void (*count)(const TileArgs &a, int W, hipStream_t st);  // exact counts into cnt
@@ -101,6 +102,10 @@ struct TileOps {
                     hipStream_t st);
     void (*gather)(const TileArgs &a, int W, int order, const double *shift, const void *phi, const double *c,
                    int mode, const KickArgs &k, void *psi_out, hipStream_t st);
+    // the kick and drifts of a step in particle order, the force summed over the ns passes in list order
+    // (acc_list[3 e + c], e = own_at[p]): mom = alpha mom + beta acc / ns, psi += (dd2 + dd1n) steps of mom
+    void (*kick_listed)(const unsigned *own_at, const void *acc_list, void *psi, void *mom, size_t np,
+                        const KickArgs &k, hipStream_t st);
     size_t entry_bytes;
 };
 template <typename T> void tile_ops(TileOps &ops);

@@ -190,7 +190,7 @@ __global__ __launch_bounds__(256) void k_tile_bin(const T *__restrict__ psi, con
                                                   const double *__restrict__ shifts, int ns, T L, TileGeom g,
                                                   unsigned *cnt, const unsigned *__restrict__ start,
                                                   const unsigned *__restrict__ gstart, unsigned *cur, unsigned *flag,
-                                                  Entry<T> *__restrict__ ent, size_t ent_cap) {
+                                                  Entry<T> *__restrict__ ent, size_t ent_cap, unsigned *own_at) {
     __shared__ unsigned vals[BSLOTS], base[BSLOTS], lend[BSLOTS];
     __shared__ int tbs[3];
     for (int s = threadIdx.x; s < BSLOTS; s += 256) vals[s] = 0;
@@ -307,6 +307,7 @@ __global__ __launch_bounds__(256) void k_tile_bin(const T *__restrict__ psi, con
             e.z = X[2];
             e.idx = (int)p;
             ent[at] = e;
+            if (c == 8 && own_at) own_at[p] = at;
         } else {
             over = true;
         }
@@ -1292,6 +1293,28 @@ __global__ __launch_bounds__((gather_threads<T, W>()), (gather_min_waves<T, W>()
     }
 }
 
+// The kick and drifts of a step in particle order (run_nbody.jl:156-158), the last interlacing pass having
+// added its force into the list-ordered hand-over: the state is read and written in order, and each
+// particle's summed force (12 contiguous bytes) is fetched through its own-list entry.  The arithmetic is
+// the kick gather's, in its order.
+template <typename T>
+__global__ __launch_bounds__(256) void k_kick_listed(const unsigned *__restrict__ own_at, const T *__restrict__ accl,
+                                                     T *__restrict__ psi, T *__restrict__ mom, size_t np, T alpha,
+                                                     T beta, T dd2, T dd1n, int has_next, int ns) {
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const size_t e = 3 * (size_t)own_at[p];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const T ac = accl[e + c] / (T)ns;
+        const T mm = alpha * mom[p + c * np] + beta * ac;
+        T sp = psi[p + c * np] + dd2 * mm;
+        if (has_next) sp = sp + dd1n * mm;
+        mom[p + c * np] = mm;
+        psi[p + c * np] = sp;
+    }
+}
+
 // ------------------------------------------------------------------------------------ launchers
 template <typename T>
 struct TileLaunch {
@@ -1300,14 +1323,14 @@ struct TileLaunch {
         dim3 grid((a.n + 15) / 16, (a.n + 3) / 4, (a.n + 3) / 4);
         k_tile_bin<T, W, false><<<grid, 256, 0, st>>>((const T *)a.psi, (const T *)a.q1, a.n, a.shifts, a.ns, (T)a.L,
                                                       a.g, a.cnt, a.start, a.gstart, a.cur, a.flag, (Entry<T> *)a.ent,
-                                                      a.ent_cap);
+                                                      a.ent_cap, a.own_at);
     }
     template <int W>
     static void fill_w(const TileArgs &a, hipStream_t st) {
         dim3 grid((a.n + 15) / 16, (a.n + 3) / 4, (a.n + 3) / 4);
         k_tile_bin<T, W, true><<<grid, 256, 0, st>>>((const T *)a.psi, (const T *)a.q1, a.n, a.shifts, a.ns, (T)a.L,
                                                      a.g, a.cnt, a.start, a.gstart, a.cur, a.flag, (Entry<T> *)a.ent,
-                                                     a.ent_cap);
+                                                     a.ent_cap, a.own_at);
     }
     static void count(const TileArgs &a, int W, hipStream_t st) {
         if (W == 2) bin_w<2>(a, st); else if (W == 3) bin_w<3>(a, st); else bin_w<4>(a, st);
@@ -1450,7 +1473,16 @@ struct TileLaunch {
     }
 };
 
+template <typename T>
+static void kick_listed_l(const unsigned *own_at, const void *acc_list, void *psi, void *mom, size_t np,
+                          const KickArgs &k, hipStream_t st) {
+    k_kick_listed<T><<<(unsigned)((np + 255) / 256), 256, 0, st>>>(own_at, (const T *)acc_list, (T *)psi, (T *)mom, np,
+                                                                   (T)k.alpha, (T)k.beta, (T)k.dd2, (T)k.dd1_next,
+                                                                   k.has_next, k.nshifts);
+}
+
 template <typename T> void tile_ops(TileOps &ops) {
+    ops.kick_listed = kick_listed_l<T>;
     ops.count = TileLaunch<T>::count;
     ops.scan = TileLaunch<T>::scan;
     ops.fill = TileLaunch<T>::fill;
