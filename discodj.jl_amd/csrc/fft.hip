@@ -25,6 +25,15 @@ template <typename T, int N> struct FftShape {
     static constexpr bool TWL = (PY::EF::LDS_ELEMS + N) * sizeof(cx<T>) <= 80 * 1024;
 };
 
+// Shape of the persistent z pass (k_zm_pipe): 4 waves of one line-group each, 1024 points per wave
+// (4096 / N lines per workgroup), about 43 KiB of LDS with the twiddle table, three workgroups per CU
+template <typename T, int N> struct ZPipeShape {
+    static constexpr bool ok = sizeof(T) == 4 && (N == 256 || N == 512 || N == 1024);
+    static constexpr int B = ok ? 4096 / N : 1, NT = 256;
+    using P = Passes<T, N, B, NT>;
+    static constexpr int GROUPS_PER_CU = 3;
+};
+
 // The hardware hands workgroup b to XCD b % 8.  Logical block (b % 8) (nb / 8) + b / 8 puts runs of
 // consecutive logical blocks on one XCD, where they run close in time: neighbouring line groups
 // whose transposed loads and stores share 128-byte lines then meet in that XCD's L2 instead of
@@ -158,6 +167,105 @@ __global__ __launch_bounds__((FftShape<T, N>::NTY)) void k_zm(const cx<T> *in, c
     }
 }
 
+// The z pass as a persistent kernel (three workgroups per CU, block b, b + G, b + 2G, ... for G
+// workgroups): each wave holds the next block's lines in registers while it transforms and stores
+// the current block's, so a block's loads overlap the previous block's transforms and stores
+// instead of waiting for another workgroup on the CU.  The multiplier's kz-dependent table values
+// are the same for every block and are read once; the per-line ones (kx, ky) are read before the
+// next block's loads are issued, so that waiting for them does not wait for those loads.  The
+// arithmetic of the multiplier is KMult::kphi's, in its order.  FP32 on 256^3 to 1024^3 meshes (a
+// lane's elements keep one kz per step), the potential multiplier without window deconvolution or
+// transfer function.
+template <typename T, int N, int MODE, bool FWD, bool INV>
+__global__ __launch_bounds__((ZPipeShape<T, N>::NT), (ZPipeShape<T, N>::GROUPS_PER_CU)) void k_zm_pipe(
+    const cx<T> *in, cx<T> *out, const cx<T> *__restrict__ tw, KMult<T> K, int nblocks) {
+    using S = ZPipeShape<T, N>;
+    using P = typename S::P;
+    using EF = typename P::EF;
+    constexpr int B = S::B, H = N / 2 + 1, LW = EF::LW, E = LW * N, CNT = E / 64;
+    static_assert(N % 64 == 0 && E % 64 == 0 && EF::wave_ok(), "lanes keep one kz per step");
+    __shared__ cx<T> lds[EF::LDS_ELEMS];
+    __shared__ cx<T> twl[N];
+    const cx<T> *t = stage_twiddles<T, N, true>(tw, twl, threadIdx.x, S::NT);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const bool lap0 = K.lap_order == 0;
+    // element c of a lane: wave line wave LW + (64 c) / N, kz = (64 c) % N + lane
+    T zt[CNT];
+    if constexpr (MODE == ZM_POT) {
+#pragma unroll
+        for (int c = 0; c < CNT; c++) {
+            const int kz = (64 * c) % N + lane;
+            zt[c] = lap0 ? K.k2f[kz] : K.tf[kz];
+        }
+    }
+    cx<T> pf[CNT];
+    auto fetch = [&](int blk) {
+        const cx<T> *src = in + (size_t)N * B * blk + (size_t)wave * E;
+#pragma unroll
+        for (int c = 0; c < CNT; c++) pf[c] = src[lane + 64 * c];
+    };
+    int b = blockIdx.x;
+    if (b < nblocks) fetch(b);
+    __syncthreads();  // the twiddle table
+    while (b < nblocks) {
+#pragma unroll
+        for (int c = 0; c < CNT; c++) lds[EF::idx(wave * LW + (64 * c) / N, (64 * c) % N + lane)] = pf[c];
+        // per-line multiplier parts: th[kx] + tf[ky] (or k2h[kx] + k2f[ky])
+        T lt[LW];
+        int lkx[LW], lky[LW];
+        if constexpr (MODE == ZM_POT) {
+#pragma unroll
+            for (int w = 0; w < LW; w++) {
+                const int L = b * B + wave * LW + w, ky = L / H, kx = L - ky * H;
+                lkx[w] = kx;
+                lky[w] = ky;
+                lt[w] = lap0 ? K.k2h[kx] + K.k2f[ky] : K.th[kx] + K.tf[ky];
+            }
+        }
+        const int bn = b + (int)gridDim.x;
+        if (bn < nblocks) fetch(bn);
+        EF::wave_sync();
+        if constexpr (FWD) EF::template run_wave<-1, 1>(lds, t, threadIdx.x);
+        if constexpr (MODE == ZM_POT) {
+#pragma unroll
+            for (int c = 0; c < CNT; c++) {
+                const int w = (64 * c) / N, kz = (64 * c) % N + lane;
+                const int i = EF::idx(wave * LW + w, kz);
+                T kp;
+                if (lkx[w] == 0 && lky[w] == 0 && kz == 0) {
+                    kp = (T)0;
+                } else {
+                    const T il = lap0 ? (T)-1 / (lt[w] + zt[c]) : (T)1 / (lt[w] + zt[c]);
+                    kp = il * K.scale;
+                }
+                lds[i] = scale(lds[i], kp);
+            }
+            EF::wave_sync();
+        }
+        if constexpr (INV) EF::template run_wave<+1, 1>(lds, t, threadIdx.x);
+        cx<T> *dst = out + (size_t)N * B * b + (size_t)wave * E;
+#pragma unroll
+        for (int c = 0; c < CNT; c++) dst[lane + 64 * c] = lds[EF::idx(wave * LW + (64 * c) / N, (64 * c) % N + lane)];
+        EF::wave_sync();  // the reads above before the next block's writes
+        b = bn;
+    }
+}
+
+// workgroups of the persistent z pass: per_cu on every CU of the current device, a multiple of 8
+static int persistent_groups(int per_cu, int nb) {
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0, n = 0;
+        cus = (hipGetDevice(&dev) == hipSuccess &&
+               hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0)
+                  ? n
+                  : 256;
+    }
+    int g = ((per_cu * cus) / 8) * 8;
+    if (g < 8) g = 8;
+    return g < nb ? g : nb;
+}
+
 template <typename T, int N>
 struct FftN {
     using S = FftShape<T, N>;
@@ -185,6 +293,14 @@ struct FftN {
         cx<T> *o = (cx<T> *)out;
         const cx<T> *t = (const cx<T> *)tw;
         dim3 g(P::ZBLOCKS()), b(S::NTY);
+        if constexpr (ZPipeShape<T, N>::ok) {
+            if (mode == ZM_POT && fwd && inv && !K.kg && !K.mf) {
+                using ZP = ZPipeShape<T, N>;
+                const int nb = ZP::P::ZBLOCKS();
+                k_zm_pipe<T, N, ZM_POT, true, true><<<persistent_groups(ZP::GROUPS_PER_CU, nb), ZP::NT, 0, st>>>(i, o, t, K, nb);
+                return;
+            }
+        }
 #define ZM(M, F, I) k_zm<T, N, M, F, I><<<g, b, 0, st>>>(i, o, t, K)
         if (mode == ZM_POT && fwd && inv) ZM(ZM_POT, true, true);
         else if (mode == ZM_NONE && fwd && !inv) ZM(ZM_NONE, true, false);

@@ -158,6 +158,21 @@ def test_tile_pipeline_large_mesh(W, order, aa):
     assert G.rel(acc, ref) < 2e-5
 
 
+@pytest.mark.parametrize("res, lap", [(256, 0), (256, 6), (512, 4), (1024, 2)])
+def test_persistent_z_pass_meshes(res, lap):
+    """FP32 meshes of 256^3 to 1024^3 run the Poisson solve's z pass as a persistent kernel that keeps
+    the next block's lines in flight (exact Laplacian and FD Laplacians): the force against the FP64
+    device force, whose z pass runs one block per workgroup."""
+    n, L = 64, 300.0
+    psi = _random_psi(n, L, 4.0, res + lap, np.float32)
+    cfg = D.PMConfig(res_pm=res, n_part=n, boxsize=L, worder=3, grad_order=4, lap_order=lap, antialias=1)
+    acc = D.pm_acceleration(psi, cfg, precision=32)
+    ref = D.pm_acceleration(psi.astype(np.float64), cfg, precision=64)
+    # FP32 round-off of the larger transforms: about 2e-5 of the largest force (a block transformed
+    # from the wrong lines would differ at order one)
+    assert G.rel(acc, ref) < 1e-4
+
+
 @pytest.mark.parametrize("n, nr, method, W, grad", [
     (32, 2, "fourier", 4, 0), (32, 3, "fourier", 3, 4), (24, 2, "fourier", 2, 6), (32, 2, "linear", 3, 4),
     (24, 3, "linear", 4, 0)])
