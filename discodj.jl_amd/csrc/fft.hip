@@ -23,6 +23,11 @@ template <typename T, int N> struct FftShape {
     // the N-entry twiddle table moves into LDS when two workgroups of the y/z tile plus table
     // still fit a CU (160 KiB); otherwise the passes read it through the caches
     static constexpr bool TWL = (PY::EF::LDS_ELEMS + N) * sizeof(cx<T>) <= 80 * 1024;
+    // the c2r x pass at 1024^3 (FP32): 16 lines per workgroup and 512 threads, so that its transposed
+    // loads read whole 128-byte lines of a block's own (1.99 against 2.12 ms with 8 lines and 256)
+    static constexpr int BXI = (sizeof(T) == 4 && N == 1024) ? 16 : B;
+    static constexpr int NTXI = BXI == B ? NT : 512;
+    using PXI = Passes<T, N, BXI, NTXI>;
 };
 
 // Shape of the persistent z pass (k_zm_pipe): 4 waves of one line-group each, 1024 points per wave
@@ -75,12 +80,12 @@ __global__ __launch_bounds__(256) void k_xf(const IN *__restrict__ mesh, cx<T> *
 }
 
 template <typename T, int N>
-__global__ __launch_bounds__(256) void k_xi(const cx<T> *__restrict__ A, T *__restrict__ mesh,
+__global__ __launch_bounds__((FftShape<T, N>::NTXI)) void k_xi(const cx<T> *__restrict__ A, T *__restrict__ mesh,
                                             const cx<T> *__restrict__ tw) {
     using S = FftShape<T, N>;
-    using P = typename S::P;
+    using P = typename S::PXI;
     __shared__ cx<T> lds[P::EX::LDS_ELEMS];
-    const int lb = xcd_block(blockIdx.x, gridDim.x), yb = lb % (N / S::B), z = lb / (N / S::B);
+    const int lb = xcd_block(blockIdx.x, gridDim.x), yb = lb % (N / S::BXI), z = lb / (N / S::BXI);
     P::xi_load_pre(A, lds, tw, yb, z, threadIdx.x);
     __syncthreads();
     if constexpr (P::EX::wave_ok()) {
@@ -278,7 +283,8 @@ struct FftN {
             k_xf<T, N, T><<<P::XBLOCKS(), 256, 0, st>>>((const T *)mesh, (cx<T> *)A, (const cx<T> *)tw, fscale);
     }
     static void xi(const void *A, void *mesh, const void *tw, hipStream_t st) {
-        k_xi<T, N><<<P::XBLOCKS(), 256, 0, st>>>((const cx<T> *)A, (T *)mesh, (const cx<T> *)tw);
+        using PX = typename S::PXI;
+        k_xi<T, N><<<PX::XBLOCKS(), S::NTXI, 0, st>>>((const cx<T> *)A, (T *)mesh, (const cx<T> *)tw);
     }
     static void yf(const void *A, void *Bm, const void *tw, hipStream_t st) {
         k_yf<T, N><<<P::YBLOCKS(), S::NTY, 0, st>>>((const cx<T> *)A, (cx<T> *)Bm, (const cx<T> *)tw);

@@ -41,7 +41,7 @@ def test_fft3d_vs_numpy(N, prec):
 
 
 def test_fft3d_1024_fp32_vs_fp64():
-    """The 1024^3 mesh (its FP32 x passes run 16 lines per workgroup) against the FP64 device transform
+    """The 1024^3 mesh (its FP32 c2r x pass runs 16 lines per workgroup) against the FP64 device transform
     of the same field, and the FP32 round trip (numpy's FFT of 1024^3 would take minutes)."""
     N = 1024
     x = np.random.default_rng(5).normal(size=(N, N, N)).astype(np.float32)
