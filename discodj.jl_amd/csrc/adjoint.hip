@@ -48,14 +48,27 @@ template <int W, typename T> __device__ __forceinline__ void wslope(T d, T &w, T
     }
 }
 
-// per-axis stencil of particle p: cells (wrapped), weights, slopes
+// Particle of a thread: the grid is (ceil(n / blockDim.x), n, n) over Lagrangian (i, j, k), so that no
+// thread divides its index (the 64-bit divisions and remainders took most of these kernels' instructions)
+struct AdjPart {
+    int ii[3];
+    size_t p;
+};
+__device__ __forceinline__ bool adj_particle(int n, AdjPart &q) {
+    q.ii[0] = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    q.ii[1] = (int)blockIdx.y;
+    q.ii[2] = (int)blockIdx.z;
+    q.p = (size_t)q.ii[0] + (size_t)n * ((size_t)q.ii[1] + (size_t)n * q.ii[2]);
+    return q.ii[0] < n;
+}
+
+// per-axis stencil of particle q: cells (wrapped: b + lo + o lies in [-1, N + 2]), weights, slopes
 template <typename T, int W>
-__device__ __forceinline__ void adj_stencil(const T *psi, const T *q1, int n, size_t n3, size_t p, int N, T L,
+__device__ __forceinline__ void adj_stencil(const T *psi, const T *q1, size_t n3, const AdjPart &q, int N, T L,
                                             const T (&sh)[3], int (&cell)[3][W], T (&w)[3][W], T (&dw)[3][W]) {
-    const int ii[3] = {(int)(p % n), (int)((p / n) % n), (int)(p / ((size_t)n * n))};
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-        const T X = (psi[p + a * n3] + q1[ii[a]]) + sh[a];
+        const T X = (psi[q.p + a * n3] + q1[q.ii[a]]) + sh[a];
         const T g = adj_jmod(X, L) / L * (T)N;
         const int b = (W % 2 == 0) ? (int)floor(g) : (int)rint(g);
         const int lo = W == 2 ? 0 : -1;
@@ -63,8 +76,7 @@ __device__ __forceinline__ void adj_stencil(const T *psi, const T *q1, int n, si
         for (int o = 0; o < W; o++) {
             const int bo = b + lo + o;
             wslope<W>(g - (T)bo, w[a][o], dw[a][o]);
-            const int cc = bo % N;
-            cell[a][o] = cc < 0 ? cc + N : cc;
+            cell[a][o] = bo < 0 ? bo + N : (bo >= N ? bo - N : bo);
         }
     }
 }
@@ -75,12 +87,13 @@ __global__ __launch_bounds__(256) void k_adj_scatter3(const T *__restrict__ psi,
                                                       T L, T sx, T sy, T sz, const T *__restrict__ abar, T s,
                                                       T *__restrict__ fbar) {
     const size_t n3 = (size_t)n * n * n, M = (size_t)N * N * N;
-    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n3) return;
+    AdjPart q;
+    if (!adj_particle(n, q)) return;
+    const size_t p = q.p;
     const T sh[3] = {sx, sy, sz};
     int cell[3][W];
     T w[3][W], dw[3][W];
-    adj_stencil<T, W>(psi, q1, n, n3, p, N, L, sh, cell, w, dw);
+    adj_stencil<T, W>(psi, q1, n3, q, N, L, sh, cell, w, dw);
     const T v0 = abar[p] * s, v1 = abar[p + n3] * s, v2 = abar[p + 2 * n3] * s;
 #pragma unroll
     for (int oz = 0; oz < W; oz++)
@@ -155,12 +168,13 @@ __global__ __launch_bounds__(256) void k_adj_gather(const T *__restrict__ psi, c
                                                     const T *__restrict__ phi, const T *__restrict__ abar, T s, T c1,
                                                     T c2, T c3, T *__restrict__ psibar) {
     const size_t n3 = (size_t)n * n * n;
-    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n3) return;
+    AdjPart q;
+    if (!adj_particle(n, q)) return;
+    const size_t p = q.p;
     const T sh[3] = {sx, sy, sz};
     int cell[3][W];
     T w[3][W], dw[3][W];
-    adj_stencil<T, W>(psi, q1, n, n3, p, N, L, sh, cell, w, dw);
+    adj_stencil<T, W>(psi, q1, n3, q, N, L, sh, cell, w, dw);
     T ab[3] = {0, 0, 0};
     if (phi) {
 #pragma unroll
@@ -206,26 +220,41 @@ __global__ __launch_bounds__(256) void k_adj_gather_f(const T *__restrict__ psi,
                                                       const T *__restrict__ F, const T *__restrict__ abar, T s,
                                                       T *__restrict__ psibar) {
     const size_t n3 = (size_t)n * n * n, M = (size_t)N * N * N;
-    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n3) return;
+    AdjPart q;
+    if (!adj_particle(n, q)) return;
+    const size_t p = q.p;
     const T sh[3] = {sx, sy, sz};
     int cell[3][W];
     T w[3][W], dw[3][W];
-    adj_stencil<T, W>(psi, q1, n, n3, p, N, L, sh, cell, w, dw);
+    adj_stencil<T, W>(psi, q1, n3, q, N, L, sh, cell, w, dw);
     const T ab[3] = {abar[p] * s, abar[p + n3] * s, abar[p + 2 * n3] * s};
+    const T *F1 = F + M, *F2 = F + 2 * M;
+    // the W^2 rows of a z plane of the stencil: all their mesh values are read before any is used
+    // (cell offsets < N^3 <= 2^30)
     T g0 = 0, g1 = 0, g2 = 0;
 #pragma unroll
-    for (int oz = 0; oz < W; oz++)
+    for (int oz = 0; oz < W; oz++) {
+        T vr[W][W], vf0[W][W], vf1[W][W], vf2[W][W];
 #pragma unroll
         for (int oy = 0; oy < W; oy++)
 #pragma unroll
             for (int ox = 0; ox < W; ox++) {
-                const size_t c = (size_t)cell[0][ox] + (size_t)N * ((size_t)cell[1][oy] + (size_t)N * cell[2][oz]);
-                const T v = rhobar[c] + (ab[0] * F[c] + ab[1] * F[c + M] + ab[2] * F[c + 2 * M]);
+                const unsigned c = (unsigned)cell[0][ox] + (unsigned)N * ((unsigned)cell[1][oy] + (unsigned)N * (unsigned)cell[2][oz]);
+                vr[oy][ox] = rhobar[c];
+                vf0[oy][ox] = F[c];
+                vf1[oy][ox] = F1[c];
+                vf2[oy][ox] = F2[c];
+            }
+#pragma unroll
+        for (int oy = 0; oy < W; oy++)
+#pragma unroll
+            for (int ox = 0; ox < W; ox++) {
+                const T v = vr[oy][ox] + (ab[0] * vf0[oy][ox] + ab[1] * vf1[oy][ox] + ab[2] * vf2[oy][ox]);
                 g0 += dw[0][ox] * w[1][oy] * w[2][oz] * v;
                 g1 += w[0][ox] * dw[1][oy] * w[2][oz] * v;
                 g2 += w[0][ox] * w[1][oy] * dw[2][oz] * v;
             }
+    }
     const T k = (T)N / L;
     psibar[p] += k * g0;
     psibar[p + n3] += k * g1;
@@ -298,14 +327,16 @@ __global__ __launch_bounds__(256) void k_loss_total(double *part) {
 }
 
 static unsigned agrid(size_t n) { return (unsigned)((n + 255) / 256); }
+// particle kernels: blocks of up to 256 threads along i, one block row per (j, k)
+static unsigned pblock(int n) { return n >= 256 ? 256u : (unsigned)(((n + 63) / 64) * 64); }
+static dim3 pgrid(int n) { return dim3((unsigned)((n + pblock(n) - 1) / pblock(n)), (unsigned)n, (unsigned)n); }
 
 template <typename T>
 struct AdjLaunch {
     static void scatter3(const void *psi, const void *q1, int n, int N, double L, const double *sh, const void *abar,
                          double s, int W, void *fbar, hipStream_t st) {
-        const size_t n3 = (size_t)n * n * n;
 #define ADJ_S(WW)                                                                                                   \
-    k_adj_scatter3<T, WW><<<agrid(n3), 256, 0, st>>>((const T *)psi, (const T *)q1, n, N, (T)L, (T)sh[0], (T)sh[1], \
+    k_adj_scatter3<T, WW><<<pgrid(n), pblock(n), 0, st>>>((const T *)psi, (const T *)q1, n, N, (T)L, (T)sh[0], (T)sh[1], \
                                                      (T)sh[2], (const T *)abar, (T)s, (T *)fbar)
         if (W == 2) ADJ_S(2);
         else if (W == 3) ADJ_S(3);
@@ -318,9 +349,8 @@ struct AdjLaunch {
     static void gather_grad(const void *psi, const void *q1, int n, int N, double L, const double *sh,
                             const void *rhobar, const void *phi, const void *abar, double s, const double *c, int W,
                             void *psibar, hipStream_t st) {
-        const size_t n3 = (size_t)n * n * n;
 #define ADJ_G(WW)                                                                                                   \
-    k_adj_gather<T, WW><<<agrid(n3), 256, 0, st>>>((const T *)psi, (const T *)q1, n, N, (T)L, (T)sh[0], (T)sh[1],   \
+    k_adj_gather<T, WW><<<pgrid(n), pblock(n), 0, st>>>((const T *)psi, (const T *)q1, n, N, (T)L, (T)sh[0], (T)sh[1],   \
                                                    (T)sh[2], (const T *)rhobar, (const T *)phi, (const T *)abar, (T)s, \
                                                    (T)c[0], (T)c[1], (T)c[2], (T *)psibar)
         if (W == 2) ADJ_G(2);
@@ -331,9 +361,8 @@ struct AdjLaunch {
     static void gather_grad_f(const void *psi, const void *q1, int n, int N, double L, const double *sh,
                               const void *rhobar, const void *F, const void *abar, double s, int W, void *psibar,
                               hipStream_t st) {
-        const size_t n3 = (size_t)n * n * n;
 #define ADJ_F(WW)                                                                                                    \
-    k_adj_gather_f<T, WW><<<agrid(n3), 256, 0, st>>>((const T *)psi, (const T *)q1, n, N, (T)L, (T)sh[0], (T)sh[1], \
+    k_adj_gather_f<T, WW><<<pgrid(n), pblock(n), 0, st>>>((const T *)psi, (const T *)q1, n, N, (T)L, (T)sh[0], (T)sh[1], \
                                                      (T)sh[2], (const T *)rhobar, (const T *)F, (const T *)abar, (T)s, \
                                                      (T *)psibar)
         if (W == 2) ADJ_F(2);
