@@ -155,11 +155,11 @@ struct Passes {
     // X[k] and X[NC-k] of line l and writes Z[k] = (X[k] + conj X[NC-k]) + i (X[k] - conj X[NC-k]) W_N^-k
     // and Z[NC-k] likewise; the imaginary parts of X[0] and X[NC] are dropped (c2r of a Hermitian line).
     // All reads of a thread are issued before its first LDS write.
-    DJ_HD static void xi_load_pre(const cx<T> *A, cx<T> *lds, const cx<T> *twN, int yb, int z, int tid) {
-        constexpr int HP = NC / 2 + 1, TOTAL = B * HP, CNT = (TOTAL + NT - 1) / NT;
-        cx<T> va[CNT], vb[CNT];
+    static constexpr int XI_CNT = (B * (NC / 2 + 1) + NT - 1) / NT;  // work items per thread
+    DJ_HD static void xi_fetch(const cx<T> *A, int yb, int z, int tid, cx<T> (&va)[XI_CNT], cx<T> (&vb)[XI_CNT]) {
+        constexpr int HP = NC / 2 + 1, TOTAL = B * HP;
 #pragma unroll
-        for (int c = 0; c < CNT; c++) {
+        for (int c = 0; c < XI_CNT; c++) {
             const int i = tid + c * NT;
             if (TOTAL % NT == 0 || i < TOTAL) {
                 const int k = i / B, l = i - k * B;
@@ -168,8 +168,12 @@ struct Passes {
                 vb[c] = A[row + (size_t)N * (NC - k)];
             }
         }
+    }
+    DJ_HD static void xi_pre(cx<T> *lds, const cx<T> *twN, int tid, const cx<T> (&va)[XI_CNT],
+                             const cx<T> (&vb)[XI_CNT]) {
+        constexpr int HP = NC / 2 + 1, TOTAL = B * HP;
 #pragma unroll
-        for (int c = 0; c < CNT; c++) {
+        for (int c = 0; c < XI_CNT; c++) {
             const int i = tid + c * NT;
             if (!(TOTAL % NT == 0 || i < TOTAL)) continue;
             const int k = i / B, l = i - k * B;
@@ -184,6 +188,12 @@ struct Passes {
                 lds[EX::idx(l, NC - k)] = {e2.x - o2.y, e2.y + o2.x};
             }
         }
+    }
+    // all reads of a thread are issued before its first LDS write
+    DJ_HD static void xi_load_pre(const cx<T> *A, cx<T> *lds, const cx<T> *twN, int yb, int z, int tid) {
+        cx<T> va[XI_CNT], vb[XI_CNT];
+        xi_fetch(A, yb, z, tid, va, vb);
+        xi_pre(lds, twN, tid, va, vb);
     }
     DJ_HD static void xi_store(const cx<T> *lds, T *mesh, int yb, int z, int tid) {
         const size_t base = (size_t)N * N * z + (size_t)N * yb * B;
