@@ -232,7 +232,7 @@ struct IcWork {
     void *twn = nullptr, *twe = nullptr, *kt = nullptr, *pt = nullptr;
     void *A = nullptr, *Bm = nullptr, *F = nullptr, *Fb = nullptr, *T1 = nullptr, *fLb = nullptr;
     void *w = nullptr, *p1 = nullptr, *p2 = nullptr;
-    void *Ae = nullptr, *Be = nullptr, *Pe = nullptr, *mu = nullptr, *ab = nullptr, *fld[3] = {nullptr, nullptr, nullptr};
+    void *Ae = nullptr, *Be = nullptr, *mu = nullptr, *ab = nullptr, *fld[3] = {nullptr, nullptr, nullptr};
 };
 void ic_work_free(IcWork &ws);
 // White noise: `white` (n^3 real values, on the host, or on the device when white_on_device), or else

@@ -119,6 +119,59 @@ __global__ void k_ic_crop(const cx<double> *in, cx<double> *out, int n, int ext,
     out[e] = v;
 }
 
+// One A_ik spectrum on the ext grid from phi: the steps of k_ic_mult<IC_PSI1> (n grid, axis i), k_ic_pad
+// (scale s) and k_ic_mult<IC_DERIV> (ext grid, axis k), in their order, in one pass over the ext spectrum
+// (the two intermediate spectra are not written)
+__global__ void k_ic_deriv_pad(const cx<double> *__restrict__ F, cx<double> *__restrict__ out, int n, int ext,
+                               double L, int i, int k, double s) {
+    const int He = ext / 2 + 1, Hn = n / 2 + 1, h = n / 2;
+    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x, tot = (size_t)ext * He * ext;
+    if (e >= tot) return;
+    const BIdx b = bidx(e, ext);
+    auto src = [&](int m) { return m < h ? m : (m > ext - h ? m - ext + n : -1); };
+    const int sx = b.kx < h ? b.kx : -1, sy = src(b.ky), sz = src(b.kz);
+    const double dk = 2 * M_PI / L;
+    cx<double> v = {0, 0};
+    if (sx >= 0 && sy >= 0 && sz >= 0) {
+        const BIdx bn = {sx, sy, sz};
+        const double g = kdir(bn, i, n, dk);
+        const cx<double> x = F[(size_t)sz + (size_t)n * (sx + (size_t)Hn * sy)];
+        cx<double> r = {-g * x.y, g * x.x};
+        r = {-r.x, -r.y};
+        v = scale(r, s);
+    }
+    const double g2 = kdir(b, k, ext, dk);
+    out[e] = {-g2 * v.y, g2 * v.x};
+}
+
+// The transpose of the steps above for the r2c of an A_ik cotangent: k_ic_halfw (ext grid, scale sme),
+// k_ic_mult<IC_PSI1> (ext grid, axis k), k_ic_crop (scale s), k_ic_mult<IC_DERIV> (n grid, axis i) and
+// k_ic_axpy into Fb, in their order, in one pass over the n spectrum (the ext entries that the crop
+// drops are not read)
+__global__ void k_ic_deriv_adj(const cx<double> *__restrict__ Be, cx<double> *__restrict__ Fb, int n, int ext,
+                               double L, int i, int k, double sme, double s) {
+    const int He = ext / 2 + 1, Hn = n / 2 + 1, h = n / 2;
+    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x, tot = (size_t)n * Hn * n;
+    if (e >= tot) return;
+    const BIdx b = bidx(e, n);
+    auto src = [&](int m) { return m < h ? m : (m > h ? m - n + ext : -1); };
+    const int sx = b.kx < h ? b.kx : -1, sy = src(b.ky), sz = src(b.kz);
+    const double dk = 2 * M_PI / L;
+    cx<double> v = {0, 0};
+    if (sx >= 0 && sy >= 0 && sz >= 0) {
+        const double w = (sx == 0 || sx == ext / 2) ? 1.0 : 2.0;
+        const cx<double> x = scale(Be[(size_t)sz + (size_t)ext * (sx + (size_t)He * sy)], sme * w);
+        const BIdx be = {sx, sy, sz};
+        const double g = kdir(be, k, ext, dk);
+        cx<double> r = {-g * x.y, g * x.x};
+        r = {-r.x, -r.y};
+        v = scale(r, s);
+    }
+    const double g2 = kdir(b, i, n, dk);
+    const cx<double> t = {-g2 * v.y, g2 * v.x};
+    Fb[e] = {Fb[e].x + t.x, Fb[e].y + t.y};
+}
+
 // mu op= a * b  (op 0 assign, 1 add, 2 subtract); the products and sums in the reference's order
 __global__ void k_ic_mu(double *mu, const double *a, const double *b, size_t n, int op) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -356,7 +409,6 @@ static hipError_t ic_work_ensure(IcWork &ws, int n, int order, int ntab) {
         ws.fLb = alloc(HC * 16);
         ws.Ae = alloc(HE * 16);
         ws.Be = alloc(HE * 16);
-        ws.Pe = alloc(HE * 16);
         ws.mu = alloc(ME * 8);
         ws.ab = alloc(ME * 8);
         for (auto &f : ws.fld) f = alloc(ME * 8);
@@ -432,15 +484,13 @@ int ics_lpt_device(IcWork &ws, int n, double L, int order, const double *white, 
     }
     if (order >= 2) {
         void *twe = ws.twe;
-        cx<double> *Ae = (cx<double> *)ws.Ae, *Be = (cx<double> *)ws.Be, *Pe = (cx<double> *)ws.Pe;
+        cx<double> *Ae = (cx<double> *)ws.Ae, *Be = (cx<double> *)ws.Be;
         double *mu = (double *)ws.mu;
         double *fld[3] = {(double *)ws.fld[0], (double *)ws.fld[1], (double *)ws.fld[2]};
         const double pad_s = std::pow((double)ext / n, 3), crop_s = std::pow((double)n / ext, 3);
         // A_ik = irfft_ext(i k_k pad(psi1_i)); psi1_i spectrum = -(i k_i) phi
         auto deriv = [&](int i, int k, double *out) {
-            k_ic_mult<IC_PSI1><<<g1(HC), 256, 0, st>>>(F, Bm, n, L, i);
-            k_ic_pad<<<g1(HE), 256, 0, st>>>(Bm, Pe, n, ext, pad_s);
-            k_ic_mult<IC_DERIV><<<g1(HE), 256, 0, st>>>(Pe, Be, ext, L, k);
+            k_ic_deriv_pad<<<g1(HE), 256, 0, st>>>(F, Be, n, ext, L, i, k, pad_s);
             inv(fe, Be, Ae, out, twe);
             k_ic_scale_real<<<g1(ME), 256, 0, st>>>(out, ME, 1.0 / (double)ME);
         };
@@ -532,7 +582,7 @@ int ics_lpt_vjp_device(IcWork &ws, int n, double L, int order, const double *whi
     }
     if (order >= 2) {
         void *twe = ws.twe;
-        cx<double> *Ae = (cx<double> *)ws.Ae, *Be = (cx<double> *)ws.Be, *Pe = (cx<double> *)ws.Pe;
+        cx<double> *Ae = (cx<double> *)ws.Ae, *Be = (cx<double> *)ws.Be;
         double *mub = (double *)ws.mu, *ab = (double *)ws.ab;
         double *fld[3] = {(double *)ws.fld[0], (double *)ws.fld[1], (double *)ws.fld[2]};
         cx<double> *fLb = (cx<double> *)ws.fLb;
@@ -552,20 +602,14 @@ int ics_lpt_vjp_device(IcWork &ws, int n, double L, int order, const double *whi
         k_ic_scale_real<<<g1(ME), 256, 0, st>>>(mub, ME, ((3.0 - 2) / 2 - 2.0) / ((2 + 1.5) * 1.0));
         // A_ik = c2r_ext((i k_k) pad(-(i k_i) phi) pad_s) / ME
         auto deriv = [&](int i, int k, double *out) {
-            k_ic_mult<IC_PSI1><<<g1(HC), 256, 0, st>>>(F, Bm, n, L, i);
-            k_ic_pad<<<g1(HE), 256, 0, st>>>(Bm, Pe, n, ext, pad_s);
-            k_ic_mult<IC_DERIV><<<g1(HE), 256, 0, st>>>(Pe, Be, ext, L, k);
+            k_ic_deriv_pad<<<g1(HE), 256, 0, st>>>(F, Be, n, ext, L, i, k, pad_s);
             inv(fe, Be, Ae, out, twe);
             k_ic_scale_real<<<g1(ME), 256, 0, st>>>(out, ME, 1.0 / (double)ME);
         };
         // Abar_ik -> phibar: w r2c_ext(Abar) / ME, conj(i k_k), crop pad_s, conj(-(i k_i))
         auto deriv_adj = [&](int i, int k, double *abar) {
             fwd(fe, abar, Ae, Be, twe);
-            k_ic_halfw<<<g1(HE), 256, 0, st>>>(Be, ext, 0, 1.0 / (double)ME);
-            k_ic_mult<IC_PSI1><<<g1(HE), 256, 0, st>>>(Be, Pe, ext, L, k);
-            k_ic_crop<<<g1(HC), 256, 0, st>>>(Pe, Bm, n, ext, pad_s);
-            k_ic_mult<IC_DERIV><<<g1(HC), 256, 0, st>>>(Bm, T1, n, L, i);
-            k_ic_axpy<<<g1(HC), 256, 0, st>>>(Fb, T1, HC);
+            k_ic_deriv_adj<<<g1(HC), 256, 0, st>>>(Be, Fb, n, ext, L, i, k, 1.0 / (double)ME, pad_s);
         };
         // mu = A11 A22 + A11 A33 + A22 A33 - A12 A21 - A13 A31 - A23 A32
         deriv(0, 0, fld[0]);
