@@ -13,6 +13,8 @@ namespace dj {
 struct FftOps {
     void (*xf)(const void *mesh, bool fixed, void *A, const void *tw, double fscale, hipStream_t st);
     void (*xi)(const void *A, void *mesh, const void *tw, hipStream_t st);
+    // the same with the output multiplied by s as it is stored
+    void (*xi_scaled)(const void *A, void *mesh, const void *tw, double s, hipStream_t st);
     void (*yf)(const void *A, void *B, const void *tw, hipStream_t st);
     void (*yi)(const void *B, void *A, const void *tw, hipStream_t st);
     void (*zm)(int mode, bool fwd, bool inv, const void *in, void *out, const void *tw, const void *K,

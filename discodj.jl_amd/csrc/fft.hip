@@ -85,9 +85,10 @@ __global__ __launch_bounds__(256) void k_xf(const IN *__restrict__ mesh, cx<T> *
     P::xf_store(lds, t, A, yb, z, threadIdx.x);
 }
 
-template <typename T, int N>
+// SC: the output multiplied by s as it is stored
+template <typename T, int N, bool SC = false>
 __global__ __launch_bounds__((FftShape<T, N>::NTXI)) void k_xi(const cx<T> *__restrict__ A, T *__restrict__ mesh,
-                                            const cx<T> *__restrict__ tw) {
+                                            const cx<T> *__restrict__ tw, T s = (T)1) {
     using S = FftShape<T, N>;
     using P = typename S::PXI;
     __shared__ cx<T> lds[P::EX::LDS_ELEMS];
@@ -109,10 +110,10 @@ __global__ __launch_bounds__((FftShape<T, N>::NTXI)) void k_xi(const cx<T> *__re
     __syncthreads();
     if constexpr (P::EX::wave_ok()) {
         P::EX::template run_wave<+1, 2>(lds, t, threadIdx.x);
-        P::xi_store_w(lds, mesh, yb, z, threadIdx.x);
+        P::template xi_store_w<SC>(lds, mesh, yb, z, threadIdx.x, s);
     } else {
         P::EX::template run<+1, 2>(lds, t, threadIdx.x);
-        P::xi_store(lds, mesh, yb, z, threadIdx.x);
+        P::template xi_store<SC>(lds, mesh, yb, z, threadIdx.x, s);
     }
 }
 
@@ -305,6 +306,10 @@ struct FftN {
         using PX = typename S::PXI;
         k_xi<T, N><<<PX::XBLOCKS(), S::NTXI, 0, st>>>((const cx<T> *)A, (T *)mesh, (const cx<T> *)tw);
     }
+    static void xi_scaled(const void *A, void *mesh, const void *tw, double s, hipStream_t st) {
+        using PX = typename S::PXI;
+        k_xi<T, N, true><<<PX::XBLOCKS(), S::NTXI, 0, st>>>((const cx<T> *)A, (T *)mesh, (const cx<T> *)tw, (T)s);
+    }
     static void yf(const void *A, void *Bm, const void *tw, hipStream_t st) {
         k_yf<T, N><<<P::YBLOCKS(), S::NTY, 0, st>>>((const cx<T> *)A, (cx<T> *)Bm, (const cx<T> *)tw);
     }
@@ -344,6 +349,7 @@ bool fft_dispatch(int N, FftOps &ops) {
     case n:                             \
         ops.xf = FftN<T, n>::xf;        \
         ops.xi = FftN<T, n>::xi;        \
+        ops.xi_scaled = FftN<T, n>::xi_scaled; \
         ops.yf = FftN<T, n>::yf;        \
         ops.yi = FftN<T, n>::yi;        \
         ops.zm = FftN<T, n>::zm;        \

@@ -195,7 +195,9 @@ struct Passes {
         xi_fetch(A, yb, z, tid, va, vb);
         xi_pre(lds, twN, tid, va, vb);
     }
-    DJ_HD static void xi_store(const cx<T> *lds, T *mesh, int yb, int z, int tid) {
+    // SC: each value multiplied by s as it is stored (a normalisation that would otherwise be a pass of its own)
+    template <bool SC = false>
+    DJ_HD static void xi_store(const cx<T> *lds, T *mesh, int yb, int z, int tid, T s = (T)1) {
         const size_t base = (size_t)N * N * z + (size_t)N * yb * B;
         copy_batched<cx<T>, B * NC, NT>(
             tid,
@@ -206,8 +208,8 @@ struct Passes {
             [&](int i, cx<T> v) {
                 const int l = i / NC, m = i - l * NC;
                 const size_t g = base + (size_t)N * l + 2 * m;
-                mesh[g] = v.x;
-                mesh[g + 1] = v.y;
+                mesh[g] = SC ? v.x * s : v.x;
+                mesh[g + 1] = SC ? v.y * s : v.y;
             });
     }
 
@@ -301,7 +303,8 @@ struct Passes {
                 lds[EX::idx(l, m)] = v;
             });
     }
-    DJ_HD static void xi_store_w(const cx<T> *lds, T *mesh, int yb, int z, int tid) {
+    template <bool SC = false>
+    DJ_HD static void xi_store_w(const cx<T> *lds, T *mesh, int yb, int z, int tid, T s = (T)1) {
         const size_t base = (size_t)N * N * z + (size_t)N * yb * B;
         copy_wave<cx<T>, EX::LW * NC>(
             tid >> 6, tid & 63,
@@ -312,7 +315,7 @@ struct Passes {
             [&](int i, cx<T> v) {
                 const int l = i / NC, m = i - l * NC;
                 const size_t g = base + (size_t)N * l + 2 * m;
-                st_stream((cx<T> *)(mesh + g), v);  // g even: a pair of reals
+                st_stream((cx<T> *)(mesh + g), SC ? cx<T>{v.x * s, v.y * s} : v);  // g even: a pair of reals
             });
     }
     DJ_HD static void y_load_a_w(const cx<T> *A, cx<T> *lds, int zb, int kx, int tid) {

@@ -172,17 +172,13 @@ __global__ void k_ic_deriv_adj(const cx<double> *__restrict__ Be, cx<double> *__
     Fb[e] = {Fb[e].x + t.x, Fb[e].y + t.y};
 }
 
-// mu op= a * b  (op 0 assign, 1 add, 2 subtract); the products and sums in the reference's order
-__global__ void k_ic_mu(double *mu, const double *a, const double *b, size_t n, int op) {
+// mu op= a * b  (op 0 assign, 1 add, 2 subtract; 3 subtract, then multiply by s); the products and sums
+// in the reference's order
+__global__ void k_ic_mu(double *mu, const double *a, const double *b, size_t n, int op, double s = 1.0) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double p = a[i] * b[i];
-    mu[i] = op == 0 ? p : op == 1 ? mu[i] + p : mu[i] - p;
-}
-
-__global__ void k_ic_scale_real(double *x, size_t n, double s) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) x[i] *= s;
+    mu[i] = op == 0 ? p : op == 1 ? mu[i] + p : op == 2 ? mu[i] - p : (mu[i] - p) * s;
 }
 
 template <typename T>
@@ -438,8 +434,7 @@ static hipError_t ic_white(IcWork &ws, FftOps &fn, int n, const double *white, b
     k_field_to_blayout<<<g1(HC), 256, 0, st>>>((const cx<double> *)ws.F, n, std::pow((double)n, 1.5), Bm);
     fn.zm(ZM_NONE, false, true, Bm, Bm, ws.twn, &none, st);
     fn.yi(Bm, ws.A, ws.twn, st);
-    fn.xi(ws.A, ws.w, ws.twn, st);
-    k_ic_scale_real<<<g1(M), 256, 0, st>>>((double *)ws.w, M, 1.0 / (double)M);
+    fn.xi_scaled(ws.A, ws.w, ws.twn, 1.0 / (double)M, st);
     return hipGetLastError();
 }
 
@@ -467,10 +462,11 @@ int ics_lpt_device(IcWork &ws, int n, double L, int order, const double *white, 
         f.yf(a, b, tw, st);
         f.zm(ZM_NONE, true, false, b, b, tw, &none, st);
     };
-    auto inv = [&](FftOps &f, cx<double> *b, cx<double> *a, double *real, void *tw) {  // destroys b
+    // inverse transform (destroys b), the output multiplied by sc as it is stored
+    auto inv_s = [&](FftOps &f, cx<double> *b, cx<double> *a, double *real, void *tw, double sc) {
         f.zm(ZM_NONE, false, true, b, b, tw, &none, st);
         f.yi(b, a, tw, st);
-        f.xi(a, real, tw, st);
+        f.xi_scaled(a, real, tw, sc, st);
     };
     if (err == hipSuccess) err = ic_white(ws, fn, n, white, white_on_device, seed, st);
     if (err != hipSuccess) goto done;
@@ -479,8 +475,7 @@ int ics_lpt_device(IcWork &ws, int n, double L, int order, const double *white, 
     k_ic_scale<<<g1(HC), 256, 0, st>>>(F, n, L, kt, pt, ntab);
     for (int c = 0; c < 3; c++) {
         k_ic_mult<IC_PSI1><<<g1(HC), 256, 0, st>>>(F, Bm, n, L, c);
-        inv(fn, Bm, A, p1 + c * M, twn);
-        k_ic_scale_real<<<g1(M), 256, 0, st>>>(p1 + c * M, M, 1.0 / (double)M);
+        inv_s(fn, Bm, A, p1 + c * M, twn, 1.0 / (double)M);
     }
     if (order >= 2) {
         void *twe = ws.twe;
@@ -491,8 +486,7 @@ int ics_lpt_device(IcWork &ws, int n, double L, int order, const double *white, 
         // A_ik = irfft_ext(i k_k pad(psi1_i)); psi1_i spectrum = -(i k_i) phi
         auto deriv = [&](int i, int k, double *out) {
             k_ic_deriv_pad<<<g1(HE), 256, 0, st>>>(F, Be, n, ext, L, i, k, pad_s);
-            inv(fe, Be, Ae, out, twe);
-            k_ic_scale_real<<<g1(ME), 256, 0, st>>>(out, ME, 1.0 / (double)ME);
+            inv_s(fe, Be, Ae, out, twe, 1.0 / (double)ME);
         };
         // mu = A11 A22 + A11 A33 + A22 A33 - A12 A21 - A13 A31 - A23 A32 (evaluated left to right)
         deriv(0, 0, fld[0]);
@@ -501,20 +495,19 @@ int ics_lpt_device(IcWork &ws, int n, double L, int order, const double *white, 
         k_ic_mu<<<g1(ME), 256, 0, st>>>(mu, fld[0], fld[1], ME, 0);
         k_ic_mu<<<g1(ME), 256, 0, st>>>(mu, fld[0], fld[2], ME, 1);
         k_ic_mu<<<g1(ME), 256, 0, st>>>(mu, fld[1], fld[2], ME, 1);
+        // the last product also applies the order-2 factor ((3-2)/2 - 2) / ((2 + 3/2)(2 - 1)) = -3/7
         const int pairs[3][2][2] = {{{0, 1}, {1, 0}}, {{0, 2}, {2, 0}}, {{1, 2}, {2, 1}}};
-        for (auto &pr : pairs) {
-            deriv(pr[0][0], pr[0][1], fld[0]);
-            deriv(pr[1][0], pr[1][1], fld[1]);
-            k_ic_mu<<<g1(ME), 256, 0, st>>>(mu, fld[0], fld[1], ME, 2);
+        for (int q = 0; q < 3; q++) {
+            deriv(pairs[q][0][0], pairs[q][0][1], fld[0]);
+            deriv(pairs[q][1][0], pairs[q][1][1], fld[1]);
+            k_ic_mu<<<g1(ME), 256, 0, st>>>(mu, fld[0], fld[1], ME, q < 2 ? 2 : 3,
+                                            ((3.0 - 2) / 2 - 2.0) / ((2 + 1.5) * 1.0));
         }
-        // the order-2 factor ((3-2)/2 - 2) / ((2 + 3/2)(2 - 1)) = -3/7
-        k_ic_scale_real<<<g1(ME), 256, 0, st>>>(mu, ME, ((3.0 - 2) / 2 - 2.0) / ((2 + 1.5) * 1.0));
         fwd(fe, mu, Ae, Be, twe);
         k_ic_crop<<<g1(HC), 256, 0, st>>>(Be, A, n, ext, crop_s);  // fL in A
         for (int c = 0; c < 3; c++) {
             k_ic_mult<IC_PSI2><<<g1(HC), 256, 0, st>>>(A, Bm, n, L, c);
-            inv(fn, Bm, F, p2 + c * M, twn);  // F is free now
-            k_ic_scale_real<<<g1(M), 256, 0, st>>>(p2 + c * M, M, 1.0 / (double)M);
+            inv_s(fn, Bm, F, p2 + c * M, twn, 1.0 / (double)M);  // F is free now
         }
     }
     if (prec == 64)
@@ -561,6 +554,12 @@ int ics_lpt_vjp_device(IcWork &ws, int n, double L, int order, const double *whi
         f.yi(b, a, tw, st);
         f.xi(a, real, tw, st);
     };
+    // the same with the output multiplied by sc as it is stored (a normalisation pass saved)
+    auto inv_s = [&](FftOps &f, cx<double> *b, cx<double> *a, double *real, void *tw, double sc) {
+        f.zm(ZM_NONE, false, true, b, b, tw, &none, st);
+        f.yi(b, a, tw, st);
+        f.xi_scaled(a, real, tw, sc, st);
+    };
     if (err == hipSuccess) err = ic_white(ws, fn, n, white, white_on_device, seed, st);
     if (err != hipSuccess) goto done;
     // forward phi(k), then the state cotangents split into psi1bar, psi2bar
@@ -598,13 +597,11 @@ int ics_lpt_vjp_device(IcWork &ws, int n, double L, int order, const double *whi
         // fL = crop(r2c_ext(-3/7 mu)) crop_s:  mubar = -3/7 c2r_ext(pad(fLbar) crop_s / w)
         k_ic_pad<<<g1(HE), 256, 0, st>>>(fLb, Be, n, ext, crop_s);
         k_ic_halfw<<<g1(HE), 256, 0, st>>>(Be, ext, 1, 1.0);
-        inv(fe, Be, Ae, mub, twe);
-        k_ic_scale_real<<<g1(ME), 256, 0, st>>>(mub, ME, ((3.0 - 2) / 2 - 2.0) / ((2 + 1.5) * 1.0));
+        inv_s(fe, Be, Ae, mub, twe, ((3.0 - 2) / 2 - 2.0) / ((2 + 1.5) * 1.0));
         // A_ik = c2r_ext((i k_k) pad(-(i k_i) phi) pad_s) / ME
         auto deriv = [&](int i, int k, double *out) {
             k_ic_deriv_pad<<<g1(HE), 256, 0, st>>>(F, Be, n, ext, L, i, k, pad_s);
-            inv(fe, Be, Ae, out, twe);
-            k_ic_scale_real<<<g1(ME), 256, 0, st>>>(out, ME, 1.0 / (double)ME);
+            inv_s(fe, Be, Ae, out, twe, 1.0 / (double)ME);
         };
         // Abar_ik -> phibar: w r2c_ext(Abar) / ME, conj(i k_k), crop pad_s, conj(-(i k_i))
         auto deriv_adj = [&](int i, int k, double *abar) {
