@@ -1380,10 +1380,9 @@ dj_status dj_run_checkpointed(dj_ctx *c, int32_t n_steps, const double *dd1, con
     c->ck_beta.assign(beta, beta + n_steps);
     c->ck_dd2.assign(dd2, dd2 + n_steps);
     for (int s = 0; s < n_steps; s++) {
+        // the step's checkpoint (psi, mom before its drift) written by the drift as it reads them
         char *ck = (char *)c->ckpt + 2 * pb * (size_t)s;
-        HIPCHK(c, hipMemcpyAsync(ck, c->psi, pb, hipMemcpyDeviceToDevice, c->st));
-        HIPCHK(c, hipMemcpyAsync(ck + pb, c->mom, pb, hipMemcpyDeviceToDevice, c->st));
-        c->part.drift(c->psi, c->mom, c->n3, dd1[s], c->st);
+        c->part.drift_to(c->psi, c->psi, c->mom, ck, ck + pb, c->n3, dd1[s], c->st);
         KickArgs k{alpha[s], beta[s], dd2[s], 0.0, 0, c->nshifts};
         dj_status r = force(c, c->psi, FORCE_KICK, k);
         if (r) return r;
@@ -1411,8 +1410,7 @@ dj_status dj_vjp_run(dj_ctx *c, void *psi_bar, void *pi_bar) {
     for (int s = c->ck_steps - 1; s >= 0; s--) {
         const char *ck = (const char *)c->ckpt + 2 * pb * (size_t)s;
         // the drifted positions of the step's force, recomputed from its checkpoint
-        HIPCHK(c, hipMemcpyAsync(c->vb_psi1, ck, pb, hipMemcpyDeviceToDevice, c->st));
-        c->part.drift(c->vb_psi1, ck + pb, c->n3, c->ck_dd1[s], c->st);
+        c->part.drift_to(c->vb_psi1, ck, ck + pb, nullptr, nullptr, c->n3, c->ck_dd1[s], c->st);
         c->adj.kick(c->vb_psibar, c->vb_pibar, c->vb_abar, np3, c->ck_alpha[s], c->ck_beta[s], c->ck_dd2[s], c->st);
         r = vjp_force(c, c->vb_psi1, c->vb_abar, c->vb_psibar);
         if (r) return r;

@@ -54,6 +54,9 @@ struct PartOps {
     void (*gather_comp)(const PartArgs &a, int W, const void *field, int comp, hipStream_t st);
     // psi += dd * mom
     void (*drift)(void *psi, const void *mom, int64_t n3, double dd, hipStream_t st);
+    // out = psi + dd * mom (out may be psi); psi and mom also copied to ck_psi, ck_mom when both are non-null
+    void (*drift_to)(void *out, const void *psi, const void *mom, void *ck_psi, void *ck_mom, int64_t n3, double dd,
+                     hipStream_t st);
     // mom = alpha mom + beta acc/ns ; psi += dd2 mom [; psi += dd1_next mom]
     void (*kick_drift)(void *psi, void *mom, const void *acc, int64_t n3, const KickArgs &k, hipStream_t st);
     // x *= s

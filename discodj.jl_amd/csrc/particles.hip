@@ -218,6 +218,21 @@ __global__ __launch_bounds__(256) void k_drift(T *__restrict__ psi, const T *__r
     if (i < n3) psi[i] = psi[i] + dd * mom[i];
 }
 
+// out = psi + dd mom (out may be psi), the same arithmetic as k_drift; with CK, psi and mom are also
+// copied to ckp and ckm as they are read (the checkpoint of a step, without copy passes of its own)
+template <typename T, bool CK>
+__global__ __launch_bounds__(256) void k_drift_to(T *out, const T *psi, const T *__restrict__ mom, T *__restrict__ ckp,
+                                                  T *__restrict__ ckm, int64_t n3, T dd) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n3) return;
+    const T p = psi[i], m = mom[i];
+    if constexpr (CK) {
+        ckp[i] = p;
+        ckm[i] = m;
+    }
+    out[i] = p + dd * m;
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_kick_drift(T *__restrict__ psi, T *__restrict__ mom,
                                                     const T *__restrict__ acc, int64_t n3, T alpha, T beta,
@@ -318,6 +333,17 @@ static void drift(void *psi, const void *mom, int64_t n3, double dd, hipStream_t
 }
 
 template <typename T>
+static void drift_to(void *out, const void *psi, const void *mom, void *ck_psi, void *ck_mom, int64_t n3, double dd,
+                     hipStream_t st) {
+    if (ck_psi && ck_mom)
+        k_drift_to<T, true><<<lgrid(n3), 256, 0, st>>>((T *)out, (const T *)psi, (const T *)mom, (T *)ck_psi,
+                                                      (T *)ck_mom, n3, (T)dd);
+    else
+        k_drift_to<T, false><<<lgrid(n3), 256, 0, st>>>((T *)out, (const T *)psi, (const T *)mom, nullptr, nullptr,
+                                                       n3, (T)dd);
+}
+
+template <typename T>
 static void kick_drift(void *psi, void *mom, const void *acc, int64_t n3, const KickArgs &k, hipStream_t st) {
     k_kick_drift<T><<<lgrid(n3), 256, 0, st>>>((T *)psi, (T *)mom, (const T *)acc, n3, (T)k.alpha, (T)k.beta,
                                                (T)k.dd2, (T)k.dd1_next, k.has_next, k.nshifts);
@@ -341,6 +367,7 @@ template <typename T> void part_ops(PartOps &ops) {
     ops.gather_fd = gather_fd<T>;
     ops.gather_comp = gather_comp<T>;
     ops.drift = drift<T>;
+    ops.drift_to = drift_to<T>;
     ops.kick_drift = kick_drift<T>;
     ops.scale = scale<T>;
     ops.zero = zero;
