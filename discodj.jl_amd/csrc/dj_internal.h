@@ -238,6 +238,16 @@ struct IcWork {
     void *A = nullptr, *Bm = nullptr, *F = nullptr, *Fb = nullptr, *T1 = nullptr, *fLb = nullptr;
     void *w = nullptr, *p1 = nullptr, *p2 = nullptr;
     void *Ae = nullptr, *Be = nullptr, *mu = nullptr, *ab = nullptr, *fld[3] = {nullptr, nullptr, nullptr};
+    // The nine second-order fields A_ik of the last forward (ext grid), kept for the reverse pass of the
+    // same ICs, and what they were computed from: n, L, order, the P(k) table, and the seed or (wkey) a
+    // copy of the white noise.  fk_room: 0 not tried, 1 allocated, -1 no room (the forward then uses fld).
+    void *fk[9] = {}, *wkey = nullptr, *flag = nullptr;
+    int fk_room = 0;
+    bool fk_valid = false, fk_seeded = false;
+    int fk_n = 0, fk_order = 0;
+    uint32_t fk_seed = 0;
+    double fk_L = 0;
+    std::vector<double> fk_kt, fk_pt;
 };
 void ic_work_free(IcWork &ws);
 // White noise: `white` (n^3 real values, on the host, or on the device when white_on_device), or else

@@ -12,6 +12,7 @@
 // Spectra are in the library's x-halved B layout  B[kz + N (kx + H ky)].
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -419,6 +420,36 @@ static hipError_t ic_work_ensure(IcWork &ws, int n, int order, int ntab) {
     return hipSuccess;
 }
 
+// Room for the nine A_ik fields and the white-noise key, allocated once per workspace; false (and no
+// retry) when the device has no room for them, the forward then computing into its three scratch fields
+static bool ic_fields_ensure(IcWork &ws, int n) {
+    if (ws.fk_room) return ws.fk_room > 0;
+    const int ext = 3 * n / 2;
+    const size_t M = (size_t)n * n * n, ME = (size_t)ext * ext * ext;
+    hipError_t err = hipSuccess;
+    auto alloc = [&](size_t bytes) -> void * {
+        void *p = nullptr;
+        if (err == hipSuccess) err = hipMalloc(&p, bytes);
+        if (p) ws.bufs.push_back(p);
+        return p;
+    };
+    for (auto &f : ws.fk) f = alloc(ME * 8);
+    ws.wkey = alloc(M * 8);
+    ws.flag = alloc(sizeof(unsigned));
+    if (err != hipSuccess) {
+        (void)hipGetLastError();  // an out-of-memory here is not an error of the call
+        ws.fk_room = -1;
+        return false;
+    }
+    ws.fk_room = 1;
+    return true;
+}
+
+__global__ void k_ic_differs(const unsigned long long *a, const unsigned long long *b, size_t n, unsigned *flag) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && a[i] != b[i]) atomicOr(flag, 1u);
+}
+
 // white noise into ws.w: copied from the host or the device, or the N-GenIC field of seed
 static hipError_t ic_white(IcWork &ws, FftOps &fn, int n, const double *white, bool on_device, uint32_t seed,
                            hipStream_t st) {
@@ -468,7 +499,10 @@ int ics_lpt_device(IcWork &ws, int n, double L, int order, const double *white, 
         f.yi(b, a, tw, st);
         f.xi_scaled(a, real, tw, sc, st);
     };
+    const bool keep = order >= 2 && ic_fields_ensure(ws, n);
+    ws.fk_valid = false;
     if (err == hipSuccess) err = ic_white(ws, fn, n, white, white_on_device, seed, st);
+    if (err == hipSuccess && keep && white) err = hipMemcpyAsync(ws.wkey, w, M * 8, hipMemcpyDeviceToDevice, st);
     if (err != hipSuccess) goto done;
     // phi(k)
     fwd(fn, w, A, F, twn);
@@ -482,6 +516,10 @@ int ics_lpt_device(IcWork &ws, int n, double L, int order, const double *white, 
         cx<double> *Ae = (cx<double> *)ws.Ae, *Be = (cx<double> *)ws.Be;
         double *mu = (double *)ws.mu;
         double *fld[3] = {(double *)ws.fld[0], (double *)ws.fld[1], (double *)ws.fld[2]};
+        // the nine A_ik: into the kept fields (A11, A22, A33, then A12, A21, A13, A31, A23, A32), or
+        // through the scratch fields
+        double *fa[9];
+        for (int j = 0; j < 9; j++) fa[j] = keep ? (double *)ws.fk[j] : fld[j < 3 ? j : (j - 3) % 2];
         const double pad_s = std::pow((double)ext / n, 3), crop_s = std::pow((double)n / ext, 3);
         // A_ik = irfft_ext(i k_k pad(psi1_i)); psi1_i spectrum = -(i k_i) phi
         auto deriv = [&](int i, int k, double *out) {
@@ -489,18 +527,18 @@ int ics_lpt_device(IcWork &ws, int n, double L, int order, const double *white, 
             inv_s(fe, Be, Ae, out, twe, 1.0 / (double)ME);
         };
         // mu = A11 A22 + A11 A33 + A22 A33 - A12 A21 - A13 A31 - A23 A32 (evaluated left to right)
-        deriv(0, 0, fld[0]);
-        deriv(1, 1, fld[1]);
-        deriv(2, 2, fld[2]);
-        k_ic_mu<<<g1(ME), 256, 0, st>>>(mu, fld[0], fld[1], ME, 0);
-        k_ic_mu<<<g1(ME), 256, 0, st>>>(mu, fld[0], fld[2], ME, 1);
-        k_ic_mu<<<g1(ME), 256, 0, st>>>(mu, fld[1], fld[2], ME, 1);
+        deriv(0, 0, fa[0]);
+        deriv(1, 1, fa[1]);
+        deriv(2, 2, fa[2]);
+        k_ic_mu<<<g1(ME), 256, 0, st>>>(mu, fa[0], fa[1], ME, 0);
+        k_ic_mu<<<g1(ME), 256, 0, st>>>(mu, fa[0], fa[2], ME, 1);
+        k_ic_mu<<<g1(ME), 256, 0, st>>>(mu, fa[1], fa[2], ME, 1);
         // the last product also applies the order-2 factor ((3-2)/2 - 2) / ((2 + 3/2)(2 - 1)) = -3/7
         const int pairs[3][2][2] = {{{0, 1}, {1, 0}}, {{0, 2}, {2, 0}}, {{1, 2}, {2, 1}}};
         for (int q = 0; q < 3; q++) {
-            deriv(pairs[q][0][0], pairs[q][0][1], fld[0]);
-            deriv(pairs[q][1][0], pairs[q][1][1], fld[1]);
-            k_ic_mu<<<g1(ME), 256, 0, st>>>(mu, fld[0], fld[1], ME, q < 2 ? 2 : 3,
+            deriv(pairs[q][0][0], pairs[q][0][1], fa[3 + 2 * q]);
+            deriv(pairs[q][1][0], pairs[q][1][1], fa[4 + 2 * q]);
+            k_ic_mu<<<g1(ME), 256, 0, st>>>(mu, fa[3 + 2 * q], fa[4 + 2 * q], ME, q < 2 ? 2 : 3,
                                             ((3.0 - 2) / 2 - 2.0) / ((2 + 1.5) * 1.0));
         }
         fwd(fe, mu, Ae, Be, twe);
@@ -516,6 +554,16 @@ int ics_lpt_device(IcWork &ws, int n, double L, int order, const double *white, 
         k_ic_combine<float><<<g1(3 * M), 256, 0, st>>>(p1, p2, (float *)psi_out, (float *)pi_out, 3 * M, D, order);
     err = hipGetLastError();
     if (err == hipSuccess) err = hipStreamSynchronize(st);
+    if (err == hipSuccess && keep) {
+        ws.fk_valid = true;
+        ws.fk_seeded = !white;
+        ws.fk_seed = seed;
+        ws.fk_n = n;
+        ws.fk_order = order;
+        ws.fk_L = L;
+        ws.fk_kt.assign(kt_h, kt_h + ntab);
+        ws.fk_pt.assign(pt_h, pt_h + ntab);
+    }
 done:
     (void)hipStreamSynchronize(st);
     return (int)err;
@@ -560,11 +608,29 @@ int ics_lpt_vjp_device(IcWork &ws, int n, double L, int order, const double *whi
         f.yi(b, a, tw, st);
         f.xi_scaled(a, real, tw, sc, st);
     };
-    if (err == hipSuccess) err = ic_white(ws, fn, n, white, white_on_device, seed, st);
+    // The A_ik of the last forward serve when they were computed from the same inputs: n, L, order, the
+    // same P(k) table values, and the same seed or white-noise values (compared bit for bit on the device)
+    bool hit = order >= 2 && ws.fk_valid && ws.fk_n == n && ws.fk_order == order && ws.fk_L == L &&
+               ws.fk_seeded == !white && (white || ws.fk_seed == seed) && ws.fk_kt.size() == (size_t)ntab &&
+               std::equal(kt_h, kt_h + ntab, ws.fk_kt.begin()) && std::equal(pt_h, pt_h + ntab, ws.fk_pt.begin());
+    if (err == hipSuccess && (!hit || white)) err = ic_white(ws, fn, n, white, white_on_device, seed, st);
+    if (err == hipSuccess && hit && white) {
+        unsigned differs = 1;
+        err = hipMemsetAsync(ws.flag, 0, sizeof(unsigned), st);
+        if (err == hipSuccess) {
+            k_ic_differs<<<g1(M), 256, 0, st>>>((const unsigned long long *)w, (const unsigned long long *)ws.wkey, M,
+                                                (unsigned *)ws.flag);
+            err = hipMemcpyAsync(&differs, ws.flag, sizeof(unsigned), hipMemcpyDeviceToHost, st);
+        }
+        if (err == hipSuccess) err = hipStreamSynchronize(st);
+        hit = err == hipSuccess && !differs;
+    }
     if (err != hipSuccess) goto done;
-    // forward phi(k), then the state cotangents split into psi1bar, psi2bar
-    fwd(fn, w, A, F, twn);
-    k_ic_scale<<<g1(HC), 256, 0, st>>>(F, n, L, kt, pt, ntab);
+    // forward phi(k) (only the recomputed A_ik need it), then the state cotangents split into psi1bar, psi2bar
+    if (!hit) {
+        fwd(fn, w, A, F, twn);
+        k_ic_scale<<<g1(HC), 256, 0, st>>>(F, n, L, kt, pt, ntab);
+    }
     if (prec == 64)
         k_ic_combine_adj<double><<<g1(3 * M), 256, 0, st>>>((const double *)psibar, (const double *)pibar, p1b, p2b,
                                                           3 * M, D, order);
@@ -608,23 +674,28 @@ int ics_lpt_vjp_device(IcWork &ws, int n, double L, int order, const double *whi
             fwd(fe, abar, Ae, Be, twe);
             k_ic_deriv_adj<<<g1(HC), 256, 0, st>>>(Be, Fb, n, ext, L, i, k, 1.0 / (double)ME, pad_s);
         };
+        // field j of the forward's order (A11, A22, A33, A12, A21, A13, A31, A23, A32): kept, or recomputed
+        // into a scratch field
+        auto field = [&](int j, int i, int k, double *scratch) -> const double * {
+            if (hit) return (const double *)ws.fk[j];
+            deriv(i, k, scratch);
+            return scratch;
+        };
         // mu = A11 A22 + A11 A33 + A22 A33 - A12 A21 - A13 A31 - A23 A32
-        deriv(0, 0, fld[0]);
-        deriv(1, 1, fld[1]);
-        deriv(2, 2, fld[2]);
+        const double *dg[3];
+        for (int d = 0; d < 3; d++) dg[d] = field(d, d, d, fld[d]);
         for (int d = 0; d < 3; d++) {
-            const double *o1 = fld[(d + 1) % 3], *o2 = fld[(d + 2) % 3];
+            const double *o1 = dg[(d + 1) % 3], *o2 = dg[(d + 2) % 3];
             k_ic_partner<<<g1(ME), 256, 0, st>>>(ab, mub, o1, o2, ME, 0);
             deriv_adj(d, d, ab);
         }
         const int pairs[3][2] = {{0, 1}, {0, 2}, {1, 2}};
-        for (auto &pr : pairs) {
-            const int i = pr[0], k = pr[1];
-            deriv(i, k, fld[0]);
-            deriv(k, i, fld[1]);
-            k_ic_partner<<<g1(ME), 256, 0, st>>>(ab, mub, fld[1], nullptr, ME, 1);
+        for (int q = 0; q < 3; q++) {
+            const int i = pairs[q][0], k = pairs[q][1];
+            const double *aik = field(3 + 2 * q, i, k, fld[0]), *aki = field(4 + 2 * q, k, i, fld[1]);
+            k_ic_partner<<<g1(ME), 256, 0, st>>>(ab, mub, aki, nullptr, ME, 1);
             deriv_adj(i, k, ab);
-            k_ic_partner<<<g1(ME), 256, 0, st>>>(ab, mub, fld[0], nullptr, ME, 1);
+            k_ic_partner<<<g1(ME), 256, 0, st>>>(ab, mub, aik, nullptr, ME, 1);
             deriv_adj(k, i, ab);
         }
     }

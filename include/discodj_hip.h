@@ -130,7 +130,9 @@ dj_status dj_ics_lpt(dj_ctx *ctx, uint32_t seed, const double *white, const doub
                      int32_t ntab, double D, int32_t order);
 /* dj_ics_lpt with the white noise in device memory (d_white: n^3 FP64 values, or NULL for the N-GenIC field
  * of seed).  The IC pipeline keeps its device buffers in the context between calls (about 20 n^3 doubles
- * for order 2: 5.3 GB at n = 256, 42 GB at n = 512), until dj_destroy. */
+ * for order 2: 5.3 GB at n = 256, 42 GB at n = 512), until dj_destroy.  For order 2 it also keeps the nine
+ * second-order fields A_ik on the 3n/2 grid and a copy of the white noise (31.4 n^3 doubles more: 4.2 GB
+ * at n = 256, 34 GB at n = 512) for dj_vjp_ics_lpt; without room for them it runs as before. */
 dj_status dj_ics_lpt_device(dj_ctx *ctx, uint32_t seed, const double *d_white, const double *k_tab,
                             const double *pk_tab, int32_t ntab, double D, int32_t order);
 
@@ -226,7 +228,10 @@ dj_status dj_run_checkpointed(dj_ctx *ctx, int32_t n_steps, const double *dd1, c
  * the context's resident cotangents, left resident */
 dj_status dj_vjp_run(dj_ctx *ctx, void *psi_bar, void *pi_bar);
 /* reverse of dj_ics_lpt (same seed / white, tables, D, order): white_bar_out (n^3 FP64) = the cotangent of
- * the real white-noise field given psi_bar, pi_bar of the ICs (both NULL: the resident cotangents) */
+ * the real white-noise field given psi_bar, pi_bar of the ICs (both NULL: the resident cotangents).  When
+ * the context's last dj_ics_lpt(_device) had the same inputs (n, L, order, table values, and the same seed
+ * or white-noise values, compared bit for bit on the device), the A_ik fields it kept are used instead of
+ * recomputing them; the result is the same bits either way. */
 dj_status dj_vjp_ics_lpt(dj_ctx *ctx, uint32_t seed, const double *white, const double *k_tab, const double *pk_tab,
                          int32_t ntab, double D, int32_t order, const void *psi_bar, const void *pi_bar,
                          double *white_bar_out);

@@ -157,6 +157,55 @@ def test_vjp_ics_fd(order, n):
     assert _rel(g, fdo) < 1e-9, (g, fdo)
 
 
+def test_vjp_ics_reuses_forward_fields_only_for_the_same_inputs():
+    """The IC reverse pass takes the nine second-order fields A_ik from the last forward when that forward
+    had the same inputs (seed or white-noise values, P(k) table, D aside) and recomputes them otherwise:
+    the cotangent is the same bits either way, and a forward with other noise, another seed, another
+    table or a white noise changed in place is not taken for the same one."""
+    n, L, a = 16, 32.0, 0.05
+    rng = np.random.default_rng(7)
+    white = O.ngenic_wnoise_real(42, n)
+    other = white + 1e-3 * rng.normal(size=white.shape)
+    pb, mb = rng.normal(size=(3, n, n, n)), rng.normal(size=(3, n, n, n))
+    oc = O.Cosmology(*COSMO)
+    k, pk = O.linear_power_spectrum(oc)
+    Dg = oc.Dplus(a)
+    with D.PMSolverHIP(D.PMConfig(res_pm=2 * n, n_part=n, boxsize=L), precision=64) as S:
+        # recomputed (no forward yet), then after the forward of the same noise (kept fields)
+        ref = S.vjp_ics_lpt(k, pk, Dg, pb, mb, white=white)
+        S.ics_lpt(k, pk, Dg, white=white)
+        assert np.array_equal(S.vjp_ics_lpt(k, pk, Dg, pb, mb, white=white), ref)
+        # a second reverse pass from the same forward
+        assert np.array_equal(S.vjp_ics_lpt(k, pk, Dg, pb, mb, white=white), ref)
+        # the forward of other noise: its fields must not serve the first noise
+        ref_o = S.vjp_ics_lpt(k, pk, Dg, pb, mb, white=other)
+        assert not np.array_equal(ref_o, ref)
+        S.ics_lpt(k, pk, Dg, white=other)
+        assert np.array_equal(S.vjp_ics_lpt(k, pk, Dg, pb, mb, white=white), ref)
+        assert np.array_equal(S.vjp_ics_lpt(k, pk, Dg, pb, mb, white=other), ref_o)
+        # another P(k) table
+        S.ics_lpt(k, pk, Dg, white=white)
+        assert not np.array_equal(S.vjp_ics_lpt(k, 1.01 * pk, Dg, pb, mb, white=white), ref)
+        # N-GenIC noise of a seed (generated on the device): the same seed hits, another seed does not
+        S.ics_lpt(k, pk, Dg, seed=43)
+        ref_s = S.vjp_ics_lpt(k, pk, Dg, pb, mb, seed=42)
+        S.ics_lpt(k, pk, Dg, seed=42)
+        assert np.array_equal(S.vjp_ics_lpt(k, pk, Dg, pb, mb, seed=42), ref_s)
+        S.ics_lpt(k, pk, Dg, seed=43)
+        assert np.array_equal(S.vjp_ics_lpt(k, pk, Dg, pb, mb, seed=42), ref_s)
+        assert G.rel(ref_s, ref) < 1e-9
+        # a forward from a device array: its values are the key, and the array changed in place and
+        # used again is another noise
+        wd = D.DeviceArray.from_numpy(white)
+        S.ics_lpt_device(k, pk, Dg, wd)
+        assert np.array_equal(S.vjp_ics_lpt(k, pk, Dg, pb, mb, white=white), ref)
+        wd.copy_from(other)
+        S.ics_lpt_device(k, pk, Dg, wd)
+        assert np.array_equal(S.vjp_ics_lpt(k, pk, Dg, pb, mb, white=white), ref)
+        assert np.array_equal(S.vjp_ics_lpt(k, pk, Dg, pb, mb, white=other), ref_o)
+        wd.free()
+
+
 def test_density_loss_gradient_directions():
     """The differentiation workload's check at reduced size: the gradient of sum (delta - target)^2 with
     respect to the white noise through 2LPT and 10 BullFrog CIC steps (grad order 4), against central
