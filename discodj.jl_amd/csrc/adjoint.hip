@@ -135,12 +135,17 @@ __global__ __launch_bounds__(256) void k_adj_div(const T *__restrict__ fbar, int
     phibar[e] = r;
 }
 
+// A cell's four values for the weight-gradient gather: rhobar and the three force components, one
+// vector load per stencil cell instead of four scalar ones
+template <typename T> struct alignas(4 * sizeof(T)) Cell4 { T r, f0, f1, f2; };
+
 // The three force meshes of the FD gradient, F_a(c) = -(c1 (phi(c + e_a) - phi(c - e_a)) + c2 (...) + c3 (...)),
-// summed in k_adj_gather's order, so that k_adj_gather_f over them gives k_adj_gather's values bit for bit
-// while each particle reads 4 W^3 mesh values instead of W^3 (1 + 6 H) (FD order 2 H)
+// summed in k_adj_gather's order and packed with rhobar into V[c] = {rhobar, F_0, F_1, F_2}, so that
+// k_adj_gather_v over them gives k_adj_gather's values bit for bit while each particle reads W^3 cells of
+// four values instead of W^3 (1 + 6 H) mesh values (FD order 2 H)
 template <typename T>
-__global__ __launch_bounds__(256) void k_fd_force3(const T *__restrict__ phi, int N, T c1, T c2, T c3,
-                                                   T *__restrict__ F) {
+__global__ __launch_bounds__(256) void k_fd_force3_packed(const T *__restrict__ phi, const T *__restrict__ rhobar,
+                                                          int N, T c1, T c2, T c3, Cell4<T> *__restrict__ V) {
     const size_t M = (size_t)N * N * N;
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= M) return;
@@ -155,9 +160,7 @@ __global__ __launch_bounds__(256) void k_fd_force3(const T *__restrict__ phi, in
         f1 += ct[t - 1] * (P(x, wr(y + t), z) - P(x, wr(y - t), z));
         f2 += ct[t - 1] * (P(x, y, wr(z + t)) - P(x, y, wr(z - t)));
     }
-    F[e] = -f0;
-    F[e + M] = -f1;
-    F[e + 2 * M] = -f2;
+    V[e] = Cell4<T>{rhobar[e], -f0, -f1, -f2};
 }
 
 // psibar_p += N/L sum_c grad W(X_p - c) (rhobar_c + sum_a abar_p,a s F_a(c)),
@@ -250,6 +253,48 @@ __global__ __launch_bounds__(256) void k_adj_gather_f(const T *__restrict__ psi,
 #pragma unroll
             for (int ox = 0; ox < W; ox++) {
                 const T v = vr[oy][ox] + (ab[0] * vf0[oy][ox] + ab[1] * vf1[oy][ox] + ab[2] * vf2[oy][ox]);
+                g0 += dw[0][ox] * w[1][oy] * w[2][oz] * v;
+                g1 += w[0][ox] * dw[1][oy] * w[2][oz] * v;
+                g2 += w[0][ox] * w[1][oy] * dw[2][oz] * v;
+            }
+    }
+    const T k = (T)N / L;
+    psibar[p] += k * g0;
+    psibar[p + n3] += k * g1;
+    psibar[p + 2 * n3] += k * g2;
+}
+
+// k_adj_gather_f over the packed cells of k_fd_force3_packed (the same values, the same sums)
+template <typename T, int W>
+__global__ __launch_bounds__(256) void k_adj_gather_v(const T *__restrict__ psi, const T *__restrict__ q1, int n, int N,
+                                                      T L, T sx, T sy, T sz, const Cell4<T> *__restrict__ V,
+                                                      const T *__restrict__ abar, T s, T *__restrict__ psibar) {
+    const size_t n3 = (size_t)n * n * n;
+    AdjPart q;
+    if (!adj_particle(n, q)) return;
+    const size_t p = q.p;
+    const T sh[3] = {sx, sy, sz};
+    int cell[3][W];
+    T w[3][W], dw[3][W];
+    adj_stencil<T, W>(psi, q1, n3, q, N, L, sh, cell, w, dw);
+    const T ab[3] = {abar[p] * s, abar[p + n3] * s, abar[p + 2 * n3] * s};
+    // a z plane's W^2 cells are read before any is used (cell offsets < N^3 <= 2^30)
+    T g0 = 0, g1 = 0, g2 = 0;
+#pragma unroll
+    for (int oz = 0; oz < W; oz++) {
+        Cell4<T> vc[W][W];
+#pragma unroll
+        for (int oy = 0; oy < W; oy++)
+#pragma unroll
+            for (int ox = 0; ox < W; ox++)
+                vc[oy][ox] = V[(unsigned)cell[0][ox] +
+                               (unsigned)N * ((unsigned)cell[1][oy] + (unsigned)N * (unsigned)cell[2][oz])];
+#pragma unroll
+        for (int oy = 0; oy < W; oy++)
+#pragma unroll
+            for (int ox = 0; ox < W; ox++) {
+                const Cell4<T> &u = vc[oy][ox];
+                const T v = u.r + (ab[0] * u.f0 + ab[1] * u.f1 + ab[2] * u.f2);
                 g0 += dw[0][ox] * w[1][oy] * w[2][oz] * v;
                 g1 += w[0][ox] * dw[1][oy] * w[2][oz] * v;
                 g2 += w[0][ox] * w[1][oy] * dw[2][oz] * v;
@@ -370,6 +415,21 @@ struct AdjLaunch {
         else ADJ_F(4);
 #undef ADJ_F
     }
+    static void gather_grad_v(const void *psi, const void *q1, int n, int N, double L, const double *sh, const void *V,
+                              const void *abar, double s, int W, void *psibar, hipStream_t st) {
+#define ADJ_V(WW)                                                                                                    \
+    k_adj_gather_v<T, WW><<<pgrid(n), pblock(n), 0, st>>>((const T *)psi, (const T *)q1, n, N, (T)L, (T)sh[0], (T)sh[1], \
+                                                     (T)sh[2], (const Cell4<T> *)V, (const T *)abar, (T)s, (T *)psibar)
+        if (W == 2) ADJ_V(2);
+        else if (W == 3) ADJ_V(3);
+        else ADJ_V(4);
+#undef ADJ_V
+    }
+    static void fd_force3_packed(const void *phi, const void *rhobar, int N, const double *c, void *V,
+                                 hipStream_t st) {
+        k_fd_force3_packed<T><<<agrid((size_t)N * N * N), 256, 0, st>>>((const T *)phi, (const T *)rhobar, N, (T)c[0],
+                                                                        (T)c[1], (T)c[2], (Cell4<T> *)V);
+    }
     static void axpy(void *y, const void *x, size_t n, double s, hipStream_t st) {
         k_adj_axpy<T><<<agrid(n), 256, 0, st>>>((T *)y, (const T *)x, n, (T)s);
     }
@@ -379,9 +439,6 @@ struct AdjLaunch {
     }
     static void drift(const void *psibar, void *pibar, size_t n, double dd1, hipStream_t st) {
         k_adj_drift<T><<<agrid(n), 256, 0, st>>>((const T *)psibar, (T *)pibar, n, (T)dd1);
-    }
-    static void fd_force3(const void *phi, int N, const double *c, void *F, hipStream_t st) {
-        k_fd_force3<T><<<agrid((size_t)N * N * N), 256, 0, st>>>((const T *)phi, N, (T)c[0], (T)c[1], (T)c[2], (T *)F);
     }
     static void mse(const void *mesh, const double *target, size_t n, double rho, void *mesh_bar, double *part,
                     hipStream_t st) {
@@ -399,7 +456,8 @@ template <typename T> void adjoint_ops(AdjointOps &ops) {
     ops.kick = AdjLaunch<T>::kick;
     ops.drift = AdjLaunch<T>::drift;
     ops.mse = AdjLaunch<T>::mse;
-    ops.fd_force3 = AdjLaunch<T>::fd_force3;
+    ops.fd_force3_packed = AdjLaunch<T>::fd_force3_packed;
+    ops.gather_grad_v = AdjLaunch<T>::gather_grad_v;
 }
 
 template void adjoint_ops<float>(AdjointOps &);

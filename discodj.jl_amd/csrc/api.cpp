@@ -94,7 +94,7 @@ struct dj_ctx {
     AdjointOps adj{};
     void *vb_psi1 = nullptr, *vb_abar = nullptr, *vb_psibar = nullptr, *vb_pibar = nullptr;
     void *vb_fbar = nullptr, *vb_pbar = nullptr;
-    void *vb_fik = nullptr;  // the three force meshes (spectral or FD gradient), recomputed in the reverse pass
+    void *vb_fik = nullptr;  // the three force meshes (spectral gradient) or the packed cells (FD), per reverse force
     bool vb_resident = false;  // vb_psibar / vb_pibar hold cotangents left resident by dj_vjp_deposit / dj_vjp_run
     double *loss_work = nullptr;  // block sums of the density loss (LOSS_PARTS doubles); the vector deposit's scales
     void *vb_pl = nullptr;  // (3, (n_resample n)^3): the cotangent of the resampled points' positions
@@ -1165,7 +1165,7 @@ dj_status adj_buffers(dj_ctx *c) {
         if (!*b) HIPCHK(c, hipMalloc(b, pb));
     if (!c->vb_fbar) HIPCHK(c, hipMalloc(&c->vb_fbar, 3 * mb));
     if (!c->vb_pbar) HIPCHK(c, hipMalloc(&c->vb_pbar, mb));
-    if (!c->vb_fik) HIPCHK(c, hipMalloc(&c->vb_fik, 3 * mb));
+    if (!c->vb_fik) HIPCHK(c, hipMalloc(&c->vb_fik, 4 * mb));  // three force meshes, or four packed values per cell
     if (!c->loss_work) HIPCHK(c, hipMalloc(&c->loss_work, sizeof(double) * LOSS_PARTS));
     return DJ_OK;
 }
@@ -1253,10 +1253,10 @@ dj_status vjp_force(dj_ctx *c, const void *psi, const void *abar, void *psibar) 
         c->fft.zm(ZM_POT, true, true, c->B, c->B, c->tw, c->kmult, c->st);
         c->fft.yi(c->B, c->A, c->tw, c->st);
         c->fft.xi(c->A, c->vb_pbar, c->tw, c->st);
-        // the forward's force meshes from phi, then the weight-gradient gather over them
-        c->adj.fd_force3(c->mesh, N, c->fdc, c->vb_fik, c->st);
-        c->adj.gather_grad_f(psi, c->q1, c->n, N, cfg.boxsize, c->shifts[s], c->vb_pbar, c->vb_fik, abar, 1.0 / ns,
-                             cfg.worder, psibar, c->st);
+        // the forward's force meshes from phi, packed with rhobar, then the weight-gradient gather over them
+        c->adj.fd_force3_packed(c->mesh, c->vb_pbar, N, c->fdc, c->vb_fik, c->st);
+        c->adj.gather_grad_v(psi, c->q1, c->n, N, cfg.boxsize, c->shifts[s], c->vb_fik, abar, 1.0 / ns, cfg.worder,
+                             psibar, c->st);
         if (cfg.n_resample > 1) {
             r = c->tsize == 8 ? resample_vjp<double>(c, c->vb_pbar, psibar)
                               : resample_vjp<float>(c, c->vb_pbar, psibar);

@@ -167,8 +167,12 @@ struct AdjointOps {
                  hipStream_t st);
     // first drift backward: Pibar += dd1 Psibar
     void (*drift)(const void *psibar, void *pibar, size_t n, double dd1, hipStream_t st);
-    // F_a = -D_a phi (FD coefficients c[3]) into F (3 N^3), summed as gather_grad sums them
-    void (*fd_force3)(const void *phi, int N, const double *c, void *F, hipStream_t st);
+    // F_a = -D_a phi (FD coefficients c[3]), summed as gather_grad sums them, packed with rhobar:
+    // V[c] = {rhobar, F_0, F_1, F_2} (4 N^3 values)
+    void (*fd_force3_packed)(const void *phi, const void *rhobar, int N, const double *c, void *V, hipStream_t st);
+    // gather_grad_f over the packed cells V of fd_force3_packed
+    void (*gather_grad_v)(const void *psi, const void *q1, int n, int N, double L, const double *sh, const void *V,
+                          const void *abar, double s, int W, void *psibar, hipStream_t st);
     // L = sum (mesh / rho - 1 - target)^2 into part[LOSS_PARTS - 1] (part: LOSS_PARTS doubles), mesh_bar = dL/dmesh;
     // target nullable (zero)
     void (*mse)(const void *mesh, const double *target, size_t n, double rho, void *mesh_bar, double *part,
