@@ -264,11 +264,14 @@ __global__ __launch_bounds__(256) void k_adj_gather_f(const T *__restrict__ psi,
     psibar[p + 2 * n3] += k * g2;
 }
 
-// k_adj_gather_f over the packed cells of k_fd_force3_packed (the same values, the same sums)
-template <typename T, int W>
+// k_adj_gather_f over the packed cells of k_fd_force3_packed (the same values, the same sums).  TAIL: the
+// step's reverse ends here for each particle, with k_adj_drift's update of pibar and, when kick, the
+// previous step's k_adj_kick (abar of that step into abar, pibar), each with their kernels' arithmetic
+template <typename T, int W, bool TAIL>
 __global__ __launch_bounds__(256) void k_adj_gather_v(const T *__restrict__ psi, const T *__restrict__ q1, int n, int N,
-                                                      T L, T sx, T sy, T sz, const Cell4<T> *__restrict__ V,
-                                                      const T *__restrict__ abar, T s, T *__restrict__ psibar) {
+                                                      T L, T sx, T sy, T sz, const Cell4<T> *__restrict__ V, T *abar,
+                                                      T s, T *__restrict__ psibar, T *__restrict__ pibar, T dd1,
+                                                      int kick, T alpha, T beta, T dd2) {
     const size_t n3 = (size_t)n * n * n;
     AdjPart q;
     if (!adj_particle(n, q)) return;
@@ -301,9 +304,26 @@ __global__ __launch_bounds__(256) void k_adj_gather_v(const T *__restrict__ psi,
             }
     }
     const T k = (T)N / L;
-    psibar[p] += k * g0;
-    psibar[p + n3] += k * g1;
-    psibar[p + 2 * n3] += k * g2;
+    if constexpr (!TAIL) {
+        psibar[p] += k * g0;
+        psibar[p + n3] += k * g1;
+        psibar[p + 2 * n3] += k * g2;
+    } else {
+        const T g[3] = {g0, g1, g2};
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const size_t i = p + c * n3;
+            const T ps = psibar[i] + k * g[c];
+            psibar[i] = ps;
+            T pb = pibar[i] + dd1 * ps;
+            if (kick) {
+                const T pt = pb + dd2 * ps;
+                abar[i] = beta * pt;
+                pb = alpha * pt;
+            }
+            pibar[i] = pb;
+        }
+    }
 }
 
 // y += s x
@@ -416,13 +436,22 @@ struct AdjLaunch {
 #undef ADJ_F
     }
     static void gather_grad_v(const void *psi, const void *q1, int n, int N, double L, const double *sh, const void *V,
-                              const void *abar, double s, int W, void *psibar, hipStream_t st) {
-#define ADJ_V(WW)                                                                                                    \
-    k_adj_gather_v<T, WW><<<pgrid(n), pblock(n), 0, st>>>((const T *)psi, (const T *)q1, n, N, (T)L, (T)sh[0], (T)sh[1], \
-                                                     (T)sh[2], (const Cell4<T> *)V, (const T *)abar, (T)s, (T *)psibar)
-        if (W == 2) ADJ_V(2);
-        else if (W == 3) ADJ_V(3);
-        else ADJ_V(4);
+                              void *abar, double s, int W, void *psibar, const AdjTail *tl, hipStream_t st) {
+        const AdjTail t0{};
+        const AdjTail &t = tl ? *tl : t0;
+#define ADJ_V(WW, TL)                                                                                                  \
+    k_adj_gather_v<T, WW, TL><<<pgrid(n), pblock(n), 0, st>>>(                                                         \
+        (const T *)psi, (const T *)q1, n, N, (T)L, (T)sh[0], (T)sh[1], (T)sh[2], (const Cell4<T> *)V, (T *)abar, (T)s, \
+        (T *)psibar, (T *)t.pibar, (T)t.dd1, t.kick, (T)t.alpha, (T)t.beta, (T)t.dd2)
+        if (tl) {
+            if (W == 2) ADJ_V(2, true);
+            else if (W == 3) ADJ_V(3, true);
+            else ADJ_V(4, true);
+        } else {
+            if (W == 2) ADJ_V(2, false);
+            else if (W == 3) ADJ_V(3, false);
+            else ADJ_V(4, false);
+        }
 #undef ADJ_V
     }
     static void fd_force3_packed(const void *phi, const void *rhobar, int N, const double *c, void *V,

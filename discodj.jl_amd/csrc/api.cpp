@@ -1173,7 +1173,11 @@ dj_status adj_buffers(dj_ctx *c) {
 // psibar += (d acc / d psi)^T abar at psi (device arrays), for the averaged interlacing passes: per pass the
 // potential phi is recomputed with the forward kernels, abar / ns is deposited to Fbar, phibar = sum_a D_a
 // Fbar_a goes through the same Poisson operator (self-adjoint), and one gather adds the weight-gradient terms.
-dj_status vjp_force(dj_ctx *c, const void *psi, const void *abar, void *psibar) {
+// tail (nullable): the step's end (AdjTail), applied in the last pass's gather when the path allows it
+// (FD gradient, no resampling), which *tail_done reports; abar is then overwritten.
+dj_status vjp_force(dj_ctx *c, const void *psi, void *abar, void *psibar, const AdjTail *tail = nullptr,
+                    bool *tail_done = nullptr) {
+    if (tail_done) *tail_done = false;
     const dj_pm_config &cfg = c->cfg;
     const bool fixed = cfg.deposit == DJ_DEPOSIT_FIXED64;
     const int ns = c->nshifts, N = c->res;
@@ -1255,8 +1259,10 @@ dj_status vjp_force(dj_ctx *c, const void *psi, const void *abar, void *psibar) 
         c->fft.xi(c->A, c->vb_pbar, c->tw, c->st);
         // the forward's force meshes from phi, packed with rhobar, then the weight-gradient gather over them
         c->adj.fd_force3_packed(c->mesh, c->vb_pbar, N, c->fdc, c->vb_fik, c->st);
+        const bool tl = tail && s == ns - 1 && cfg.n_resample <= 1;
         c->adj.gather_grad_v(psi, c->q1, c->n, N, cfg.boxsize, c->shifts[s], c->vb_fik, abar, 1.0 / ns, cfg.worder,
-                             psibar, c->st);
+                             psibar, tl ? tail : nullptr, c->st);
+        if (tl && tail_done) *tail_done = true;
         if (cfg.n_resample > 1) {
             r = c->tsize == 8 ? resample_vjp<double>(c, c->vb_pbar, psibar)
                               : resample_vjp<float>(c, c->vb_pbar, psibar);
@@ -1407,14 +1413,27 @@ dj_status dj_vjp_run(dj_ctx *c, void *psi_bar, void *pi_bar) {
         HIPCHK(c, hipMemcpyAsync(c->vb_psibar, psi_bar, pb, hipMemcpyHostToDevice, c->st));
         HIPCHK(c, hipMemcpyAsync(c->vb_pibar, pi_bar, pb, hipMemcpyHostToDevice, c->st));
     }
+    bool kicked = false;  // the step's kick backward already made at the end of the later step's reverse
     for (int s = c->ck_steps - 1; s >= 0; s--) {
         const char *ck = (const char *)c->ckpt + 2 * pb * (size_t)s;
         // the drifted positions of the step's force, recomputed from its checkpoint
         c->part.drift_to(c->vb_psi1, ck, ck + pb, nullptr, nullptr, c->n3, c->ck_dd1[s], c->st);
-        c->adj.kick(c->vb_psibar, c->vb_pibar, c->vb_abar, np3, c->ck_alpha[s], c->ck_beta[s], c->ck_dd2[s], c->st);
-        r = vjp_force(c, c->vb_psi1, c->vb_abar, c->vb_psibar);
+        if (!kicked)
+            c->adj.kick(c->vb_psibar, c->vb_pibar, c->vb_abar, np3, c->ck_alpha[s], c->ck_beta[s], c->ck_dd2[s], c->st);
+        AdjTail tail;
+        tail.pibar = c->vb_pibar;
+        tail.dd1 = c->ck_dd1[s];
+        tail.kick = s > 0;
+        if (s > 0) {
+            tail.alpha = c->ck_alpha[s - 1];
+            tail.beta = c->ck_beta[s - 1];
+            tail.dd2 = c->ck_dd2[s - 1];
+        }
+        bool done = false;
+        r = vjp_force(c, c->vb_psi1, c->vb_abar, c->vb_psibar, &tail, &done);
         if (r) return r;
-        c->adj.drift(c->vb_psibar, c->vb_pibar, np3, c->ck_dd1[s], c->st);
+        if (!done) c->adj.drift(c->vb_psibar, c->vb_pibar, np3, c->ck_dd1[s], c->st);
+        kicked = done && s > 0;
     }
     r = check_launch(c, "run adjoint");
     if (r) return r;

@@ -147,6 +147,14 @@ struct ResampleOps {
 template <typename T> void resample_ops(ResampleOps &ops);
 
 // reverse mode of the force and the DKD step (adjoint.hip); particle arrays (3, n^3), meshes N^3
+// The end of a DKD step's reverse pass after its force: Pibar += dd1 Psibar (first drift backward), then,
+// when kick, the previous step's kick backward with its alpha, beta, dd2 (abar of that step, Pibar)
+struct AdjTail {
+    void *pibar = nullptr;
+    double dd1 = 0;
+    int kick = 0;
+    double alpha = 0, beta = 0, dd2 = 0;
+};
 struct AdjointOps {
     // Fbar_a += sum_p W(X_p - c) abar_p,a s   (3 meshes)
     void (*scatter3)(const void *psi, const void *q1, int n, int N, double L, const double *sh, const void *abar,
@@ -170,9 +178,10 @@ struct AdjointOps {
     // F_a = -D_a phi (FD coefficients c[3]), summed as gather_grad sums them, packed with rhobar:
     // V[c] = {rhobar, F_0, F_1, F_2} (4 N^3 values)
     void (*fd_force3_packed)(const void *phi, const void *rhobar, int N, const double *c, void *V, hipStream_t st);
-    // gather_grad_f over the packed cells V of fd_force3_packed
+    // gather_grad_f over the packed cells V of fd_force3_packed; with tail (nullable), the step's drift and
+    // the previous step's kick backward per particle (AdjTail), abar then overwritten with that step's
     void (*gather_grad_v)(const void *psi, const void *q1, int n, int N, double L, const double *sh, const void *V,
-                          const void *abar, double s, int W, void *psibar, hipStream_t st);
+                          void *abar, double s, int W, void *psibar, const AdjTail *tail, hipStream_t st);
     // L = sum (mesh / rho - 1 - target)^2 into part[LOSS_PARTS - 1] (part: LOSS_PARTS doubles), mesh_bar = dL/dmesh;
     // target nullable (zero)
     void (*mse)(const void *mesh, const double *target, size_t n, double rho, void *mesh_bar, double *part,
