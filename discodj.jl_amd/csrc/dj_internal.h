@@ -12,6 +12,9 @@ namespace dj {
 // FFT passes of one mesh size and precision (fft.hip)
 struct FftOps {
     void (*xf)(const void *mesh, bool fixed, void *A, const void *tw, double fscale, hipStream_t st);
+    // FP64: xf of the real field -(mub a) (neg) or mub (a + b) (b nullable), formed as the pass loads it
+    void (*xf_partner)(const double *mub, const double *a, const double *b, int neg, void *A, const void *tw,
+                       hipStream_t st);
     void (*xi)(const void *A, void *mesh, const void *tw, hipStream_t st);
     // the same with the output multiplied by s as it is stored
     void (*xi_scaled)(const void *A, void *mesh, const void *tw, double s, hipStream_t st);
@@ -250,7 +253,7 @@ struct IcWork {
     void *twn = nullptr, *twe = nullptr, *kt = nullptr, *pt = nullptr;
     void *A = nullptr, *Bm = nullptr, *F = nullptr, *Fb = nullptr, *T1 = nullptr, *fLb = nullptr;
     void *w = nullptr, *p1 = nullptr, *p2 = nullptr;
-    void *Ae = nullptr, *Be = nullptr, *mu = nullptr, *ab = nullptr, *fld[3] = {nullptr, nullptr, nullptr};
+    void *Ae = nullptr, *Be = nullptr, *mu = nullptr, *fld[3] = {nullptr, nullptr, nullptr};
     // The nine second-order fields A_ik of the last forward (ext grid), kept for the reverse pass of the
     // same ICs, and what they were computed from: n, L, order, the P(k) table, and the seed or (wkey) a
     // copy of the white noise.  fk_room: 0 not tried, 1 allocated, -1 no room (the forward then uses fld).

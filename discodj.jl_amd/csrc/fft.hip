@@ -85,6 +85,42 @@ __global__ __launch_bounds__(256) void k_xf(const IN *__restrict__ mesh, cx<T> *
     P::xf_store(lds, t, A, yb, z, threadIdx.x);
 }
 
+// The r2c x pass of a real field computed as it is loaded (src(g): the reals g, g + 1 as a complex)
+template <typename T, int N, typename SRC>
+__global__ __launch_bounds__(256) void k_xf_src(SRC src, cx<T> *__restrict__ A, const cx<T> *__restrict__ tw) {
+    using S = FftShape<T, N>;
+    using P = typename S::P;
+    __shared__ cx<T> lds[P::EX::LDS_ELEMS];
+    constexpr bool TWX = sizeof(T) == 4;
+    __shared__ cx<T> twl[TWX ? N : 1];
+    const cx<T> *t = stage_twiddles<T, N, TWX>(tw, twl, threadIdx.x, 256);
+    const int lb = xcd_block(blockIdx.x, gridDim.x), yb = lb % (N / S::B), z = lb / (N / S::B);
+    if constexpr (P::EX::wave_ok()) {
+        P::xf_load_src_w(src, lds, yb, z, threadIdx.x);
+        if constexpr (TWX) __syncthreads();
+        else P::EX::wave_sync();
+        P::EX::template run_wave<-1, 2>(lds, t, threadIdx.x);
+        __syncthreads();
+    } else {
+        P::xf_load_src(src, lds, yb, z, threadIdx.x);
+        __syncthreads();
+        P::EX::template run<-1, 2>(lds, t, threadIdx.x);
+    }
+    P::xf_store(lds, t, A, yb, z, threadIdx.x);
+}
+
+// The 2LPT source's partner product of the IC reverse pass, -(mubar a) (neg) or mubar (a + b) (b non-null),
+// as a source of k_xf_src: the values k_ic_partner writes, without the field in memory
+struct PartnerSrc {
+    const double *mub, *a, *b;
+    int neg;
+    __device__ double at(size_t i) const {
+        const double p = b ? a[i] + b[i] : a[i];
+        return neg ? -(mub[i] * p) : mub[i] * p;
+    }
+    __device__ cx<double> operator()(size_t g) const { return cx<double>{at(g), at(g + 1)}; }
+};
+
 // SC: the output multiplied by s as it is stored
 template <typename T, int N, bool SC = false>
 __global__ __launch_bounds__((FftShape<T, N>::NTXI)) void k_xi(const cx<T> *__restrict__ A, T *__restrict__ mesh,
@@ -302,6 +338,12 @@ struct FftN {
         else
             k_xf<T, N, T><<<P::XBLOCKS(), 256, 0, st>>>((const T *)mesh, (cx<T> *)A, (const cx<T> *)tw, fscale);
     }
+    static void xf_partner(const double *mub, const double *a, const double *b, int neg, void *A, const void *tw,
+                           hipStream_t st) {
+        if constexpr (sizeof(T) == 8)
+            k_xf_src<T, N, PartnerSrc><<<P::XBLOCKS(), 256, 0, st>>>(PartnerSrc{mub, a, b, neg}, (cx<T> *)A,
+                                                                     (const cx<T> *)tw);
+    }
     static void xi(const void *A, void *mesh, const void *tw, hipStream_t st) {
         using PX = typename S::PXI;
         k_xi<T, N><<<PX::XBLOCKS(), S::NTXI, 0, st>>>((const cx<T> *)A, (T *)mesh, (const cx<T> *)tw);
@@ -350,6 +392,7 @@ bool fft_dispatch(int N, FftOps &ops) {
         ops.xf = FftN<T, n>::xf;        \
         ops.xi = FftN<T, n>::xi;        \
         ops.xi_scaled = FftN<T, n>::xi_scaled; \
+        ops.xf_partner = FftN<T, n>::xf_partner; \
         ops.yf = FftN<T, n>::yf;        \
         ops.yi = FftN<T, n>::yi;        \
         ops.zm = FftN<T, n>::zm;        \

@@ -131,6 +131,35 @@ struct Passes {
                 lds[EX::idx(l, m)] = v;
             });
     }
+    // the same loads from a computed real field: src(g) returns the pair of reals (g, g + 1) as a complex
+    template <typename SRC>
+    DJ_HD static void xf_load_src(const SRC &src, cx<T> *lds, int yb, int z, int tid) {
+        const size_t base = (size_t)N * N * z + (size_t)N * yb * B;
+        copy_batched<cx<T>, B * NC, NT>(
+            tid,
+            [&](int i) {
+                const int l = i / NC, m = i - l * NC;
+                return src(base + (size_t)N * l + 2 * m);
+            },
+            [&](int i, cx<T> v) {
+                const int l = i / NC, m = i - l * NC;
+                lds[EX::idx(l, m)] = v;
+            });
+    }
+    template <typename SRC>
+    DJ_HD static void xf_load_src_w(const SRC &src, cx<T> *lds, int yb, int z, int tid) {
+        const size_t base = (size_t)N * N * z + (size_t)N * yb * B;
+        copy_wave<cx<T>, EX::LW * NC>(
+            tid >> 6, tid & 63,
+            [&](int i) {
+                const int l = i / NC, m = i - l * NC;
+                return src(base + (size_t)N * l + 2 * m);
+            },
+            [&](int i, cx<T> v) {
+                const int l = i / NC, m = i - l * NC;
+                lds[EX::idx(l, m)] = v;
+            });
+    }
     // X[k] = E[k] + W_N^k O[k], E = (Z[k] + conj Z[NC-k]) / 2, O = (Z[k] - conj Z[NC-k]) / 2i
     DJ_HD static void xf_store(const cx<T> *lds, const cx<T> *twN, cx<T> *A, int yb, int z, int tid) {
         copy_batched<cx<T>, B * H, NT>(

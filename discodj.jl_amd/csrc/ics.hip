@@ -182,6 +182,23 @@ __global__ void k_ic_mu(double *mu, const double *a, const double *b, size_t n, 
     mu[i] = op == 0 ? p : op == 1 ? mu[i] + p : op == 2 ? mu[i] - p : (mu[i] - p) * s;
 }
 
+// mu from the nine A_ik in one pass (A11, A22, A33, A12, A21, A13, A31, A23, A32): the products and sums of
+// the six k_ic_mu calls in their order, (A11 A22 + A11 A33 + A22 A33 - A12 A21 - A13 A31 - A23 A32) s
+struct IcFields9 {
+    const double *f[9];
+};
+__global__ void k_ic_mu9(double *__restrict__ mu, IcFields9 A, size_t n, double s) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double a11 = A.f[0][i], a22 = A.f[1][i], a33 = A.f[2][i];
+    double m = a11 * a22;
+    m = m + a11 * a33;
+    m = m + a22 * a33;
+    m = m - A.f[3][i] * A.f[4][i];
+    m = m - A.f[5][i] * A.f[6][i];
+    mu[i] = (m - A.f[7][i] * A.f[8][i]) * s;
+}
+
 template <typename T>
 __global__ void k_ic_combine(const double *p1, const double *p2, T *psi, T *pi, size_t n3, double D, int order) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -209,13 +226,6 @@ __global__ void k_ic_halfw(cx<double> *B, int N, int divide, double s) {
 __global__ void k_ic_axpy(cx<double> *out, const cx<double> *in, size_t n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) out[i] = {out[i].x + in[i].x, out[i].y + in[i].y};
-}
-// Abar = mubar * (a + b) (b may be null), or -mubar * a when neg
-__global__ void k_ic_partner(double *out, const double *mubar, const double *a, const double *b, size_t n, int neg) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double p = b ? a[i] + b[i] : a[i];
-    out[i] = neg ? -(mubar[i] * p) : mubar[i] * p;
 }
 // (Psibar, Pibar) -> (psi1bar, psi2bar) of Psi = D psi1 + D^2 psi2, Pi = psi1 + 2 D psi2
 template <typename T>
@@ -407,7 +417,6 @@ static hipError_t ic_work_ensure(IcWork &ws, int n, int order, int ntab) {
         ws.Ae = alloc(HE * 16);
         ws.Be = alloc(HE * 16);
         ws.mu = alloc(ME * 8);
-        ws.ab = alloc(ME * 8);
         for (auto &f : ws.fld) f = alloc(ME * 8);
     }
     if (err != hipSuccess) {
@@ -527,19 +536,31 @@ int ics_lpt_device(IcWork &ws, int n, double L, int order, const double *white, 
             inv_s(fe, Be, Ae, out, twe, 1.0 / (double)ME);
         };
         // mu = A11 A22 + A11 A33 + A22 A33 - A12 A21 - A13 A31 - A23 A32 (evaluated left to right)
-        deriv(0, 0, fa[0]);
-        deriv(1, 1, fa[1]);
-        deriv(2, 2, fa[2]);
-        k_ic_mu<<<g1(ME), 256, 0, st>>>(mu, fa[0], fa[1], ME, 0);
-        k_ic_mu<<<g1(ME), 256, 0, st>>>(mu, fa[0], fa[2], ME, 1);
-        k_ic_mu<<<g1(ME), 256, 0, st>>>(mu, fa[1], fa[2], ME, 1);
         // the last product also applies the order-2 factor ((3-2)/2 - 2) / ((2 + 3/2)(2 - 1)) = -3/7
+        const double f2 = ((3.0 - 2) / 2 - 2.0) / ((2 + 1.5) * 1.0);
         const int pairs[3][2][2] = {{{0, 1}, {1, 0}}, {{0, 2}, {2, 0}}, {{1, 2}, {2, 1}}};
-        for (int q = 0; q < 3; q++) {
-            deriv(pairs[q][0][0], pairs[q][0][1], fa[3 + 2 * q]);
-            deriv(pairs[q][1][0], pairs[q][1][1], fa[4 + 2 * q]);
-            k_ic_mu<<<g1(ME), 256, 0, st>>>(mu, fa[3 + 2 * q], fa[4 + 2 * q], ME, q < 2 ? 2 : 3,
-                                            ((3.0 - 2) / 2 - 2.0) / ((2 + 1.5) * 1.0));
+        if (keep) {
+            // all nine fields first, then mu in one pass
+            IcFields9 A9;
+            for (int d = 0; d < 3; d++) deriv(d, d, fa[d]);
+            for (int q = 0; q < 3; q++) {
+                deriv(pairs[q][0][0], pairs[q][0][1], fa[3 + 2 * q]);
+                deriv(pairs[q][1][0], pairs[q][1][1], fa[4 + 2 * q]);
+            }
+            for (int j = 0; j < 9; j++) A9.f[j] = fa[j];
+            k_ic_mu9<<<g1(ME), 256, 0, st>>>(mu, A9, ME, f2);
+        } else {
+            deriv(0, 0, fa[0]);
+            deriv(1, 1, fa[1]);
+            deriv(2, 2, fa[2]);
+            k_ic_mu<<<g1(ME), 256, 0, st>>>(mu, fa[0], fa[1], ME, 0);
+            k_ic_mu<<<g1(ME), 256, 0, st>>>(mu, fa[0], fa[2], ME, 1);
+            k_ic_mu<<<g1(ME), 256, 0, st>>>(mu, fa[1], fa[2], ME, 1);
+            for (int q = 0; q < 3; q++) {
+                deriv(pairs[q][0][0], pairs[q][0][1], fa[3 + 2 * q]);
+                deriv(pairs[q][1][0], pairs[q][1][1], fa[4 + 2 * q]);
+                k_ic_mu<<<g1(ME), 256, 0, st>>>(mu, fa[3 + 2 * q], fa[4 + 2 * q], ME, q < 2 ? 2 : 3, f2);
+            }
         }
         fwd(fe, mu, Ae, Be, twe);
         k_ic_crop<<<g1(HC), 256, 0, st>>>(Be, A, n, ext, crop_s);  // fL in A
@@ -648,7 +669,7 @@ int ics_lpt_vjp_device(IcWork &ws, int n, double L, int order, const double *whi
     if (order >= 2) {
         void *twe = ws.twe;
         cx<double> *Ae = (cx<double> *)ws.Ae, *Be = (cx<double> *)ws.Be;
-        double *mub = (double *)ws.mu, *ab = (double *)ws.ab;
+        double *mub = (double *)ws.mu;
         double *fld[3] = {(double *)ws.fld[0], (double *)ws.fld[1], (double *)ws.fld[2]};
         cx<double> *fLb = (cx<double> *)ws.fLb;
         const double pad_s = std::pow((double)ext / n, 3), crop_s = std::pow((double)n / ext, 3);
@@ -670,8 +691,11 @@ int ics_lpt_vjp_device(IcWork &ws, int n, double L, int order, const double *whi
             inv_s(fe, Be, Ae, out, twe, 1.0 / (double)ME);
         };
         // Abar_ik -> phibar: w r2c_ext(Abar) / ME, conj(i k_k), crop pad_s, conj(-(i k_i))
-        auto deriv_adj = [&](int i, int k, double *abar) {
-            fwd(fe, abar, Ae, Be, twe);
+        // with the field Abar_ik = -(mubar a) (neg) or mubar (a + b) formed by the transform as it loads it
+        auto deriv_adj = [&](int i, int k, const double *a, const double *b, int neg) {
+            fe.xf_partner(mub, a, b, neg, Ae, twe, st);
+            fe.yf(Ae, Be, twe, st);
+            fe.zm(ZM_NONE, true, false, Be, Be, twe, &none, st);
             k_ic_deriv_adj<<<g1(HC), 256, 0, st>>>(Be, Fb, n, ext, L, i, k, 1.0 / (double)ME, pad_s);
         };
         // field j of the forward's order (A11, A22, A33, A12, A21, A13, A31, A23, A32): kept, or recomputed
@@ -684,19 +708,13 @@ int ics_lpt_vjp_device(IcWork &ws, int n, double L, int order, const double *whi
         // mu = A11 A22 + A11 A33 + A22 A33 - A12 A21 - A13 A31 - A23 A32
         const double *dg[3];
         for (int d = 0; d < 3; d++) dg[d] = field(d, d, d, fld[d]);
-        for (int d = 0; d < 3; d++) {
-            const double *o1 = dg[(d + 1) % 3], *o2 = dg[(d + 2) % 3];
-            k_ic_partner<<<g1(ME), 256, 0, st>>>(ab, mub, o1, o2, ME, 0);
-            deriv_adj(d, d, ab);
-        }
+        for (int d = 0; d < 3; d++) deriv_adj(d, d, dg[(d + 1) % 3], dg[(d + 2) % 3], 0);
         const int pairs[3][2] = {{0, 1}, {0, 2}, {1, 2}};
         for (int q = 0; q < 3; q++) {
             const int i = pairs[q][0], k = pairs[q][1];
             const double *aik = field(3 + 2 * q, i, k, fld[0]), *aki = field(4 + 2 * q, k, i, fld[1]);
-            k_ic_partner<<<g1(ME), 256, 0, st>>>(ab, mub, aki, nullptr, ME, 1);
-            deriv_adj(i, k, ab);
-            k_ic_partner<<<g1(ME), 256, 0, st>>>(ab, mub, aik, nullptr, ME, 1);
-            deriv_adj(k, i, ab);
+            deriv_adj(i, k, aki, nullptr, 1);
+            deriv_adj(k, i, aik, nullptr, 1);
         }
     }
     // phi = r2c(w) s(k):  wbar = c2r(phibar s / w)
