@@ -111,6 +111,7 @@ struct dj_ctx {
     void *nn_pf = nullptr, *nn_pb = nullptr;  // (layers, 4, 64) x 8 bytes, packed for two-channel inputs
     void *nn_t[10] = {};                      // z1 z2 z3 h1 h2 h3 and four gradient tensors (N^3 x 16 BF16)
     void *nn_dphi = nullptr, *nn_dens = nullptr, *nn_tgt = nullptr, *nn_sum = nullptr;
+    void *nn_wpart = nullptr;  // the weight gradient's per-block partial sums (NnOps::wgrad_scratch_bytes)
     void *acc_list = nullptr;  // (entries, 3): the acceleration between interlacing passes, in tile-list order
     unsigned *own_at = nullptr;  // (Np): each particle's entry in its own tile list (written by the bin's fill)
     size_t acc_list_cap = 0;
@@ -345,7 +346,7 @@ void free_all(dj_ctx *c) {
                     c->nn_p, c->nn_g, c->nn_m, c->nn_v, c->nn_mask, c->nn_wf, c->nn_wb, c->nn_pf, c->nn_pb,
                     c->nn_t[0], c->nn_t[1],
                     c->nn_t[2], c->nn_t[3], c->nn_t[4], c->nn_t[5], c->nn_t[6], c->nn_t[7], c->nn_t[8], c->nn_t[9],
-                    c->nn_dphi, c->nn_dens, c->nn_tgt, c->nn_sum, c->acc_list, c->kgain, c->pk_work,
+                    c->nn_dphi, c->nn_dens, c->nn_tgt, c->nn_sum, c->nn_wpart, c->acc_list, c->kgain, c->pk_work,
                     c->pg_tw, c->pg_A, c->pg_B, c->loss_work, c->vb_pl, c->own_at};
     // teardown: failures here have no caller to report to
     for (void *b : bufs)
@@ -1515,7 +1516,8 @@ dj_status nn_alloc(dj_ctx *c) {
     HIPCHK(c, hipMalloc(&c->nn_dphi, c->M * 4));
     HIPCHK(c, hipMalloc(&c->nn_dens, c->M * 4));
     HIPCHK(c, hipMalloc(&c->nn_tgt, (size_t)c->n3 * 4));
-    HIPCHK(c, hipMalloc(&c->nn_sum, sizeof(double)));
+    HIPCHK(c, hipMalloc(&c->nn_sum, sizeof(double) * c->nn.residual_sum_doubles()));
+    HIPCHK(c, hipMalloc(&c->nn_wpart, c->nn.wgrad_scratch_bytes()));
     // the real parameters: layer 1 has inputs 0, 1; layer 4 has output 0
     std::vector<float> mask((size_t)NN_LAYERS * NN_L, 0.0f);
     for (int l = 0; l < NN_LAYERS; l++)
@@ -1562,19 +1564,19 @@ dj_status nn_backward_dev(dj_ctx *c, const void *gphi, const void *dens, double 
     HIPCHK(c, hipMemsetAsync(c->nn_g, 0, sizeof(float) * NN_LAYERS * NN_L, c->st));
     float *g3 = nn_layer(c->nn_g, 3), *g2 = nn_layer(c->nn_g, 2), *g1 = nn_layer(c->nn_g, 1), *g0 = nn_layer(c->nn_g, 0);
     // layer 4: dphi = os (W4 * h3 + b4)
-    c->nn.wgrad(0, 1, N, t[5], nullptr, 0.0, 0.0, nullptr, gphi, os, g3, g3 + NN_W, c->st);
+    c->nn.wgrad(0, 1, N, t[5], nullptr, 0.0, 0.0, nullptr, gphi, os, g3, g3 + NN_W, c->nn_wpart, c->st);
     c->nn.conv(2, 2, N, nullptr, gphi, os, 0.0, nn_pop(c->nn_pb, 3), nullptr, nullptr, t[2], t[7], t[6], nullptr, 0.0,
                c->st);  // gh3 (t6), gz3 (t7)
     // layer 3: h3 = h2 + relu(z3)
-    c->nn.wgrad(0, 0, N, t[4], nullptr, 0.0, 0.0, t[7], nullptr, 0.0, g2, g2 + NN_W, c->st);
+    c->nn.wgrad(0, 0, N, t[4], nullptr, 0.0, 0.0, t[7], nullptr, 0.0, g2, g2 + NN_W, c->nn_wpart, c->st);
     c->nn.conv(0, 2, N, t[7], nullptr, 0.0, 0.0, nn_op(c->nn_wb, 2), nullptr, t[6], t[1], t[9], t[8], nullptr, 0.0,
                c->st);  // gh2 (t8), gz2 (t9)
     // layer 2: h2 = h1 + relu(z2)
-    c->nn.wgrad(0, 0, N, t[3], nullptr, 0.0, 0.0, t[9], nullptr, 0.0, g1, g1 + NN_W, c->st);
+    c->nn.wgrad(0, 0, N, t[3], nullptr, 0.0, 0.0, t[9], nullptr, 0.0, g1, g1 + NN_W, c->nn_wpart, c->st);
     c->nn.conv(0, 2, N, t[9], nullptr, 0.0, 0.0, nn_op(c->nn_wb, 1), nullptr, t[8], t[0], t[7], t[6], nullptr, 0.0,
                c->st);  // gh1 (t6), gz1 (t7)
     // layer 1: h1 = relu(z1), input (log(1 + delta), a)
-    c->nn.wgrad(1, 0, N, nullptr, dens, inv_rho, a, t[7], nullptr, 0.0, g0, g0 + NN_W, c->st);
+    c->nn.wgrad(1, 0, N, nullptr, dens, inv_rho, a, t[7], nullptr, 0.0, g0, g0 + NN_W, c->nn_wpart, c->st);
     return check_launch(c, "learned correction backward");
 }
 
@@ -1712,13 +1714,19 @@ dj_status dj_nn_train_step(dj_ctx *c, const void *psi, const void *acc_target, d
     c->nn_on = on;
     if (r) return r;
     // loss = weight x mean over components of (acc - target)^2, abar = dL / d acc
-    HIPCHK(c, hipMemsetAsync(c->nn_sum, 0, sizeof(double), c->st));
     c->nn.residual(c->acc, c->nn_tgt, c->vb_abar, (size_t)c->n3, 2.0 * weight / (double)c->n3, c->nn_sum, c->st);
-    // dL / dphi: the gather's transpose (deposit of abar) and the FD stencil's transpose
+    // dL / dphi: the gather's transpose (deposit of abar) and the FD stencil's transpose.  The deposit runs
+    // on the snapshot's tile lists in fixed point (order-free sums, so a training run repeats bit for bit),
+    // or with device atomics on a single-tile axis
     const double zero[3] = {0, 0, 0};
-    HIPCHK(c, hipMemsetAsync(c->vb_fbar, 0, 3 * c->M * 4, c->st));
-    c->adj.scatter3(c->vb_psi1, c->q1, c->n, c->res, c->cfg.boxsize, zero, c->vb_abar, 1.0, c->cfg.worder, c->vb_fbar,
-                    c->st);
+    r = tile_sort(c, c->vb_psi1, c->d_shifts, c->nshifts, c->cfg.worder);
+    if (r) return r;
+    TileArgs ta = tile_args(c, c->vb_psi1, c->d_shifts, c->nshifts);
+    if (!c->tiles.deposit3(ta, c->cfg.worder, zero, c->vb_abar, 1.0, c->vb_fbar, c->loss_work, c->st)) {
+        c->part.zero(c->vb_fbar, 3 * c->M * 4, c->st);
+        c->adj.scatter3(c->vb_psi1, c->q1, c->n, c->res, c->cfg.boxsize, zero, c->vb_abar, 1.0, c->cfg.worder,
+                        c->vb_fbar, c->st);
+    }
     c->adj.divergence(c->vb_fbar, c->res, c->fdc, c->vb_pbar, c->st);
     r = nn_backward_dev(c, c->vb_pbar, c->nn_dens, a);
     if (r) return r;
@@ -1727,7 +1735,8 @@ dj_status dj_nn_train_step(dj_ctx *c, const void *psi, const void *acc_target, d
                1.0 - std::pow(b1, step), 1.0 - std::pow(b2, step), c->st);
     nn_swizzle_all(c);
     double sum = 0;
-    HIPCHK(c, hipMemcpyAsync(&sum, c->nn_sum, sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(&sum, (double *)c->nn_sum + (c->nn.residual_sum_doubles() - 1), sizeof(double),
+                             hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     if (loss_out) *loss_out = weight * sum / (double)c->n3;
     return check_launch(c, "learned correction training step");

@@ -200,15 +200,19 @@ struct NnOps {
     void (*conv)(int in, int epi, int N, const void *hin, const void *fin, double inv_rho, double cval, const void *wsw,
                  const void *bias, const void *res, const void *zmask, void *out_z, void *out_h, void *out_f,
                  double oscale, hipStream_t st);
-    // dW[27][16][16] += sum_v gz(v) x(v + off)^T, db += sum_v gz(v)  (ga: gz is a 1-channel field gf * gscale)
+    // dW[27][16][16] += sum_v gz(v) x(v + off)^T, db += sum_v gz(v)  (ga: gz is a 1-channel field gf * gscale),
+    // summed in a fixed order through part (wgrad_scratch_bytes())
     void (*wgrad)(int in, int ga, int N, const void *hin, const void *fin, double inv_rho, double cval, const void *gz,
-                  const void *gf, double gscale, void *dW, void *db, hipStream_t st);
+                  const void *gf, double gscale, void *dW, void *db, void *part, hipStream_t st);
+    size_t (*wgrad_scratch_bytes)();
     // lane-ordered BF16 operands of a layer: per tap (fwd, bwd) and packed for two-channel inputs
     void (*swizzle)(const void *W, void *fwd, void *bwd, void *pfwd, void *pbwd, hipStream_t st);
     void (*adam)(void *p, const void *g, void *m, void *v, const void *mask, int n, double lr, double b1, double b2,
                  double eps, double c1, double c2, hipStream_t st);
     void (*axpy)(void *out, const void *in, size_t n, double s, hipStream_t st);
+    // sum: residual_sum_doubles() doubles, the total of r^2 in the last one (summed in a fixed order)
     void (*residual)(const void *acc, const void *tgt, void *abar, size_t n, double gs, void *sum, hipStream_t st);
+    size_t (*residual_sum_doubles)();
 };
 void nn_ops(NnOps &ops);
 

@@ -199,27 +199,34 @@ __global__ __launch_bounds__(256, 2) void k_nn_conv(int N, const unsigned short 
     }
 }
 
+// One block's partial weight and bias gradients (27 x 16 x 16 + 16 floats), and the slices of blocks the
+// first reduction pass sums
+constexpr int NN_WP = 27 * 16 * 16 + 16, NN_WSLICES = 16, NN_WBLOCKS = 2048;
+
 // Weight gradient dW[t][co][ci] += sum_v gz(v, co) x(v + off(t), ci) and db[co] += sum_v gz(v, co).
 // A block takes rows of 64 voxels (x chunks), stages gz (64 x 16) and the 9 neighbouring input rows
 // (66 x 16 each, periodic) in LDS, and each of its 4 waves accumulates the MFMAs of taps w, w+4, ...
-// over the voxels (K = 16 voxels per MFMA) in registers; the sums go to global memory once per block.
+// over the voxels (K = 16 voxels per MFMA) in registers.  Each block writes its sums to its row of part,
+// and k_nn_wreduce1/2 add the rows in a fixed order: no atomics, so a training run repeats bit for bit.
 template <int IN, int GA>
 __global__ __launch_bounds__(256) void k_nn_wgrad(int N, const unsigned short *__restrict__ hin, const float *__restrict__ fin,
                                                   float inv_rho, float cval, const unsigned short *__restrict__ gz,
-                                                  const float *__restrict__ gf, float gscale, float *__restrict__ dW,
-                                                  float *__restrict__ db) {
+                                                  const float *__restrict__ gf, float gscale, float *__restrict__ part) {
     __shared__ unsigned short sA[64 * 16];
     __shared__ unsigned short sB[9 * 66 * 16];
-    __shared__ float sdb[16];
+    __shared__ float sdbp[4][16];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int chunks = N / 64, nrows = chunks * N * N;
-    if (threadIdx.x < 16) sdb[threadIdx.x] = 0;
+    float dbacc = 0;  // threads < 16: channel threadIdx.x's bias gradient over the block's rows, in row order
     f4v acc[7];
 #pragma unroll
     for (int i = 0; i < 7; i++) acc[i] = {0, 0, 0, 0};
     for (int r = blockIdx.x; r < nrows; r += gridDim.x) {
         const int x0 = (r % chunks) * 64, y = (r / chunks) % N, z = r / (chunks * N);
         __syncthreads();  // previous row's LDS reads are done
+        // the previous row's four partial bias sums, in order (written before that row's MFMAs)
+        if (r != (int)blockIdx.x && threadIdx.x < 16)
+            dbacc += ((sdbp[0][threadIdx.x] + sdbp[1][threadIdx.x]) + sdbp[2][threadIdx.x]) + sdbp[3][threadIdx.x];
         // gz of the row: 64 voxels x 16 channels (GA: a 1-channel float field in channel 0, scaled)
         for (int e = threadIdx.x; e < 64 * 16; e += 256) {
             const int vx = e >> 4, c = e & 15;
@@ -244,7 +251,7 @@ __global__ __launch_bounds__(256) void k_nn_wgrad(int N, const unsigned short *_
             const int c = threadIdx.x & 15, q = threadIdx.x >> 4;
             float s = 0;
             for (int vx = q * 16; vx < q * 16 + 16; vx++) s += bf2f(sA[vx * 16 + c]);
-            atomicAdd(&sdb[c], s);
+            sdbp[q][c] = s;
         }
         const int cg = 4 * (lane >> 4), cl = lane & 15;
 #pragma unroll
@@ -265,18 +272,47 @@ __global__ __launch_bounds__(256) void k_nn_wgrad(int N, const unsigned short *_
             }
         }
     }
-    // D[co = cg + j][ci = lane & 15] of tap t
+    // D[co = cg + j][ci = lane & 15] of tap t: each element is one lane's
+    float *mine = part + (size_t)blockIdx.x * NN_WP;
     const int cg = 4 * (lane >> 4), cl = lane & 15;
 #pragma unroll
     for (int i = 0; i < 7; i++) {
         const int t = wave + 4 * i;
         if (t < 27) {
 #pragma unroll
-            for (int j = 0; j < 4; j++) atomicAdd(&dW[(t * 16 + cg + j) * 16 + cl], acc[i][j]);
+            for (int j = 0; j < 4; j++) mine[(t * 16 + cg + j) * 16 + cl] = acc[i][j];
         }
     }
     __syncthreads();
-    if (threadIdx.x < 16) atomicAdd(&db[threadIdx.x], sdb[threadIdx.x]);
+    if (threadIdx.x < 16) {
+        if ((int)blockIdx.x < nrows)  // the last row's partial sums
+            dbacc += ((sdbp[0][threadIdx.x] + sdbp[1][threadIdx.x]) + sdbp[2][threadIdx.x]) + sdbp[3][threadIdx.x];
+        mine[27 * 16 * 16 + threadIdx.x] = dbacc;
+    }
+}
+
+// Fixed-order sums of the blocks' rows of part: slice y of the nb rows into part2[y] (each thread sums
+// rows g, g + 4, ... of the slice, then the four threads' sums in order), then the slices in order into
+// dW (the first 27 x 16 x 16 values) and db
+__global__ __launch_bounds__(256) void k_nn_wreduce1(const float *__restrict__ part, int nb, float *__restrict__ part2) {
+    __shared__ float s4[4][64];
+    const int o = threadIdx.x & 63, g = threadIdx.x >> 6, i = blockIdx.x * 64 + o;
+    const int per = (nb + NN_WSLICES - 1) / NN_WSLICES, b0 = blockIdx.y * per, b1 = min(nb, b0 + per);
+    float a = 0;
+    if (i < NN_WP)
+        for (int b = b0 + g; b < b1; b += 4) a += part[(size_t)b * NN_WP + i];
+    s4[g][o] = a;
+    __syncthreads();
+    if (g == 0 && i < NN_WP) part2[(size_t)blockIdx.y * NN_WP + i] = ((s4[0][o] + s4[1][o]) + s4[2][o]) + s4[3][o];
+}
+__global__ __launch_bounds__(256) void k_nn_wreduce2(const float *__restrict__ part2, float *__restrict__ dW,
+                                                     float *__restrict__ db) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= NN_WP) return;
+    float a = 0;
+    for (int sl = 0; sl < NN_WSLICES; sl++) a += part2[(size_t)sl * NN_WP + i];
+    if (i < 27 * 16 * 16) dW[i] += a;
+    else db[i - 27 * 16 * 16] += a;
 }
 
 // lane-ordered BF16 A operands of the forward (W[t][co][ci] at lane (co, ci / 4)) and of the
@@ -331,17 +367,38 @@ __global__ void k_nn_axpy(float *__restrict__ out, const float *__restrict__ in,
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) out[i] += s * in[i];
 }
-__global__ void k_nn_residual(const float *__restrict__ acc, const float *__restrict__ tgt, float *__restrict__ abar, size_t n,
-                              float gs, double *__restrict__ sum) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+// Each of the NN_LOSS_BLOCKS blocks sums its grid-stride share of r^2 in a fixed order into sum[block], and
+// k_nn_loss_total adds the block sums in a fixed order into sum[NN_LOSS_BLOCKS]: the loss repeats bit for bit
+constexpr int NN_LOSS_BLOCKS = 1024;
+__global__ __launch_bounds__(256) void k_nn_residual(const float *__restrict__ acc, const float *__restrict__ tgt,
+                                                     float *__restrict__ abar, size_t n, float gs,
+                                                     double *__restrict__ sum) {
+    __shared__ double red[256];
     double s = 0;
-    if (i < n) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)NN_LOSS_BLOCKS * 256) {
         const float r = acc[i] - tgt[i];
         abar[i] = gs * r;
-        s = (double)r * r;
+        s += (double)r * r;
     }
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o);
-    if ((threadIdx.x & 63) == 0) atomicAdd(sum, s);
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) sum[blockIdx.x] = red[0];
+}
+__global__ __launch_bounds__(256) void k_nn_loss_total(double *sum) {
+    __shared__ double red[256];
+    double s = 0;
+    for (int i = threadIdx.x; i < NN_LOSS_BLOCKS; i += 256) s += sum[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) sum[NN_LOSS_BLOCKS] = red[0];
 }
 
 static unsigned ngrid(size_t n, int b) { return (unsigned)((n + b - 1) / b); }
@@ -367,17 +424,20 @@ struct NnLaunch {
 #undef NN_C
     }
     static void wgrad(int in, int ga, int N, const void *hin, const void *fin, double inv_rho, double cval, const void *gz,
-                      const void *gf, double gscale, void *dW, void *db, hipStream_t st) {
-        const int nrows = (N / 64) * N * N, blocks = nrows < 2048 ? nrows : 2048;
+                      const void *gf, double gscale, void *dW, void *db, void *part, hipStream_t st) {
+        const int nrows = (N / 64) * N * N, blocks = nrows < NN_WBLOCKS ? nrows : NN_WBLOCKS;
+        float *p1 = (float *)part, *p2 = p1 + (size_t)NN_WBLOCKS * NN_WP;
 #define NN_W(I, G)                                                                                                     \
     k_nn_wgrad<I, G><<<blocks, 256, 0, st>>>(N, (const unsigned short *)hin, (const float *)fin, (float)inv_rho,        \
-                                             (float)cval, (const unsigned short *)gz, (const float *)gf, (float)gscale, \
-                                             (float *)dW, (float *)db)
+                                             (float)cval, (const unsigned short *)gz, (const float *)gf, (float)gscale, p1)
         if (in == NN_IN_FIELD) NN_W(NN_IN_FIELD, 0);
         else if (ga) NN_W(NN_IN_H, 1);
         else NN_W(NN_IN_H, 0);
 #undef NN_W
+        k_nn_wreduce1<<<dim3(ngrid(NN_WP, 64), NN_WSLICES), 256, 0, st>>>(p1, blocks, p2);
+        k_nn_wreduce2<<<ngrid(NN_WP, 256), 256, 0, st>>>(p2, (float *)dW, (float *)db);
     }
+    static size_t wgrad_scratch_bytes() { return sizeof(float) * (size_t)NN_WP * (NN_WBLOCKS + NN_WSLICES); }
     static void swizzle(const void *W, void *fwd, void *bwd, void *pfwd, void *pbwd, hipStream_t st) {
         k_nn_swizzle<<<ngrid(27 * 64, 256), 256, 0, st>>>((const float *)W, (s4v *)fwd, (s4v *)bwd);
         k_nn_swizzle_packed<<<1, 256, 0, st>>>((const float *)W, (s4v *)pfwd, (s4v *)pbwd);
@@ -392,18 +452,22 @@ struct NnLaunch {
         k_nn_axpy<<<ngrid(n, 256), 256, 0, st>>>((float *)out, (const float *)in, n, (float)s);
     }
     static void residual(const void *acc, const void *tgt, void *abar, size_t n, double gs, void *sum, hipStream_t st) {
-        k_nn_residual<<<ngrid(n, 256), 256, 0, st>>>((const float *)acc, (const float *)tgt, (float *)abar, n, (float)gs,
-                                                    (double *)sum);
+        k_nn_residual<<<NN_LOSS_BLOCKS, 256, 0, st>>>((const float *)acc, (const float *)tgt, (float *)abar, n, (float)gs,
+                                                     (double *)sum);
+        k_nn_loss_total<<<1, 256, 0, st>>>((double *)sum);
     }
+    static size_t residual_sum_doubles() { return NN_LOSS_BLOCKS + 1; }
 };
 
 void nn_ops(NnOps &ops) {
     ops.conv = NnLaunch::conv;
     ops.wgrad = NnLaunch::wgrad;
+    ops.wgrad_scratch_bytes = NnLaunch::wgrad_scratch_bytes;
     ops.swizzle = NnLaunch::swizzle;
     ops.adam = NnLaunch::adam;
     ops.axpy = NnLaunch::axpy;
     ops.residual = NnLaunch::residual;
+    ops.residual_sum_doubles = NnLaunch::residual_sum_doubles;
 }
 
 }  // namespace dj
