@@ -15,6 +15,12 @@ struct FftOps {
     // FP64: xf of the real field -(mub a) (neg) or mub (a + b) (b nullable), formed as the pass loads it
     void (*xf_partner)(const double *mub, const double *a, const double *b, int neg, void *A, const void *tw,
                        hipStream_t st);
+    // FP64, on the ext grid of the 2LPT source: the inverse z pass of A_ik formed from phi (F, n grid) as it
+    // is loaded, and the forward z pass of an A_ik cotangent whose output is added into Fb (n grid)
+    void (*zm_deriv_pad)(const void *F, void *Be, int n, double L, int i, int k, double s, const void *tw,
+                         hipStream_t st);
+    void (*zm_deriv_adj)(const void *Be, void *Fb, int n, double L, int i, int k, double sme, double s,
+                         const void *tw, hipStream_t st);
     void (*xi)(const void *A, void *mesh, const void *tw, hipStream_t st);
     // the same with the output multiplied by s as it is stored
     void (*xi_scaled)(const void *A, void *mesh, const void *tw, double s, hipStream_t st);

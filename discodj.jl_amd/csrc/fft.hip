@@ -3,6 +3,7 @@
 
 #include "dj_internal.h"
 #include "fft_passes.h"
+#include "ic_modes.h"
 
 namespace dj {
 
@@ -228,6 +229,38 @@ __global__ __launch_bounds__((FftShape<T, N>::NTY)) void k_zm(const cx<T> *in, c
     }
 }
 
+// The z pass (forward or inverse) of lines read from src(e) and written to dst(e, v), e the B-layout index:
+// the FP64 2LPT derivative spectra formed as the pass loads them, or their cotangents cropped and added
+// into the n-grid spectrum as it stores them
+template <typename T, int N, bool FWD, typename SRC, typename DST>
+__global__ __launch_bounds__((FftShape<T, N>::NTY)) void k_zm_io(SRC src, DST dst, const cx<T> *__restrict__ tw) {
+    using S = FftShape<T, N>;
+    using P = typename S::PY;
+    __shared__ cx<T> lds[P::EF::LDS_ELEMS];
+    __shared__ cx<T> twl[S::TWL ? N : 1];
+    const int b = blockIdx.x;
+    const cx<T> *t = stage_twiddles<T, N, S::TWL>(tw, twl, threadIdx.x, S::NTY);
+    if constexpr (P::EF::wave_ok()) {
+        P::z_load_src_w(src, lds, b, threadIdx.x);
+        __syncthreads();  // the twiddle table; from here on every wave works on its own lines
+        P::EF::template run_wave<FWD ? -1 : +1, 1>(lds, t, threadIdx.x);
+        P::z_store_dst_w(lds, dst, b, threadIdx.x);
+    } else {
+        P::z_load_src(src, lds, b, threadIdx.x);
+        __syncthreads();
+        P::EF::template run<FWD ? -1 : +1, 1>(lds, t, threadIdx.x);
+        P::z_store_dst(lds, dst, b, threadIdx.x);
+    }
+}
+template <typename T> struct StoreTo {
+    cx<T> *out;
+    __device__ void operator()(size_t e, cx<T> v) const { out[e] = v; }
+};
+template <typename T> struct LoadFrom {
+    const cx<T> *in;
+    __device__ cx<T> operator()(size_t e) const { return in[e]; }
+};
+
 // The z pass as a persistent kernel (three workgroups per CU, block b, b + G, b + 2G, ... for G
 // workgroups): each wave holds the next block's lines in registers while it transforms and stores
 // the current block's, so a block's loads overlap the previous block's transforms and stores
@@ -344,6 +377,20 @@ struct FftN {
             k_xf_src<T, N, PartnerSrc><<<P::XBLOCKS(), 256, 0, st>>>(PartnerSrc{mub, a, b, neg}, (cx<T> *)A,
                                                                      (const cx<T> *)tw);
     }
+    // inverse z pass of the A_ik spectrum (ext grid N) formed from phi (F, n grid) as it is loaded, into Be
+    static void zm_deriv_pad(const void *F, void *Be, int n, double L, int i, int k, double s, const void *tw,
+                             hipStream_t st) {
+        if constexpr (sizeof(T) == 8)
+            k_zm_io<T, N, false, DerivPadSrc, StoreTo<T>><<<P::ZBLOCKS(), S::NTY, 0, st>>>(
+                DerivPadSrc{(const cx<double> *)F, n, N, i, k, L, s}, StoreTo<T>{(cx<T> *)Be}, (const cx<T> *)tw);
+    }
+    // forward z pass of Be (ext grid N) whose output is cropped, multiplied and added into Fb (n grid)
+    static void zm_deriv_adj(const void *Be, void *Fb, int n, double L, int i, int k, double sme, double s,
+                             const void *tw, hipStream_t st) {
+        if constexpr (sizeof(T) == 8)
+            k_zm_io<T, N, true, LoadFrom<T>, DerivAdjDst><<<P::ZBLOCKS(), S::NTY, 0, st>>>(
+                LoadFrom<T>{(const cx<T> *)Be}, DerivAdjDst{(cx<double> *)Fb, n, N, i, k, L, sme, s}, (const cx<T> *)tw);
+    }
     static void xi(const void *A, void *mesh, const void *tw, hipStream_t st) {
         using PX = typename S::PXI;
         k_xi<T, N><<<PX::XBLOCKS(), S::NTXI, 0, st>>>((const cx<T> *)A, (T *)mesh, (const cx<T> *)tw);
@@ -393,6 +440,8 @@ bool fft_dispatch(int N, FftOps &ops) {
         ops.xi = FftN<T, n>::xi;        \
         ops.xi_scaled = FftN<T, n>::xi_scaled; \
         ops.xf_partner = FftN<T, n>::xf_partner; \
+        ops.zm_deriv_pad = FftN<T, n>::zm_deriv_pad; \
+        ops.zm_deriv_adj = FftN<T, n>::zm_deriv_adj; \
         ops.yf = FftN<T, n>::yf;        \
         ops.yi = FftN<T, n>::yi;        \
         ops.zm = FftN<T, n>::zm;        \

@@ -302,6 +302,35 @@ struct Passes {
             if ((B * N) % NT == 0 || i < B * N) z_mult_one<MODE>(lds, K, b, i);
         }
     }
+    // the z lines of block b from / to functors of the B-layout entry index (src(e), dst(e, v))
+    template <typename SRC>
+    DJ_HD static void z_load_src(const SRC &src, cx<T> *lds, int b, int tid) {
+        const size_t base = (size_t)N * B * b;
+        copy_batched<cx<T>, B * N, NT>(
+            tid, [&](int i) { return src(base + i); },
+            [&](int i, cx<T> v) { const int l = i / N, zz = i - l * N; lds[EF::idx(l, zz)] = v; });
+    }
+    template <typename DST>
+    DJ_HD static void z_store_dst(const cx<T> *lds, const DST &dst, int b, int tid) {
+        const size_t base = (size_t)N * B * b;
+        copy_batched<cx<T>, B * N, NT>(
+            tid, [&](int i) { const int l = i / N, zz = i - l * N; return lds[EF::idx(l, zz)]; },
+            [&](int i, cx<T> v) { dst(base + i, v); });
+    }
+    template <typename SRC>
+    DJ_HD static void z_load_src_w(const SRC &src, cx<T> *lds, int b, int tid) {
+        const size_t base = (size_t)N * B * b;
+        copy_wave<cx<T>, EF::LW * N>(
+            tid >> 6, tid & 63, [&](int i) { return src(base + i); },
+            [&](int i, cx<T> v) { const int l = i / N, zz = i - l * N; lds[EF::idx(l, zz)] = v; });
+    }
+    template <typename DST>
+    DJ_HD static void z_store_dst_w(const cx<T> *lds, const DST &dst, int b, int tid) {
+        const size_t base = (size_t)N * B * b;
+        copy_wave<cx<T>, EF::LW * N>(
+            tid >> 6, tid & 63, [&](int i) { const int l = i / N, zz = i - l * N; return lds[EF::idx(l, zz)]; },
+            [&](int i, cx<T> v) { dst(base + i, v); });
+    }
     DJ_HD static void z_store(const cx<T> *lds, cx<T> *Bm, int b, int tid) {
         cx<T> *dst = Bm + (size_t)N * B * b;
         copy_batched<cx<T>, B * N, NT>(

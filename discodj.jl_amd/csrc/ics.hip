@@ -18,22 +18,10 @@
 
 #include "dj_internal.h"
 #include "fft_passes.h"
+#include "ic_modes.h"
 
 namespace dj {
 
-struct BIdx {
-    int kx, ky, kz;
-};
-__device__ __forceinline__ BIdx bidx(size_t e, int N) {
-    const int H = N / 2 + 1;
-    BIdx r;
-    r.kz = (int)(e % N);
-    size_t t = e / N;
-    r.kx = (int)(t % H);
-    r.ky = (int)(t / H);
-    return r;
-}
-__device__ __forceinline__ int sgn_full(int m, int N) { return m < N / 2 ? m : m - N; }
 
 // phi(k) = F(w)(k) * scale(k), k grid of grf.jl:190-216, P interpolated as np.interp (clamped)
 __global__ void k_ic_scale(cx<double> *B, int N, double L, const double *kt, const double *pt, int ntab) {
@@ -65,12 +53,6 @@ __global__ void k_ic_scale(cx<double> *B, int N, double L, const double *kt, con
     B[e] = scale(B[e], s);
 }
 
-// i k_c along a physical axis with the Nyquist entry zeroed (nlpt_core.jl:49-67)
-__device__ __forceinline__ double kdir(const BIdx &b, int c, int N, double dk) {
-    if (c == 0) return b.kx == N / 2 ? 0.0 : b.kx * dk;
-    int m = c == 1 ? b.ky : b.kz;
-    return m == N / 2 ? 0.0 : sgn_full(m, N) * dk;
-}
 
 enum { IC_PSI1 = 0, IC_DERIV = 1, IC_PSI2 = 2, IC_PSI2_ADJ = 3 };
 
@@ -118,59 +100,6 @@ __global__ void k_ic_crop(const cx<double> *in, cx<double> *out, int n, int ext,
     cx<double> v = {0, 0};
     if (sx >= 0 && sy >= 0 && sz >= 0) v = scale(in[(size_t)sz + (size_t)ext * (sx + (size_t)He * sy)], s);
     out[e] = v;
-}
-
-// One A_ik spectrum on the ext grid from phi: the steps of k_ic_mult<IC_PSI1> (n grid, axis i), k_ic_pad
-// (scale s) and k_ic_mult<IC_DERIV> (ext grid, axis k), in their order, in one pass over the ext spectrum
-// (the two intermediate spectra are not written)
-__global__ void k_ic_deriv_pad(const cx<double> *__restrict__ F, cx<double> *__restrict__ out, int n, int ext,
-                               double L, int i, int k, double s) {
-    const int He = ext / 2 + 1, Hn = n / 2 + 1, h = n / 2;
-    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x, tot = (size_t)ext * He * ext;
-    if (e >= tot) return;
-    const BIdx b = bidx(e, ext);
-    auto src = [&](int m) { return m < h ? m : (m > ext - h ? m - ext + n : -1); };
-    const int sx = b.kx < h ? b.kx : -1, sy = src(b.ky), sz = src(b.kz);
-    const double dk = 2 * M_PI / L;
-    cx<double> v = {0, 0};
-    if (sx >= 0 && sy >= 0 && sz >= 0) {
-        const BIdx bn = {sx, sy, sz};
-        const double g = kdir(bn, i, n, dk);
-        const cx<double> x = F[(size_t)sz + (size_t)n * (sx + (size_t)Hn * sy)];
-        cx<double> r = {-g * x.y, g * x.x};
-        r = {-r.x, -r.y};
-        v = scale(r, s);
-    }
-    const double g2 = kdir(b, k, ext, dk);
-    out[e] = {-g2 * v.y, g2 * v.x};
-}
-
-// The transpose of the steps above for the r2c of an A_ik cotangent: k_ic_halfw (ext grid, scale sme),
-// k_ic_mult<IC_PSI1> (ext grid, axis k), k_ic_crop (scale s), k_ic_mult<IC_DERIV> (n grid, axis i) and
-// k_ic_axpy into Fb, in their order, in one pass over the n spectrum (the ext entries that the crop
-// drops are not read)
-__global__ void k_ic_deriv_adj(const cx<double> *__restrict__ Be, cx<double> *__restrict__ Fb, int n, int ext,
-                               double L, int i, int k, double sme, double s) {
-    const int He = ext / 2 + 1, Hn = n / 2 + 1, h = n / 2;
-    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x, tot = (size_t)n * Hn * n;
-    if (e >= tot) return;
-    const BIdx b = bidx(e, n);
-    auto src = [&](int m) { return m < h ? m : (m > h ? m - n + ext : -1); };
-    const int sx = b.kx < h ? b.kx : -1, sy = src(b.ky), sz = src(b.kz);
-    const double dk = 2 * M_PI / L;
-    cx<double> v = {0, 0};
-    if (sx >= 0 && sy >= 0 && sz >= 0) {
-        const double w = (sx == 0 || sx == ext / 2) ? 1.0 : 2.0;
-        const cx<double> x = scale(Be[(size_t)sz + (size_t)ext * (sx + (size_t)He * sy)], sme * w);
-        const BIdx be = {sx, sy, sz};
-        const double g = kdir(be, k, ext, dk);
-        cx<double> r = {-g * x.y, g * x.x};
-        r = {-r.x, -r.y};
-        v = scale(r, s);
-    }
-    const double g2 = kdir(b, i, n, dk);
-    const cx<double> t = {-g2 * v.y, g2 * v.x};
-    Fb[e] = {Fb[e].x + t.x, Fb[e].y + t.y};
 }
 
 // mu op= a * b  (op 0 assign, 1 add, 2 subtract; 3 subtract, then multiply by s); the products and sums
@@ -486,8 +415,7 @@ int ics_lpt_device(IcWork &ws, int n, double L, int order, const double *white, 
     const int ext = 3 * n / 2;
     if (!fft_dispatch<double>(n, fn)) return (int)hipErrorInvalidValue;
     if (order >= 2 && !fft_dispatch<double>(ext, fe)) return (int)hipErrorInvalidValue;
-    const size_t M = (size_t)n * n * n, HC = (size_t)n * (n / 2 + 1) * n, ME = (size_t)ext * ext * ext,
-                 HE = (size_t)ext * (ext / 2 + 1) * ext;
+    const size_t M = (size_t)n * n * n, HC = (size_t)n * (n / 2 + 1) * n, ME = (size_t)ext * ext * ext;
     hipError_t err = ic_work_ensure(ws, n, order, ntab);
     if (err != hipSuccess) return (int)err;
     void *twn = ws.twn;
@@ -532,8 +460,9 @@ int ics_lpt_device(IcWork &ws, int n, double L, int order, const double *white, 
         const double pad_s = std::pow((double)ext / n, 3), crop_s = std::pow((double)n / ext, 3);
         // A_ik = irfft_ext(i k_k pad(psi1_i)); psi1_i spectrum = -(i k_i) phi
         auto deriv = [&](int i, int k, double *out) {
-            k_ic_deriv_pad<<<g1(HE), 256, 0, st>>>(F, Be, n, ext, L, i, k, pad_s);
-            inv_s(fe, Be, Ae, out, twe, 1.0 / (double)ME);
+            fe.zm_deriv_pad(F, Be, n, L, i, k, pad_s, twe, st);
+            fe.yi(Be, Ae, twe, st);
+            fe.xi_scaled(Ae, out, twe, 1.0 / (double)ME, st);
         };
         // mu = A11 A22 + A11 A33 + A22 A33 - A12 A21 - A13 A31 - A23 A32 (evaluated left to right)
         // the last product also applies the order-2 factor ((3-2)/2 - 2) / ((2 + 3/2)(2 - 1)) = -3/7
@@ -687,16 +616,16 @@ int ics_lpt_vjp_device(IcWork &ws, int n, double L, int order, const double *whi
         inv_s(fe, Be, Ae, mub, twe, ((3.0 - 2) / 2 - 2.0) / ((2 + 1.5) * 1.0));
         // A_ik = c2r_ext((i k_k) pad(-(i k_i) phi) pad_s) / ME
         auto deriv = [&](int i, int k, double *out) {
-            k_ic_deriv_pad<<<g1(HE), 256, 0, st>>>(F, Be, n, ext, L, i, k, pad_s);
-            inv_s(fe, Be, Ae, out, twe, 1.0 / (double)ME);
+            fe.zm_deriv_pad(F, Be, n, L, i, k, pad_s, twe, st);
+            fe.yi(Be, Ae, twe, st);
+            fe.xi_scaled(Ae, out, twe, 1.0 / (double)ME, st);
         };
         // Abar_ik -> phibar: w r2c_ext(Abar) / ME, conj(i k_k), crop pad_s, conj(-(i k_i))
         // with the field Abar_ik = -(mubar a) (neg) or mubar (a + b) formed by the transform as it loads it
         auto deriv_adj = [&](int i, int k, const double *a, const double *b, int neg) {
             fe.xf_partner(mub, a, b, neg, Ae, twe, st);
             fe.yf(Ae, Be, twe, st);
-            fe.zm(ZM_NONE, true, false, Be, Be, twe, &none, st);
-            k_ic_deriv_adj<<<g1(HC), 256, 0, st>>>(Be, Fb, n, ext, L, i, k, 1.0 / (double)ME, pad_s);
+            fe.zm_deriv_adj(Be, Fb, n, L, i, k, 1.0 / (double)ME, pad_s, twe, st);
         };
         // field j of the forward's order (A11, A22, A33, A12, A21, A13, A31, A23, A32): kept, or recomputed
         // into a scratch field
