@@ -21,6 +21,11 @@ struct FftOps {
                          hipStream_t st);
     void (*zm_deriv_adj)(const void *Be, void *Fb, int n, double L, int i, int k, double sme, double s,
                          const void *tw, hipStream_t st);
+    // FP64 IC spectra (mode: IC_PSI1 / IC_PSI2, and IC_DERIV / IC_PSI2_ADJ of ic_modes.h): the inverse z pass
+    // of the multiplied spectrum, and the forward z pass whose output is weighted, multiplied, accumulated
+    void (*zm_mult_inv)(int mode, const void *in, void *out, int c, double L, const void *tw, hipStream_t st);
+    void (*zm_fwd_mult_acc)(int mode, const void *in, void *acc, int c, double L, double s, const void *tw,
+                            hipStream_t st);
     void (*xi)(const void *A, void *mesh, const void *tw, hipStream_t st);
     // the same with the output multiplied by s as it is stored
     void (*xi_scaled)(const void *A, void *mesh, const void *tw, double s, hipStream_t st);
@@ -261,7 +266,7 @@ struct IcWork {
     int n = 0, order = 0, ntab = 0;
     std::vector<void *> bufs;
     void *twn = nullptr, *twe = nullptr, *kt = nullptr, *pt = nullptr;
-    void *A = nullptr, *Bm = nullptr, *F = nullptr, *Fb = nullptr, *T1 = nullptr, *fLb = nullptr;
+    void *A = nullptr, *Bm = nullptr, *F = nullptr, *Fb = nullptr, *fLb = nullptr;
     void *w = nullptr, *p1 = nullptr, *p2 = nullptr;
     void *Ae = nullptr, *Be = nullptr, *mu = nullptr, *fld[3] = {nullptr, nullptr, nullptr};
     // The nine second-order fields A_ik of the last forward (ext grid), kept for the reverse pass of the

@@ -391,6 +391,33 @@ struct FftN {
             k_zm_io<T, N, true, LoadFrom<T>, DerivAdjDst><<<P::ZBLOCKS(), S::NTY, 0, st>>>(
                 LoadFrom<T>{(const cx<T> *)Be}, DerivAdjDst{(cx<double> *)Fb, n, N, i, k, L, sme, s}, (const cx<T> *)tw);
     }
+    // FP64 IC spectra on this grid: the inverse z pass of mult<mode>(in) into out, and the forward z pass of
+    // in whose output is weighted (s), multiplied by mult<mode> and added into acc
+    static void zm_mult_inv(int mode, const void *in, void *out, int c, double L, const void *tw, hipStream_t st) {
+        if constexpr (sizeof(T) == 8) {
+            const cx<T> *i = (const cx<T> *)in, *t = (const cx<T> *)tw;
+            StoreTo<T> o{(cx<T> *)out};
+            if (mode == IC_PSI1)
+                k_zm_io<T, N, false, MultSrc<IC_PSI1>, StoreTo<T>><<<P::ZBLOCKS(), S::NTY, 0, st>>>(
+                    MultSrc<IC_PSI1>{i, N, c, L}, o, t);
+            else
+                k_zm_io<T, N, false, MultSrc<IC_PSI2>, StoreTo<T>><<<P::ZBLOCKS(), S::NTY, 0, st>>>(
+                    MultSrc<IC_PSI2>{i, N, c, L}, o, t);
+        }
+    }
+    static void zm_fwd_mult_acc(int mode, const void *in, void *acc, int c, double L, double s, const void *tw,
+                                hipStream_t st) {
+        if constexpr (sizeof(T) == 8) {
+            const cx<T> *t = (const cx<T> *)tw;
+            LoadFrom<T> i{(const cx<T> *)in};
+            if (mode == IC_DERIV)
+                k_zm_io<T, N, true, LoadFrom<T>, MultAccDst<IC_DERIV>><<<P::ZBLOCKS(), S::NTY, 0, st>>>(
+                    i, MultAccDst<IC_DERIV>{(cx<T> *)acc, N, c, L, s}, t);
+            else
+                k_zm_io<T, N, true, LoadFrom<T>, MultAccDst<IC_PSI2_ADJ>><<<P::ZBLOCKS(), S::NTY, 0, st>>>(
+                    i, MultAccDst<IC_PSI2_ADJ>{(cx<T> *)acc, N, c, L, s}, t);
+        }
+    }
     static void xi(const void *A, void *mesh, const void *tw, hipStream_t st) {
         using PX = typename S::PXI;
         k_xi<T, N><<<PX::XBLOCKS(), S::NTXI, 0, st>>>((const cx<T> *)A, (T *)mesh, (const cx<T> *)tw);
@@ -442,6 +469,8 @@ bool fft_dispatch(int N, FftOps &ops) {
         ops.xf_partner = FftN<T, n>::xf_partner; \
         ops.zm_deriv_pad = FftN<T, n>::zm_deriv_pad; \
         ops.zm_deriv_adj = FftN<T, n>::zm_deriv_adj; \
+        ops.zm_mult_inv = FftN<T, n>::zm_mult_inv; \
+        ops.zm_fwd_mult_acc = FftN<T, n>::zm_fwd_mult_acc; \
         ops.yf = FftN<T, n>::yf;        \
         ops.yi = FftN<T, n>::yi;        \
         ops.zm = FftN<T, n>::zm;        \

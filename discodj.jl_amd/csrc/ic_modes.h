@@ -30,6 +30,46 @@ __device__ __forceinline__ double kdir(const BIdx &b, int c, int N, double dk) {
     return m == N / 2 ? 0.0 : sgn_full(m, N) * dk;
 }
 
+enum { IC_PSI1 = 0, IC_DERIV = 1, IC_PSI2 = 2, IC_PSI2_ADJ = 3 };
+
+// mult(k) v at entry e:  PSI1: -(i k_c) ;  DERIV: i k_c ;  PSI2: (-1/k^2) (i k_c) ;  PSI2_ADJ: the complex
+// conjugate of PSI2's multiplier, (1/k^2) (i k_c)  (the adjoint of a complex multiplier is its conjugate)
+template <int MODE>
+__device__ __forceinline__ cx<double> ic_mult_at(size_t e, cx<double> v, int N, double L, int c) {
+    const BIdx b = bidx(e, N);
+    const double dk = 2 * M_PI / L;
+    const double g = kdir(b, c, N, dk);
+    cx<double> r = {-g * v.y, g * v.x};  // (i g) v
+    if constexpr (MODE == IC_PSI1) r = {-r.x, -r.y};
+    if constexpr (MODE == IC_PSI2 || MODE == IC_PSI2_ADJ) {
+        const double kx = b.kx * dk, ky = sgn_full(b.ky, N) * dk, kz = sgn_full(b.kz, N) * dk;
+        const double k2 = (kx * kx + ky * ky) + kz * kz;
+        const double il = -1.0 / (e == 0 ? 1.0 : k2);
+        r = scale(r, MODE == IC_PSI2 ? il : -il);
+    }
+    return r;
+}
+// the multiplier applied to in[e] as a z pass loads it
+template <int MODE> struct MultSrc {
+    const cx<double> *in;
+    int N, c;
+    double L;
+    __device__ cx<double> operator()(size_t e) const { return ic_mult_at<MODE>(e, in[e], N, L, c); }
+};
+// a z pass's output v at e: the half-spectrum weight times s (1 at kx = 0 and N/2, else 2), the multiplier,
+// then added into acc[e]
+template <int MODE> struct MultAccDst {
+    cx<double> *acc;
+    int N, c;
+    double L, s;
+    __device__ void operator()(size_t e, cx<double> v) const {
+        const int H = N / 2 + 1, kx = (int)((e / N) % H);
+        const double w = (kx == 0 || kx == N / 2) ? 1.0 : 2.0;
+        const cx<double> r = ic_mult_at<MODE>(e, scale(v, s * w), N, L, c);
+        acc[e] = {acc[e].x + r.x, acc[e].y + r.y};
+    }
+};
+
 // Entry e of one A_ik spectrum on the ext grid from phi (F, n grid): the steps of the PSI1 multiplier
 // (n grid, axis i), the pad (scale s) and the DERIV multiplier (ext grid, axis k), in their order
 struct DerivPadSrc {

@@ -54,29 +54,6 @@ __global__ void k_ic_scale(cx<double> *B, int N, double L, const double *kt, con
 }
 
 
-enum { IC_PSI1 = 0, IC_DERIV = 1, IC_PSI2 = 2, IC_PSI2_ADJ = 3 };
-
-// out = mult(k) in:  PSI1: -(i k_c) ;  DERIV: i k_c ;  PSI2: (-1/k^2) (i k_c) ;  PSI2_ADJ: the complex
-// conjugate of PSI2's multiplier, (1/k^2) (i k_c)  (the adjoint of a complex multiplier is its conjugate)
-template <int MODE>
-__global__ void k_ic_mult(const cx<double> *in, cx<double> *out, int N, double L, int c) {
-    const int H = N / 2 + 1;
-    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x, tot = (size_t)N * H * N;
-    if (e >= tot) return;
-    BIdx b = bidx(e, N);
-    double dk = 2 * M_PI / L;
-    double g = kdir(b, c, N, dk);
-    cx<double> v = in[e];
-    cx<double> r = {-g * v.y, g * v.x};  // (i g) v
-    if constexpr (MODE == IC_PSI1) r = {-r.x, -r.y};
-    if constexpr (MODE == IC_PSI2 || MODE == IC_PSI2_ADJ) {
-        double kx = b.kx * dk, ky = sgn_full(b.ky, N) * dk, kz = sgn_full(b.kz, N) * dk;
-        double k2 = (kx * kx + ky * ky) + kz * kz;
-        double il = -1.0 / (e == 0 ? 1.0 : k2);
-        r = scale(r, MODE == IC_PSI2 ? il : -il);
-    }
-    out[e] = r;
-}
 
 // zero-pad (n -> e) or crop (e -> n) between B layouts, Nyquist planes dropped, amplitude rescaled
 __global__ void k_ic_pad(const cx<double> *in, cx<double> *out, int n, int ext, double s) {
@@ -151,10 +128,6 @@ __global__ void k_ic_halfw(cx<double> *B, int N, int divide, double s) {
     const int kx = (int)((e / N) % H);
     const double w = (kx == 0 || kx == N / 2) ? 1.0 : 2.0;
     B[e] = scale(B[e], divide ? s / w : s * w);
-}
-__global__ void k_ic_axpy(cx<double> *out, const cx<double> *in, size_t n) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = {out[i].x + in[i].x, out[i].y + in[i].y};
 }
 // (Psibar, Pibar) -> (psi1bar, psi2bar) of Psi = D psi1 + D^2 psi2, Pi = psi1 + 2 D psi2
 template <typename T>
@@ -336,7 +309,6 @@ static hipError_t ic_work_ensure(IcWork &ws, int n, int order, int ntab) {
     ws.Bm = alloc(HC * 16);
     ws.F = alloc(HC * 16);
     ws.Fb = alloc(HC * 16);
-    ws.T1 = alloc(HC * 16);
     ws.w = alloc(M * 8);
     ws.p1 = alloc(3 * M * 8);
     ws.p2 = alloc(3 * M * 8);
@@ -430,12 +402,6 @@ int ics_lpt_device(IcWork &ws, int n, double L, int order, const double *white, 
         f.yf(a, b, tw, st);
         f.zm(ZM_NONE, true, false, b, b, tw, &none, st);
     };
-    // inverse transform (destroys b), the output multiplied by sc as it is stored
-    auto inv_s = [&](FftOps &f, cx<double> *b, cx<double> *a, double *real, void *tw, double sc) {
-        f.zm(ZM_NONE, false, true, b, b, tw, &none, st);
-        f.yi(b, a, tw, st);
-        f.xi_scaled(a, real, tw, sc, st);
-    };
     const bool keep = order >= 2 && ic_fields_ensure(ws, n);
     ws.fk_valid = false;
     if (err == hipSuccess) err = ic_white(ws, fn, n, white, white_on_device, seed, st);
@@ -445,8 +411,9 @@ int ics_lpt_device(IcWork &ws, int n, double L, int order, const double *white, 
     fwd(fn, w, A, F, twn);
     k_ic_scale<<<g1(HC), 256, 0, st>>>(F, n, L, kt, pt, ntab);
     for (int c = 0; c < 3; c++) {
-        k_ic_mult<IC_PSI1><<<g1(HC), 256, 0, st>>>(F, Bm, n, L, c);
-        inv_s(fn, Bm, A, p1 + c * M, twn, 1.0 / (double)M);
+        fn.zm_mult_inv(IC_PSI1, F, Bm, c, L, twn, st);  // the multiplier applied as the z pass loads
+        fn.yi(Bm, A, twn, st);
+        fn.xi_scaled(A, p1 + c * M, twn, 1.0 / (double)M, st);
     }
     if (order >= 2) {
         void *twe = ws.twe;
@@ -494,8 +461,9 @@ int ics_lpt_device(IcWork &ws, int n, double L, int order, const double *white, 
         fwd(fe, mu, Ae, Be, twe);
         k_ic_crop<<<g1(HC), 256, 0, st>>>(Be, A, n, ext, crop_s);  // fL in A
         for (int c = 0; c < 3; c++) {
-            k_ic_mult<IC_PSI2><<<g1(HC), 256, 0, st>>>(A, Bm, n, L, c);
-            inv_s(fn, Bm, F, p2 + c * M, twn, 1.0 / (double)M);  // F is free now
+            fn.zm_mult_inv(IC_PSI2, A, Bm, c, L, twn, st);
+            fn.yi(Bm, F, twn, st);  // F is free now
+            fn.xi_scaled(F, p2 + c * M, twn, 1.0 / (double)M, st);
         }
     }
     if (prec == 64)
@@ -536,7 +504,7 @@ int ics_lpt_vjp_device(IcWork &ws, int n, double L, int order, const double *whi
     if (err != hipSuccess) return (int)err;
     void *twn = ws.twn;
     cx<double> *A = (cx<double> *)ws.A, *Bm = (cx<double> *)ws.Bm, *F = (cx<double> *)ws.F,
-               *Fb = (cx<double> *)ws.Fb, *T1 = (cx<double> *)ws.T1;
+               *Fb = (cx<double> *)ws.Fb;
     double *w = (double *)ws.w, *p1b = (double *)ws.p1, *p2b = (double *)ws.p2;
     double *kt = (double *)ws.kt, *pt = (double *)ws.pt;
     err = hipMemcpyAsync(kt, kt_h, ntab * 8, hipMemcpyHostToDevice, st);
@@ -590,10 +558,10 @@ int ics_lpt_vjp_device(IcWork &ws, int n, double L, int order, const double *whi
     (void)hipMemsetAsync(Fb, 0, HC * 16, st);
     // psi1_c = c2r(-(i k_c) phi) / M:  phibar += (i k_c) w r2c(psi1bar_c) / M
     for (int c = 0; c < 3; c++) {
-        fwd(fn, p1b + c * M, A, Bm, twn);
-        k_ic_halfw<<<g1(HC), 256, 0, st>>>(Bm, n, 0, 1.0 / (double)M);
-        k_ic_mult<IC_DERIV><<<g1(HC), 256, 0, st>>>(Bm, T1, n, L, c);
-        k_ic_axpy<<<g1(HC), 256, 0, st>>>(Fb, T1, HC);
+        // the weights, the multiplier and the accumulation applied as the z pass stores
+        fn.xf(p1b + c * M, false, A, twn, 1.0, st);
+        fn.yf(A, Bm, twn, st);
+        fn.zm_fwd_mult_acc(IC_DERIV, Bm, Fb, c, L, 1.0 / (double)M, twn, st);
     }
     if (order >= 2) {
         void *twe = ws.twe;
@@ -605,10 +573,9 @@ int ics_lpt_vjp_device(IcWork &ws, int n, double L, int order, const double *whi
         // psi2_c = c2r((-1/k^2)(i k_c) fL) / M:  fLbar = sum_c (1/k^2)(i k_c) w r2c(psi2bar_c) / M
         (void)hipMemsetAsync(fLb, 0, HC * 16, st);
         for (int c = 0; c < 3; c++) {
-            fwd(fn, p2b + c * M, A, Bm, twn);
-            k_ic_halfw<<<g1(HC), 256, 0, st>>>(Bm, n, 0, 1.0 / (double)M);
-            k_ic_mult<IC_PSI2_ADJ><<<g1(HC), 256, 0, st>>>(Bm, T1, n, L, c);
-            k_ic_axpy<<<g1(HC), 256, 0, st>>>(fLb, T1, HC);
+            fn.xf(p2b + c * M, false, A, twn, 1.0, st);
+            fn.yf(A, Bm, twn, st);
+            fn.zm_fwd_mult_acc(IC_PSI2_ADJ, Bm, fLb, c, L, 1.0 / (double)M, twn, st);
         }
         // fL = crop(r2c_ext(-3/7 mu)) crop_s:  mubar = -3/7 c2r_ext(pad(fLbar) crop_s / w)
         k_ic_pad<<<g1(HE), 256, 0, st>>>(fLb, Be, n, ext, crop_s);
