@@ -381,15 +381,15 @@ struct FftN {
     static void zm_deriv_pad(const void *F, void *Be, int n, double L, int i, int k, double s, const void *tw,
                              hipStream_t st) {
         if constexpr (sizeof(T) == 8)
-            k_zm_io<T, N, false, DerivPadSrc, StoreTo<T>><<<P::ZBLOCKS(), S::NTY, 0, st>>>(
-                DerivPadSrc{(const cx<double> *)F, n, N, i, k, L, s}, StoreTo<T>{(cx<T> *)Be}, (const cx<T> *)tw);
+            k_zm_io<T, N, false, DerivPadSrc<N>, StoreTo<T>><<<P::ZBLOCKS(), S::NTY, 0, st>>>(
+                DerivPadSrc<N>{(const cx<double> *)F, i, k, L, s}, StoreTo<T>{(cx<T> *)Be}, (const cx<T> *)tw);
     }
     // forward z pass of Be (ext grid N) whose output is cropped, multiplied and added into Fb (n grid)
     static void zm_deriv_adj(const void *Be, void *Fb, int n, double L, int i, int k, double sme, double s,
                              const void *tw, hipStream_t st) {
         if constexpr (sizeof(T) == 8)
-            k_zm_io<T, N, true, LoadFrom<T>, DerivAdjDst><<<P::ZBLOCKS(), S::NTY, 0, st>>>(
-                LoadFrom<T>{(const cx<T> *)Be}, DerivAdjDst{(cx<double> *)Fb, n, N, i, k, L, sme, s}, (const cx<T> *)tw);
+            k_zm_io<T, N, true, LoadFrom<T>, DerivAdjDst<N>><<<P::ZBLOCKS(), S::NTY, 0, st>>>(
+                LoadFrom<T>{(const cx<T> *)Be}, DerivAdjDst<N>{(cx<double> *)Fb, i, k, L, sme, s}, (const cx<T> *)tw);
     }
     // FP64 IC spectra on this grid: the inverse z pass of mult<mode>(in) into out, and the forward z pass of
     // in whose output is weighted (s), multiplied by mult<mode> and added into acc
@@ -398,11 +398,11 @@ struct FftN {
             const cx<T> *i = (const cx<T> *)in, *t = (const cx<T> *)tw;
             StoreTo<T> o{(cx<T> *)out};
             if (mode == IC_PSI1)
-                k_zm_io<T, N, false, MultSrc<IC_PSI1>, StoreTo<T>><<<P::ZBLOCKS(), S::NTY, 0, st>>>(
-                    MultSrc<IC_PSI1>{i, N, c, L}, o, t);
+                k_zm_io<T, N, false, MultSrc<IC_PSI1, N>, StoreTo<T>><<<P::ZBLOCKS(), S::NTY, 0, st>>>(
+                    MultSrc<IC_PSI1, N>{i, c, L}, o, t);
             else
-                k_zm_io<T, N, false, MultSrc<IC_PSI2>, StoreTo<T>><<<P::ZBLOCKS(), S::NTY, 0, st>>>(
-                    MultSrc<IC_PSI2>{i, N, c, L}, o, t);
+                k_zm_io<T, N, false, MultSrc<IC_PSI2, N>, StoreTo<T>><<<P::ZBLOCKS(), S::NTY, 0, st>>>(
+                    MultSrc<IC_PSI2, N>{i, c, L}, o, t);
         }
     }
     static void zm_fwd_mult_acc(int mode, const void *in, void *acc, int c, double L, double s, const void *tw,
@@ -411,11 +411,11 @@ struct FftN {
             const cx<T> *t = (const cx<T> *)tw;
             LoadFrom<T> i{(const cx<T> *)in};
             if (mode == IC_DERIV)
-                k_zm_io<T, N, true, LoadFrom<T>, MultAccDst<IC_DERIV>><<<P::ZBLOCKS(), S::NTY, 0, st>>>(
-                    i, MultAccDst<IC_DERIV>{(cx<T> *)acc, N, c, L, s}, t);
+                k_zm_io<T, N, true, LoadFrom<T>, MultAccDst<IC_DERIV, N>><<<P::ZBLOCKS(), S::NTY, 0, st>>>(
+                    i, MultAccDst<IC_DERIV, N>{(cx<T> *)acc, c, L, s}, t);
             else
-                k_zm_io<T, N, true, LoadFrom<T>, MultAccDst<IC_PSI2_ADJ>><<<P::ZBLOCKS(), S::NTY, 0, st>>>(
-                    i, MultAccDst<IC_PSI2_ADJ>{(cx<T> *)acc, N, c, L, s}, t);
+                k_zm_io<T, N, true, LoadFrom<T>, MultAccDst<IC_PSI2_ADJ, N>><<<P::ZBLOCKS(), S::NTY, 0, st>>>(
+                    i, MultAccDst<IC_PSI2_ADJ, N>{(cx<T> *)acc, c, L, s}, t);
         }
     }
     static void xi(const void *A, void *mesh, const void *tw, hipStream_t st) {
