@@ -100,6 +100,10 @@ struct dj_ctx {
     void *vb_pl = nullptr;  // (3, (n_resample n)^3): the cotangent of the resampled points' positions
     void *ckpt = nullptr;  // (steps, 2, 3 n^3): Psi and Pi before each step
     int ck_steps = 0;
+    // each checkpointed step's particle tile lists (the sort of its force), kept for the reverse pass when
+    // the device has room; ck_tl_ok[s]: slot s holds step s's lists of the last checkpointed run
+    std::vector<TileLists> ck_tl;
+    std::vector<char> ck_tl_ok;
     std::vector<double> ck_dd1, ck_alpha, ck_beta, ck_dd2;
     // learned correction: FP32 master parameters, Adam moments, BF16 lane-ordered operands, activations
     NnOps nn{};
@@ -354,6 +358,7 @@ void free_all(dj_ctx *c) {
     ic_work_free(c->icw);
     free_lists(c->tl);
     free_lists(c->tlr);
+    for (auto &t : c->ck_tl) free_lists(t);
     for (auto &p : c->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
     for (auto e : c->pool) (void)hipEventDestroy(e);
     if (c->st) (void)hipStreamDestroy(c->st);
@@ -1176,15 +1181,16 @@ dj_status adj_buffers(dj_ctx *c) {
 // Fbar_a goes through the same Poisson operator (self-adjoint), and one gather adds the weight-gradient terms.
 // tail (nullable): the step's end (AdjTail), applied in the last pass's gather when the path allows it
 // (FD gradient, no resampling), which *tail_done reports; abar is then overwritten.
+// presorted: the context's tile lists already hold psi's sort (the checkpointed step's own lists)
 dj_status vjp_force(dj_ctx *c, const void *psi, void *abar, void *psibar, const AdjTail *tail = nullptr,
-                    bool *tail_done = nullptr) {
+                    bool *tail_done = nullptr, bool presorted = false) {
     if (tail_done) *tail_done = false;
     const dj_pm_config &cfg = c->cfg;
     const bool fixed = cfg.deposit == DJ_DEPOSIT_FIXED64;
     const int ns = c->nshifts, N = c->res;
     void *dens = fixed ? c->meshfix : c->mesh;
     const double fscale = std::ldexp(1.0, -(cfg.fixed_frac_bits ? cfg.fixed_frac_bits : 32));
-    dj_status r = tile_sort(c, psi, c->d_shifts, ns, cfg.worder);
+    dj_status r = presorted ? DJ_OK : tile_sort(c, psi, c->d_shifts, ns, cfg.worder);
     if (r) return r;
     for (int s = 0; s < ns; s++) {
         TileArgs ta = tile_args(c, psi, c->d_shifts, ns);
@@ -1386,6 +1392,28 @@ dj_status dj_run_checkpointed(dj_ctx *c, int32_t n_steps, const double *dd1, con
     c->ck_alpha.assign(alpha, alpha + n_steps);
     c->ck_beta.assign(beta, beta + n_steps);
     c->ck_dd2.assign(dd2, dd2 + n_steps);
+    // Each step's tile lists are kept (swapped out of the context after its force) when the device has room
+    // for them: about twice the current sort's reserve per step, with 8 GB to spare
+    bool keep = c->cfg.n_resample <= 1;
+    if (keep) {
+        size_t fr = 0, tot = 0;
+        const size_t per = (c->tl.ent_cap ? c->tl.ent_cap : 4 * (size_t)c->n3) * c->tiles.entry_bytes;
+        size_t have = 0;
+        for (auto &t : c->ck_tl) have += t.ent_cap * c->tiles.entry_bytes;
+        keep = hipMemGetInfo(&fr, &tot) == hipSuccess && fr + have > 2 * per * (size_t)n_steps + ((size_t)8 << 30);
+    }
+    if (!keep) {
+        for (auto &t : c->ck_tl) free_lists(t);
+        c->ck_tl.clear();
+    } else {
+        if ((int)c->ck_tl.size() < n_steps) c->ck_tl.resize(n_steps);
+        for (int s = 0; s < n_steps; s++)
+            if (!c->ck_tl[s].cnt) {
+                dj_status ra = alloc_lists(c, c->ck_tl[s], c->tl.ent_cap);
+                if (ra) return ra;
+            }
+    }
+    c->ck_tl_ok.assign(n_steps, 0);
     for (int s = 0; s < n_steps; s++) {
         // the step's checkpoint (psi, mom before its drift) written by the drift as it reads them
         char *ck = (char *)c->ckpt + 2 * pb * (size_t)s;
@@ -1393,6 +1421,10 @@ dj_status dj_run_checkpointed(dj_ctx *c, int32_t n_steps, const double *dd1, con
         KickArgs k{alpha[s], beta[s], dd2[s], 0.0, 0, c->nshifts};
         dj_status r = force(c, c->psi, FORCE_KICK, k);
         if (r) return r;
+        if (keep) {
+            std::swap(c->tl, c->ck_tl[s]);  // the next sort reserves from the swapped-in set's counts
+            c->ck_tl_ok[s] = 1;
+        }
     }
     HIPCHK(c, hipStreamSynchronize(c->st));
     drain_prof(c);
@@ -1431,7 +1463,11 @@ dj_status dj_vjp_run(dj_ctx *c, void *psi_bar, void *pi_bar) {
             tail.dd2 = c->ck_dd2[s - 1];
         }
         bool done = false;
-        r = vjp_force(c, c->vb_psi1, c->vb_abar, c->vb_psibar, &tail, &done);
+        // the step's kept lists: the drifted checkpoint is the position its force sorted, bit for bit
+        const bool kept = s < (int)c->ck_tl_ok.size() && c->ck_tl_ok[s] && s < (int)c->ck_tl.size();
+        if (kept) std::swap(c->tl, c->ck_tl[s]);
+        r = vjp_force(c, c->vb_psi1, c->vb_abar, c->vb_psibar, &tail, &done, kept);
+        if (kept) std::swap(c->tl, c->ck_tl[s]);
         if (r) return r;
         if (!done) c->adj.drift(c->vb_psibar, c->vb_pibar, np3, c->ck_dd1[s], c->st);
         kicked = done && s > 0;
