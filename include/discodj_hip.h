@@ -220,7 +220,9 @@ dj_status dj_vjp_deposit_device(dj_ctx *ctx, const void *d_mesh_bar);
  * dj_vjp_run / dj_vjp_ics_lpt(_device).  Replaces dj_deposit, a loss on the host and dj_vjp_deposit. */
 dj_status dj_density_loss_device(dj_ctx *ctx, const double *d_target, double *loss_out);
 /* dj_run's n_steps DKD steps from the current state, keeping (Psi, Pi) before each step on the device
- * (2 x 3 n^3 values per step) for dj_vjp_run; the final state is the context's state as after dj_run */
+ * (2 x 3 n^3 values per step) for dj_vjp_run, and, when the device has room for them, each step's tile
+ * lists (about 2.5 n^3 entries of 4 values per step), which dj_vjp_run then uses instead of sorting the
+ * step's positions again; the final state is the context's state as after dj_run */
 dj_status dj_run_checkpointed(dj_ctx *ctx, int32_t n_steps, const double *dd1, const double *alpha,
                               const double *beta, const double *dd2);
 /* reverse of the last dj_run_checkpointed: (psi_bar, pi_bar), the cotangents of the final (Psi, Pi), are
